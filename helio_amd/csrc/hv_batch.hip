@@ -1,0 +1,109 @@
+// C ABI of the batched multi-view cull and the timing/shape hooks of the
+// plain pass. Both run the kernels of hv_cull.hip through CullBatch.
+#include "hv_cull.h"
+
+using namespace hv;
+
+struct hv_batch {
+    CullBatch* batch = nullptr;
+    bool prepared = false;
+};
+
+static hv_status set_shape(CullBatch* b, uint32_t wg, uint32_t ipt) {
+    const bool wg_ok = wg == 64 || wg == 128 || wg == 256 || wg == 512;
+    const bool ipt_ok = ipt == 1 || ipt == 2 || ipt == 4 || ipt == 8;
+    if (!wg_ok || !ipt_ok) return set_error(HV_EINVAL, "shape wg=%u ipt=%u not supported", wg, ipt);
+    b->shape.wg = wg;
+    b->shape.ipt = ipt;
+    return HV_OK;
+}
+
+extern "C" {
+
+hv_status hv_batch_create(hv_ctx* ctx, const hv_cull_config* config, uint32_t max_views, hv_batch** out) {
+    if (!ctx || !config || !out) return set_error(HV_EINVAL, "hv_batch_create: NULL argument");
+    if (!max_views || max_views > kMaxViews)
+        return set_error(HV_ELIMIT, "hv_batch_create: max_views %u not in 1..%u", max_views, kMaxViews);
+    if (config->lod_count < 1 || config->lod_count > HV_MAX_LOD_LEVELS)
+        return set_error(HV_EINVAL, "hv_batch_create: lod_count %u not in 1..8", config->lod_count);
+    hv_batch* b = new hv_batch();
+    b->batch = new CullBatch(ctx, *config, max_views);
+    *out = b;
+    return HV_OK;
+}
+
+void hv_batch_destroy(hv_batch* b) {
+    if (!b) return;
+    delete b->batch;
+    delete b;
+}
+
+hv_status hv_batch_set_shape(hv_batch* b, uint32_t wg, uint32_t ipt) {
+    if (!b) return set_error(HV_EINVAL, "hv_batch_set_shape: NULL batch");
+    b->prepared = false;
+    return set_shape(b->batch, wg, ipt);
+}
+
+hv_status hv_batch_prepare(hv_batch* b, const hv_view_request* views, uint32_t count) {
+    if (!b || (!views && count)) return set_error(HV_EINVAL, "hv_batch_prepare: NULL argument");
+    std::vector<ViewDesc> d(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        d[i].scene = views[i].scene;
+        d[i].camera = views[i].camera;
+        d[i].first = views[i].first;
+        d[i].count = views[i].count;
+        d[i].hiz = views[i].hiz;
+    }
+    b->prepared = false;
+    HV_TRY(b->batch->prepare(d.data(), count));
+    b->prepared = count > 0;
+    return HV_OK;
+}
+
+hv_status hv_batch_execute(hv_batch* b, void* stream, uint32_t use_graph) {
+    if (!b || !b->prepared) return set_error(HV_ESTATE, "hv_batch_execute: not prepared");
+    hipStream_t s = stream ? (hipStream_t)stream : b->batch->ctx()->stream;
+    return use_graph ? b->batch->execute_graph(s) : b->batch->execute(s);
+}
+
+hv_status hv_batch_execute_timed(hv_batch* b, void* stream, float ms[3]) {
+    if (!b || !b->prepared || !ms) return set_error(HV_ESTATE, "hv_batch_execute_timed: not prepared");
+    hipStream_t s = stream ? (hipStream_t)stream : b->batch->ctx()->stream;
+    return b->batch->execute(s, ms);
+}
+
+hv_status hv_batch_outputs(const hv_batch* b, uint32_t view, hv_cull_outputs* out) {
+    if (!b || !b->prepared) return set_error(HV_ESTATE, "hv_batch_outputs: not prepared");
+    return b->batch->outputs(view, out);
+}
+
+hv_status hv_batch_read(hv_batch* b, uint32_t view, uint32_t* ids, uint32_t ids_capacity, uint32_t* id_count,
+                        hv_draw_indexed_indirect* args, uint32_t args_capacity, uint32_t stats[HV_STAT_COUNT]) {
+    if (!b || !b->prepared) return set_error(HV_ESTATE, "hv_batch_read: not prepared");
+    return b->batch->read(view, ids, ids_capacity, id_count, args, args_capacity, stats);
+}
+
+hv_status hv_batch_wait(hv_batch* b) {
+    if (!b) return set_error(HV_EINVAL, "hv_batch_wait: NULL batch");
+    return b->batch->wait();
+}
+
+hv_status hv_batch_device_bytes(const hv_batch* b, uint64_t* bytes) {
+    if (!b || !bytes) return set_error(HV_EINVAL, "hv_batch_device_bytes: NULL argument");
+    *bytes = b->batch->device_bytes();
+    return HV_OK;
+}
+
+hv_status hv_inst_cull_execute_timed(hv_inst_cull_pass* pass, void* stream, float ms[3]) {
+    if (!pass || !pass->prepared || !ms) return set_error(HV_ESTATE, "hv_inst_cull_execute_timed: not prepared");
+    hipStream_t s = stream ? (hipStream_t)stream : pass->batch->ctx()->stream;
+    return pass->batch->execute(s, ms);
+}
+
+hv_status hv_inst_cull_set_shape(hv_inst_cull_pass* pass, uint32_t wg, uint32_t ipt) {
+    if (!pass) return set_error(HV_EINVAL, "hv_inst_cull_set_shape: NULL pass");
+    pass->prepared = false;
+    return set_shape(pass->batch, wg, ipt);
+}
+
+}  // extern "C"

@@ -1,0 +1,115 @@
+// Device-visible view record and host-side batch runtime of the per-instance
+// compacting cull (frustum + sub-pixel [+ Hi-Z occlusion] + distance LOD +
+// bucket-major compaction into DrawIndexedIndirect args).
+#pragma once
+
+#include <vector>
+
+#include "hv_internal.h"
+
+namespace hv {
+
+constexpr uint32_t kMaxViews = 1024;      // views per batch launch
+constexpr uint32_t kSpan = 16384;         // staged entries per count/scatter block
+constexpr uint32_t kMaxBuckets = 14336;   // mesh_count * lod_count per view (LDS histogram, < 64 KiB with statics)
+constexpr uint32_t kCounterStride = 16;   // u32 counters per view
+
+// Per-view counters (u32, kCounterStride per view):
+//   [0..7] final stats (layout of indirect_dispatch.wgsl:69-77), written once
+//   [8]    visible ids (after occlusion), written once
+//   [9]    staged entries (atomic during classify; reset by the scatter kernel)
+//   [10]   finished count blocks (atomic; reset by the scatter kernel)
+enum : uint32_t { kCtrVisible = 8, kCtrStaged = 9, kCtrDone = 10 };
+
+struct alignas(16) CullView {
+    float planes[24];        // extract_frustum_planes(view_proj)
+    float vp[16];            // view_proj, column-major
+    float p00, p11;          // proj[0][0], proj[1][1]
+    float cam[3];            // camera position
+    float lod_d[7];          // LOD distance thresholds
+    uint32_t lod_count;      // 1..8
+    uint32_t flags;          // HV_CULL_*
+    uint32_t nb;             // buckets = mesh_count * lod_count
+    uint32_t first, count;   // candidate range in the scene
+    uint32_t item_begin;     // first classify work item of this view
+    uint32_t n_items;        // classify work items of this view
+    uint32_t screen_h;       // occlusion: r_px scale
+    uint32_t hiz_mip_count;  // CullParams.hiz_mip_count = mip_levels(screen)
+    uint32_t hiz_w, hiz_h, hiz_levels;
+    uint32_t hiz_off[12];    // float offsets of pyramid levels 1.. in hiz_packed
+    const float4* bounds;
+    const uint32_t* meta;
+    const hv_instance_aabb* aabb;     // NULL: sphere_to_aabb(bounds)
+    const hv_mesh_lod* mesh_lods;
+    const float* hiz0;
+    const float* hiz_packed;
+    uint2* stage;                      // (instance id, bucket), capacity count
+    uint32_t* ids;                     // capacity count
+    hv_draw_indexed_indirect* args;    // nb
+    uint32_t* counters;                // kCounterStride
+    uint32_t* item_stats;              // n_items x 8
+    uint32_t* block_hist;              // max_blocks x nb
+    uint32_t max_blocks;
+    uint32_t _pad[3];
+};
+
+struct BatchHdr {
+    uint32_t n_views;
+    uint32_t n_items;
+    uint32_t _pad[2];
+};
+
+// One view of a batch, as described by the host.
+struct ViewDesc {
+    const hv_scene* scene = nullptr;
+    const hv_camera_uniforms* camera = nullptr;
+    uint32_t first = 0, count = 0;
+    const hv_hiz* hiz = nullptr;
+};
+
+// Host runtime of one batch of views (a single view for the plain pass).
+class CullBatch {
+  public:
+    CullBatch(hv_ctx* ctx, const hv_cull_config& cfg, uint32_t max_views);
+    ~CullBatch();
+    hv_status prepare(const ViewDesc* views, uint32_t n);
+    // Eager launches on `stream` (the plain path). If `ms` is non-NULL the
+    // three kernels are bracketed by events and their durations returned
+    // (ms[0] classify, ms[1] count+scan, ms[2] scatter); this synchronises.
+    hv_status execute(hipStream_t stream, float* ms = nullptr);
+    // The same three kernels captured once into a hipGraph and replayed; the
+    // view table lives in device memory, so a new prepare() needs no re-capture.
+    hv_status execute_graph(hipStream_t stream);
+    hv_status outputs(uint32_t view, hv_cull_outputs* out) const;
+    hv_status read(uint32_t view, uint32_t* ids, uint32_t ids_capacity, uint32_t* id_count,
+                   hv_draw_indexed_indirect* args, uint32_t args_capacity, uint32_t* stats);
+    uint32_t n_views() const { return n_views_; }
+    hv_status wait();
+    uint64_t device_bytes() const;
+    const CullView& view(uint32_t v) const { return host_views_[v]; }
+    hv_ctx* ctx() const { return ctx_; }
+    CullShape shape;
+
+  private:
+    hv_status launch(hipStream_t stream, float* ms);
+    hv_ctx* ctx_;
+    hv_cull_config cfg_;
+    uint32_t max_views_;
+    uint32_t n_views_ = 0, n_items_ = 0, max_nb_ = 0;
+    uint32_t grid1_ = 0, grid23_ = 0;
+    std::vector<CullView> host_views_;
+    DevBuf d_table_, d_stage_, d_ids_, d_args_, d_counters_, d_item_stats_, d_block_hist_;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    uint32_t graph_key_[4] = {0, 0, 0, 0};
+    hipEvent_t ev_[4] = {};
+    hipEvent_t done_ = nullptr;    // recorded after every launch
+    bool done_recorded_ = false;
+};
+
+}  // namespace hv
+
+struct hv_inst_cull_pass {
+    hv::CullBatch* batch = nullptr;
+    bool prepared = false;
+};

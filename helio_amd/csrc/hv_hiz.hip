@@ -1,0 +1,316 @@
+// Hi-Z pyramid for gfx950 in one launch.
+//
+// Reference: hiz_depth_copy.wgsl:13-19 (mip 0 = depth) and hiz_build.wgsl:12-23
+// dispatched once per level (crates/helio-pass-hiz/src/lib.rs:204-225,516-531):
+//   dst(x, y) = op(op(s(2x, 2y), s(min(2x+1, sw-1), 2y)),
+//                  op(s(2x, min(2y+1, sh-1)), s(min(2x+1, sw-1), min(2y+1, sh-1))))
+// with dst size max(1, w >> (k+1)), so an odd trailing row/column is dropped.
+// Level 0 is the depth buffer itself (no copy).
+//
+// Here one workgroup reduces a TILE x TILE block of level 0 through
+// log2(TILE) levels (the first one or two in registers from float4 loads, the
+// rest through a 16x16 LDS tile); the last workgroup to finish (device-scope
+// counter, reset by itself) builds the remaining small levels in LDS. Every
+// texel is computed from the previous level's texels with the reference's
+// formula and operand order, so the chain is bit-identical for any input.
+#include <algorithm>
+
+#include "hv_internal.h"
+
+namespace hv {
+namespace {
+
+struct HizArgs {
+    const float* level0;
+    float* packed;
+    uint32_t* counter;
+    uint32_t w, h, levels;
+    uint32_t n_tiles;
+    uint32_t off[16];  // float offset of level m >= 1 in packed
+};
+
+template <bool MIN>
+__device__ __forceinline__ float op(float a, float b) {
+    return MIN ? ((b < a) ? b : a) : ((a < b) ? b : a);
+}
+template <bool MIN>
+__device__ __forceinline__ float reduce4(float s00, float s10, float s01, float s11) {
+    return op<MIN>(op<MIN>(s00, s10), op<MIN>(s01, s11));
+}
+
+__device__ __forceinline__ uint32_t lvl(uint32_t d, uint32_t k) { return max(d >> k, 1u); }
+
+__device__ __forceinline__ void put(const HizArgs& a, uint32_t k, uint32_t x, uint32_t y, float v) {
+    const uint32_t dw = lvl(a.w, k), dh = lvl(a.h, k);
+    if (x < dw && y < dh) a.packed[a.off[k] + (size_t)y * dw + x] = v;
+}
+
+// Reduce a whole level held in LDS (sw x sh texels) into the next one, with
+// the reference's clamps.
+template <bool MIN>
+__device__ __forceinline__ float lds_reduce(const float* src, uint32_t pitch, uint32_t x, uint32_t y, uint32_t sw,
+                                            uint32_t sh) {
+    const uint32_t x0 = 2 * x, y0 = 2 * y;
+    const uint32_t x1 = min(x0 + 1, sw - 1), y1 = min(y0 + 1, sh - 1);
+    return reduce4<MIN>(src[y0 * pitch + x0], src[y0 * pitch + x1], src[y1 * pitch + x0], src[y1 * pitch + x1]);
+}
+
+template <int TILE, bool MIN, bool VEC4>
+__global__ __launch_bounds__(256) void k_hiz(HizArgs a) {
+    constexpr uint32_t PER = TILE / 16;           // level-0 texels per thread per axis (4 or 2)
+    constexpr uint32_t TL = (TILE == 64) ? 6 : 5; // levels built per tile
+    __shared__ float s_a[16 * 16];
+    __shared__ float s_b[8 * 8];
+    __shared__ float s_tail_a[64 * 64];
+    __shared__ float s_tail_b[32 * 32];
+    __shared__ uint32_t s_last;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t tx = blockIdx.x, ty = blockIdx.y;
+    const uint32_t lx = tid & 15u, ly = tid >> 4;
+    const uint32_t top = a.levels - 1u;  // last level index
+
+    // ---- level 0 block of PER x PER texels -> registers ----
+    float v0[PER][PER];
+    const uint32_t gx0 = tx * TILE + lx * PER, gy0 = ty * TILE + ly * PER;
+#pragma unroll
+    for (uint32_t r = 0; r < PER; ++r) {
+        const uint32_t gy = gy0 + r;
+        if (VEC4 && PER == 4 && gy < a.h && gx0 + 3 < a.w) {
+            const float4 q = *reinterpret_cast<const float4*>(a.level0 + (size_t)gy * a.w + gx0);
+            v0[r][0] = q.x; v0[r][1] = q.y; v0[r][2] = q.z; v0[r][3] = q.w;
+        } else {
+#pragma unroll
+            for (uint32_t c = 0; c < PER; ++c) {
+                const uint32_t gx = gx0 + c;
+                v0[r][c] = (gy < a.h && gx < a.w) ? a.level0[(size_t)gy * a.w + gx] : 0.0f;
+            }
+        }
+    }
+    // ---- level 1 in registers ----
+    constexpr uint32_t P1 = PER / 2;
+    float v1[P1][P1];
+    const uint32_t sw0 = a.w, sh0 = a.h;
+#pragma unroll
+    for (uint32_t r = 0; r < P1; ++r)
+#pragma unroll
+        for (uint32_t c = 0; c < P1; ++c) {
+            const uint32_t cx1 = (sw0 == 1u) ? 2 * c : 2 * c + 1, ry1 = (sh0 == 1u) ? 2 * r : 2 * r + 1;
+            v1[r][c] = reduce4<MIN>(v0[2 * r][2 * c], v0[2 * r][cx1], v0[ry1][2 * c], v0[ry1][cx1]);
+            if (top >= 1u) put(a, 1, tx * (TILE / 2) + lx * P1 + c, ty * (TILE / 2) + ly * P1 + r, v1[r][c]);
+        }
+    // ---- level 2 in registers (TILE 64) -> LDS 16x16 ----
+    uint32_t k;  // level held in s_a
+    if (PER == 4) {
+        const uint32_t sw1 = lvl(a.w, 1), sh1 = lvl(a.h, 1);
+        const uint32_t c1 = (sw1 == 1u) ? 0 : 1, r1 = (sh1 == 1u) ? 0 : 1;
+        const float v2 = reduce4<MIN>(v1[0][0], v1[0][c1 % P1], v1[r1 % P1][0], v1[r1 % P1][c1 % P1]);
+        if (top >= 2u) put(a, 2, tx * 16 + lx, ty * 16 + ly, v2);
+        s_a[ly * 16 + lx] = v2;
+        k = 2;
+    } else {
+        s_a[ly * 16 + lx] = v1[0][0];
+        k = 1;
+    }
+    __syncthreads();
+    // ---- remaining tile levels through LDS: 16 -> 8 -> 4 -> 2 -> 1 ----
+    float* src = s_a;
+    float* dst = s_b;
+    uint32_t edge = 16;  // texels per axis of the tile at level k
+    for (; k < TL; ++k) {
+        const uint32_t ne = edge / 2;
+        if (tid < ne * ne) {
+            const uint32_t x = tid % ne, y = tid / ne;
+            const uint32_t sw = lvl(a.w, k), sh = lvl(a.h, k);
+            // In-tile clamp: a level one texel wide stays on its texel.
+            const uint32_t x1 = (sw == 1u) ? 2 * x : 2 * x + 1, y1 = (sh == 1u) ? 2 * y : 2 * y + 1;
+            const float v = reduce4<MIN>(src[2 * y * edge + 2 * x], src[2 * y * edge + x1], src[y1 * edge + 2 * x],
+                                         src[y1 * edge + x1]);
+            if (k + 1 <= top) put(a, k + 1, tx * ne + x, ty * ne + y, v);
+            dst[y * ne + x] = v;
+        }
+        __syncthreads();
+        float* t = src;
+        src = dst;
+        dst = t;
+        edge = ne;
+    }
+    if (top <= TL) return;
+    // ---- tail: the last tile to finish builds levels TL+1 .. top ----
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) s_last = (atomicAdd(a.counter, 1u) + 1u == a.n_tiles) ? 1u : 0u;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    uint32_t sw = lvl(a.w, TL), sh = lvl(a.h, TL);  // <= 64 x 64 (checked on the host)
+    const float* lv = a.packed + a.off[TL];
+    for (uint32_t i = tid; i < sw * sh; i += 256u) s_tail_a[i] = lv[i];
+    __syncthreads();
+    // Ping-pong: level TL in the 64x64 buffer, TL+1 (<= 32x32) in the other, ...
+    for (uint32_t kk = TL; kk < top; ++kk) {
+        const bool even = ((kk - TL) & 1u) == 0u;
+        const float* ts = even ? s_tail_a : s_tail_b;
+        float* td = even ? s_tail_b : s_tail_a;
+        const uint32_t dw = lvl(a.w, kk + 1), dh = lvl(a.h, kk + 1);
+        for (uint32_t i = tid; i < dw * dh; i += 256u) {
+            const uint32_t x = i % dw, y = i / dw;
+            const float v = lds_reduce<MIN>(ts, sw, x, y, sw, sh);
+            a.packed[a.off[kk + 1] + i] = v;
+            td[i] = v;
+        }
+        __syncthreads();
+        sw = dw;
+        sh = dh;
+    }
+    if (tid == 0) *a.counter = 0u;
+}
+
+using HizFn = void (*)(HizArgs);
+
+template <int TILE>
+HizFn pick(bool min, bool vec4) {
+    if (min) return vec4 ? k_hiz<TILE, true, true> : k_hiz<TILE, true, false>;
+    return vec4 ? k_hiz<TILE, false, true> : k_hiz<TILE, false, false>;
+}
+
+}  // namespace
+}  // namespace hv
+
+using namespace hv;
+
+static hipStream_t pick_stream(hv_ctx* ctx, void* s) { return s ? static_cast<hipStream_t>(s) : ctx->stream; }
+
+extern "C" {
+
+hv_status hv_hiz_create(hv_ctx* ctx, uint32_t width, uint32_t height, uint32_t mode, hv_hiz** out) {
+    if (!ctx || !out) return set_error(HV_EINVAL, "hv_hiz_create: NULL argument");
+    if (!width || !height) return set_error(HV_EINVAL, "hv_hiz_create: empty %ux%u", width, height);
+    if (width > 4096 || height > 4096) return set_error(HV_ELIMIT, "hv_hiz_create: %ux%u exceeds 4096", width, height);
+    if (mode != HV_HIZ_MAX && mode != HV_HIZ_MIN) return set_error(HV_EINVAL, "hv_hiz_create: mode %u", mode);
+    HV_HIP(hipSetDevice(ctx->device));
+    hv_hiz* z = new hv_hiz();
+    z->ctx = ctx;
+    z->width = width;
+    z->height = height;
+    z->mode = mode;
+    // Texture mip count min(bits(max(w, h)), 12): crates/helio-core/src/graph/resource_lifetime.rs:83-88.
+    z->levels = std::min(hv_mip_levels(width, height), 12u);
+    uint64_t off = 0;
+    for (uint32_t m = 1; m < z->levels; ++m) {
+        z->level_off[m] = off;
+        off += (uint64_t)std::max(width >> m, 1u) * std::max(height >> m, 1u);
+    }
+    hv_status st = z->packed.reserve(std::max<uint64_t>(off, 1) * 4, false, ctx->stream);
+    if (st == HV_OK) st = z->counter.reserve(4, false, ctx->stream);
+    if (st == HV_OK) {
+        hipError_t e = hipMemsetAsync(z->counter.ptr, 0, 4, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = hip_error(e, "hv_hiz_create", __FILE__, __LINE__);
+    }
+    if (st != HV_OK) {
+        hv_hiz_destroy(z);
+        return st;
+    }
+    *out = z;
+    return HV_OK;
+}
+
+void hv_hiz_destroy(hv_hiz* z) {
+    if (!z) return;
+    (void)hipSetDevice(z->ctx->device);
+    (void)hipDeviceSynchronize();
+    z->own_level0.release();
+    z->packed.release();
+    z->counter.release();
+    delete z;
+}
+
+hv_status hv_hiz_bind_depth(hv_hiz* z, const float* depth_device) {
+    if (!z) return set_error(HV_EINVAL, "hv_hiz_bind_depth: NULL pyramid");
+    if (depth_device) {
+        z->level0 = depth_device;
+    } else {
+        HV_TRY(z->own_level0.reserve((uint64_t)z->width * z->height * 4, false, z->ctx->stream));
+        z->level0 = z->own_level0.as<float>();
+    }
+    z->last_generation = ~0ull;
+    return HV_OK;
+}
+
+hv_status hv_hiz_upload_depth(hv_hiz* z, const float* depth_host, void* stream) {
+    if (!z || !depth_host) return set_error(HV_EINVAL, "hv_hiz_upload_depth: NULL argument");
+    HV_HIP(hipSetDevice(z->ctx->device));
+    HV_TRY(z->own_level0.reserve((uint64_t)z->width * z->height * 4, false, z->ctx->stream));
+    const hipStream_t s = pick_stream(z->ctx, stream);
+    HV_HIP(hipMemcpyAsync(z->own_level0.ptr, depth_host, (size_t)z->width * z->height * 4, hipMemcpyHostToDevice, s));
+    HV_HIP(hipStreamSynchronize(s));
+    // New depth content alone does not force a rebuild: like the reference,
+    // the camera generation decides (crates/helio-pass-hiz/src/lib.rs:472-478).
+    z->level0 = z->own_level0.as<float>();
+    return HV_OK;
+}
+
+hv_status hv_hiz_prepare(hv_hiz* z, uint64_t camera_generation) {
+    if (!z) return set_error(HV_EINVAL, "hv_hiz_prepare: NULL pyramid");
+    z->skip_next = (camera_generation != ~0ull && camera_generation == z->last_generation);
+    z->last_generation = camera_generation;
+    return HV_OK;
+}
+
+hv_status hv_hiz_execute(hv_hiz* z, void* stream) {
+    if (!z) return set_error(HV_EINVAL, "hv_hiz_execute: NULL pyramid");
+    if (!z->level0) return set_error(HV_ESTATE, "hv_hiz_execute: no depth bound");
+    if (z->skip_next) {
+        z->skip_next = false;
+        return HV_OK;
+    }
+    if (z->levels < 2) return HV_OK;
+    HV_HIP(hipSetDevice(z->ctx->device));
+    uint32_t tile = z->ctx->hiz_shape.tile;
+    if (tile != 32 && tile != 64) tile = 64;
+    // The tail workgroup holds level log2(tile) in a 64x64 LDS tile.
+    if (tile == 32 && (z->width > 2048 || z->height > 2048)) tile = 64;
+    HizArgs a{};
+    a.level0 = z->level0;
+    a.packed = z->packed.as<float>();
+    a.counter = z->counter.as<uint32_t>();
+    a.w = z->width;
+    a.h = z->height;
+    a.levels = z->levels;
+    for (int m = 0; m < 16; ++m) a.off[m] = (uint32_t)z->level_off[m];
+    const dim3 grid((z->width + tile - 1) / tile, (z->height + tile - 1) / tile);
+    a.n_tiles = grid.x * grid.y;
+    const bool vec4 = (z->width % 4u) == 0u && (reinterpret_cast<uintptr_t>(z->level0) % 16u) == 0u;
+    HizFn fn = (tile == 64) ? pick<64>(z->mode == HV_HIZ_MIN, vec4) : pick<32>(z->mode == HV_HIZ_MIN, vec4);
+    hipLaunchKernelGGL(fn, grid, dim3(256), 0, pick_stream(z->ctx, stream), a);
+    HV_HIP(hipGetLastError());
+    return HV_OK;
+}
+
+hv_status hv_hiz_levels(const hv_hiz* z, uint32_t* level_count) {
+    if (!z || !level_count) return set_error(HV_EINVAL, "hv_hiz_levels: NULL argument");
+    *level_count = z->levels;
+    return HV_OK;
+}
+
+hv_status hv_hiz_level_size(const hv_hiz* z, uint32_t level, uint32_t* width, uint32_t* height) {
+    if (!z || level >= z->levels) return set_error(HV_EINVAL, "hv_hiz_level_size: bad level");
+    if (width) *width = std::max(z->width >> level, 1u);
+    if (height) *height = std::max(z->height >> level, 1u);
+    return HV_OK;
+}
+
+hv_status hv_hiz_read_level(hv_hiz* z, uint32_t level, float* out, uint64_t capacity) {
+    if (!z || !out || level >= z->levels) return set_error(HV_EINVAL, "hv_hiz_read_level: bad argument");
+    const uint64_t w = std::max(z->width >> level, 1u), h = std::max(z->height >> level, 1u);
+    if (capacity < w * h) return set_error(HV_ELIMIT, "hv_hiz_read_level: capacity %llu < %llu",
+                                           (unsigned long long)capacity, (unsigned long long)(w * h));
+    HV_HIP(hipSetDevice(z->ctx->device));
+    HV_HIP(hipDeviceSynchronize());
+    const float* src = level == 0 ? z->level0 : z->packed.as<float>() + z->level_off[level];
+    if (!src) return set_error(HV_ESTATE, "hv_hiz_read_level: no depth bound");
+    HV_HIP(hipMemcpy(out, src, w * h * 4, hipMemcpyDeviceToHost));
+    return HV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,287 @@
+"""Seeded synthetic workloads for the visibility path.
+
+Scenes, cameras and occluder depth are generated on the host and handed to
+both the HIP path and the CPU oracle as the same bytes. Camera matrices follow
+glam 0.33's ``perspective_rh`` / ``look_at_rh`` / ``orthographic_rh`` (as used
+by crates/helio/src/scene/camera.rs:124-125 and crates/libhelio/src/camera.rs:33-56),
+computed in double and rounded to f32; reversed Z swaps near and far.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from . import layouts as L
+
+SEED_BASE = 0xC0FFEE00
+CUBE = 1000.0
+
+
+def rng(config: int, index: int = 0) -> np.random.Generator:
+    """PCG64 stream for (config, index): seed 0xC0FFEE00 + config, sub-stream index."""
+    return np.random.Generator(np.random.PCG64([SEED_BASE + config, index]))
+
+
+# ---------------------------------------------------------------------------
+# Matrices (column-major float64 4x4 arrays, m[col][row]).
+# ---------------------------------------------------------------------------
+
+
+def perspective_rh(fov_y: float, aspect: float, z_near: float, z_far: float) -> np.ndarray:
+    h = math.cos(0.5 * fov_y) / math.sin(0.5 * fov_y)
+    w = h / aspect
+    r = z_far / (z_near - z_far)
+    return np.array([[w, 0, 0, 0], [0, h, 0, 0], [0, 0, r, -1.0], [0, 0, r * z_near, 0]], np.float64)
+
+
+def orthographic_rh(left, right, bottom, top, near, far) -> np.ndarray:
+    rw = 1.0 / (right - left)
+    rh = 1.0 / (top - bottom)
+    r = 1.0 / (near - far)
+    return np.array([[rw + rw, 0, 0, 0], [0, rh + rh, 0, 0], [0, 0, r, 0],
+                     [-(left + right) * rw, -(top + bottom) * rh, r * near, 1.0]], np.float64)
+
+
+def look_to_rh(eye, direction, up) -> np.ndarray:
+    f = np.asarray(direction, np.float64)
+    f = f / np.linalg.norm(f)
+    s = np.cross(f, up)
+    s = s / np.linalg.norm(s)
+    u = np.cross(s, f)
+    eye = np.asarray(eye, np.float64)
+    return np.array([[s[0], u[0], -f[0], 0], [s[1], u[1], -f[1], 0], [s[2], u[2], -f[2], 0],
+                     [-eye @ s, -eye @ u, eye @ f, 1.0]], np.float64)
+
+
+def _mul(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Column-major product a * b (glam Mat4 multiplication)."""
+    return (a.T @ b.T).T
+
+
+def camera_uniforms(view: np.ndarray, proj: np.ndarray, position, near: float, far: float) -> np.ndarray:
+    """GpuCameraUniforms::new (crates/libhelio/src/camera.rs:33-56) from double matrices."""
+    vp = _mul(proj, view)
+    inv = np.linalg.inv(vp.T).T
+    fwd = -view[2][:3]  # -(view.z_axis).truncate(); numpy row c is matrix column c
+    fwd = fwd / np.linalg.norm(fwd)
+    cam = np.zeros(1, L.CAMERA)
+    cam["view"] = view.reshape(16).astype(np.float32)
+    cam["proj"] = proj.reshape(16).astype(np.float32)
+    cam["view_proj"] = vp.reshape(16).astype(np.float32)
+    cam["inv_view_proj"] = inv.reshape(16).astype(np.float32)
+    cam["position_near"] = np.array([*position, near], np.float32)
+    cam["forward_far"] = np.array([*fwd, far], np.float32)
+    cam["prev_view_proj"] = vp.reshape(16).astype(np.float32)
+    return cam
+
+
+@dataclass
+class View:
+    camera: np.ndarray          # one L.CAMERA record
+    view_proj64: np.ndarray     # double column-major 4x4 (for depth synthesis)
+    position: np.ndarray        # double xyz
+    kind: str = "perspective"
+
+
+def _yaw_pitch_dir(yaw: float, pitch: float) -> np.ndarray:
+    return np.array([math.cos(pitch) * math.cos(yaw), math.sin(pitch), math.cos(pitch) * math.sin(yaw)])
+
+
+def perspective_view(g: np.random.Generator, fov_deg=60.0, aspect=16 / 9, near=0.1, far=1000.0,
+                     reversed_z=False, position=None, direction=None) -> View:
+    pos = g.uniform(0.0, CUBE, 3) if position is None else np.asarray(position, np.float64)
+    if direction is None:
+        yaw = g.uniform(0.0, 2 * math.pi)
+        pitch = g.uniform(-math.pi / 4, math.pi / 4)
+        direction = _yaw_pitch_dir(yaw, pitch)
+    pos32 = pos.astype(np.float32).astype(np.float64)
+    view = look_to_rh(pos32, direction, np.array([0.0, 1.0, 0.0]))
+    fov = math.radians(fov_deg)
+    proj = perspective_rh(fov, aspect, far, near) if reversed_z else perspective_rh(fov, aspect, near, far)
+    cam = camera_uniforms(view, proj, pos32, near, far)
+    return View(cam, _mul(proj, view), pos32, "perspective")
+
+
+def ortho_view(g: np.random.Generator, center, half_extent: float, depth: float, direction) -> View:
+    """A CSM-style orthographic view looking along `direction` at `center`."""
+    direction = np.asarray(direction, np.float64)
+    direction = direction / np.linalg.norm(direction)
+    eye = np.asarray(center, np.float64) - direction * depth * 0.5
+    eye32 = eye.astype(np.float32).astype(np.float64)
+    up = np.array([0.0, 1.0, 0.0]) if abs(direction[1]) < 0.99 else np.array([1.0, 0.0, 0.0])
+    view = look_to_rh(eye32, direction, up)
+    proj = orthographic_rh(-half_extent, half_extent, -half_extent, half_extent, 0.1, depth)
+    cam = camera_uniforms(view, proj, eye32, 0.1, depth)
+    return View(cam, _mul(proj, view), eye32, "ortho")
+
+
+def cube_face_views(position, near=0.1, far=200.0) -> list[View]:
+    """Six 90-degree 1:1 views of a point light."""
+    dirs = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+    ups = [(0, -1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (0, -1, 0), (0, -1, 0)]
+    pos32 = np.asarray(position, np.float64).astype(np.float32).astype(np.float64)
+    out = []
+    for d, u in zip(dirs, ups):
+        view = look_to_rh(pos32, np.array(d, np.float64), np.array(u, np.float64))
+        proj = perspective_rh(math.pi / 2, 1.0, near, far)
+        out.append(View(camera_uniforms(view, proj, pos32, near, far), _mul(proj, view), pos32, "cube"))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Instance scenes (configs 0/1/3/4 distribution).
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class InstanceScene:
+    bounds: np.ndarray      # (n, 4) f32: center xyz, radius
+    mesh_ids: np.ndarray    # (n,) u32
+    flags: np.ndarray       # (n,) u32 (bit0 = casts shadow)
+    mesh_lods: np.ndarray   # (mesh_count * lod_count,) L.MESH_LOD
+    mesh_count: int
+    lod_count: int
+
+    @property
+    def n(self) -> int:
+        return len(self.bounds)
+
+    def meta(self) -> np.ndarray:
+        """The library's resident meta word: mesh_id | casts_shadow << 31."""
+        return (self.mesh_ids | ((self.flags & 1) << 31)).astype(np.uint32)
+
+    def instances(self) -> np.ndarray:
+        """The same scene as GpuInstanceData records (identity transforms)."""
+        inst = np.zeros(self.n, L.INSTANCE)
+        model = np.zeros((self.n, 16), np.float32)
+        model[:, 0] = model[:, 5] = model[:, 10] = model[:, 15] = 1.0
+        model[:, 12:15] = self.bounds[:, :3]
+        inst["model"] = model
+        inst["normal_mat"][:, [0, 5, 10]] = 1.0
+        inst["bounds"] = self.bounds
+        inst["mesh_id"] = self.mesh_ids
+        inst["flags"] = self.flags
+        inst["lightmap_index"] = 0xFFFFFFFF
+        return inst
+
+
+def mesh_lod_table(g: np.random.Generator, mesh_count: int, lod_count: int) -> np.ndarray:
+    """Per-(mesh, LOD) index ranges: triangles halve per LOD, ranges laid out contiguously."""
+    base_tris = g.integers(64, 4096, mesh_count)
+    t = np.zeros(mesh_count * lod_count, L.MESH_LOD)
+    first_index = 0
+    base_vertex = 0
+    for m in range(mesh_count):
+        for l in range(lod_count):
+            tris = max(int(base_tris[m]) >> l, 1)
+            t[m * lod_count + l] = (3 * tris, first_index, base_vertex)
+            first_index += 3 * tris
+        base_vertex += int(base_tris[m]) * 2
+    return t
+
+
+def instance_scene(n: int, config: int = 0, index: int = 0, mesh_count: int = 256, lod_count: int = 4,
+                   cube: float = CUBE) -> InstanceScene:
+    """Centers U[0, cube)^3, radii log-uniform 0.1-10 m, mesh ids uniform, half shadow casters."""
+    g = rng(config, index)
+    centers = g.uniform(0.0, cube, (n, 3)).astype(np.float32)
+    radii = np.exp(g.uniform(math.log(0.1), math.log(10.0), n)).astype(np.float32)
+    mesh_ids = g.integers(0, mesh_count, n, dtype=np.uint32)
+    flags = g.integers(0, 2, n, dtype=np.uint32)
+    bounds = np.empty((n, 4), np.float32)
+    bounds[:, :3] = centers
+    bounds[:, 3] = radii
+    return InstanceScene(bounds, mesh_ids, flags, mesh_lod_table(g, mesh_count, lod_count), mesh_count, lod_count)
+
+
+# ---------------------------------------------------------------------------
+# Occluder depth (config 1): seeded boxes rasterised per pixel (nearest hit).
+# ---------------------------------------------------------------------------
+
+_SYNTH = None
+
+
+def _synth_lib():
+    global _SYNTH
+    if _SYNTH is None:
+        path = Path(__file__).resolve().parent / "libhv_synth.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} is missing: build with `python -m helio_amd._build`")
+        lib = C.CDLL(str(path))
+        lib.hvs_rasterize_boxes.restype = C.c_int
+        lib.hvs_rasterize_boxes.argtypes = [C.c_void_p] * 3 + [C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32,
+                                                               C.c_int, C.c_void_p, C.c_uint32]
+        _SYNTH = lib
+    return _SYNTH
+
+
+def occluder_boxes(g: np.random.Generator, count: int = 2000, size=(1.0, 50.0), cube: float = CUBE) -> np.ndarray:
+    c = g.uniform(0.0, cube, (count, 3))
+    s = g.uniform(size[0], size[1], (count, 3))
+    boxes = np.empty((count, 6), np.float32)
+    boxes[:, :3] = c - s / 2
+    boxes[:, 3:] = c + s / 2
+    return boxes
+
+
+def rasterize_depth(view: View, boxes: np.ndarray, width: int, height: int, reversed_z: bool,
+                    threads: int = 8) -> np.ndarray:
+    """Depth of the nearest box per pixel centre; cleared to far (0 reversed, 1 standard)."""
+    vp = np.ascontiguousarray(view.view_proj64.reshape(16), np.float64)
+    inv = np.ascontiguousarray(np.linalg.inv(view.view_proj64.T).T.reshape(16), np.float64)
+    cam = np.ascontiguousarray(view.position, np.float64)
+    depth = np.full((height, width), 0.0 if reversed_z else 1.0, np.float32)
+    b = np.ascontiguousarray(boxes, np.float32)
+    rc = _synth_lib().hvs_rasterize_boxes(vp.ctypes.data, inv.ctypes.data, cam.ctypes.data, width, height,
+                                          b.ctypes.data, len(b), int(reversed_z), depth.ctypes.data, threads)
+    if rc != 0:
+        raise RuntimeError("hvs_rasterize_boxes failed")
+    return depth
+
+
+def occluder_view_and_depth(config: int = 1, width: int = 1920, height: int = 1080, boxes: int = 2000,
+                            reversed_z: bool = True, threads: int = 8):
+    """Camera + rasterised depth of `boxes` seeded occluders (config-1 recipe)."""
+    g = rng(config, 1)
+    view = perspective_view(g, reversed_z=reversed_z)
+    occ = occluder_boxes(g, boxes)
+    return view, rasterize_depth(view, occ, width, height, reversed_z, threads), occ
+
+
+# ---------------------------------------------------------------------------
+# Batched multi-view workload (config 3): one resident scene, 64 views.
+# ---------------------------------------------------------------------------
+
+
+def multiview_views(n_instances: int, config: int = 3, n_cameras: int = 16, n_suns: int = 6, cascades: int = 4,
+                    n_points: int = 4, min_count: int = 1000, max_count: int = 2_000_000):
+    """64 views = 16 perspective cameras + 6 directional lights x 4 CSM cascades
+    (orthographic) + 4 point lights x 6 cube faces. Each view culls a contiguous
+    candidate range of length floor(exp(U[ln 1e3, ln 2e6])) at a seeded offset.
+    Returns a list of (View, first, count)."""
+    g = rng(config, 2)
+    views: list[View] = []
+    for _ in range(n_cameras):
+        views.append(perspective_view(g))
+    for _ in range(n_suns):
+        d = g.normal(size=3)
+        d[1] = -abs(d[1]) - 0.3
+        center = g.uniform(0.0, CUBE, 3)
+        for c in range(cascades):
+            half = 25.0 * (4.0 ** c)
+            views.append(ortho_view(g, center, half, 2.0 * CUBE, d))
+    for _ in range(n_points):
+        views.extend(cube_face_views(g.uniform(0.0, CUBE, 3)))
+    out = []
+    max_count = min(max_count, n_instances)
+    min_count = min(min_count, max_count)
+    for v in views:
+        cnt = int(math.floor(math.exp(g.uniform(math.log(min_count), math.log(max_count)))))
+        cnt = max(min(cnt, n_instances), 1)
+        first = int(g.integers(0, n_instances - cnt + 1))
+        out.append((v, first, cnt))
+    return out

@@ -1,0 +1,410 @@
+/*
+ * hv — MI355X-native GPU-driven visibility: instance frustum culling with
+ * distance LOD, the Hi-Z pyramid, Hi-Z occlusion, meshlet (virtual-geometry)
+ * culling and stream compaction into DrawIndexedIndirect argument buffers.
+ *
+ * This is the C ABI of libhv.so. Every struct below keeps the byte layout of the
+ * GPU-shared struct it mirrors in the reference (t4ce/Helio, crate libhelio and
+ * the pass crates); every function cites the reference interface it replaces.
+ * Only plain pointers, sizes and status codes cross this boundary.
+ *
+ * Conventions
+ *  - Every entry point returns hv_status (HV_OK == 0). On failure the message is
+ *    available from hv_last_error() (thread-local). No exception crosses the ABI.
+ *  - `stream` arguments are hipStream_t values passed as void*; NULL means the
+ *    context's own stream.
+ *  - `*_prepare` never touches the device queue except for small uniform
+ *    uploads; `*_execute` only enqueues work (record-only, no host blocking),
+ *    mirroring RenderPass::prepare / RenderPass::execute.
+ *  - Matrices are column-major float[16] (glam::Mat4::to_cols_array()).
+ *  - Handles are not internally synchronised: one caller thread per handle.
+ *    hv_server is the exception and is safe to call from any thread.
+ */
+#ifndef HV_HV_H
+#define HV_HV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Status codes. Replaces helio_core::Result / Error                          */
+/* (crates/helio-core/src/error.rs:37,62) and the assert!/expect panics of    */
+/* the passes (crates/helio-pass-virtual-geometry/src/rendering.rs:836-845).  */
+/* ------------------------------------------------------------------------- */
+typedef int32_t hv_status;
+#define HV_OK 0
+#define HV_EINVAL (-1)   /* bad argument, size or layout */
+#define HV_ENOMEM (-2)   /* device allocation failed or admission refused */
+#define HV_ELIMIT (-3)   /* grid, capacity or table limit exceeded */
+#define HV_EDEVICE (-4)  /* HIP runtime error */
+#define HV_EAGAIN (-5)   /* result not ready yet (server poll) */
+#define HV_ESTATE (-6)   /* call order violated (execute before prepare, ...) */
+
+/* ------------------------------------------------------------------------- */
+/* GPU-shared layouts (libhelio). Sizes are static-asserted in the library.   */
+/* ------------------------------------------------------------------------- */
+
+/* GpuCameraUniforms, crates/libhelio/src/camera.rs:12-29. 368 bytes (the
+ * reference doc comment says 256; the struct is 4 mat4 + 3 vec4 + 1 mat4). */
+typedef struct hv_camera_uniforms {
+    float view[16];           /* @0   */
+    float proj[16];           /* @64  */
+    float view_proj[16];      /* @128 */
+    float inv_view_proj[16];  /* @192 */
+    float position_near[4];   /* @256 xyz = position, w = near */
+    float forward_far[4];     /* @272 xyz = forward,  w = far  */
+    float jitter_frame[4];    /* @288 */
+    float prev_view_proj[16]; /* @304 */
+} hv_camera_uniforms;
+
+/* GpuInstanceData, crates/libhelio/src/instance.rs:30-45. 144 bytes. */
+typedef struct hv_instance_data {
+    float model[16];      /* @0   column-major model matrix */
+    float normal_mat[12]; /* @64  3 x vec4 (xyz used) */
+    float bounds[4];      /* @112 world-space sphere: xyz center, w radius */
+    uint32_t mesh_id;     /* @128 */
+    uint32_t material_id; /* @132 */
+    uint32_t flags;       /* @136 bit0 = casts shadow */
+    uint32_t lightmap_index; /* @140 */
+} hv_instance_data;
+
+/* GpuInstanceAabb, crates/libhelio/src/instance.rs:60-65. 32 bytes. */
+typedef struct hv_instance_aabb {
+    float min[3];
+    float _pad0;
+    float max[3];
+    float _pad1;
+} hv_instance_aabb;
+
+/* GpuDrawCall, crates/libhelio/src/draw.rs:28-36. 20 bytes. */
+typedef struct hv_draw_call {
+    uint32_t index_count;
+    uint32_t first_index;
+    int32_t vertex_offset;
+    uint32_t first_instance;
+    uint32_t instance_count;
+} hv_draw_call;
+
+/* DrawIndexedIndirectArgs, crates/libhelio/src/draw.rs:44-50. 20 bytes. */
+typedef struct hv_draw_indexed_indirect {
+    uint32_t index_count;
+    uint32_t instance_count;
+    uint32_t first_index;
+    int32_t base_vertex;
+    uint32_t first_instance;
+} hv_draw_indexed_indirect;
+
+/* GpuMeshletEntry, crates/libhelio/src/meshlet.rs:43-72. 64 bytes. */
+typedef struct hv_meshlet_entry {
+    float center[3];
+    float radius;
+    float cone_apex[3];
+    float cone_cutoff; /* > 1 disables the cone test */
+    float cone_axis[3];
+    float lod_error;   /* not read by the cull (vg_cull.wgsl:135-264) */
+    uint32_t first_index;
+    uint32_t index_count;
+    int32_t vertex_offset;
+    uint32_t instance_index; /* reserved */
+} hv_meshlet_entry;
+
+#define HV_VG_LOD_LEVELS 8
+#define HV_VG_MESHLETS_PER_WORK_ITEM 64
+#define HV_VG_COUNTER_COUNT 11
+
+/* GpuVgObject, crates/libhelio/src/meshlet.rs:81-102. 128 bytes. */
+typedef struct hv_vg_object {
+    uint32_t instance_index;
+    uint32_t lod_count;
+    uint32_t max_meshlet_count;
+    uint32_t selected_lod_plus_one; /* GPU scratch (reference field `reserved`) */
+    float local_bounds[4];
+    float lod_errors[HV_VG_LOD_LEVELS];
+    uint32_t lod_first_meshlets[HV_VG_LOD_LEVELS];
+    uint32_t lod_meshlet_counts[HV_VG_LOD_LEVELS];
+} hv_vg_object;
+
+/* GpuVgDraw, crates/libhelio/src/meshlet.rs:111-116. 16 bytes. */
+typedef struct hv_vg_draw {
+    uint32_t instance_index;
+    uint32_t meshlet_index;
+    uint32_t lod_level;
+    uint32_t reserved;
+} hv_vg_draw;
+
+/* GpuVgWorkItem, crates/libhelio/src/meshlet.rs:126-129. 8 bytes. */
+typedef struct hv_vg_work_item {
+    uint32_t object_index;
+    uint32_t local_meshlet_base;
+} hv_vg_work_item;
+
+/* InstanceCullData, crates/helio-pass-virtual-geometry/src/lib.rs:122-129. 16 bytes. */
+typedef struct hv_instance_cull_data {
+    float max_scale;
+    float min_scale;
+    uint32_t cone_cull_enabled;
+    uint32_t valid_transform;
+} hv_instance_cull_data;
+
+/* CullUniforms of the indirect-dispatch pass,
+ * crates/helio-pass-indirect-dispatch/src/lib.rs:16-22. 112 bytes. */
+typedef struct hv_id_cull_uniforms {
+    float frustum_planes[6][4];
+    uint32_t draw_count;
+    uint32_t _pad[3];
+} hv_id_cull_uniforms;
+
+/* CullParams of the occlusion pass,
+ * crates/helio-pass-occlusion-cull/src/lib.rs:20-35. 56 bytes. */
+typedef struct hv_occl_cull_params {
+    uint32_t screen_width;
+    uint32_t screen_height;
+    uint32_t draw_count;
+    uint32_t hiz_mip_count;
+    uint32_t static_hiz_available; /* must be 0: the baked-PVS branch is not built */
+    uint32_t grid_resolution[3];
+    float world_bounds_min[3];
+    float world_bounds_max[3];
+} hv_occl_cull_params;
+
+/* CullUniforms of the virtual-geometry pass,
+ * crates/helio-pass-virtual-geometry/src/lib.rs:257-272. 48 bytes. */
+typedef struct hv_vg_cull_uniforms {
+    uint32_t object_count;
+    uint32_t screen_width;
+    uint32_t screen_height;
+    uint32_t hiz_mip_count;
+    uint32_t draw_capacity;
+    float lod_error_threshold_px;
+    uint32_t object_dispatch_width;
+    uint32_t work_item_count;
+    uint32_t work_dispatch_width;
+    uint32_t _pad[3];
+} hv_vg_cull_uniforms;
+
+/* HiZUniforms, crates/helio-pass-hiz/src/lib.rs:29-34. 16 bytes. */
+typedef struct hv_hiz_uniforms {
+    uint32_t src_size[2];
+    uint32_t dst_size[2];
+} hv_hiz_uniforms;
+
+/* Cull statistics, 8 x u32, layout of indirect_dispatch.wgsl:69-77:
+ * 0 total, 1 frustum_culled, 2 subpixel_culled, 3 frustum_visible,
+ * 4 occlusion_culled, 5 shadow_total, 6 shadow_frustum_visible,
+ * 7 shadow_occlusion_culled. */
+#define HV_STAT_COUNT 8
+
+/* ------------------------------------------------------------------------- */
+/* Per-(mesh, LOD) draw range used to fill bucket args. No reference struct:  */
+/* the compacting per-instance cull is this library's extension of           */
+/* indirect_dispatch.wgsl (which culls per draw call and never selects LOD). */
+/* ------------------------------------------------------------------------- */
+typedef struct hv_mesh_lod {
+    uint32_t index_count;
+    uint32_t first_index;
+    int32_t base_vertex;
+} hv_mesh_lod;
+
+/* ------------------------------------------------------------------------- */
+/* Context, errors, device memory plumbing                                   */
+/* ------------------------------------------------------------------------- */
+typedef struct hv_ctx hv_ctx;
+
+/* Thread-local message of the last failure on this thread. Never NULL. */
+const char* hv_last_error(void);
+
+/* Device context: owns a stream, the launch-shape table and the device
+ * properties. Replaces the wgpu::Device/Queue pair handed to every pass
+ * (crates/helio-core/src/context.rs:450-494). */
+hv_status hv_ctx_create(int32_t device, hv_ctx** out);
+void hv_ctx_destroy(hv_ctx* ctx);
+hv_status hv_ctx_stream(hv_ctx* ctx, void** stream);
+hv_status hv_ctx_synchronize(hv_ctx* ctx);
+/* Number of compute units and the L2/LLC sizes seen by the library. */
+hv_status hv_ctx_device_info(hv_ctx* ctx, uint32_t* cu_count, uint64_t* hbm_bytes);
+
+/* Device buffers, the counterpart of wgpu::Device::create_buffer and
+ * Queue::write_buffer (crates/helio-core/src/context.rs:498). */
+hv_status hv_device_alloc(hv_ctx* ctx, uint64_t bytes, void** dptr);
+hv_status hv_device_free(hv_ctx* ctx, void* dptr);
+hv_status hv_memcpy_h2d(hv_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream);
+hv_status hv_memcpy_d2h(hv_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream);
+hv_status hv_memset_d32(hv_ctx* ctx, void* dst, uint32_t value, uint64_t count, void* stream);
+
+/* Stream-ordered timing: record a HIP event on `stream` into slot `slot`
+ * (0..63) and read the elapsed ms between two recorded slots. */
+hv_status hv_timer_record(hv_ctx* ctx, uint32_t slot, void* stream);
+hv_status hv_timer_elapsed_ms(hv_ctx* ctx, uint32_t slot_begin, uint32_t slot_end, float* ms);
+
+/* ------------------------------------------------------------------------- */
+/* Math helpers shared with the host side of the reference passes            */
+/* ------------------------------------------------------------------------- */
+
+/* extract_frustum_planes, crates/helio-pass-indirect-dispatch/src/lib.rs:271-300:
+ * Gribb-Hartmann rows of view_proj, normalised when |n| > 1e-10.
+ * Order L, R, B, T, near (row 2, z in [0,1]), far. */
+hv_status hv_extract_frustum_planes(const float view_proj[16], float planes[24]);
+
+/* mip_levels, crates/helio-pass-hiz/src/lib.rs:376-379 and
+ * crates/helio-pass-occlusion-cull/src/lib.rs:413-416. */
+uint32_t hv_mip_levels(uint32_t width, uint32_t height);
+
+/* InstanceCullData::from_instance,
+ * crates/helio-pass-virtual-geometry/src/lib.rs:131-168 (Rust f32 semantics). */
+hv_status hv_instance_cull_data_from_instances(const hv_instance_data* instances, uint32_t count,
+                                               hv_instance_cull_data* out);
+
+/* ------------------------------------------------------------------------- */
+/* Resident scene (GpuScene / GrowableBuffer residency,                     */
+/* crates/helio-core/src/scene/gpu_scene.rs:142-248,                         */
+/* crates/helio-core/src/scene/managers.rs:19-249).                          */
+/* Instances are converted AoS -> SoA on the device at upload: bounds as     */
+/* float4 (16 B) and a 4-byte meta word (mesh_id | casts_shadow << 31), so   */
+/* the cull reads 20 B per instance instead of 144 + 32 B.                   */
+/* ------------------------------------------------------------------------- */
+typedef struct hv_scene hv_scene;
+
+hv_status hv_scene_create(hv_ctx* ctx, uint32_t capacity, hv_scene** out);
+void hv_scene_destroy(hv_scene* scene);
+/* Upload instances [first, first + count) (a dirty range; the scene grows x2
+ * like GrowableBuffer). `aabbs` may be NULL, meaning sphere_to_aabb(bounds)
+ * (crates/helio/src/scene/helpers.rs:54-65); a non-NULL array that differs from
+ * sphere_to_aabb anywhere switches the scene to explicit AABBs. mesh_id must be
+ * below 2^31. */
+hv_status hv_scene_upload(hv_scene* scene, const hv_instance_data* instances,
+                          const hv_instance_aabb* aabbs, uint32_t first, uint32_t count,
+                          void* stream);
+/* Upload already-SoA data: bounds (count x float4), mesh ids and flags. */
+hv_status hv_scene_upload_soa(hv_scene* scene, const float* bounds4, const uint32_t* mesh_ids,
+                              const uint32_t* flags, uint32_t first, uint32_t count, void* stream);
+/* Per-(mesh, LOD) draw ranges: table[mesh * lod_count + lod]. */
+hv_status hv_scene_set_mesh_lods(hv_scene* scene, const hv_mesh_lod* table, uint32_t mesh_count,
+                                 uint32_t lod_count);
+hv_status hv_scene_info(const hv_scene* scene, uint32_t* instance_count, uint32_t* mesh_count,
+                        uint32_t* lod_count, uint64_t* device_bytes);
+
+/* ------------------------------------------------------------------------- */
+/* Hi-Z pyramid (HiZBuildPass, crates/helio-pass-hiz/src/lib.rs:45-548;       */
+/* hiz_depth_copy.wgsl:13-19, hiz_build.wgsl:12-23).                         */
+/* Level 0 is the depth buffer itself (no copy); levels 1.. are packed in one */
+/* device buffer. HV_HIZ_MAX is the reference (standard Z, max reduction);   */
+/* HV_HIZ_MIN is its mirror for reversed Z (min reduction).                   */
+/* ------------------------------------------------------------------------- */
+#define HV_HIZ_MAX 0u
+#define HV_HIZ_MIN 1u
+
+typedef struct hv_hiz hv_hiz;
+
+hv_status hv_hiz_create(hv_ctx* ctx, uint32_t width, uint32_t height, uint32_t mode, hv_hiz** out);
+void hv_hiz_destroy(hv_hiz* hiz);
+/* Bind a caller-owned device depth buffer (width x height f32, row-major) as
+ * level 0, or pass NULL to use the pyramid's own level-0 buffer. */
+hv_status hv_hiz_bind_depth(hv_hiz* hiz, const float* depth_device);
+/* Copy host depth into the pyramid's own level-0 buffer. */
+hv_status hv_hiz_upload_depth(hv_hiz* hiz, const float* depth_host, void* stream);
+/* Camera-static reuse (crates/helio-pass-hiz/src/lib.rs:472-478): when
+ * `camera_generation` equals the one of the last build, execute is a no-op. */
+hv_status hv_hiz_prepare(hv_hiz* hiz, uint64_t camera_generation);
+hv_status hv_hiz_execute(hv_hiz* hiz, void* stream);
+hv_status hv_hiz_levels(const hv_hiz* hiz, uint32_t* level_count);
+hv_status hv_hiz_level_size(const hv_hiz* hiz, uint32_t level, uint32_t* width, uint32_t* height);
+hv_status hv_hiz_read_level(hv_hiz* hiz, uint32_t level, float* out, uint64_t capacity_floats);
+
+/* ------------------------------------------------------------------------- */
+/* Per-instance cull + distance LOD + compaction (one view).                  */
+/* Predicate: indirect_dispatch.wgsl:79-137 applied per instance; with        */
+/* HV_CULL_OCCLUSION also occlusion_cull.wgsl:101-220 against an hv_hiz.      */
+/* Output: visible instance ids grouped bucket-major (bucket = mesh_id *      */
+/* lod_count + lod; order inside a bucket unspecified, like the reference's  */
+/* atomic append), one hv_draw_indexed_indirect per bucket with               */
+/* first_instance = exclusive prefix of bucket counts, and the 8 stats.       */
+/* ------------------------------------------------------------------------- */
+#define HV_CULL_OCCLUSION 1u  /* test survivors against the bound pyramid */
+#define HV_CULL_REVERSED_Z 2u /* pyramid is HV_HIZ_MIN, compare mirrored */
+#define HV_MAX_LOD_LEVELS 8
+
+typedef struct hv_cull_config {
+    uint32_t lod_count;                            /* 1..8 */
+    float lod_distances[HV_MAX_LOD_LEVELS - 1];    /* ascending, lod = #{t : d >= t} */
+    uint32_t flags;                                /* HV_CULL_* */
+    uint32_t screen_width;                         /* used by occlusion */
+    uint32_t screen_height;
+} hv_cull_config;
+
+typedef struct hv_inst_cull_pass hv_inst_cull_pass;
+
+/* Device pointers of a view's outputs (valid until the next prepare). */
+typedef struct hv_cull_outputs {
+    uint32_t* ids;                   /* capacity: candidate count */
+    hv_draw_indexed_indirect* args;  /* mesh_count * lod_count entries */
+    uint32_t* stats;                 /* HV_STAT_COUNT entries, then visible count at [8] */
+    uint32_t bucket_count;
+    uint32_t id_capacity;
+} hv_cull_outputs;
+
+/* IndirectDispatchPass::new, crates/helio-pass-indirect-dispatch/src/lib.rs:39-161. */
+hv_status hv_inst_cull_create(hv_ctx* ctx, const hv_cull_config* config, hv_inst_cull_pass** out);
+void hv_inst_cull_destroy(hv_inst_cull_pass* pass);
+/* IndirectDispatchPass::prepare, crates/helio-pass-indirect-dispatch/src/lib.rs:178-191:
+ * frustum planes from camera->view_proj, candidate range [first, first+count). */
+hv_status hv_inst_cull_prepare(hv_inst_cull_pass* pass, const hv_scene* scene,
+                               const hv_camera_uniforms* camera, uint32_t first, uint32_t count,
+                               const hv_hiz* hiz);
+/* IndirectDispatchPass::execute, crates/helio-pass-indirect-dispatch/src/lib.rs:193-258. */
+hv_status hv_inst_cull_execute(hv_inst_cull_pass* pass, void* stream);
+hv_status hv_inst_cull_outputs(const hv_inst_cull_pass* pass, hv_cull_outputs* out);
+/* Blocking readback (the async cull-stats readback of
+ * crates/helio/src/renderer/render.rs:436-481 made synchronous). Any pointer
+ * may be NULL to skip that output. */
+hv_status hv_inst_cull_read(hv_inst_cull_pass* pass, uint32_t* ids, uint32_t ids_capacity,
+                            uint32_t* id_count, hv_draw_indexed_indirect* args,
+                            uint32_t args_capacity, uint32_t stats[HV_STAT_COUNT]);
+
+
+/* ------------------------------------------------------------------------- */
+/* Batched multi-view cull: many views (cameras, shadow cascades, cube faces) */
+/* of possibly different resident scenes in ONE launch sequence, optionally  */
+/* replayed from a captured hipGraph. Per-view results are identical to the  */
+/* plain per-view pass. No reference counterpart: the reference dispatches   */
+/* one compute pass per view (closest analog: shadow_cull.wgsl:57-96, which  */
+/* loops over faces inside one dispatch).                                    */
+/* ------------------------------------------------------------------------- */
+typedef struct hv_view_request {
+    const hv_scene* scene;
+    const hv_camera_uniforms* camera;
+    uint32_t first;   /* candidate range [first, first + count) of the scene */
+    uint32_t count;
+    const hv_hiz* hiz; /* required when the config has HV_CULL_OCCLUSION */
+} hv_view_request;
+
+typedef struct hv_batch hv_batch;
+
+hv_status hv_batch_create(hv_ctx* ctx, const hv_cull_config* config, uint32_t max_views, hv_batch** out);
+void hv_batch_destroy(hv_batch* batch);
+/* Launch shape of the classify kernel: wg in {64,128,256,512}, ipt in {1,2,4,8}. */
+hv_status hv_batch_set_shape(hv_batch* batch, uint32_t wg, uint32_t ipt);
+hv_status hv_batch_prepare(hv_batch* batch, const hv_view_request* views, uint32_t count);
+/* use_graph != 0: replay the captured graph (captured on first use). */
+hv_status hv_batch_execute(hv_batch* batch, void* stream, uint32_t use_graph);
+/* Eager execute with HIP events around the three kernels; synchronises and
+ * returns ms[0] classify, ms[1] bucket count + scan, ms[2] scatter. */
+hv_status hv_batch_execute_timed(hv_batch* batch, void* stream, float ms[3]);
+hv_status hv_batch_outputs(const hv_batch* batch, uint32_t view, hv_cull_outputs* out);
+hv_status hv_batch_read(hv_batch* batch, uint32_t view, uint32_t* ids, uint32_t ids_capacity, uint32_t* id_count,
+                        hv_draw_indexed_indirect* args, uint32_t args_capacity, uint32_t stats[HV_STAT_COUNT]);
+/* Block until the last execute of this batch finished. */
+hv_status hv_batch_wait(hv_batch* batch);
+hv_status hv_batch_device_bytes(const hv_batch* batch, uint64_t* bytes);
+
+/* The same timing hook for the plain per-view pass. */
+hv_status hv_inst_cull_execute_timed(hv_inst_cull_pass* pass, void* stream, float ms[3]);
+hv_status hv_inst_cull_set_shape(hv_inst_cull_pass* pass, uint32_t wg, uint32_t ipt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HV_HV_H */

@@ -1,0 +1,145 @@
+"""Parity of the per-instance compacting cull (HIP) with the CPU oracle.
+
+Bit-exact: the visible-id multiset per bucket, every DrawIndexedIndirect arg
+and the 8 stats must equal the oracle's on the same seeded bytes.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from helio_amd import layouts as L
+from helio_amd import synth
+from helio_amd.passes import HiZBuildPass, InstanceCullPass, Scene
+from tests.conftest import args_as_u32, canonical_ids
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(ctx, sc, cam, first=0, count=None, soa=True, aabbs=None, hiz=None, occl=False, rev=False,
+         lod=(25.0, 75.0, 200.0)):
+    count = sc.n - first if count is None else count
+    scene = Scene(ctx, max(sc.n, 1))
+    if soa and aabbs is None:
+        scene.upload_soa(sc.bounds, sc.mesh_ids, sc.flags)
+    else:
+        scene.upload(sc.instances(), aabbs)
+    scene.set_mesh_lods(sc.mesh_lods, sc.mesh_count, sc.lod_count)
+    p = InstanceCullPass(ctx, lod, occlusion=occl, reversed_z=rev)
+    p.prepare(scene, cam, first, count, hiz)
+    p.execute()
+    ids, args, stats = p.read()
+    return ids, args, stats
+
+
+def _oracle(sc, cam, first=0, count=None, pyr=None, flags=0, aabb6=None, lod=(25.0, 75.0, 200.0)):
+    count = sc.n - first if count is None else count
+    ov = oracle.view_from_camera(cam, lod_distances=lod, flags=flags, mesh_lods=sc.mesh_lods,
+                                 mesh_count=sc.mesh_count)
+    return oracle.cull_view(ov, sc.bounds, sc.meta(), first, count, pyr, aabb6, threads=4)
+
+
+def _check(gpu, ref):
+    ids, args, stats = gpu
+    rids, rargs, rstats = ref
+    assert np.array_equal(stats, rstats), (stats, rstats)
+    assert np.array_equal(args_as_u32(args), rargs)
+    assert len(ids) == len(rids)
+    assert np.array_equal(canonical_ids(ids, args), rids)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_config0_parity(ctx, seed):
+    """Config 0: 65,536 instances, one random camera, LOD 25/75/200 m, 256 meshes x 4 LODs."""
+    sc = synth.instance_scene(65536, config=0)
+    cam = synth.perspective_view(synth.rng(0, 100 + seed)).camera
+    print(f"config0 seed={seed}", flush=True)
+    _check(_run(ctx, sc, cam), _oracle(sc, cam))
+
+
+def test_camera_inside_cloud_many_visible(ctx):
+    """A camera at the cube centre looking along +z sees many instances in every LOD."""
+    sc = synth.instance_scene(200_000, config=0, index=7, cube=300.0)
+    cam = synth.perspective_view(synth.rng(0, 9), position=(150, 150, 0), direction=(0, 0, 1)).camera
+    gpu, ref = _run(ctx, sc, cam), _oracle(sc, cam)
+    _check(gpu, ref)
+    assert len(ref[0]) > 20_000
+    assert (ref[1].reshape(-1, 4, 5)[:, :, 1].sum(0) > 0).all()
+
+
+@pytest.mark.parametrize("first,count", [(0, 0), (5, 1), (1000, 63), (3, 64), (77, 65), (123, 4097)])
+def test_ragged_ranges(ctx, first, count):
+    sc = synth.instance_scene(10_000, config=0, index=3, cube=200.0)
+    cam = synth.perspective_view(synth.rng(0, 5), position=(100, 100, -20), direction=(0, 0, 1)).camera
+    _check(_run(ctx, sc, cam, first, count), _oracle(sc, cam, first, count))
+
+
+def test_aos_upload_matches_soa(ctx):
+    sc = synth.instance_scene(30_000, config=0, index=4, cube=250.0)
+    cam = synth.perspective_view(synth.rng(0, 6), position=(125, 125, -10), direction=(0, 0, 1)).camera
+    a = _run(ctx, sc, cam, soa=True)
+    b = _run(ctx, sc, cam, soa=False)
+    for x, y in zip(a, b):
+        assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8)) or \
+            np.array_equal(canonical_ids(a[0], a[1]), canonical_ids(b[0], b[1]))
+    _check(a, _oracle(sc, cam))
+
+
+def test_explicit_aabbs(ctx):
+    """AABBs that differ from sphere_to_aabb switch to the explicit-AABB kernel."""
+    sc = synth.instance_scene(20_000, config=0, index=8, cube=200.0)
+    g = synth.rng(0, 11)
+    aabbs = np.zeros(sc.n, L.AABB)
+    ext = g.uniform(0.05, 3.0, (sc.n, 3)).astype(np.float32)
+    aabbs["min"] = sc.bounds[:, :3] - ext
+    aabbs["max"] = sc.bounds[:, :3] + ext
+    cam = synth.perspective_view(g, position=(100, 100, -10), direction=(0, 0, 1)).camera
+    aabb6 = np.concatenate([aabbs["min"], aabbs["max"]], axis=1).astype(np.float32)
+    _check(_run(ctx, sc, cam, soa=False, aabbs=aabbs), _oracle(sc, cam, aabb6=aabb6))
+
+
+def test_degenerate_sphere_path(ctx):
+    """Tiny radii far from the origin make c - r == c + r: the sphere test applies."""
+    sc = synth.instance_scene(4096, config=0, index=12, cube=200.0)
+    sc.bounds[:, :3] += 3.0e6
+    sc.bounds[::2, 3] = 1.0e-3
+    cam = synth.perspective_view(synth.rng(0, 12), position=(3.0e6 + 100, 3.0e6 + 100, 3.0e6 - 5),
+                                 direction=(0, 0, 1), far=1.0e4).camera
+    _check(_run(ctx, sc, cam), _oracle(sc, cam))
+
+
+def test_repeat_execute_is_stable(ctx):
+    """Counters reset themselves: executing twice gives the same result."""
+    sc = synth.instance_scene(50_000, config=0, index=13, cube=300.0)
+    cam = synth.perspective_view(synth.rng(0, 13), position=(150, 150, -5), direction=(0, 0, 1)).camera
+    scene = Scene(ctx, sc.n)
+    scene.upload_soa(sc.bounds, sc.mesh_ids, sc.flags)
+    scene.set_mesh_lods(sc.mesh_lods, sc.mesh_count, sc.lod_count)
+    p = InstanceCullPass(ctx)
+    p.prepare(scene, cam)
+    p.execute()
+    a = p.read()
+    p.execute()
+    p.execute()
+    b = p.read()
+    assert np.array_equal(a[2], b[2])
+    assert np.array_equal(args_as_u32(a[1]), args_as_u32(b[1]))
+    assert np.array_equal(canonical_ids(a[0], a[1]), canonical_ids(b[0], b[1]))
+    _check(b, _oracle(sc, cam))
+
+
+@pytest.mark.parametrize("rev", [True, False])
+def test_occlusion_parity(ctx, rev):
+    """Config-1 recipe at reduced size: frustum + sub-pixel, then Hi-Z occlusion."""
+    view, depth, _ = synth.occluder_view_and_depth(reversed_z=rev, boxes=600)
+    sc = synth.instance_scene(262_144, config=1)
+    hiz = HiZBuildPass(ctx, 1920, 1080, L.HIZ_MIN if rev else L.HIZ_MAX)
+    hiz.upload_depth(depth)
+    hiz.execute()
+    flags = L.CULL_OCCLUSION | (L.CULL_REVERSED_Z if rev else 0)
+    gpu = _run(ctx, sc, view.camera, hiz=hiz, occl=True, rev=rev)
+    pyr = oracle.pyramid(depth, 1 if rev else 0)
+    ref = _oracle(sc, view.camera, pyr=pyr, flags=flags)
+    _check(gpu, ref)
+    assert ref[2][4] > 0, "no instance was occluded: the test would not pin the Hi-Z path"
