@@ -54,6 +54,39 @@ class ViewRequest(C.Structure):
     ]
 
 
+class VgFrameData(C.Structure):
+    _fields_ = [
+        ("meshlets", vp),
+        ("objects", vp),
+        ("instances", vp),
+        ("work_items", vp),
+        ("meshlet_count", u32),
+        ("object_count", u32),
+        ("work_item_count", u32),
+        ("max_draw_count", u32),
+        ("buffer_version", u64),
+        ("instance_version", u64),
+        ("instance_dirty_start", u32),
+        ("instance_dirty_count", u32),
+    ]
+
+
+class VgOutputs(C.Structure):
+    _fields_ = [("indirect", vp), ("metadata", vp), ("counters", vp), ("capacity", u32)]
+
+
+class DrawBuffers(C.Structure):
+    _fields_ = [
+        ("instances", vp),
+        ("aabbs", vp),
+        ("draw_calls", vp),
+        ("indirect", vp),
+        ("stats", vp),
+        ("draw_count", u32),
+        ("instance_capacity", u32),
+    ]
+
+
 # name -> (restype, argtypes)
 _SIGS: dict[str, tuple] = {
     "hv_last_error": (C.c_char_p, []),
@@ -105,6 +138,24 @@ _SIGS: dict[str, tuple] = {
     "hv_batch_read": (i32, [vp, u32, vp, u32, P(u32), vp, u32, vp]),
     "hv_batch_wait": (i32, [vp]),
     "hv_batch_device_bytes": (i32, [vp, P(u64)]),
+    "hv_vg_create": (i32, [vp, u32, P(vp)]),
+    "hv_vg_destroy": (None, [vp]),
+    "hv_vg_prepare": (i32, [vp, P(VgFrameData), vp, u32, u32, f32, vp, u32]),
+    "hv_vg_execute": (i32, [vp, vp]),
+    "hv_vg_execute_timed": (i32, [vp, vp, P(f32)]),
+    "hv_vg_outputs": (i32, [vp, P(VgOutputs)]),
+    "hv_vg_read": (i32, [vp, vp, vp, u32, vp, vp, u32]),
+    "hv_vg_build_work_items": (i32, [vp, u32, vp, u32, P(u32)]),
+    "hv_indirect_dispatch_create": (i32, [vp, P(vp)]),
+    "hv_indirect_dispatch_destroy": (None, [vp]),
+    "hv_indirect_dispatch_prepare": (i32, [vp, vp, u32]),
+    "hv_indirect_dispatch_uniforms": (i32, [vp, vp]),
+    "hv_indirect_dispatch_execute": (i32, [vp, P(DrawBuffers), vp]),
+    "hv_occlusion_cull_create": (i32, [vp, u32, u32, P(vp)]),
+    "hv_occlusion_cull_destroy": (None, [vp]),
+    "hv_occlusion_cull_prepare": (i32, [vp, vp, u32, u64]),
+    "hv_occlusion_cull_params": (i32, [vp, vp]),
+    "hv_occlusion_cull_execute": (i32, [vp, P(DrawBuffers), vp, vp]),
 }
 
 _lock = threading.Lock()

@@ -22,47 +22,23 @@
 #include <cstring>
 
 #include "hv_cull.h"
+#include "hv_devmath.h"
 
 namespace hv {
 namespace dev {
 
 constexpr uint32_t kInvalid = 0xffffffffu;
 
-// WGSL max/min/clamp semantics: max(a,b) = (a < b) ? b : a.
-__device__ __forceinline__ float wmax(float a, float b) { return (a < b) ? b : a; }
-__device__ __forceinline__ float wmin(float a, float b) { return (b < a) ? b : a; }
-__device__ __forceinline__ float wclamp(float e, float lo, float hi) { return wmin(wmax(e, lo), hi); }
-__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
-    return (ax * bx + ay * by) + az * bz;
-}
-// Row r of view_proj * (x, y, z, 1), summed left to right.
-__device__ __forceinline__ float mrow(const float* m, int r, float x, float y, float z) {
-    return ((m[r] * x + m[4 + r] * y) + m[8 + r] * z) + m[12 + r] * 1.0f;
-}
-
-// ceil(log2(d)) for d >= 1 from the float's bits; NaN -> 0; +inf -> large.
-__device__ __forceinline__ uint32_t ceil_log2(float d) {
-    if (!(d >= 1.0f)) return 0u;
-    const uint32_t bits = __float_as_uint(d);
-    const uint32_t e = (bits >> 23) & 0xffu;
-    if (e == 0xffu) return 0xffffffffu;
-    return (e - 127u) + ((bits & 0x7fffffu) != 0u ? 1u : 0u);
-}
-
-__device__ __forceinline__ uint32_t mbcnt(uint64_t mask) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
-// Nearest tap of pyramid level m (sampler: nearest, clamp-to-edge; level
-// clamped to the last one): texel = clamp(floor(u * w_m), 0, w_m - 1).
-__device__ __forceinline__ float hiz_tap(const CullView* __restrict__ V, uint32_t m, float u, float v) {
-    if (m >= V->hiz_levels) m = V->hiz_levels - 1u;
-    const uint32_t wm = max(V->hiz_w >> m, 1u), hm = max(V->hiz_h >> m, 1u);
-    const float fx = floorf(u * (float)wm), fy = floorf(v * (float)hm);
-    const int x = (fx >= 0.0f) ? (fx < (float)wm ? (int)fx : (int)wm - 1) : 0;
-    const int y = (fy >= 0.0f) ? (fy < (float)hm ? (int)fy : (int)hm - 1) : 0;
-    const float* base = (m == 0u) ? V->hiz0 : V->hiz_packed + V->hiz_off[m];
-    return base[(size_t)y * wm + (uint32_t)x];
+__device__ __forceinline__ Pyramid view_pyramid(const CullView* __restrict__ V) {
+    Pyramid p;
+    p.level0 = V->hiz0;
+    p.packed = V->hiz_packed;
+    p.w = V->hiz_w;
+    p.h = V->hiz_h;
+    p.levels = V->hiz_levels;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) p.off[i] = V->hiz_off[i];
+    return p;
 }
 
 // occlusion_cull.wgsl:158-220 for one sphere (reversed-Z mirror when rev).
@@ -97,13 +73,11 @@ __device__ __forceinline__ bool occluded(const CullView* __restrict__ V, const f
     const float uv_half = ndc_r * 0.5f;
     const float u0 = wclamp(u - uv_half, 0.0f, 1.0f), v0 = wclamp(vv - uv_half, 0.0f, 1.0f);
     const float u1 = wclamp(u + uv_half, 0.0f, 1.0f), v1 = wclamp(vv + uv_half, 0.0f, 1.0f);
-    const float h00 = hiz_tap(V, mip, u0, v0);
-    const float h01 = hiz_tap(V, mip, u1, v0);
-    const float h10 = hiz_tap(V, mip, u0, v1);
-    const float h11 = hiz_tap(V, mip, u1, v1);
+    const Pyramid p = view_pyramid(V);
+    const float hz = hiz_reduce4(rev, hiz_tap(p, mip, u0, v0), hiz_tap(p, mip, u1, v0), hiz_tap(p, mip, u0, v1),
+                                 hiz_tap(p, mip, u1, v1));
     constexpr float bias = 1.0f / 65536.0f;
-    if (rev) return near_z < wmin(wmin(h00, h01), wmin(h10, h11)) - bias;
-    return near_z > wmax(wmax(h00, h01), wmax(h10, h11)) + bias;
+    return rev ? (near_z < hz - bias) : (near_z > hz + bias);
 }
 
 // Flag bits produced per instance.

@@ -12,7 +12,8 @@ import ctypes as C
 import numpy as np
 
 from . import layouts as L
-from ._lib import CullConfig, CullOutputs, ViewRequest, call, load, ptr
+from ._lib import (CullConfig, CullOutputs, DrawBuffers, VgFrameData, VgOutputs, ViewRequest, call, load,
+                   ptr)
 
 
 class Context:
@@ -319,3 +320,217 @@ class BatchCull:
         b = C.c_uint64()
         call("hv_batch_device_bytes", self.handle, C.byref(b))
         return b.value
+
+
+def build_vg_work_items(objects: np.ndarray) -> np.ndarray:
+    """append_work_items over all objects (crates/helio/src/scene/virtual_geometry/rebuild.rs:45-58)."""
+    objs = np.ascontiguousarray(objects, dtype=L.VG_OBJECT)
+    n = C.c_uint32()
+    call("hv_vg_build_work_items", ptr(objs), len(objs), None, 0, C.byref(n))
+    out = np.zeros(n.value, L.VG_WORK_ITEM)
+    call("hv_vg_build_work_items", ptr(objs), len(objs), ptr(out), len(out), C.byref(n))
+    return out
+
+
+class VirtualGeometryPass:
+    """VirtualGeometryPass cull stages (crates/helio-pass-virtual-geometry/src/rendering.rs:708-1165)."""
+
+    def __init__(self, ctx: Context, publication_limit: int = 262_144):
+        self.ctx = ctx
+        h = C.c_void_p()
+        call("hv_vg_create", ctx.handle, publication_limit, C.byref(h))
+        self.handle = h
+        self.version = 0
+        self.instance_version = 0
+        self._frame = None
+
+    def close(self) -> None:
+        if self.handle:
+            load().hv_vg_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_frame(self, meshlets, objects, instances, work_items=None, max_draw_count=None) -> None:
+        """Publish new topology (advances buffer_version)."""
+        m = np.ascontiguousarray(meshlets, dtype=L.MESHLET)
+        o = np.ascontiguousarray(objects, dtype=L.VG_OBJECT)
+        i = np.ascontiguousarray(instances, dtype=L.INSTANCE)
+        w = build_vg_work_items(o) if work_items is None else np.ascontiguousarray(work_items, dtype=L.VG_WORK_ITEM)
+        if max_draw_count is None:
+            max_draw_count = int(o["max_meshlet_count"].astype(np.int64).sum())
+        self.version += 1
+        self._frame = (m, o, i, w, int(max_draw_count))
+        self._dirty = (0, 0)
+
+    def update_instances(self, start: int, instances: np.ndarray) -> None:
+        """Transform-only change of instances[start:start+len] (advances instance_version)."""
+        m, o, i, w, mx = self._frame
+        i = i.copy()
+        i[start:start + len(instances)] = instances
+        self._frame = (m, o, i, w, mx)
+        self.instance_version += 1
+        self._dirty = (start, len(instances))
+
+    def prepare(self, camera: np.ndarray, width: int = 1920, height: int = 1080, threshold_px: float = 0.5,
+                hiz: HiZBuildPass | None = None, reversed_z: bool = False) -> None:
+        m, o, i, w, mx = self._frame
+        fd = VgFrameData()
+        fd.meshlets, fd.objects, fd.instances, fd.work_items = ptr(m), ptr(o), ptr(i), ptr(w)
+        fd.meshlet_count, fd.object_count, fd.work_item_count = len(m), len(o), len(w)
+        fd.max_draw_count = mx
+        fd.buffer_version = self.version
+        fd.instance_version = self.instance_version
+        fd.instance_dirty_start, fd.instance_dirty_count = self._dirty
+        cam = np.ascontiguousarray(camera, dtype=L.CAMERA).reshape(1)
+        self._cam = cam
+        call("hv_vg_prepare", self.handle, C.byref(fd), ptr(cam), width, height, threshold_px,
+             None if hiz is None else hiz.handle, L.CULL_REVERSED_Z if reversed_z else 0)
+
+    def execute(self, stream: int | None = None) -> None:
+        call("hv_vg_execute", self.handle, stream)
+
+    def execute_timed(self, stream: int | None = None) -> np.ndarray:
+        ms = (C.c_float * 2)()
+        call("hv_vg_execute_timed", self.handle, stream, ms)
+        return np.array(ms[:], np.float64)
+
+    def outputs(self) -> VgOutputs:
+        o = VgOutputs()
+        call("hv_vg_outputs", self.handle, C.byref(o))
+        return o
+
+    def read(self):
+        """(indirect, metadata, counters[11], selected_lod_plus_one per object)."""
+        cap = self.outputs().capacity
+        ind = np.zeros(max(cap, 1), L.DRAW_INDEXED_INDIRECT)
+        md = np.zeros(max(cap, 1), L.VG_DRAW)
+        ctr = np.zeros(11, np.uint32)
+        nobj = len(self._frame[1])
+        sel = np.zeros(max(nobj, 1), np.uint32)
+        call("hv_vg_read", self.handle, ptr(ind), ptr(md), len(ind), ptr(ctr), ptr(sel), len(sel))
+        n = min(int(ctr[0]), cap)
+        return ind[:n].copy(), md[:n].copy(), ctr, sel[:nobj].copy()
+
+
+class DeviceBuffer:
+    """A device allocation made through the C ABI (wgpu::Buffer counterpart)."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        call("hv_device_alloc", ctx.handle, max(self.nbytes, 4), C.byref(p))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, ctx: Context, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(ctx, a.nbytes)
+        b.write(a)
+        return b
+
+    def write(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        call("hv_memcpy_h2d", self.ctx.handle, self.ptr, ptr(a), a.nbytes, None)
+
+    def read(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype)
+        assert out.nbytes <= self.nbytes
+        call("hv_memcpy_d2h", self.ctx.handle, ptr(out), self.ptr, out.nbytes, None)
+        return out
+
+    def fill_u32(self, value: int = 0) -> None:
+        call("hv_memset_d32", self.ctx.handle, self.ptr, value, self.nbytes // 4, None)
+        self.ctx.synchronize()
+
+    def close(self) -> None:
+        if self.ptr:
+            call("hv_device_free", self.ctx.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def draw_buffers(instances: DeviceBuffer, aabbs: DeviceBuffer, draw_calls: DeviceBuffer, indirect: DeviceBuffer,
+                 stats: DeviceBuffer, draw_count: int, instance_capacity: int) -> DrawBuffers:
+    b = DrawBuffers()
+    b.instances, b.aabbs, b.draw_calls = instances.ptr, aabbs.ptr, draw_calls.ptr
+    b.indirect, b.stats = indirect.ptr, stats.ptr
+    b.draw_count, b.instance_capacity = draw_count, instance_capacity
+    return b
+
+
+class IndirectDispatchPass:
+    """IndirectDispatchPass, reference-exact (non-compacting per draw call)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = C.c_void_p()
+        call("hv_indirect_dispatch_create", ctx.handle, C.byref(h))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            load().hv_indirect_dispatch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def prepare(self, camera: np.ndarray, draw_count: int) -> None:
+        cam = np.ascontiguousarray(camera, dtype=L.CAMERA).reshape(1)
+        call("hv_indirect_dispatch_prepare", self.handle, ptr(cam), draw_count)
+
+    def uniforms(self) -> np.ndarray:
+        out = np.zeros(1, L.ID_CULL_UNIFORMS)
+        call("hv_indirect_dispatch_uniforms", self.handle, ptr(out))
+        return out
+
+    def execute(self, buffers: DrawBuffers, stream: int | None = None) -> None:
+        call("hv_indirect_dispatch_execute", self.handle, C.byref(buffers), stream)
+
+
+class OcclusionCullPass:
+    """OcclusionCullPass, reference-exact (zeroes instance_count of occluded draws)."""
+
+    def __init__(self, ctx: Context, width: int, height: int):
+        self.ctx = ctx
+        h = C.c_void_p()
+        call("hv_occlusion_cull_create", ctx.handle, width, height, C.byref(h))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            load().hv_occlusion_cull_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def prepare(self, camera: np.ndarray, draw_count: int, frame_num: int) -> None:
+        cam = np.ascontiguousarray(camera, dtype=L.CAMERA).reshape(1)
+        call("hv_occlusion_cull_prepare", self.handle, ptr(cam), draw_count, C.c_uint64(frame_num))
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(1, L.OCCL_CULL_PARAMS)
+        call("hv_occlusion_cull_params", self.handle, ptr(out))
+        return out
+
+    def execute(self, buffers: DrawBuffers, hiz: HiZBuildPass, stream: int | None = None) -> None:
+        call("hv_occlusion_cull_execute", self.handle, C.byref(buffers), hiz.handle, stream)

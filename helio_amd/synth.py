@@ -285,3 +285,121 @@ def multiview_views(n_instances: int, config: int = 3, n_cameras: int = 16, n_su
         first = int(g.integers(0, n_instances - cnt + 1))
         out.append((v, first, cnt))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Virtual geometry (config 2): objects x meshlets with per-object LOD ranges.
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class VgScene:
+    meshlets: np.ndarray    # L.MESHLET
+    objects: np.ndarray     # L.VG_OBJECT
+    instances: np.ndarray   # L.INSTANCE (one per object)
+
+
+def _rotations(g: np.random.Generator, n: int) -> np.ndarray:
+    q = g.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y + w * z), 2 * (x * z - w * y)], axis=1),  # column 0
+        np.stack([2 * (x * y - w * z), 1 - 2 * (x * x + z * z), 2 * (y * z + w * x)], axis=1),  # column 1
+        np.stack([2 * (x * z + w * y), 2 * (y * z - w * x), 1 - 2 * (x * x + y * y)], axis=1),  # column 2
+    ], axis=1)  # (n, col, row)
+
+
+def vg_scene(n_objects: int = 8192, lod_counts=(384, 96, 32), config: int = 2, index: int = 0,
+             cube: float = CUBE, error_range=(1e-4, 1e-1)) -> VgScene:
+    """Seeded objects with rigid + uniform-scale transforms, each owning its own
+    meshlets split into LOD ranges (finest first). Meshlet spheres lie inside
+    the object sphere; cone axes are unit-random with cutoff U[-0.2, 0.9];
+    object LOD errors are log-uniform and ascending; 124 triangles per meshlet."""
+    g = rng(config, index)
+    per_obj = int(sum(lod_counts))
+    n_m = n_objects * per_obj
+    rot = _rotations(g, n_objects)
+    scale = g.uniform(0.5, 2.0, n_objects)
+    pos = g.uniform(0.0, cube, (n_objects, 3))
+    obj_r = g.uniform(2.0, 15.0, n_objects)
+
+    inst = np.zeros(n_objects, L.INSTANCE)
+    model = np.zeros((n_objects, 16), np.float64)
+    for c in range(3):
+        model[:, 4 * c:4 * c + 3] = rot[:, c, :] * scale[:, None]
+    model[:, 12:15] = pos
+    model[:, 15] = 1.0
+    inst["model"] = model.astype(np.float32)
+    nm = np.zeros((n_objects, 12), np.float64)
+    for c in range(3):
+        nm[:, 4 * c:4 * c + 3] = rot[:, c, :] / scale[:, None]  # inverse-transpose of R*s
+    inst["normal_mat"] = nm.astype(np.float32)
+    inst["bounds"][:, :3] = pos.astype(np.float32)
+    inst["bounds"][:, 3] = (obj_r * scale).astype(np.float32)
+    inst["lightmap_index"] = 0xFFFFFFFF
+
+    objs = np.zeros(n_objects, L.VG_OBJECT)
+    objs["instance_index"] = np.arange(n_objects, dtype=np.uint32)
+    objs["lod_count"] = len(lod_counts)
+    objs["max_meshlet_count"] = max(lod_counts)
+    objs["local_bounds"][:, 3] = obj_r.astype(np.float32)
+    errs = np.sort(np.exp(g.uniform(math.log(error_range[0]), math.log(error_range[1]),
+                                    (n_objects, len(lod_counts) - 1))), axis=1)
+    objs["lod_errors"][:, 1:len(lod_counts)] = errs.astype(np.float32)
+    base = np.arange(n_objects, dtype=np.uint64) * per_obj
+    off = 0
+    for l, c in enumerate(lod_counts):
+        objs["lod_first_meshlets"][:, l] = (base + off).astype(np.uint32)
+        objs["lod_meshlet_counts"][:, l] = c
+        off += c
+
+    ms = np.zeros(n_m, L.MESHLET)
+    r_obj = np.repeat(obj_r, per_obj)
+    r_m = r_obj * g.uniform(0.05, 0.2, n_m)
+    d = g.normal(size=(n_m, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    dist = (r_obj - r_m) * g.uniform(0.0, 1.0, n_m) ** (1.0 / 3.0)
+    centers = d * dist[:, None]
+    ms["center"] = centers.astype(np.float32)
+    ms["radius"] = r_m.astype(np.float32)
+    ms["cone_apex"] = centers.astype(np.float32)
+    ax = g.normal(size=(n_m, 3))
+    ax /= np.linalg.norm(ax, axis=1, keepdims=True)
+    ms["cone_axis"] = ax.astype(np.float32)
+    ms["cone_cutoff"] = g.uniform(-0.2, 0.9, n_m).astype(np.float32)
+    ms["lod_error"] = 0.0
+    ms["first_index"] = (np.arange(n_m, dtype=np.uint64) * 372 % (1 << 32)).astype(np.uint32)
+    ms["index_count"] = 372
+    ms["vertex_offset"] = (np.repeat(np.arange(n_objects), per_obj) * 256).astype(np.int32)
+    return VgScene(ms, objs, inst)
+
+
+def benchmark_vg_case(object_count: int, meshlets_per_object: int):
+    """The headless VG cull benchmark's case buffers
+    (crates/examples/meshlet_cull_benchmark.rs:459-560): identity camera at the
+    origin (60 deg, 16:9, 0.1-1000), every meshlet at z = -10 with radius 0.25
+    and the cone disabled, one LOD, a 1x1 far Hi-Z. Every meshlet is visible."""
+    proj = perspective_rh(math.radians(60), 16 / 9, 0.1, 1000.0)
+    cam = camera_uniforms(np.eye(4), proj, (0.0, 0.0, 0.0), 0.1, 1000.0)
+    inst = np.zeros(object_count, L.INSTANCE)
+    m = np.zeros(16, np.float32)
+    m[[0, 5, 10, 15]] = 1
+    inst["model"] = m
+    inst["normal_mat"][:, [0, 5, 10]] = 1
+    inst["bounds"] = [0.0, 0.0, -10.0, 1.0]
+    inst["lightmap_index"] = 0xFFFFFFFF
+    ms = np.zeros(meshlets_per_object, L.MESHLET)
+    ms["center"] = [0.0, 0.0, -10.0]
+    ms["radius"] = 0.25
+    ms["cone_apex"] = [0.0, 0.0, -10.0]
+    ms["cone_cutoff"] = 2.0
+    ms["cone_axis"] = [0.0, 0.0, 1.0]
+    ms["index_count"] = 3
+    objs = np.zeros(object_count, L.VG_OBJECT)
+    objs["instance_index"] = np.arange(object_count, dtype=np.uint32)
+    objs["lod_count"] = 1
+    objs["max_meshlet_count"] = meshlets_per_object
+    objs["local_bounds"] = [0.0, 0.0, -10.0, 1.0]
+    objs["lod_meshlet_counts"][:, 0] = meshlets_per_object
+    return cam, VgScene(ms, objs, inst)

@@ -404,6 +404,117 @@ hv_status hv_batch_device_bytes(const hv_batch* batch, uint64_t* bytes);
 hv_status hv_inst_cull_execute_timed(hv_inst_cull_pass* pass, void* stream, float ms[3]);
 hv_status hv_inst_cull_set_shape(hv_inst_cull_pass* pass, uint32_t wg, uint32_t ipt);
 
+/* ------------------------------------------------------------------------- */
+/* Virtual geometry: per-object frustum + projected-error LOD select, then    */
+/* per-meshlet frustum + normal cone + Hi-Z cull with atomic publication      */
+/* (VirtualGeometryPass, crates/helio-pass-virtual-geometry/src/rendering.rs; */
+/* vg_cull.wgsl:126-405).                                                     */
+/* ------------------------------------------------------------------------- */
+
+/* VgFrameData, crates/libhelio/src/frame.rs:491-517 (slices as typed pointers). */
+typedef struct hv_vg_frame_data {
+    const hv_meshlet_entry* meshlets;
+    const hv_vg_object* objects;
+    const hv_instance_data* instances;   /* one per object */
+    const hv_vg_work_item* work_items;
+    uint32_t meshlet_count;
+    uint32_t object_count;
+    uint32_t work_item_count;
+    uint32_t max_draw_count;
+    uint64_t buffer_version;
+    uint64_t instance_version;
+    uint32_t instance_dirty_start;
+    uint32_t instance_dirty_count;
+} hv_vg_frame_data;
+
+typedef struct hv_vg_buffers {
+    hv_draw_indexed_indirect* indirect; /* capacity entries; first_instance = slot */
+    hv_vg_draw* metadata;               /* capacity entries */
+    uint32_t* counters;                 /* HV_VG_COUNTER_COUNT: [0] attempted, [1] rejected,
+                                           [2..9] selected-LOD histogram, [10] max LOD */
+    uint32_t capacity;
+} hv_vg_buffers;
+
+typedef struct hv_vg_pass hv_vg_pass;
+
+/* VirtualGeometryPass::new_with_budget (publication budget in meshlets,
+ * crates/helio-pass-virtual-geometry/src/lib.rs:80-117). 0 is rejected like
+ * VirtualGeometryBudget::new. */
+hv_status hv_vg_create(hv_ctx* ctx, uint32_t publication_limit, hv_vg_pass** out);
+void hv_vg_destroy(hv_vg_pass* pass);
+/* VirtualGeometryPass::prepare (rendering.rs:708-955): uploads on a new
+ * buffer_version, a dirty instance range on a new instance_version, derives
+ * InstanceCullData on the host, writes the cull uniforms. flags: 0 or
+ * HV_CULL_REVERSED_Z (the pyramid must then be HV_HIZ_MIN). */
+hv_status hv_vg_prepare(hv_vg_pass* pass, const hv_vg_frame_data* vg, const hv_camera_uniforms* camera,
+                        uint32_t width, uint32_t height, float lod_error_threshold_px, const hv_hiz* hiz,
+                        uint32_t flags);
+/* VirtualGeometryPass::execute (rendering.rs:1031-1165): clear counters,
+ * select objects, cull meshlets. */
+hv_status hv_vg_execute(hv_vg_pass* pass, void* stream);
+/* Execute with HIP events: ms[0] object select, ms[1] meshlet cull. Synchronises. */
+hv_status hv_vg_execute_timed(hv_vg_pass* pass, void* stream, float ms[2]);
+hv_status hv_vg_outputs(const hv_vg_pass* pass, hv_vg_buffers* out);
+/* Blocking readback; any pointer may be NULL. selected: one u32 per object
+ * (selected_lod_plus_one). */
+hv_status hv_vg_read(hv_vg_pass* pass, hv_draw_indexed_indirect* indirect, hv_vg_draw* metadata,
+                     uint32_t capacity, uint32_t counters[HV_VG_COUNTER_COUNT], uint32_t* selected,
+                     uint32_t selected_capacity);
+/* append_work_items for every object (crates/helio/src/scene/virtual_geometry/rebuild.rs:45-58):
+ * one record per 64-meshlet span of max_meshlet_count. */
+hv_status hv_vg_build_work_items(const hv_vg_object* objects, uint32_t object_count, hv_vg_work_item* out,
+                                 uint32_t capacity, uint32_t* count);
+
+/* ------------------------------------------------------------------------- */
+/* Reference-exact per-draw passes over the reference's own buffers (AoS      */
+/* GpuInstanceData, GpuDrawCall, GpuInstanceAabb, DrawIndexedIndirect), as    */
+/* device pointers owned by the caller.                                       */
+/* ------------------------------------------------------------------------- */
+typedef struct hv_draw_buffers {
+    const hv_instance_data* instances;  /* indexed by draw_calls[i].first_instance */
+    const hv_instance_aabb* aabbs;      /* same indexing */
+    const hv_draw_call* draw_calls;
+    hv_draw_indexed_indirect* indirect; /* one per draw call */
+    uint32_t* stats;                    /* HV_STAT_COUNT counters, cleared by the caller per frame */
+    uint32_t draw_count;
+    uint32_t instance_capacity;         /* entries in instances/aabbs; a first_instance beyond it
+                                           reads zeros, like WGSL's robust buffer access */
+} hv_draw_buffers;
+
+typedef struct hv_indirect_dispatch_pass hv_indirect_dispatch_pass;
+
+/* IndirectDispatchPass (crates/helio-pass-indirect-dispatch/src/lib.rs:24-258;
+ * indirect_dispatch.wgsl:100-176): per draw call, the representative instance
+ * (first_instance) is frustum (AABB unless degenerate, else sphere) and
+ * sub-pixel tested; every draw gets its args, instance_count 0 when culled. */
+hv_status hv_indirect_dispatch_create(hv_ctx* ctx, hv_indirect_dispatch_pass** out);
+void hv_indirect_dispatch_destroy(hv_indirect_dispatch_pass* pass);
+/* IndirectDispatchPass::prepare (:178-191): planes from the camera. */
+hv_status hv_indirect_dispatch_prepare(hv_indirect_dispatch_pass* pass, const hv_camera_uniforms* camera,
+                                       uint32_t draw_count);
+/* The CullUniforms written by the last prepare (112 bytes). */
+hv_status hv_indirect_dispatch_uniforms(const hv_indirect_dispatch_pass* pass, hv_id_cull_uniforms* out);
+/* IndirectDispatchPass::execute (:193-258). */
+hv_status hv_indirect_dispatch_execute(hv_indirect_dispatch_pass* pass, const hv_draw_buffers* buffers,
+                                       void* stream);
+
+typedef struct hv_occlusion_cull_pass hv_occlusion_cull_pass;
+
+/* OcclusionCullPass (crates/helio-pass-occlusion-cull/src/lib.rs:37-411;
+ * occlusion_cull.wgsl:146-220): per draw call, sphere -> screen rect -> mip ->
+ * 4 nearest taps -> zero instance_count when occluded (stats 4 and 7 count
+ * draws that were visible). The static-Hi-Z branch is not supported. */
+hv_status hv_occlusion_cull_create(hv_ctx* ctx, uint32_t screen_width, uint32_t screen_height,
+                                   hv_occlusion_cull_pass** out);
+void hv_occlusion_cull_destroy(hv_occlusion_cull_pass* pass);
+/* OcclusionCullPass::prepare (:297-316). frame_num 0 makes execute a no-op
+ * (temporal Hi-Z has no pyramid yet, :319-323). */
+hv_status hv_occlusion_cull_prepare(hv_occlusion_cull_pass* pass, const hv_camera_uniforms* camera,
+                                    uint32_t draw_count, uint64_t frame_num);
+hv_status hv_occlusion_cull_params(const hv_occlusion_cull_pass* pass, hv_occl_cull_params* out);
+hv_status hv_occlusion_cull_execute(hv_occlusion_cull_pass* pass, const hv_draw_buffers* buffers, const hv_hiz* hiz,
+                                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,171 @@
+"""Virtual-geometry cull (HIP) against the oracle's restatement of vg_cull.wgsl.
+
+Slots come from an atomic counter in both the reference and this path, so
+parity is defined on the published multiset: counters exact, the sorted
+(instance, meshlet, lod, index_count, first_index, base_vertex) rows exact,
+and every published command's first_instance equals its slot.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from helio_amd import layouts as L
+from helio_amd import synth
+from helio_amd.passes import HiZBuildPass, VirtualGeometryPass, instance_cull_data
+from tests.vg_util import oracle_vg, vg_uniforms
+
+pytestmark = pytest.mark.gpu
+
+TOTAL = 1_048_576  # crates/examples/meshlet_cull_benchmark.rs:35
+
+
+def _rows(indirect, metadata):
+    ind = np.ascontiguousarray(indirect).view(np.uint32).reshape(-1, 5)
+    md = np.ascontiguousarray(metadata).view(np.uint32).reshape(-1, 4)
+    rows = np.concatenate([md[:, :3], ind[:, [0, 1, 2, 3]]], axis=1)
+    return rows[np.lexsort(rows.T[::-1])]
+
+
+def _check_slots(indirect):
+    ind = np.ascontiguousarray(indirect).view(np.uint32).reshape(-1, 5)
+    assert np.array_equal(ind[:, 4], np.arange(len(ind), dtype=np.uint32))
+
+
+@pytest.mark.parametrize("objects,per_object", [(1, TOTAL), (16, TOTAL // 16), (4096, TOTAL // 4096),
+                                                (65536, TOTAL // 65536)],
+                         ids=["single_huge", "few_huge", "balanced", "many_small"])
+def test_benchmark_cases_attempt_every_meshlet(ctx, objects, per_object):
+    """meshlet_cull_benchmark.rs:284-294: counters == [TOTAL, 0] for every case."""
+    print(f"vg bench case {objects}x{per_object}", flush=True)
+    cam, sc = synth.benchmark_vg_case(objects, per_object)
+    hiz = HiZBuildPass(ctx, 1, 1, L.HIZ_MAX)
+    hiz.upload_depth(np.ones((1, 1), np.float32))
+    vg = VirtualGeometryPass(ctx, publication_limit=TOTAL)
+    vg.set_frame(sc.meshlets, sc.objects, sc.instances, max_draw_count=TOTAL)
+    vg.prepare(cam, 1920, 1080, 2.0, hiz)
+    vg.execute()
+    ind, md, ctr, sel = vg.read()
+    assert ctr[0] == TOTAL and ctr[1] == 0
+    assert ctr[2] == objects and ctr[10] == 0
+    assert np.all(sel == 1)
+    _check_slots(ind)
+    mdu = np.ascontiguousarray(md).view(np.uint32).reshape(-1, 4)
+    # Every (object, meshlet) pair exactly once.
+    key = mdu[:, 0].astype(np.uint64) * per_object + mdu[:, 1]
+    assert np.array_equal(np.sort(key), np.arange(TOTAL, dtype=np.uint64))
+    assert np.all(ind["index_count"] == 3) and np.all(ind["instance_count"] == 1)
+
+
+def test_benchmark_overflow_probe(ctx):
+    """meshlet_cull_benchmark.rs:344-368: capacity TOTAL - 17 gives [TOTAL, 17]."""
+    cam, sc = synth.benchmark_vg_case(4096, TOTAL // 4096)
+    hiz = HiZBuildPass(ctx, 1, 1, L.HIZ_MAX)
+    hiz.upload_depth(np.ones((1, 1), np.float32))
+    vg = VirtualGeometryPass(ctx, publication_limit=TOTAL - 17)
+    vg.set_frame(sc.meshlets, sc.objects, sc.instances, max_draw_count=TOTAL)
+    vg.prepare(cam, 1920, 1080, 2.0, hiz)
+    vg.execute()
+    ind, md, ctr, _ = vg.read()
+    assert ctr[0] == TOTAL and ctr[1] == 17
+    assert len(ind) == TOTAL - 17
+    _check_slots(ind)
+    mdu = np.ascontiguousarray(md).view(np.uint32).reshape(-1, 4)
+    key = mdu[:, 0].astype(np.uint64) * 256 + mdu[:, 1]
+    assert len(np.unique(key)) == TOTAL - 17
+
+
+def _vg_case(ctx, sc, view, depth, rev, threshold=0.5, publication_limit=None):
+    hiz = HiZBuildPass(ctx, depth.shape[1], depth.shape[0], L.HIZ_MIN if rev else L.HIZ_MAX)
+    hiz.upload_depth(depth)
+    hiz.execute()
+    maxd = int(sc.objects["max_meshlet_count"].astype(np.int64).sum())
+    limit = publication_limit or maxd
+    vg = VirtualGeometryPass(ctx, publication_limit=limit)
+    vg.set_frame(sc.meshlets, sc.objects, sc.instances, max_draw_count=maxd)
+    vg.prepare(view.camera, 1920, 1080, threshold, hiz, reversed_z=rev)
+    vg.execute()
+    got = vg.read()
+    icd = instance_cull_data(sc.instances)
+    wi = vg._frame[3]
+    cap = min(maxd, limit)
+    u = vg_uniforms(len(sc.objects), len(wi), cap, threshold=threshold)
+    pyr = oracle.pyramid(depth, 1 if rev else 0)
+    ref = oracle_vg(view.camera, u, sc.meshlets, sc.objects, sc.instances, icd, wi, pyr, cap, reversed_z=rev)
+    return got, ref, vg
+
+
+def _compare(got, ref):
+    ind, md, ctr, sel = got
+    rind, rmd, rctr, robjs = ref
+    assert np.array_equal(ctr, rctr), (ctr, rctr)
+    assert np.array_equal(sel, robjs["selected_lod_plus_one"])
+    n = len(ind)
+    assert n == min(int(rctr[0]), len(rind))
+    _check_slots(ind)
+    if rctr[1] == 0:
+        rin = rind[:n].view(np.uint32).reshape(-1, 5) if rind.dtype != np.uint32 else rind[:n]
+        rmdn = rmd[:n]
+        ref_rows = np.concatenate([rmdn[:, :3], rin[:, [0, 1, 2, 3]]], axis=1)
+        ref_rows = ref_rows[np.lexsort(ref_rows.T[::-1])]
+        assert np.array_equal(_rows(ind, md), ref_rows)
+
+
+@pytest.mark.parametrize("rev", [True, False])
+def test_vg_parity_with_hiz(ctx, rev):
+    """Config-2 recipe at 1,024 objects x 512 meshlets: frustum + cone + Hi-Z + LOD."""
+    sc = synth.vg_scene(1024, config=2, index=1, cube=400.0)
+    view, depth, _ = synth.occluder_view_and_depth(config=2, reversed_z=rev, boxes=400)
+    print(f"vg parity rev={rev}", flush=True)
+    got, ref, _ = _vg_case(ctx, sc, view, depth, rev)
+    _compare(got, ref)
+    assert ref[2][0] > 1000, "too few visible meshlets to pin the path"
+
+
+def test_vg_parity_config2_full(ctx):
+    """Config 2 at full size: 8,192 objects x 512 meshlets (4,194,304 descriptors)."""
+    sc = synth.vg_scene(8192, config=2)
+    view, depth, _ = synth.occluder_view_and_depth(config=2, reversed_z=True)
+    got, ref, _ = _vg_case(ctx, sc, view, depth, True)
+    _compare(got, ref)
+
+
+def test_vg_overflow_counts(ctx):
+    """A publication budget below the visible count: counts exact, slots dense."""
+    sc = synth.vg_scene(512, config=2, index=2, cube=300.0)
+    view, depth, _ = synth.occluder_view_and_depth(config=2, reversed_z=True, boxes=50)
+    got, ref, _ = _vg_case(ctx, sc, view, depth, True, publication_limit=257)
+    assert ref[2][1] > 0
+    _compare(got, ref)
+
+
+def test_vg_dirty_instance_update(ctx):
+    """Transform-only publication (instance_version) re-derives InstanceCullData
+    for the dirty range only (rendering.rs:787-817)."""
+    sc = synth.vg_scene(256, config=2, index=3, cube=200.0)
+    view, depth, _ = synth.occluder_view_and_depth(config=2, reversed_z=True, boxes=20)
+    hiz = HiZBuildPass(ctx, 1920, 1080, L.HIZ_MIN)
+    hiz.upload_depth(depth)
+    hiz.execute()
+    vg = VirtualGeometryPass(ctx)
+    vg.set_frame(sc.meshlets, sc.objects, sc.instances)
+    vg.prepare(view.camera, hiz=hiz, reversed_z=True)
+    vg.execute()
+    moved = sc.instances[10:20].copy()
+    moved["model"][:, 12:15] += np.float32(3.0)
+    moved["model"][:, 0] *= np.float32(2.0)  # non-uniform: disables the cone test for these
+    vg.update_instances(10, moved)
+    vg.prepare(view.camera, hiz=hiz, reversed_z=True)
+    vg.execute()
+    got = vg.read()
+    inst = sc.instances.copy()
+    inst[10:20] = moved
+    icd = instance_cull_data(inst)
+    assert icd["cone_cull_enabled"][10:20].sum() == 0
+    wi = vg._frame[3]
+    maxd = int(sc.objects["max_meshlet_count"].astype(np.int64).sum())
+    u = vg_uniforms(len(sc.objects), len(wi), maxd)
+    ref = oracle_vg(view.camera, u, sc.meshlets, sc.objects, inst, icd, wi, oracle.pyramid(depth, 1), maxd,
+                    reversed_z=True)
+    _compare(got, ref)
