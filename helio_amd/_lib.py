@@ -87,6 +87,41 @@ class DrawBuffers(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("cull", CullConfig),
+        ("max_batch_views", u32),
+        ("max_inflight", u32),
+        ("max_batch_candidates", u64),
+        ("device_budget_bytes", u64),
+        ("max_buckets", u32),
+        ("use_graph", u32),
+        ("copy_ids", u32),
+    ]
+
+
+class ServerRequest(C.Structure):
+    _fields_ = [
+        ("view", ViewRequest),
+        ("ids_out", vp),
+        ("args_out", vp),
+        ("stats_out", vp),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", u64), ("completed", u64), ("rejected", u64), ("batches", u64),
+        ("latency_p50_us", C.c_double), ("latency_p99_us", C.c_double), ("latency_max_us", C.c_double),
+        ("latency_mean_us", C.c_double), ("mean_batch_views", C.c_double),
+        ("device_bytes_peak", u64), ("device_bytes_slots", u64),
+    ]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("wg", u32), ("ipt", u32), ("classify_ms", f32)]
+
+
 # name -> (restype, argtypes)
 _SIGS: dict[str, tuple] = {
     "hv_last_error": (C.c_char_p, []),
@@ -156,6 +191,17 @@ _SIGS: dict[str, tuple] = {
     "hv_occlusion_cull_prepare": (i32, [vp, vp, u32, u64]),
     "hv_occlusion_cull_params": (i32, [vp, vp]),
     "hv_occlusion_cull_execute": (i32, [vp, P(DrawBuffers), vp, vp]),
+    "hv_server_create": (i32, [vp, P(ServerConfig), P(vp)]),
+    "hv_server_destroy": (None, [vp]),
+    "hv_server_submit": (i32, [vp, P(ServerRequest), P(u64)]),
+    "hv_server_wait": (i32, [vp, u64, i32, P(u32)]),
+    "hv_server_stats_get": (i32, [vp, P(ServerStats)]),
+    "hv_server_stats_reset": (i32, [vp]),
+    "hv_server_run_trace": (i32, [vp, P(ServerRequest), P(C.c_double), u32, P(C.c_double), P(C.c_double)]),
+    "hv_ctx_set_shapes": (i32, [vp, u32, u32, u32]),
+    "hv_ctx_get_shapes": (i32, [vp, P(u32), P(u32), P(u32)]),
+    "hv_autotune_batch": (i32, [vp, u32, P(TuneResult), u32, P(u32), P(TuneResult)]),
+    "hv_autotune_hiz": (i32, [vp, u32, P(f32), P(u32)]),
 }
 
 _lock = threading.Lock()

@@ -1,0 +1,67 @@
+"""Launch-shape autotuner: classify workgroup size x instances per thread, and
+the Hi-Z tile edge, measured on the GPU and cached as JSON per device.
+
+The cache is keyed by the device's CU count and the workload signature; the
+batched launch, the plain pass and the server read their shapes from the
+context, which ``apply`` sets.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import time
+from pathlib import Path
+
+from ._lib import TuneResult, call
+from .passes import BatchCull, Context, HiZBuildPass
+
+CACHE = Path(__file__).resolve().parent / "tuned" / "shapes.json"
+
+
+def get_shapes(ctx: Context) -> dict:
+    wg, ipt, tile = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    call("hv_ctx_get_shapes", ctx.handle, C.byref(wg), C.byref(ipt), C.byref(tile))
+    return {"wg": wg.value, "ipt": ipt.value, "hiz_tile": tile.value}
+
+
+def apply(ctx: Context, shapes: dict) -> None:
+    call("hv_ctx_set_shapes", ctx.handle, int(shapes["wg"]), int(shapes["ipt"]), int(shapes["hiz_tile"]))
+
+
+def tune_batch(batch: BatchCull, reps: int = 5) -> dict:
+    res = (TuneResult * 16)()
+    n, best = C.c_uint32(), TuneResult()
+    call("hv_autotune_batch", batch.handle, reps, res, 16, C.byref(n), C.byref(best))
+    return {"best": {"wg": best.wg, "ipt": best.ipt, "classify_ms": best.classify_ms},
+            "table": [{"wg": r.wg, "ipt": r.ipt, "classify_ms": r.classify_ms} for r in res[: n.value]]}
+
+
+def tune_hiz(hiz: HiZBuildPass, reps: int = 10) -> dict:
+    ms = (C.c_float * 2)()
+    tile = C.c_uint32()
+    call("hv_autotune_hiz", hiz.handle, reps, ms, C.byref(tile))
+    return {"best_tile": tile.value, "ms_tile32": ms[0], "ms_tile64": ms[1]}
+
+
+def key(ctx: Context, workload: str) -> str:
+    cu, _ = ctx.device_info()
+    return f"cu{cu}:{workload}"
+
+
+def load_cache(path: Path = CACHE) -> dict:
+    try:
+        return json.loads(path.read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def lookup(ctx: Context, workload: str, path: Path = CACHE) -> dict | None:
+    return load_cache(path).get(key(ctx, workload))
+
+
+def store(ctx: Context, workload: str, entry: dict, path: Path = CACHE) -> None:
+    data = load_cache(path)
+    entry = dict(entry, tuned_at=time.strftime("%Y-%m-%dT%H:%M:%S"))
+    data[key(ctx, workload)] = entry
+    path.parent.mkdir(parents=True, exist_ok=True)
+    path.write_text(json.dumps(data, indent=1, sort_keys=True))

@@ -1,5 +1,8 @@
 // C ABI of the batched multi-view cull and the timing/shape hooks of the
 // plain pass. Both run the kernels of hv_cull.hip through CullBatch.
+#include <algorithm>
+#include <vector>
+
 #include "hv_cull.h"
 
 using namespace hv;
@@ -104,6 +107,61 @@ hv_status hv_inst_cull_set_shape(hv_inst_cull_pass* pass, uint32_t wg, uint32_t 
     if (!pass) return set_error(HV_EINVAL, "hv_inst_cull_set_shape: NULL pass");
     pass->prepared = false;
     return set_shape(pass->batch, wg, ipt);
+}
+
+hv_status hv_ctx_set_shapes(hv_ctx* ctx, uint32_t wg, uint32_t ipt, uint32_t hiz_tile) {
+    if (!ctx) return set_error(HV_EINVAL, "hv_ctx_set_shapes: NULL ctx");
+    CullBatch probe(ctx, hv_cull_config{1, {0}, 0, 0, 0}, 1);
+    HV_TRY(set_shape(&probe, wg, ipt));
+    if (hiz_tile != 32 && hiz_tile != 64) return set_error(HV_EINVAL, "hv_ctx_set_shapes: hiz tile %u", hiz_tile);
+    ctx->cull_shape.wg = wg;
+    ctx->cull_shape.ipt = ipt;
+    ctx->hiz_shape.tile = hiz_tile;
+    return HV_OK;
+}
+
+hv_status hv_ctx_get_shapes(hv_ctx* ctx, uint32_t* wg, uint32_t* ipt, uint32_t* hiz_tile) {
+    if (!ctx) return set_error(HV_EINVAL, "hv_ctx_get_shapes: NULL ctx");
+    if (wg) *wg = ctx->cull_shape.wg;
+    if (ipt) *ipt = ctx->cull_shape.ipt;
+    if (hiz_tile) *hiz_tile = ctx->hiz_shape.tile;
+    return HV_OK;
+}
+
+hv_status hv_autotune_batch(hv_batch* b, uint32_t reps, hv_tune_result* results, uint32_t capacity, uint32_t* count,
+                            hv_tune_result* best) {
+    if (!b || !b->prepared) return set_error(HV_ESTATE, "hv_autotune_batch: prepare the batch first");
+    if (reps == 0) reps = 5;
+    static const uint32_t kWg[] = {64, 128, 256, 512};
+    static const uint32_t kIpt[] = {1, 2, 4, 8};
+    CullBatch* cb = b->batch;
+    hipStream_t s = cb->ctx()->stream;
+    hv_tune_result top{0, 0, 1e30f};
+    uint32_t n = 0;
+    for (uint32_t wg : kWg)
+        for (uint32_t ipt : kIpt) {
+            HV_TRY(set_shape(cb, wg, ipt));
+            HV_TRY(cb->reprepare());
+            float ms[3];
+            HV_TRY(cb->execute(s, ms));  // warm-up
+            std::vector<float> t;
+            for (uint32_t r = 0; r < reps; ++r) {
+                HV_TRY(cb->execute(s, ms));
+                t.push_back(ms[0]);
+            }
+            std::sort(t.begin(), t.end());
+            const hv_tune_result res{wg, ipt, t[t.size() / 2]};
+            if (results && n < capacity) results[n] = res;
+            ++n;
+            if (res.classify_ms < top.classify_ms) top = res;
+        }
+    HV_TRY(set_shape(cb, top.wg, top.ipt));
+    HV_TRY(cb->reprepare());
+    cb->ctx()->cull_shape.wg = top.wg;
+    cb->ctx()->cull_shape.ipt = top.ipt;
+    if (count) *count = n;
+    if (best) *best = top;
+    return HV_OK;
 }
 
 }  // extern "C"

@@ -73,6 +73,11 @@ class CullBatch {
     CullBatch(hv_ctx* ctx, const hv_cull_config& cfg, uint32_t max_views);
     ~CullBatch();
     hv_status prepare(const ViewDesc* views, uint32_t n);
+    // Preallocate for up to `views` views, `candidates` candidates in total and
+    // `buckets` buckets per view, so later prepares never reallocate.
+    hv_status reserve(uint32_t views, uint64_t candidates, uint32_t buckets);
+    // Device bytes a batch of these views needs (stage, ids, args, stats, rows).
+    static uint64_t bytes_for(uint64_t candidates, uint32_t views, uint32_t buckets, uint32_t chunk);
     // Eager launches on `stream` (the plain path). If `ms` is non-NULL the
     // three kernels are bracketed by events and their durations returned
     // (ms[0] classify, ms[1] count+scan, ms[2] scatter); this synchronises.
@@ -85,19 +90,25 @@ class CullBatch {
                    hv_draw_indexed_indirect* args, uint32_t args_capacity, uint32_t* stats);
     uint32_t n_views() const { return n_views_; }
     hv_status wait();
+    // Re-run prepare on the last views (after a shape change).
+    hv_status reprepare() { std::vector<ViewDesc> d = descs_; return prepare(d.data(), (uint32_t)d.size()); }
     uint64_t device_bytes() const;
     const CullView& view(uint32_t v) const { return host_views_[v]; }
     hv_ctx* ctx() const { return ctx_; }
     CullShape shape;
 
   private:
-    hv_status launch(hipStream_t stream, float* ms);
+    hv_status launch(hipStream_t stream, float* ms, bool fixed_grids);
+    hv_status ensure_occupancy();
     hv_ctx* ctx_;
     hv_cull_config cfg_;
     uint32_t max_views_;
     uint32_t n_views_ = 0, n_items_ = 0, max_nb_ = 0;
     uint32_t grid1_ = 0, grid23_ = 0;
+    uint32_t resident1_ = 0, lds_cap_ = 0;   // capacity grids used by the captured graph
+    uint32_t occ_key_ = 0;
     std::vector<CullView> host_views_;
+    std::vector<ViewDesc> descs_;
     DevBuf d_table_, d_stage_, d_ids_, d_args_, d_counters_, d_item_stats_, d_block_hist_;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;

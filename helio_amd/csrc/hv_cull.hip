@@ -475,6 +475,48 @@ uint64_t CullBatch::device_bytes() const {
            d_item_stats_.bytes + d_block_hist_.bytes;
 }
 
+uint64_t CullBatch::bytes_for(uint64_t candidates, uint32_t views, uint32_t buckets, uint32_t chunk) {
+    const uint64_t items = candidates / chunk + views;
+    const uint64_t rows = candidates / kSpan + views;
+    return candidates * 12 + (uint64_t)views * (buckets * 20ull + kCounterStride * 4 + sizeof(CullView)) +
+           items * 32 + rows * buckets * 4;
+}
+
+hv_status CullBatch::reserve(uint32_t views, uint64_t candidates, uint32_t buckets) {
+    HV_HIP(hipSetDevice(ctx_->device));
+    if (done_recorded_) HV_HIP(hipEventSynchronize(done_));
+    const hipStream_t st = ctx_->stream;
+    const uint32_t chunk = shape.chunk();
+    HV_TRY(d_stage_.reserve(std::max<uint64_t>(candidates, 1) * 8, false, st));
+    HV_TRY(d_ids_.reserve(std::max<uint64_t>(candidates, 1) * 4, false, st));
+    HV_TRY(d_args_.reserve(std::max<uint64_t>((uint64_t)views * buckets, 1) * sizeof(hv_draw_indexed_indirect), false,
+                           st));
+    HV_TRY(d_item_stats_.reserve((candidates / chunk + views + 1) * 32, false, st));
+    HV_TRY(d_block_hist_.reserve((candidates / kSpan + views + 1) * (uint64_t)buckets * 4, false, st));
+    const uint64_t ctr_bytes = (uint64_t)std::max(views, 1u) * kCounterStride * 4;
+    if (ctr_bytes > d_counters_.bytes) {
+        HV_TRY(d_counters_.reserve(ctr_bytes, false, st));
+        HV_HIP(hipMemsetAsync(d_counters_.ptr, 0, d_counters_.bytes, st));
+    }
+    HV_TRY(d_table_.reserve(sizeof(BatchHdr) + (uint64_t)std::max(views, 1u) * sizeof(CullView), false, st));
+    HV_HIP(hipStreamSynchronize(st));
+    lds_cap_ = std::max(lds_cap_, buckets * 4u);
+    return HV_OK;
+}
+
+hv_status CullBatch::ensure_occupancy() {
+    const uint32_t key = shape.wg * 16u + shape.ipt + ((cfg_.flags & HV_CULL_OCCLUSION) ? 1024u : 0u);
+    if (key == occ_key_ && resident1_) return HV_OK;
+    const bool occl = (cfg_.flags & HV_CULL_OCCLUSION) != 0;
+    int occ = 0;
+    // The AABB variant uses at least as many registers; size the grid for it.
+    HV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(classify_fn(shape, occl, true)),
+                                                        (int)shape.wg, 0));
+    resident1_ = std::max(1u, ctx_->cu_count * (uint32_t)std::max(occ, 1));
+    occ_key_ = key;
+    return HV_OK;
+}
+
 hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     if (!done_) {
         HV_HIP(hipSetDevice(ctx_->device));
@@ -583,6 +625,7 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     }
     n_views_ = n;
     n_items_ = item;
+    descs_.assign(views, views + n);
     max_nb_ = max_nb;
     // Upload the table (ordered before any later launch on the context stream).
     std::vector<uint8_t> buf(table_bytes, 0);
@@ -592,33 +635,33 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     HV_HIP(hipMemcpyAsync(d_table_.ptr, buf.data(), table_bytes, hipMemcpyHostToDevice, st));
     HV_HIP(hipStreamSynchronize(st));
     // Persistent grids.
-    const bool aabb_any = std::any_of(host_views_.begin(), host_views_.end(), [](const CullView& c) { return c.aabb; });
-    ClassifyFn fn = classify_fn(shape, occl, aabb_any);
-    int occ = 0;
-    HV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(fn), (int)shape.wg, 0));
-    const uint32_t resident = std::max(1u, ctx_->cu_count * (uint32_t)std::max(occ, 1));
-    grid1_ = std::max(1u, std::min(resident, std::max(n_items_, 1u)));
+    HV_TRY(ensure_occupancy());
+    grid1_ = std::max(1u, std::min(resident1_, std::max(n_items_, 1u)));
     uint64_t blocks_ub = 0;
     for (const auto& cv : host_views_) blocks_ub += cv.max_blocks;
     grid23_ = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(blocks_ub, 2ull * ctx_->cu_count));
+    lds_cap_ = std::max(lds_cap_, max_nb_ * 4u);
     return HV_OK;
 }
 
-hv_status CullBatch::launch(hipStream_t s, float* ms) {
+hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed) {
     const bool occl = (cfg_.flags & HV_CULL_OCCLUSION) != 0;
     const bool aabb_any = std::any_of(host_views_.begin(), host_views_.end(), [](const CullView& c) { return c.aabb; });
     ClassifyFn fn = classify_fn(shape, occl, aabb_any);
     const BatchHdr* hdr = d_table_.as<BatchHdr>();
     const CullView* views = reinterpret_cast<const CullView*>(d_table_.as<uint8_t>() + sizeof(BatchHdr));
-    const size_t lds = (size_t)max_nb_ * 4;
+    // Fixed (capacity) grids make one captured graph valid for any later table.
+    const size_t lds = fixed ? (size_t)lds_cap_ : (size_t)max_nb_ * 4;
+    const uint32_t g1 = fixed ? resident1_ : grid1_;
+    const uint32_t g23 = fixed ? 2u * ctx_->cu_count : grid23_;
     if (ms) HV_HIP(hipEventRecord(ev_[0], s));
-    hipLaunchKernelGGL(fn, dim3(grid1_), dim3(shape.wg), 0, s, hdr, views);
+    hipLaunchKernelGGL(fn, dim3(g1), dim3(shape.wg), 0, s, hdr, views);
     HV_HIP(hipGetLastError());
     if (ms) HV_HIP(hipEventRecord(ev_[1], s));
-    hipLaunchKernelGGL(dev::k_count, dim3(grid23_), dim3(256), lds, s, hdr, views);
+    hipLaunchKernelGGL(dev::k_count, dim3(g23), dim3(256), lds, s, hdr, views);
     HV_HIP(hipGetLastError());
     if (ms) HV_HIP(hipEventRecord(ev_[2], s));
-    hipLaunchKernelGGL(dev::k_scatter, dim3(grid23_), dim3(256), lds, s, hdr, views);
+    hipLaunchKernelGGL(dev::k_scatter, dim3(g23), dim3(256), lds, s, hdr, views);
     HV_HIP(hipGetLastError());
     HV_HIP(hipEventRecord(done_, s));
     done_recorded_ = true;
@@ -635,13 +678,16 @@ hv_status CullBatch::execute(hipStream_t s, float* ms) {
     HV_HIP(hipSetDevice(ctx_->device));
     if (ms && !ev_[0])
         for (auto& e : ev_) HV_HIP(hipEventCreate(&e));
-    return launch(s, ms);
+    return launch(s, ms, false);
 }
 
 hv_status CullBatch::execute_graph(hipStream_t s) {
     if (!n_views_) return set_error(HV_ESTATE, "execute_graph: nothing prepared");
     HV_HIP(hipSetDevice(ctx_->device));
-    const uint32_t key[4] = {grid1_, grid23_, max_nb_, shape.wg * 16u + shape.ipt};
+    // The classify variant (occlusion, explicit AABBs) and the LDS size are
+    // baked into the graph; grids are the capacity grids.
+    const bool aabb_any = std::any_of(host_views_.begin(), host_views_.end(), [](const CullView& c) { return c.aabb; });
+    const uint32_t key[4] = {resident1_, lds_cap_, aabb_any ? 1u : 0u, shape.wg * 16u + shape.ipt};
     if (!graph_exec_ || std::memcmp(key, graph_key_, sizeof key) != 0) {
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
         if (graph_) (void)hipGraphDestroy(graph_);
@@ -651,7 +697,7 @@ hv_status CullBatch::execute_graph(hipStream_t s) {
         HV_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
         hv_status st = HV_OK;
-        if (e == hipSuccess) st = launch(cap, nullptr);
+        if (e == hipSuccess) st = launch(cap, nullptr, true);
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(cap, &g);
         (void)hipStreamDestroy(cap);

@@ -14,6 +14,7 @@
 // texel is computed from the previous level's texels with the reference's
 // formula and operand order, so the chain is bit-identical for any input.
 #include <algorithm>
+#include <vector>
 
 #include "hv_internal.h"
 
@@ -284,6 +285,54 @@ hv_status hv_hiz_execute(hv_hiz* z, void* stream) {
     HizFn fn = (tile == 64) ? pick<64>(z->mode == HV_HIZ_MIN, vec4) : pick<32>(z->mode == HV_HIZ_MIN, vec4);
     hipLaunchKernelGGL(fn, grid, dim3(256), 0, pick_stream(z->ctx, stream), a);
     HV_HIP(hipGetLastError());
+    return HV_OK;
+}
+
+hv_status hv_autotune_hiz(hv_hiz* z, uint32_t reps, float ms[2], uint32_t* best_tile) {
+    if (!z || !z->level0) return set_error(HV_ESTATE, "hv_autotune_hiz: no depth bound");
+    if (reps == 0) reps = 10;
+    HV_HIP(hipSetDevice(z->ctx->device));
+    hv_ctx* ctx = z->ctx;
+    const uint32_t saved = ctx->hiz_shape.tile;
+    hipEvent_t a, b;
+    HV_HIP(hipEventCreate(&a));
+    HV_HIP(hipEventCreate(&b));
+    const uint32_t tiles[2] = {32, 64};
+    float med[2] = {0, 0};
+    hv_status st = HV_OK;
+    for (int k = 0; k < 2 && st == HV_OK; ++k) {
+        ctx->hiz_shape.tile = tiles[k];
+        std::vector<float> t;
+        for (uint32_t r = 0; r <= reps && st == HV_OK; ++r) {
+            z->skip_next = false;
+            hipError_t e = hipEventRecord(a, ctx->stream);
+            if (e == hipSuccess) st = hv_hiz_execute(z, ctx->stream);
+            if (st == HV_OK && e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(b);
+            float x = 0;
+            if (e == hipSuccess) e = hipEventElapsedTime(&x, a, b);
+            if (e != hipSuccess) st = hip_error(e, "hv_autotune_hiz", __FILE__, __LINE__);
+            if (r > 0) t.push_back(x);  // first run is the warm-up
+        }
+        if (st == HV_OK) {
+            std::sort(t.begin(), t.end());
+            med[k] = t[t.size() / 2];
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (st != HV_OK) {
+        ctx->hiz_shape.tile = saved;
+        return st;
+    }
+    // 32-wide tiles need the tail level to fit the 64x64 LDS tile.
+    const bool t32_ok = z->width <= 2048 && z->height <= 2048;
+    ctx->hiz_shape.tile = (t32_ok && med[0] < med[1]) ? 32u : 64u;
+    if (ms) {
+        ms[0] = med[0];
+        ms[1] = med[1];
+    }
+    if (best_tile) *best_tile = ctx->hiz_shape.tile;
     return HV_OK;
 }
 

@@ -442,6 +442,12 @@ hv_status hv_vg_prepare(hv_vg_pass* p, const hv_vg_frame_data* vg, const hv_came
 
 static hv_status vg_launch(hv_vg_pass* p, hipStream_t s, float* ms) {
     VgArgs& a = p->args;
+    // Nothing to cull: like the reference, no dispatch and no counter clear
+    // (crates/helio-pass-virtual-geometry/src/rendering.rs:1032-1038).
+    if (!p->object_count || !p->work_item_count || !p->max_draw) {
+        if (ms) ms[0] = ms[1] = 0.0f;
+        return HV_OK;
+    }
     if (ms && !p->ev[0])
         for (auto& e : p->ev) HV_HIP(hipEventCreate(&e));
     hipLaunchKernelGGL(k_vg_clear, dim3(1), dim3(64), 0, s, a.counters, a.counter_len);

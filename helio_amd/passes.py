@@ -12,8 +12,8 @@ import ctypes as C
 import numpy as np
 
 from . import layouts as L
-from ._lib import (CullConfig, CullOutputs, DrawBuffers, VgFrameData, VgOutputs, ViewRequest, call, load,
-                   ptr)
+from ._lib import (CullConfig, CullOutputs, DrawBuffers, ServerConfig, ServerRequest, ServerStats, VgFrameData,
+                   VgOutputs, ViewRequest, call, load, ptr)
 
 
 class Context:
@@ -534,3 +534,81 @@ class OcclusionCullPass:
 
     def execute(self, buffers: DrawBuffers, hiz: HiZBuildPass, stream: int | None = None) -> None:
         call("hv_occlusion_cull_execute", self.handle, C.byref(buffers), hiz.handle, stream)
+
+
+class CullServer:
+    """Serving loop: requests admitted by device memory, continuously batched
+    into graph-replayed launches on several streams (libhv.so worker thread)."""
+
+    def __init__(self, ctx: Context, max_batch_views: int = 64, max_inflight: int = 2,
+                 max_batch_candidates: int = 1 << 24, device_budget_bytes: int = 1 << 30, max_buckets: int = 1024,
+                 lod_distances=(25.0, 75.0, 200.0), use_graph: bool = True, copy_ids: bool = True):
+        self.ctx = ctx
+        c = ServerConfig()
+        c.cull = cull_config(lod_distances)
+        c.max_batch_views, c.max_inflight = max_batch_views, max_inflight
+        c.max_batch_candidates, c.device_budget_bytes = max_batch_candidates, device_budget_bytes
+        c.max_buckets, c.use_graph, c.copy_ids = max_buckets, int(use_graph), int(copy_ids)
+        h = C.c_void_p()
+        call("hv_server_create", ctx.handle, C.byref(c), C.byref(h))
+        self.handle = h
+        self._live = {}
+
+    def close(self) -> None:
+        if self.handle:
+            load().hv_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def request(scene: Scene, camera: np.ndarray, first: int, count: int, want_ids: bool = True):
+        """Build a request plus the host buffers it writes into."""
+        cam = np.ascontiguousarray(camera, dtype=L.CAMERA).reshape(1)
+        nb = scene.info()["meshes"] * scene.info()["lods"]
+        ids = np.zeros(max(count, 1), np.uint32) if want_ids else None
+        args = np.zeros(max(nb, 1), L.DRAW_INDEXED_INDIRECT)
+        stats = np.zeros(9, np.uint32)
+        r = ServerRequest()
+        r.view.scene, r.view.camera = scene.handle.value, cam.ctypes.data
+        r.view.first, r.view.count = first, count
+        r.ids_out = None if ids is None else ids.ctypes.data
+        r.args_out, r.stats_out = args.ctypes.data, stats.ctypes.data
+        return r, (cam, ids, args, stats)
+
+    def submit(self, scene: Scene, camera: np.ndarray, first: int, count: int, want_ids: bool = True) -> int:
+        r, keep = self.request(scene, camera, first, count, want_ids)
+        t = C.c_uint64()
+        call("hv_server_submit", self.handle, C.byref(r), C.byref(t))
+        self._live[t.value] = keep
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1):
+        """(ids, args, stats[8]) of a finished request."""
+        vis = C.c_uint32()
+        call("hv_server_wait", self.handle, C.c_uint64(ticket), timeout_ms, C.byref(vis))
+        cam, ids, args, stats = self._live.pop(ticket)
+        return (None if ids is None else ids[: vis.value].copy()), args, stats[:8].copy()
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        call("hv_server_stats_get", self.handle, C.byref(st))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def reset_stats(self) -> None:
+        call("hv_server_stats_reset", self.handle)
+
+    def run_trace(self, requests, arrivals_s: np.ndarray):
+        """Open-loop Poisson driver in C++: returns (latencies_s, wall_s)."""
+        n = len(requests)
+        arr = (ServerRequest * max(n, 1))(*requests)
+        a = np.ascontiguousarray(arrivals_s, np.float64)
+        lat = np.zeros(max(n, 1), np.float64)
+        wall = C.c_double()
+        call("hv_server_run_trace", self.handle, arr, a.ctypes.data_as(C.POINTER(C.c_double)), n,
+             lat.ctypes.data_as(C.POINTER(C.c_double)), C.byref(wall))
+        return lat[:n], wall.value

@@ -515,6 +515,77 @@ hv_status hv_occlusion_cull_params(const hv_occlusion_cull_pass* pass, hv_occl_c
 hv_status hv_occlusion_cull_execute(hv_occlusion_cull_pass* pass, const hv_draw_buffers* buffers, const hv_hiz* hiz,
                                     void* stream);
 
+/* ------------------------------------------------------------------------- */
+/* Serving loop: resident scenes, cull requests admitted by device memory,   */
+/* continuously batched into graph-replayed launches on several streams.     */
+/* No reference counterpart (the reference renders one frame at a time); it  */
+/* is built from the batched cull above. Thread-safe.                        */
+/* ------------------------------------------------------------------------- */
+typedef struct hv_server_config {
+    hv_cull_config cull;
+    uint32_t max_batch_views;       /* views per launch, 1..1024 (1 = plain per-request path) */
+    uint32_t max_inflight;          /* batches on the device at once (one stream each), 1..8 */
+    uint64_t max_batch_candidates;  /* candidates per launch (sizes each slot's buffers) */
+    uint64_t device_budget_bytes;   /* admission: device bytes of all admitted, unfinished requests */
+    uint32_t max_buckets;           /* mesh_count * lod_count per request (sizes args buffers) */
+    uint32_t use_graph;             /* replay each slot's captured hipGraph */
+    uint32_t copy_ids;              /* copy visible ids back for requests that give ids_out */
+} hv_server_config;
+
+typedef struct hv_server_request {
+    hv_view_request view;               /* the camera is copied at submit */
+    uint32_t* ids_out;                  /* optional host buffer, >= view.count entries */
+    hv_draw_indexed_indirect* args_out; /* optional host buffer, >= mesh_count * lod_count */
+    uint32_t* stats_out;                /* optional host buffer, HV_STAT_COUNT + 1 (visible ids at [8]) */
+} hv_server_request;
+
+typedef struct hv_server_stats {
+    uint64_t submitted, completed, rejected, batches;
+    double latency_p50_us, latency_p99_us, latency_max_us, latency_mean_us;
+    double mean_batch_views;
+    uint64_t device_bytes_peak;   /* peak admitted-request bytes */
+    uint64_t device_bytes_slots;  /* slot buffers held by the server */
+} hv_server_stats;
+
+typedef struct hv_server hv_server;
+
+hv_status hv_server_create(hv_ctx* ctx, const hv_server_config* config, hv_server** out);
+/* Drains every admitted request, then stops the worker. */
+void hv_server_destroy(hv_server* server);
+/* Queue one request. HV_ENOMEM if it can never fit the device budget or a
+ * slot, HV_EINVAL for a bad request. *ticket identifies it. */
+hv_status hv_server_submit(hv_server* server, const hv_server_request* request, uint64_t* ticket);
+/* Wait up to timeout_ms (< 0: forever) for a ticket; HV_EAGAIN on time-out.
+ * On success the request's output buffers are filled and *visible is set. */
+hv_status hv_server_wait(hv_server* server, uint64_t ticket, int32_t timeout_ms, uint32_t* visible);
+hv_status hv_server_stats_get(hv_server* server, hv_server_stats* out);
+hv_status hv_server_stats_reset(hv_server* server);
+/* Open-loop load driver: submits requests[i] at arrival_s[i] seconds after
+ * the call (spin-timed on a client thread), waits for all of them and
+ * returns the per-request latencies (seconds, submit -> result delivered). */
+hv_status hv_server_run_trace(hv_server* server, const hv_server_request* requests, const double* arrival_s,
+                              uint32_t count, double* latency_s, double* wall_s);
+
+/* ------------------------------------------------------------------------- */
+/* Launch-shape autotuner. Shapes apply to batches/passes created afterwards. */
+/* ------------------------------------------------------------------------- */
+typedef struct hv_tune_result {
+    uint32_t wg;          /* classify workgroup size */
+    uint32_t ipt;         /* instances per thread per work item */
+    float classify_ms;    /* median classify-kernel time on the tuned batch */
+} hv_tune_result;
+
+hv_status hv_ctx_set_shapes(hv_ctx* ctx, uint32_t wg, uint32_t ipt, uint32_t hiz_tile);
+hv_status hv_ctx_get_shapes(hv_ctx* ctx, uint32_t* wg, uint32_t* ipt, uint32_t* hiz_tile);
+/* Times every supported (wg, ipt) on the views of a prepared batch (`reps`
+ * timed runs each, median), leaves the batch prepared with the fastest and
+ * makes it the context default. results may be NULL. */
+hv_status hv_autotune_batch(hv_batch* batch, uint32_t reps, hv_tune_result* results, uint32_t capacity,
+                            uint32_t* count, hv_tune_result* best);
+/* Times the pyramid build with 32x32 and 64x64 tiles (ms[0], ms[1]) and makes
+ * the faster the context default. */
+hv_status hv_autotune_hiz(hv_hiz* hiz, uint32_t reps, float ms[2], uint32_t* best_tile);
+
 #ifdef __cplusplus
 }
 #endif
