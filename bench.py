@@ -90,6 +90,161 @@ def cpu_baseline(sc, views, budget_s: float, threads: int) -> dict:
     }
 
 
+def serving_leg(ctx, rates, seconds: float) -> dict:
+    """Config 4: 8 resident scenes (10k..8M instances, log-spaced), one view per
+    request (perspective or ortho, whole scene), Poisson arrivals, batched
+    server vs the plain per-request path (1 view per launch, no graph)."""
+    from helio_amd import synth
+    from helio_amd.passes import CullServer, Scene
+
+    sizes = np.unique(np.round(np.exp(np.linspace(np.log(10_000), np.log(8_000_000), 8))).astype(int))
+    scenes = []
+    for i, n in enumerate(sizes):
+        sc = synth.instance_scene(int(n), config=4, index=i)
+        s = Scene(ctx, sc.n)
+        s.upload_soa(sc.bounds, sc.mesh_ids, sc.flags)
+        s.set_mesh_lods(sc.mesh_lods, sc.mesh_count, sc.lod_count)
+        scenes.append(s)
+    g = synth.rng(4, 1000)
+
+    def trace(n):
+        reqs = []
+        for k in range(n):
+            si = int(g.integers(0, len(scenes)))
+            if g.random() < 0.25:
+                v = synth.ortho_view(g, g.uniform(0, 1000, 3), float(g.uniform(50, 400)), 2000.0, g.normal(size=3))
+            else:
+                v = synth.perspective_view(g)
+            reqs.append(CullServer.request(scenes[si], v.camera, 0, int(sizes[si]), want_ids=False))
+        return reqs
+
+    out = {"scenes": [int(x) for x in sizes], "rates": {}}
+    batched = CullServer(ctx, max_batch_views=64, max_inflight=3, max_batch_candidates=64 << 20,
+                         device_budget_bytes=16 << 30, copy_ids=False, use_graph=True)
+    plain = CullServer(ctx, max_batch_views=1, max_inflight=1, max_batch_candidates=8 << 20,
+                       device_budget_bytes=16 << 30, copy_ids=False, use_graph=False)
+    for rate in rates:
+        n = max(200, int(rate * seconds))
+        row = {}
+        for name, srv in (("batched", batched), ("plain", plain)):
+            reqs = trace(n)
+            arr = np.cumsum(np.random.default_rng(rate).exponential(1.0 / rate, n))
+            srv.reset_stats()
+            lat, wall = srv.run_trace([r for r, _ in reqs], arr)
+            st = srv.stats()
+            row[name] = {
+                "requests": n,
+                "views_per_s": n / wall,
+                "latency_ms_p50": float(np.percentile(lat, 50) * 1e3),
+                "latency_ms_p99": float(np.percentile(lat, 99) * 1e3),
+                "mean_batch_views": st["mean_batch_views"],
+                "instances_per_s": float(sum(r.view.count for r, _ in reqs) / wall),
+            }
+            log(f"serving {name} @ {rate}/s: {row[name]['views_per_s']:.0f} views/s, "
+                f"p50 {row[name]['latency_ms_p50']:.3f} ms, p99 {row[name]['latency_ms_p99']:.3f} ms, "
+                f"batch {st['mean_batch_views']:.1f}")
+        out["rates"][str(rate)] = row
+    batched.close()
+    plain.close()
+    return out
+
+
+def occlusion_leg(ctx, steps: int) -> dict:
+    """Config 1: 1920x1080 reversed-Z depth from 2,000 seeded boxes, 11-level
+    min pyramid, 1,048,576 spheres frustum + sub-pixel + Hi-Z tested."""
+    from helio_amd import layouts as L
+    from helio_amd import synth
+    from helio_amd.passes import HiZBuildPass, InstanceCullPass, Scene
+
+    view, depth, _ = synth.occluder_view_and_depth(config=1, threads=min(16, os.cpu_count() or 1))
+    sc = synth.instance_scene(1_048_576, config=1)
+    scene = Scene(ctx, sc.n)
+    scene.upload_soa(sc.bounds, sc.mesh_ids, sc.flags)
+    scene.set_mesh_lods(sc.mesh_lods, sc.mesh_count, sc.lod_count)
+    hiz = HiZBuildPass(ctx, 1920, 1080, L.HIZ_MIN)
+    hiz.upload_depth(depth)
+    # Pyramid build time (HIP events on the context stream).
+    from helio_amd._lib import call
+    hz_ms = []
+    for i in range(steps + 3):
+        call("hv_timer_record", ctx.handle, 0, None)
+        hiz.execute()
+        call("hv_timer_record", ctx.handle, 1, None)
+        import ctypes as C
+        ms = C.c_float()
+        call("hv_timer_elapsed_ms", ctx.handle, 0, 1, C.byref(ms))
+        if i >= 3:
+            hz_ms.append(ms.value)
+    p = InstanceCullPass(ctx, occlusion=True, reversed_z=True)
+    p.prepare(scene, view.camera, 0, sc.n, hiz)
+    k = np.array([p.execute_timed() for _ in range(steps)])
+    ids, args, stats = p.read()
+    hz = float(np.median(hz_ms))
+    cls = float(np.median(k[:, 0]))
+    tot = float(np.median(k.sum(1)))
+    return {
+        "pyramid_ms": hz,
+        "pyramid_gbs": 11_058_620 / (hz * 1e-3) / 1e9,
+        "cull_kernels_ms": {"classify": cls, "count_scan": float(np.median(k[:, 1])),
+                            "scatter": float(np.median(k[:, 2]))},
+        "spheres_per_s": sc.n / (tot * 1e-3),
+        "classify_gbs": sc.n * BYTES_PER_TEST / (cls * 1e-3) / 1e9,
+        "visible": int(len(ids)),
+        "stats": [int(x) for x in stats],
+    }
+
+
+def vg_leg(ctx, steps: int) -> dict:
+    """Config 2 (8,192 objects x 512 meshlets, frustum + cone + Hi-Z, 0.5 px)
+    and the reference VG benchmark's four 1,048,576-meshlet cases."""
+    from helio_amd import layouts as L
+    from helio_amd import synth
+    from helio_amd.passes import HiZBuildPass, VirtualGeometryPass
+
+    out = {"benchmark_cases": {}}
+    total = 1_048_576
+    hiz1 = HiZBuildPass(ctx, 1, 1, L.HIZ_MAX)
+    hiz1.upload_depth(np.ones((1, 1), np.float32))
+    for name, objs in (("single_huge", 1), ("few_huge", 16), ("balanced", 4096), ("many_small", 65536)):
+        cam, sc = synth.benchmark_vg_case(objs, total // objs)
+        vg = VirtualGeometryPass(ctx, publication_limit=total)
+        vg.set_frame(sc.meshlets, sc.objects, sc.instances, max_draw_count=total)
+        vg.prepare(cam, 1920, 1080, 2.0, hiz1)
+        for _ in range(5):
+            vg.execute_timed()
+        t = np.array([vg.execute_timed().sum() for _ in range(max(20, steps))])
+        med = float(np.median(t))
+        out["benchmark_cases"][name] = {"objects": objs, "meshlets_per_object": total // objs, "median_ms": med,
+                                        "p95_ms": float(np.percentile(t, 95)), "meshlets_per_second": total / (med * 1e-3)}
+        vg.close()
+    sc = synth.vg_scene(8192, config=2)
+    view, depth, _ = synth.occluder_view_and_depth(config=2, reversed_z=True, threads=min(16, os.cpu_count() or 1))
+    hiz = HiZBuildPass(ctx, 1920, 1080, L.HIZ_MIN)
+    hiz.upload_depth(depth)
+    hiz.execute()
+    vg = VirtualGeometryPass(ctx)
+    vg.set_frame(sc.meshlets, sc.objects, sc.instances)
+    vg.prepare(view.camera, 1920, 1080, 0.5, hiz, reversed_z=True)
+    for _ in range(3):
+        vg.execute_timed()
+    t = np.array([vg.execute_timed() for _ in range(max(20, steps))])
+    ind, md, ctr, sel = vg.read()
+    objs = sc.objects
+    tested = int(sum(objs["lod_meshlet_counts"][i, s - 1] for i, s in enumerate(sel) if s))
+    cull = float(np.median(t[:, 1]))
+    out["config2"] = {
+        "descriptors": int(len(sc.meshlets)),
+        "objects_selected": int((sel > 0).sum()),
+        "meshlets_tested": tested,
+        "published": int(ctr[0]),
+        "select_ms": float(np.median(t[:, 0])),
+        "cull_ms": cull,
+        "meshlets_per_s": tested / (float(np.median(t.sum(1))) * 1e-3),
+        "cull_gbs": (tested * 64 + int(ctr[0]) * 36) / (cull * 1e-3) / 1e9,
+    }
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +253,10 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--instances", type=int, default=N_INSTANCES)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the occlusion and VG legs")
+    ap.add_argument("--serving-rates", default="2000,10000,50000")
+    ap.add_argument("--serving-seconds", type=float, default=0.5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the visibility path is "
@@ -199,6 +358,19 @@ def main() -> int:
         ctx.synchronize()
         lat0.append((time.perf_counter() - t) * 1e3)
 
+    occl = vgres = None
+    if not args.no_extra:
+        log("occlusion leg")
+        occl = occlusion_leg(ctx, max(10, args.steps // 2))
+        log(f"occlusion: {occl}")
+        log("vg leg")
+        vgres = vg_leg(ctx, max(10, args.steps // 2))
+        log(f"vg: {vgres}")
+
+    serving = None
+    if not args.no_serving:
+        serving = serving_leg(ctx, [int(r) for r in args.serving_rates.split(",")], args.serving_seconds)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -253,6 +425,9 @@ def main() -> int:
             "kernel_ms": {"classify": k1, "count_scan": float(k_ms[:, 1].mean()),
                           "scatter": float(k_ms[:, 2].mean())},
         },
+        "serving": serving,
+        "occlusion_1m": occl,
+        "virtual_geometry": vgres,
         "single_view_65536": {
             "latency_ms_p50": pct(lat0, 50),
             "latency_ms_p99": pct(lat0, 99),
