@@ -1,4 +1,4 @@
-// Per-instance compacting cull for gfx950: classify -> count+scan -> scatter.
+// Per-instance compacting cull for gfx950: classify -> rows -> finalize -> scatter.
 //
 // Predicate: indirect_dispatch.wgsl:79-137 applied per instance (frustum with
 // the AABB-or-sphere rule and the sub-pixel test), optionally followed by the
@@ -7,16 +7,19 @@
 // DrawIndexedIndirect per bucket whose first_instance is the exclusive prefix
 // of the bucket counts.
 //
-// Kernels (one launch each, persistent grids, the view table in device memory):
-//   k_classify  : streams 20 B/instance (float4 bounds + u32 meta), compacts the
-//                 survivors of each CHUNK with wave ballots and ONE atomic per
-//                 workgroup chunk into a per-view staging list, writes per-chunk
-//                 stats (no stat atomics).
-//   k_count     : per view and per kSpan staged entries, an LDS bucket histogram
-//                 written as one row; the last block of a view (device-scope
-//                 done counter) sums the stats, scans the rows and writes args.
-//   k_scatter   : per block, loads its row of bucket offsets into LDS and places
-//                 each id with an LDS atomic (no global atomics).
+// Four launches, persistent grids, the view table in device memory, and no
+// global atomics or device-scope fences anywhere (on a multi-XCD CDNA part a
+// device-scope fence writes back L2, and single-address atomics serialise):
+//   k_classify  streams 20 B/instance (float4 bounds + u32 meta); each chunk of
+//               WG x IPT candidates compacts its survivors with wave ballots
+//               into its own stage region and writes one 8-word record
+//               (survivors, stats 1..7).
+//   k_rows      the chunks of a view form <= 32 rows; one block per row builds
+//               the row's bucket histogram in LDS and sums its chunk records.
+//   k_finalize  one 1024-thread block per view: stats, bucket totals over the
+//               rows, exclusive scan -> args, and per-row bucket offsets.
+//   k_scatter   one block per row: its offsets into LDS, each id placed with an
+//               LDS atomic.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -127,10 +130,13 @@ __device__ __forceinline__ uint32_t classify(const CullView* __restrict__ V, con
         draw = false;
     }
     if (!draw) return kInvalid;
+    // lod = #{t : sqrt_rn(d2) >= lod_distance[t]}, evaluated without the
+    // square root as d2 >= lod_d2[t] (the host picks lod_d2[t] as the least
+    // float whose correctly rounded square root reaches the distance).
     const float dx = cx - camx, dy = cy - camy, dz = cz - camz;
-    const float d = sqrtf(dot3(dx, dy, dz, dx, dy, dz));
+    const float d2 = dot3(dx, dy, dz, dx, dy, dz);
     uint32_t lod = 0;
-    for (uint32_t t = 0; t + 1 < lod_count; ++t) lod += (d >= lod_d[t]) ? 1u : 0u;
+    for (uint32_t t = 0; t + 1 < lod_count; ++t) lod += (d2 >= lod_d[t]) ? 1u : 0u;
     return (m & 0x7fffffffu) * lod_count + lod;
 }
 
@@ -141,7 +147,7 @@ __global__ __launch_bounds__(WG) void k_classify(const BatchHdr* __restrict__ hd
     constexpr uint32_t CHUNK = WG * IPT;
     __shared__ uint32_t s_cnt[NW];
     __shared__ uint32_t s_st[NW][8];
-    __shared__ uint32_t s_base;
+    __shared__ uint32_t s_base;  // survivors of the chunk
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
     const uint32_t n_views = hdr->n_views, n_items = hdr->n_items;
     for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
@@ -158,7 +164,7 @@ __global__ __launch_bounds__(WG) void k_classify(const BatchHdr* __restrict__ hd
 #pragma unroll
         for (int i = 0; i < 16; ++i) vp[i] = V->vp[i];
 #pragma unroll
-        for (int i = 0; i < 7; ++i) lod_d[i] = V->lod_d[i];
+        for (int i = 0; i < 7; ++i) lod_d[i] = V->lod_d2[i];
         const float p00 = V->p00, p11 = V->p11, camx = V->cam[0], camy = V->cam[1], camz = V->cam[2];
         const uint32_t lod_count = V->lod_count;
         const uint32_t local_item = item - V->item_begin;
@@ -222,36 +228,39 @@ __global__ __launch_bounds__(WG) void k_classify(const BatchHdr* __restrict__ hd
                 s_cnt[w] = tot;
                 tot += c;
             }
-            s_base = tot ? atomicAdd(&V->counters[kCtrStaged], tot) : 0u;
+            s_base = tot;
         }
+        __syncthreads();
         if (tid < 8) {
             uint32_t sum = 0;
-            if (tid == 0) sum = n;
+            if (tid == 0) sum = s_base;  // survivors of this chunk
             else
                 for (int w = 0; w < NW; ++w) sum += s_st[w][tid];
             V->item_stats[(size_t)local_item * 8 + tid] = sum;
         }
-        __syncthreads();
-        uint32_t pos = s_base + s_cnt[wave];
+        // Survivors go to this chunk's own stage region, in wave order.
+        uint2* __restrict__ out = V->stage + (size_t)local_item * CHUNK;
+        uint32_t pos = s_cnt[wave];
 #pragma unroll
         for (int k = 0; k < IPT; ++k) {
-            if (code[k] != kInvalid) V->stage[pos + mbcnt(vis_mask[k])] = make_uint2(first + k * WG + tid, code[k]);
+            if (code[k] != kInvalid) out[pos + mbcnt(vis_mask[k])] = make_uint2(first + k * WG + tid, code[k]);
             pos += (uint32_t)__popcll(vis_mask[k]);
         }
         __syncthreads();
     }
 }
 
-// Exclusive block scan (256 threads) of s[0..n) in place.
-__device__ void block_excl_scan256(uint32_t* s, uint32_t n, uint32_t* s_tmp) {
+// Exclusive scan of s[0..n) in place by a block of NT threads (s_tmp: NT words).
+template <int NT>
+__device__ void block_excl_scan(uint32_t* s, uint32_t n, uint32_t* s_tmp) {
     const uint32_t tid = threadIdx.x;
-    const uint32_t per = (n + 255u) / 256u;
+    const uint32_t per = (n + NT - 1u) / NT;
     const uint32_t beg = min(n, tid * per), end = min(n, beg + per);
     uint32_t local = 0;
     for (uint32_t i = beg; i < end; ++i) local += s[i];
     s_tmp[tid] = local;
     __syncthreads();
-    for (uint32_t off = 1; off < 256u; off <<= 1) {
+    for (uint32_t off = 1; off < (uint32_t)NT; off <<= 1) {
         const uint32_t v = tid >= off ? s_tmp[tid - off] : 0u;
         __syncthreads();
         s_tmp[tid] += v;
@@ -266,146 +275,123 @@ __device__ void block_excl_scan256(uint32_t* s, uint32_t n, uint32_t* s_tmp) {
     __syncthreads();
 }
 
-__device__ __forceinline__ uint32_t nblocks(uint32_t staged) {
-    return staged ? (staged + kSpan - 1u) / kSpan : 1u;
-}
-
-// s_pref[v] = exclusive prefix over views of nblocks(counters[ctr]);
-// s_pref[n_views] = total. Returns the total.
-__device__ uint32_t blocks_prefix(const CullView* __restrict__ views, uint32_t n_views, uint32_t ctr,
-                                  uint32_t* s_pref, uint32_t* s_tmp) {
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t v = tid; v < n_views; v += 256u) s_pref[v] = nblocks(views[v].counters[ctr]);
-    __syncthreads();
-    block_excl_scan256(s_pref, n_views, s_tmp);
-    if (tid == 0) {
-        const uint32_t last = n_views ? s_pref[n_views - 1] + nblocks(views[n_views - 1].counters[ctr]) : 0u;
-        s_pref[n_views] = last;
-    }
-    __syncthreads();
-    return s_pref[n_views];
-}
-
-__device__ __forceinline__ uint32_t find_view(const uint32_t* s_pref, uint32_t n_views, uint32_t blk) {
+// View owning global row `row` (largest v with row_begin <= row and rows > 0).
+__device__ __forceinline__ uint32_t view_of_row(const CullView* __restrict__ views, uint32_t n_views, uint32_t row) {
     uint32_t lo = 0, hi = n_views;
     while (hi - lo > 1u) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (s_pref[mid] <= blk) lo = mid;
+        if (views[mid].row_begin <= row) lo = mid;
         else hi = mid;
     }
     return lo;
 }
 
-__global__ __launch_bounds__(256) void k_count(const BatchHdr* __restrict__ hdr, const CullView* __restrict__ views) {
+// Row histogram: the bucket counts of one row of chunks, and the sum of their
+// 8-word records.
+__global__ __launch_bounds__(256) void k_rows(const BatchHdr* __restrict__ hdr, const CullView* __restrict__ views) {
     extern __shared__ uint32_t s_bins[];
-    __shared__ uint32_t s_pref[kMaxViews + 1];
-    __shared__ uint32_t s_tmp[256];
-    __shared__ uint32_t s_red[8][33];
-    __shared__ uint32_t s_last;
     const uint32_t tid = threadIdx.x;
-    const uint32_t n_views = hdr->n_views;
-    const uint32_t total = blocks_prefix(views, n_views, kCtrStaged, s_pref, s_tmp);
-    for (uint32_t blk = blockIdx.x; blk < total; blk += gridDim.x) {
-        const uint32_t v = find_view(s_pref, n_views, blk);
-        const CullView* __restrict__ V = views + v;
-        const uint32_t j = blk - s_pref[v];
-        const uint32_t staged = V->counters[kCtrStaged];
-        const uint32_t nblk = nblocks(staged);
-        const uint32_t nb = V->nb;
-        const uint32_t beg = j * kSpan, end = min(staged, beg + kSpan);
+    const uint32_t n_views = hdr->n_views, n_rows = hdr->n_rows;
+    for (uint32_t row = blockIdx.x; row < n_rows; row += gridDim.x) {
+        const CullView* __restrict__ V = views + view_of_row(views, n_views, row);
+        const uint32_t r = row - V->row_begin;
+        const uint32_t nb = V->nb, chunk = V->chunk;
+        const uint32_t i0 = r * V->items_per_row, i1 = min(V->n_items, i0 + V->items_per_row);
         for (uint32_t b = tid; b < nb; b += 256u) s_bins[b] = 0u;
         __syncthreads();
-        const uint2* __restrict__ stage = V->stage;
-        for (uint32_t e = beg + tid; e < end; e += 256u) atomicAdd(&s_bins[stage[e].y], 1u);
+        for (uint32_t it = i0; it < i1; ++it) {
+            const uint32_t cnt = V->item_stats[(size_t)it * 8];
+            const uint2* __restrict__ st = V->stage + (size_t)it * chunk;
+            for (uint32_t e = tid; e < cnt; e += 256u) atomicAdd(&s_bins[st[e].y], 1u);
+        }
+        if (tid < 8) {
+            uint32_t sum = 0;
+            for (uint32_t it = i0; it < i1; ++it) sum += V->item_stats[(size_t)it * 8 + tid];
+            V->row_stats[(size_t)r * 8 + tid] = sum;
+        }
         __syncthreads();
-        uint32_t* row = V->block_hist + (size_t)j * nb;
-        for (uint32_t b = tid; b < nb; b += 256u) row[b] = s_bins[b];
-        __threadfence();
+        uint32_t* __restrict__ out = V->block_hist + (size_t)r * nb;
+        for (uint32_t b = tid; b < nb; b += 256u) out[b] = s_bins[b];
         __syncthreads();
-        if (tid == 0) s_last = (atomicAdd(&V->counters[kCtrDone], 1u) + 1u == nblk) ? 1u : 0u;
-        __syncthreads();
-        if (s_last) {
-            __threadfence();
-            // Stats: sum the per-chunk records of this view.
-            uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            for (uint32_t it = tid; it < V->n_items; it += 256u) {
-                const uint4* rec = reinterpret_cast<const uint4*>(V->item_stats + (size_t)it * 8);
-                const uint4 a = rec[0], c = rec[1];
-                acc[0] += a.x; acc[1] += a.y; acc[2] += a.z; acc[3] += a.w;
-                acc[4] += c.x; acc[5] += c.y; acc[6] += c.z; acc[7] += c.w;
-            }
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                uint32_t x = acc[i];
-                for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-                if ((tid & 63u) == 0) s_red[i][tid >> 6] = x;
-            }
-            __syncthreads();
-            if (tid < 8) {
-                const uint32_t sum = s_red[tid][0] + s_red[tid][1] + s_red[tid][2] + s_red[tid][3];
+    }
+}
+
+// Per view: stats, visible count, bucket totals over the rows, exclusive scan
+// into the args, and each row's starting offset per bucket.
+__global__ __launch_bounds__(1024) void k_finalize(const BatchHdr* __restrict__ hdr,
+                                                   const CullView* __restrict__ views) {
+    extern __shared__ uint32_t s_bins[];
+    __shared__ uint32_t s_tmp[1024];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t n_views = hdr->n_views;
+    for (uint32_t v = blockIdx.x; v < n_views; v += gridDim.x) {
+        const CullView* __restrict__ V = views + v;
+        const uint32_t nb = V->nb, rows = V->n_rows;
+        if (tid < 8) {
+            uint32_t sum = 0;
+            for (uint32_t r = 0; r < rows; ++r) sum += V->row_stats[(size_t)r * 8 + tid];
+            if (tid == 0) {
+                V->counters[kCtrVisible] = sum;  // survivors
+                V->counters[0] = V->count;       // stats[0]: instances tested
+            } else {
                 V->counters[tid] = sum;
             }
-            if (tid == 0) V->counters[kCtrVisible] = staged;
-            // Bucket totals over the rows, then the exclusive prefix.
-            for (uint32_t b = tid; b < nb; b += 256u) {
-                uint32_t t = 0;
-                for (uint32_t r = 0; r < nblk; ++r) t += V->block_hist[(size_t)r * nb + b];
-                s_bins[b] = t;
-                V->args[b].instance_count = t;
+        }
+        const uint32_t* __restrict__ hist = V->block_hist;
+        for (uint32_t b = tid; b < nb; b += 1024u) {
+            uint32_t c[kMaxRows];
+#pragma unroll
+            for (uint32_t r = 0; r < kMaxRows; ++r) c[r] = r < rows ? hist[(size_t)r * nb + b] : 0u;
+            uint32_t t = 0;
+#pragma unroll
+            for (uint32_t r = 0; r < kMaxRows; ++r) t += c[r];
+            s_bins[b] = t;
+        }
+        __syncthreads();
+        block_excl_scan<1024>(s_bins, nb, s_tmp);
+        for (uint32_t b = tid; b < nb; b += 1024u) {
+            uint32_t run = s_bins[b];
+            const hv_mesh_lod ml = V->mesh_lods[b];
+            hv_draw_indexed_indirect a;
+            a.index_count = ml.index_count;
+            a.first_index = ml.first_index;
+            a.base_vertex = ml.base_vertex;
+            a.first_instance = run;
+            uint32_t* cell = V->block_hist + b;
+            for (uint32_t r = 0; r < rows; ++r, cell += nb) {
+                const uint32_t t = *cell;
+                *cell = run;
+                run += t;
             }
-            __syncthreads();
-            block_excl_scan256(s_bins, nb, s_tmp);
-            for (uint32_t b = tid; b < nb; b += 256u) {
-                const hv_mesh_lod ml = V->mesh_lods[b];
-                hv_draw_indexed_indirect& a = V->args[b];
-                a.index_count = ml.index_count;
-                a.first_index = ml.first_index;
-                a.base_vertex = ml.base_vertex;
-                a.first_instance = s_bins[b];
-                // Rows become this bucket's offsets for each scatter block.
-                uint32_t run = s_bins[b];
-                for (uint32_t r = 0; r < nblk; ++r) {
-                    uint32_t* cell = V->block_hist + (size_t)r * nb + b;
-                    const uint32_t t = *cell;
-                    *cell = run;
-                    run += t;
-                }
-            }
+            a.instance_count = run - a.first_instance;
+            V->args[b] = a;
         }
         __syncthreads();
     }
 }
 
+// Scatter: each row places its survivors at its bucket offsets.
 __global__ __launch_bounds__(256) void k_scatter(const BatchHdr* __restrict__ hdr,
                                                  const CullView* __restrict__ views) {
     extern __shared__ uint32_t s_cur[];
-    __shared__ uint32_t s_pref[kMaxViews + 1];
-    __shared__ uint32_t s_tmp[256];
     const uint32_t tid = threadIdx.x;
-    const uint32_t n_views = hdr->n_views;
-    const uint32_t total = blocks_prefix(views, n_views, kCtrVisible, s_pref, s_tmp);
-    if (blockIdx.x == 0) {
-        // The classify/count counters are consumed: leave them zero for the next launch.
-        for (uint32_t v = tid; v < n_views; v += 256u) {
-            views[v].counters[kCtrStaged] = 0u;
-            views[v].counters[kCtrDone] = 0u;
-        }
-    }
-    for (uint32_t blk = blockIdx.x; blk < total; blk += gridDim.x) {
-        const uint32_t v = find_view(s_pref, n_views, blk);
-        const CullView* __restrict__ V = views + v;
-        const uint32_t j = blk - s_pref[v];
-        const uint32_t staged = V->counters[kCtrVisible];
-        const uint32_t nb = V->nb;
-        const uint32_t beg = j * kSpan, end = min(staged, beg + kSpan);
-        const uint32_t* row = V->block_hist + (size_t)j * nb;
-        for (uint32_t b = tid; b < nb; b += 256u) s_cur[b] = row[b];
+    const uint32_t n_views = hdr->n_views, n_rows = hdr->n_rows;
+    for (uint32_t row = blockIdx.x; row < n_rows; row += gridDim.x) {
+        const CullView* __restrict__ V = views + view_of_row(views, n_views, row);
+        const uint32_t r = row - V->row_begin;
+        const uint32_t nb = V->nb, chunk = V->chunk;
+        const uint32_t i0 = r * V->items_per_row, i1 = min(V->n_items, i0 + V->items_per_row);
+        const uint32_t* __restrict__ offs = V->block_hist + (size_t)r * nb;
+        for (uint32_t b = tid; b < nb; b += 256u) s_cur[b] = offs[b];
         __syncthreads();
-        const uint2* __restrict__ stage = V->stage;
         uint32_t* __restrict__ ids = V->ids;
-        for (uint32_t e = beg + tid; e < end; e += 256u) {
-            const uint2 en = stage[e];
-            ids[atomicAdd(&s_cur[en.y], 1u)] = en.x;
+        for (uint32_t it = i0; it < i1; ++it) {
+            const uint32_t cnt = V->item_stats[(size_t)it * 8];
+            const uint2* __restrict__ st = V->stage + (size_t)it * chunk;
+            for (uint32_t e = tid; e < cnt; e += 256u) {
+                const uint2 en = st[e];
+                ids[atomicAdd(&s_cur[en.y], 1u)] = en.x;
+            }
         }
         __syncthreads();
     }
@@ -447,6 +433,25 @@ ClassifyFn classify_fn(const CullShape& s, bool occl, bool aabb) {
     }
 }
 
+// Least non-negative float s with sqrtf(s) >= t (binary search over the bit
+// patterns of [0, +inf]; sqrtf is monotone). NaN t gives NaN, so that
+// s >= result is false like sqrtf(s) >= NaN.
+float sqrt_threshold(float t) {
+    if (t != t) return t;
+    if (!(t > 0.0f)) return 0.0f;
+    uint32_t lo = 0u, hi = 0x7f800000u;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2u;
+        float f;
+        std::memcpy(&f, &mid, 4);
+        if (std::sqrt(f) >= t) hi = mid;
+        else lo = mid + 1u;
+    }
+    float r;
+    std::memcpy(&r, &lo, 4);
+    return r;
+}
+
 }  // namespace
 
 CullBatch::CullBatch(hv_ctx* ctx, const hv_cull_config& cfg, uint32_t max_views)
@@ -477,9 +482,8 @@ uint64_t CullBatch::device_bytes() const {
 
 uint64_t CullBatch::bytes_for(uint64_t candidates, uint32_t views, uint32_t buckets, uint32_t chunk) {
     const uint64_t items = candidates / chunk + views;
-    const uint64_t rows = candidates / kSpan + views;
-    return candidates * 12 + (uint64_t)views * (buckets * 20ull + kCounterStride * 4 + sizeof(CullView)) +
-           items * 32 + rows * buckets * 4;
+    return (items * chunk) * 8 + candidates * 4 + items * 32 +
+           (uint64_t)views * (buckets * 20ull + kCounterStride * 4 + sizeof(CullView) + kMaxRows * (buckets + 8ull) * 4);
 }
 
 hv_status CullBatch::reserve(uint32_t views, uint64_t candidates, uint32_t buckets) {
@@ -487,19 +491,16 @@ hv_status CullBatch::reserve(uint32_t views, uint64_t candidates, uint32_t bucke
     if (done_recorded_) HV_HIP(hipEventSynchronize(done_));
     const hipStream_t st = ctx_->stream;
     const uint32_t chunk = shape.chunk();
-    HV_TRY(d_stage_.reserve(std::max<uint64_t>(candidates, 1) * 8, false, st));
+    const uint64_t items = candidates / chunk + views + 1;
+    HV_TRY(d_stage_.reserve(items * chunk * 8, false, st));
     HV_TRY(d_ids_.reserve(std::max<uint64_t>(candidates, 1) * 4, false, st));
     HV_TRY(d_args_.reserve(std::max<uint64_t>((uint64_t)views * buckets, 1) * sizeof(hv_draw_indexed_indirect), false,
                            st));
-    HV_TRY(d_item_stats_.reserve((candidates / chunk + views + 1) * 32, false, st));
-    HV_TRY(d_block_hist_.reserve((candidates / kSpan + views + 1) * (uint64_t)buckets * 4, false, st));
-    const uint64_t ctr_bytes = (uint64_t)std::max(views, 1u) * kCounterStride * 4;
-    if (ctr_bytes > d_counters_.bytes) {
-        HV_TRY(d_counters_.reserve(ctr_bytes, false, st));
-        HV_HIP(hipMemsetAsync(d_counters_.ptr, 0, d_counters_.bytes, st));
-    }
+    HV_TRY(d_item_stats_.reserve(items * 32, false, st));
+    HV_TRY(d_row_stats_.reserve((uint64_t)std::max(views, 1u) * kMaxRows * 32, false, st));
+    HV_TRY(d_block_hist_.reserve((uint64_t)std::max(views, 1u) * kMaxRows * buckets * 4, false, st));
+    HV_TRY(d_counters_.reserve((uint64_t)std::max(views, 1u) * kCounterStride * 4, false, st));
     HV_TRY(d_table_.reserve(sizeof(BatchHdr) + (uint64_t)std::max(views, 1u) * sizeof(CullView), false, st));
-    HV_HIP(hipStreamSynchronize(st));
     lds_cap_ = std::max(lds_cap_, buckets * 4u);
     return HV_OK;
 }
@@ -559,26 +560,28 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
         stage_total += d.count;
         args_total += nb;
         items_total += (d.count + chunk - 1) / chunk;
-        hist_total += (uint64_t)std::max<uint32_t>(1u, (d.count + kSpan - 1) / kSpan) * nb;
+        hist_total += (uint64_t)std::min((d.count + chunk - 1) / chunk, kMaxRows) * nb;
     }
     if (items_total > 0xffffffffull) return set_error(HV_ELIMIT, "prepare: too many work items");
     const hipStream_t st = ctx_->stream;
-    HV_TRY(d_stage_.reserve(std::max<uint64_t>(stage_total, 1) * 8, false, st));
+    uint64_t rows_total = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        const uint32_t items = (views[v].count + chunk - 1) / chunk;
+        rows_total += std::min(items, kMaxRows);
+    }
+    HV_TRY(d_stage_.reserve(std::max<uint64_t>(items_total, 1) * chunk * 8, false, st));
     HV_TRY(d_ids_.reserve(std::max<uint64_t>(stage_total, 1) * 4, false, st));
     HV_TRY(d_args_.reserve(std::max<uint64_t>(args_total, 1) * sizeof(hv_draw_indexed_indirect), false, st));
     HV_TRY(d_item_stats_.reserve(std::max<uint64_t>(items_total, 1) * 32, false, st));
+    HV_TRY(d_row_stats_.reserve(std::max<uint64_t>(rows_total, 1) * 32, false, st));
     HV_TRY(d_block_hist_.reserve(std::max<uint64_t>(hist_total, 1) * 4, false, st));
-    const uint64_t ctr_bytes = (uint64_t)std::max(n, 1u) * kCounterStride * 4;
-    if (ctr_bytes > d_counters_.bytes) {
-        HV_TRY(d_counters_.reserve(ctr_bytes, false, st));
-        HV_HIP(hipMemsetAsync(d_counters_.ptr, 0, d_counters_.bytes, st));
-    }
+    HV_TRY(d_counters_.reserve((uint64_t)std::max(n, 1u) * kCounterStride * 4, false, st));
     const uint64_t table_bytes = sizeof(BatchHdr) + (uint64_t)std::max(n, 1u) * sizeof(CullView);
     HV_TRY(d_table_.reserve(table_bytes, false, st));
 
     host_views_.assign(n, CullView{});
-    uint64_t so = 0, ao = 0, io = 0, ho = 0;
-    uint32_t item = 0;
+    uint64_t so = 0, ao = 0, io = 0, ho = 0, ro = 0;
+    uint32_t item = 0, row = 0;
     for (uint32_t v = 0; v < n; ++v) {
         const ViewDesc& d = views[v];
         CullView& cv = host_views_[v];
@@ -588,7 +591,7 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
         cv.p00 = d.camera->proj[0];
         cv.p11 = d.camera->proj[5];
         std::memcpy(cv.cam, d.camera->position_near, 12);
-        for (int i = 0; i < 7; ++i) cv.lod_d[i] = cfg_.lod_distances[i];
+        for (int i = 0; i < 7; ++i) cv.lod_d2[i] = sqrt_threshold(cfg_.lod_distances[i]);
         cv.lod_count = cfg_.lod_count;
         cv.flags = cfg_.flags;
         cv.nb = d.scene->mesh_count * d.scene->lod_count;
@@ -596,7 +599,13 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
         cv.count = d.count;
         cv.item_begin = item;
         cv.n_items = (d.count + chunk - 1) / chunk;
+        cv.chunk = chunk;
+        cv.n_rows = std::min(cv.n_items, kMaxRows);
+        cv.items_per_row = cv.n_rows ? (cv.n_items + cv.n_rows - 1) / cv.n_rows : 0;
+        if (cv.n_rows) cv.n_rows = (cv.n_items + cv.items_per_row - 1) / cv.items_per_row;
+        cv.row_begin = row;
         item += cv.n_items;
+        row += cv.n_rows;
         cv.screen_h = cfg_.screen_height;
         cv.hiz_mip_count = hv_mip_levels(cfg_.screen_width, cfg_.screen_height);
         cv.bounds = d.scene->bounds.as<float4>();
@@ -611,25 +620,27 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
             cv.hiz0 = d.hiz->level0;
             cv.hiz_packed = d.hiz->packed.as<float>();
         }
-        cv.stage = d_stage_.as<uint2>() + so;
+        cv.stage = d_stage_.as<uint2>() + io * chunk;
         cv.ids = d_ids_.as<uint32_t>() + so;
         cv.args = d_args_.as<hv_draw_indexed_indirect>() + ao;
         cv.counters = d_counters_.as<uint32_t>() + (uint64_t)v * kCounterStride;
         cv.item_stats = d_item_stats_.as<uint32_t>() + io * 8;
-        cv.max_blocks = std::max<uint32_t>(1u, (d.count + kSpan - 1) / kSpan);
+        cv.row_stats = d_row_stats_.as<uint32_t>() + ro * 8;
         cv.block_hist = d_block_hist_.as<uint32_t>() + ho;
         so += d.count;
         ao += cv.nb;
         io += cv.n_items;
-        ho += (uint64_t)cv.max_blocks * cv.nb;
+        ro += cv.n_rows;
+        ho += (uint64_t)cv.n_rows * cv.nb;
     }
     n_views_ = n;
     n_items_ = item;
+    n_rows_ = row;
     descs_.assign(views, views + n);
     max_nb_ = max_nb;
     // Upload the table (ordered before any later launch on the context stream).
     std::vector<uint8_t> buf(table_bytes, 0);
-    BatchHdr hdr{n, item, {0, 0}};
+    BatchHdr hdr{n, item, row, 0};
     std::memcpy(buf.data(), &hdr, sizeof hdr);
     if (n) std::memcpy(buf.data() + sizeof hdr, host_views_.data(), (size_t)n * sizeof(CullView));
     HV_HIP(hipMemcpyAsync(d_table_.ptr, buf.data(), table_bytes, hipMemcpyHostToDevice, st));
@@ -637,9 +648,7 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     // Persistent grids.
     HV_TRY(ensure_occupancy());
     grid1_ = std::max(1u, std::min(resident1_, std::max(n_items_, 1u)));
-    uint64_t blocks_ub = 0;
-    for (const auto& cv : host_views_) blocks_ub += cv.max_blocks;
-    grid23_ = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(blocks_ub, 2ull * ctx_->cu_count));
+    grid23_ = std::max(1u, std::min(n_rows_, 2u * ctx_->cu_count));
     lds_cap_ = std::max(lds_cap_, max_nb_ * 4u);
     return HV_OK;
 }
@@ -654,11 +663,14 @@ hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed) {
     const size_t lds = fixed ? (size_t)lds_cap_ : (size_t)max_nb_ * 4;
     const uint32_t g1 = fixed ? resident1_ : grid1_;
     const uint32_t g23 = fixed ? 2u * ctx_->cu_count : grid23_;
+    const uint32_t gf = fixed ? 2u * ctx_->cu_count : std::max(1u, std::min(n_views_, 2u * ctx_->cu_count));
     if (ms) HV_HIP(hipEventRecord(ev_[0], s));
     hipLaunchKernelGGL(fn, dim3(g1), dim3(shape.wg), 0, s, hdr, views);
     HV_HIP(hipGetLastError());
     if (ms) HV_HIP(hipEventRecord(ev_[1], s));
-    hipLaunchKernelGGL(dev::k_count, dim3(g23), dim3(256), lds, s, hdr, views);
+    hipLaunchKernelGGL(dev::k_rows, dim3(g23), dim3(256), lds, s, hdr, views);
+    HV_HIP(hipGetLastError());
+    hipLaunchKernelGGL(dev::k_finalize, dim3(gf), dim3(1024), lds, s, hdr, views);
     HV_HIP(hipGetLastError());
     if (ms) HV_HIP(hipEventRecord(ev_[2], s));
     hipLaunchKernelGGL(dev::k_scatter, dim3(g23), dim3(256), lds, s, hdr, views);

@@ -9,10 +9,11 @@
 //
 // Here one workgroup reduces a TILE x TILE block of level 0 through
 // log2(TILE) levels (the first one or two in registers from float4 loads, the
-// rest through a 16x16 LDS tile); the last workgroup to finish (device-scope
-// counter, reset by itself) builds the remaining small levels in LDS. Every
-// texel is computed from the previous level's texels with the reference's
-// formula and operand order, so the chain is bit-identical for any input.
+// rest through a 16x16 LDS tile); a second launch of one workgroup builds the
+// remaining small levels in LDS (a kernel boundary instead of a device-scope
+// fence per workgroup). Every texel is computed from the previous level's
+// texels with the reference's formula and operand order, so the chain is
+// bit-identical for any input.
 #include <algorithm>
 #include <vector>
 
@@ -24,10 +25,9 @@ namespace {
 struct HizArgs {
     const float* level0;
     float* packed;
-    uint32_t* counter;
     uint32_t w, h, levels;
-    uint32_t n_tiles;
-    uint32_t off[16];  // float offset of level m >= 1 in packed
+    uint32_t first_tail;  // first level built by the tail launch (the tile levels end one below)
+    uint32_t off[16];     // float offset of level m >= 1 in packed
 };
 
 template <bool MIN>
@@ -62,9 +62,6 @@ __global__ __launch_bounds__(256) void k_hiz(HizArgs a) {
     constexpr uint32_t TL = (TILE == 64) ? 6 : 5; // levels built per tile
     __shared__ float s_a[16 * 16];
     __shared__ float s_b[8 * 8];
-    __shared__ float s_tail_a[64 * 64];
-    __shared__ float s_tail_b[32 * 32];
-    __shared__ uint32_t s_last;
     const uint32_t tid = threadIdx.x;
     const uint32_t tx = blockIdx.x, ty = blockIdx.y;
     const uint32_t lx = tid & 15u, ly = tid >> 4;
@@ -135,15 +132,16 @@ __global__ __launch_bounds__(256) void k_hiz(HizArgs a) {
         dst = t;
         edge = ne;
     }
-    if (top <= TL) return;
-    // ---- tail: the last tile to finish builds levels TL+1 .. top ----
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) s_last = (atomicAdd(a.counter, 1u) + 1u == a.n_tiles) ? 1u : 0u;
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    uint32_t sw = lvl(a.w, TL), sh = lvl(a.h, TL);  // <= 64 x 64 (checked on the host)
+}
+
+// Levels TL+1 .. top from level TL (<= 64 x 64 texels), one workgroup.
+template <bool MIN>
+__global__ __launch_bounds__(256) void k_hiz_tail(HizArgs a) {
+    __shared__ float s_tail_a[64 * 64];
+    __shared__ float s_tail_b[32 * 32];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t top = a.levels - 1u, TL = a.first_tail - 1u;
+    uint32_t sw = lvl(a.w, TL), sh = lvl(a.h, TL);
     const float* lv = a.packed + a.off[TL];
     for (uint32_t i = tid; i < sw * sh; i += 256u) s_tail_a[i] = lv[i];
     __syncthreads();
@@ -163,7 +161,6 @@ __global__ __launch_bounds__(256) void k_hiz(HizArgs a) {
         sw = dw;
         sh = dh;
     }
-    if (tid == 0) *a.counter = 0u;
 }
 
 using HizFn = void (*)(HizArgs);
@@ -202,12 +199,6 @@ hv_status hv_hiz_create(hv_ctx* ctx, uint32_t width, uint32_t height, uint32_t m
         off += (uint64_t)std::max(width >> m, 1u) * std::max(height >> m, 1u);
     }
     hv_status st = z->packed.reserve(std::max<uint64_t>(off, 1) * 4, false, ctx->stream);
-    if (st == HV_OK) st = z->counter.reserve(4, false, ctx->stream);
-    if (st == HV_OK) {
-        hipError_t e = hipMemsetAsync(z->counter.ptr, 0, 4, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) st = hip_error(e, "hv_hiz_create", __FILE__, __LINE__);
-    }
     if (st != HV_OK) {
         hv_hiz_destroy(z);
         return st;
@@ -222,7 +213,6 @@ void hv_hiz_destroy(hv_hiz* z) {
     (void)hipDeviceSynchronize();
     z->own_level0.release();
     z->packed.release();
-    z->counter.release();
     delete z;
 }
 
@@ -274,17 +264,23 @@ hv_status hv_hiz_execute(hv_hiz* z, void* stream) {
     HizArgs a{};
     a.level0 = z->level0;
     a.packed = z->packed.as<float>();
-    a.counter = z->counter.as<uint32_t>();
     a.w = z->width;
     a.h = z->height;
     a.levels = z->levels;
     for (int m = 0; m < 16; ++m) a.off[m] = (uint32_t)z->level_off[m];
     const dim3 grid((z->width + tile - 1) / tile, (z->height + tile - 1) / tile);
-    a.n_tiles = grid.x * grid.y;
+    const uint32_t tile_levels = tile == 64 ? 6u : 5u;
+    a.first_tail = tile_levels + 1u;
     const bool vec4 = (z->width % 4u) == 0u && (reinterpret_cast<uintptr_t>(z->level0) % 16u) == 0u;
-    HizFn fn = (tile == 64) ? pick<64>(z->mode == HV_HIZ_MIN, vec4) : pick<32>(z->mode == HV_HIZ_MIN, vec4);
-    hipLaunchKernelGGL(fn, grid, dim3(256), 0, pick_stream(z->ctx, stream), a);
+    const bool mn = z->mode == HV_HIZ_MIN;
+    HizFn fn = (tile == 64) ? pick<64>(mn, vec4) : pick<32>(mn, vec4);
+    const hipStream_t s = pick_stream(z->ctx, stream);
+    hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, a);
     HV_HIP(hipGetLastError());
+    if (z->levels - 1u > tile_levels) {
+        hipLaunchKernelGGL(mn ? k_hiz_tail<true> : k_hiz_tail<false>, dim3(1), dim3(256), 0, s, a);
+        HV_HIP(hipGetLastError());
+    }
     return HV_OK;
 }
 

@@ -44,7 +44,7 @@ namespace hv {
 // ---------------------------------------------------------------------------
 // Errors: thread-local message, status codes.
 // ---------------------------------------------------------------------------
-hv_status set_error(hv_status code, const char* fmt, ...);
+hv_status set_error(hv_status code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 hv_status hip_error(hipError_t e, const char* what, const char* file, int line);
 
 #define HV_HIP(call)                                                   \
@@ -117,7 +117,6 @@ struct hv_hiz {
     hv::DevBuf own_level0;
     const float* level0 = nullptr;   // bound depth buffer or own_level0
     hv::DevBuf packed;               // levels 1..levels-1
-    hv::DevBuf counter;              // tail-workgroup election counter (self-resetting)
     uint64_t level_off[16] = {};     // float offsets of level m >= 1 inside `packed`
     uint64_t last_generation = ~0ull;
     bool skip_next = false;

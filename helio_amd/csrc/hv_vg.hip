@@ -9,8 +9,10 @@
 //                64 is exactly one wave64): the work item, object, transform
 //                and InstanceCullData are wave-uniform (scalar loads), each
 //                lane loads its 64-byte descriptor with four 16-byte loads and
-//                runs cull_meshlet (:135-264). Survivors get slots from ONE
-//                atomic per workgroup (4 waves) instead of one per meshlet; the
+//                runs cull_meshlet (:135-264). A workgroup handles 8 rounds of
+//                4 spans and keeps its survivors' outputs in registers, so ONE
+//                slot reservation (a single-address atomic, which serialises
+//                device-wide) covers 32 spans instead of one per meshlet; the
 //                capacity-overflow count is derived from the same reservation.
 #include <algorithm>
 #include <cstring>
@@ -169,52 +171,70 @@ __device__ __forceinline__ bool cull_meshlet(const VgArgs& a, uint32_t mi, uint3
     return true;
 }
 
+// Spans handled per workgroup: 4 waves x kVgRounds rounds. Survivors' outputs
+// stay in registers until the workgroup's single slot reservation.
+constexpr uint32_t kVgRounds = 8;
+
 __global__ __launch_bounds__(256) void k_vg_cull(VgArgs a) {
     __shared__ uint32_t s_cnt[4];
     __shared__ uint32_t s_base;
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const uint32_t work = blockIdx.x * 4u + wave;
-    // Wave-uniform work-item decode (the lane-0 prologue of cs_cull_meshlets).
-    uint32_t first = 0, count = 0, ii = 0, lod = 0, base = 0;
-    bool item_ok = false;
-    if (work < a.work_item_count && work < a.work_item_buf_count) {
-        const hv_vg_work_item item = a.work_items[work];
-        if (item.object_index < a.object_count_buf) {
-            const hv_vg_object* o = a.objects + item.object_index;
-            const uint32_t sel = o->selected_lod_plus_one;
-            if (sel != 0u) {
-                lod = sel - 1u;
-                const uint32_t cnt = o->lod_meshlet_counts[lod];
-                if (item.local_meshlet_base < cnt) {
-                    first = o->lod_first_meshlets[lod];
-                    count = cnt;
-                    ii = o->instance_index;
-                    base = item.local_meshlet_base;
-                    item_ok = true;
+    uint64_t mask[kVgRounds];
+    uint32_t o_mi[kVgRounds], o_fi[kVgRounds], o_ic[kVgRounds], o_vo[kVgRounds];
+    uint32_t u_ii[kVgRounds], u_lod[kVgRounds];
+    uint32_t wave_total = 0;
+#pragma unroll
+    for (uint32_t round = 0; round < kVgRounds; ++round) {
+        const uint32_t work = (blockIdx.x * kVgRounds + round) * 4u + wave;
+        // Wave-uniform work-item decode (the lane-0 prologue of cs_cull_meshlets).
+        uint32_t first = 0, count = 0, ii = 0, lod = 0, base = 0;
+        bool item_ok = false;
+        if (work < a.work_item_count && work < a.work_item_buf_count) {
+            const hv_vg_work_item item = a.work_items[work];
+            if (item.object_index < a.object_count_buf) {
+                const hv_vg_object* o = a.objects + item.object_index;
+                const uint32_t sel = o->selected_lod_plus_one;
+                if (sel != 0u) {
+                    lod = sel - 1u;
+                    const uint32_t cnt = o->lod_meshlet_counts[lod];
+                    if (item.local_meshlet_base < cnt) {
+                        first = o->lod_first_meshlets[lod];
+                        count = cnt;
+                        ii = o->instance_index;
+                        base = item.local_meshlet_base;
+                        item_ok = true;
+                    }
                 }
             }
         }
-    }
-    bool visible = false;
-    uint32_t mi = 0;
-    uint4 q3 = make_uint4(0, 0, 0, 0);
-    if (item_ok && base + lane < count) {
-        mi = first + base + lane;
-        if (mi < a.meshlet_count && ii < a.instance_count && ii < a.icd_count) {
-            const hv_instance_cull_data d = a.icd[ii];
-            if (d.valid_transform != 0u) {
-                const float4* mp = reinterpret_cast<const float4*>(a.meshlets + mi);
-                const float4 q0 = mp[0], q1 = mp[1], q2 = mp[2];
-                q3 = reinterpret_cast<const uint4*>(mp)[3];
-                const hv_instance_data& inst = a.instances[ii];
-                visible = cull_meshlet(a, mi, ii, inst.model, inst.normal_mat, d.max_scale, d.cone_cull_enabled, q0,
-                                       q1, q2);
+        bool visible = false;
+        uint32_t mi = 0;
+        uint4 q3 = make_uint4(0, 0, 0, 0);
+        if (item_ok && base + lane < count) {
+            mi = first + base + lane;
+            if (mi < a.meshlet_count && ii < a.instance_count && ii < a.icd_count) {
+                const hv_instance_cull_data d = a.icd[ii];
+                if (d.valid_transform != 0u) {
+                    const float4* mp = reinterpret_cast<const float4*>(a.meshlets + mi);
+                    const float4 q0 = mp[0], q1 = mp[1], q2 = mp[2];
+                    q3 = reinterpret_cast<const uint4*>(mp)[3];
+                    const hv_instance_data& inst = a.instances[ii];
+                    visible = cull_meshlet(a, mi, ii, inst.model, inst.normal_mat, d.max_scale, d.cone_cull_enabled,
+                                           q0, q1, q2);
+                }
             }
         }
+        mask[round] = __ballot(visible);
+        wave_total += (uint32_t)__popcll(mask[round]);
+        o_mi[round] = mi;
+        o_fi[round] = q3.x;
+        o_ic[round] = q3.y;
+        o_vo[round] = q3.z;
+        u_ii[round] = ii;
+        u_lod[round] = lod;
     }
-    const uint64_t mask = __ballot(visible);
-    if (lane == 0) s_cnt[wave] = (uint32_t)__popcll(mask);
+    if (lane == 0) s_cnt[wave] = wave_total;
     __syncthreads();
     if (tid == 0) {
         uint32_t tot = 0;
@@ -232,23 +252,29 @@ __global__ __launch_bounds__(256) void k_vg_cull(VgArgs a) {
         s_base = b;
     }
     __syncthreads();
-    if (visible) {
-        const uint32_t slot = s_base + s_cnt[wave] + mbcnt(mask);
-        if (slot < a.capacity) {
-            hv_draw_indexed_indirect cmd;
-            cmd.index_count = q3.y;
-            cmd.instance_count = 1u;
-            cmd.first_index = q3.x;
-            cmd.base_vertex = (int32_t)q3.z;
-            cmd.first_instance = slot;
-            a.indirect[slot] = cmd;
-            hv_vg_draw md;
-            md.instance_index = ii;
-            md.meshlet_index = mi;
-            md.lod_level = lod;
-            md.reserved = 0u;
-            a.metadata[slot] = md;
+    uint32_t slot0 = s_base + s_cnt[wave];
+#pragma unroll
+    for (uint32_t round = 0; round < kVgRounds; ++round) {
+        const uint64_t m = mask[round];
+        if ((m >> lane) & 1ull) {
+            const uint32_t slot = slot0 + mbcnt(m);
+            if (slot < a.capacity) {
+                hv_draw_indexed_indirect cmd;
+                cmd.index_count = o_ic[round];
+                cmd.instance_count = 1u;
+                cmd.first_index = o_fi[round];
+                cmd.base_vertex = (int32_t)o_vo[round];
+                cmd.first_instance = slot;
+                a.indirect[slot] = cmd;
+                hv_vg_draw md;
+                md.instance_index = u_ii[round];
+                md.meshlet_index = o_mi[round];
+                md.lod_level = u_lod[round];
+                md.reserved = 0u;
+                a.metadata[slot] = md;
+            }
         }
+        slot0 += (uint32_t)__popcll(m);
     }
 }
 
@@ -459,7 +485,8 @@ static hv_status vg_launch(hv_vg_pass* p, hipStream_t s, float* ms) {
     }
     if (ms) HV_HIP(hipEventRecord(p->ev[1], s));
     if (p->work_item_count) {
-        hipLaunchKernelGGL(k_vg_cull, dim3((p->work_item_count + 3) / 4), dim3(256), 0, s, a);
+        const uint32_t per_block = 4u * kVgRounds;
+        hipLaunchKernelGGL(k_vg_cull, dim3((p->work_item_count + per_block - 1) / per_block), dim3(256), 0, s, a);
         HV_HIP(hipGetLastError());
     }
     HV_HIP(hipEventRecord(p->done, s));

@@ -33,6 +33,13 @@ int hvs_rasterize_boxes(const double* view_proj, const double* inv_view_proj, co
     std::vector<Rect> rects(n);
     for (uint32_t b = 0; b < n; ++b) {
         const float* bx = boxes + 6 * (size_t)b;
+        // A box around the eye is seen from inside (back faces only): it
+        // occludes nothing, as with back-face culling in a depth prepass.
+        if (cam[0] >= bx[0] && cam[0] <= bx[3] && cam[1] >= bx[1] && cam[1] <= bx[4] && cam[2] >= bx[2] &&
+            cam[2] <= bx[5]) {
+            rects[b] = {1, 1, 0, 0};
+            continue;
+        }
         double mnx = 1e30, mny = 1e30, mxx = -1e30, mxy = -1e30;
         bool behind = false;
         for (int c = 0; c < 8; ++c) {

@@ -311,7 +311,7 @@ def _rotations(g: np.random.Generator, n: int) -> np.ndarray:
 
 
 def vg_scene(n_objects: int = 8192, lod_counts=(384, 96, 32), config: int = 2, index: int = 0,
-             cube: float = CUBE, error_range=(1e-4, 1e-1)) -> VgScene:
+             cube: float = CUBE, error_range=(1e-4, 1e-1), positions: np.ndarray | None = None) -> VgScene:
     """Seeded objects with rigid + uniform-scale transforms, each owning its own
     meshlets split into LOD ranges (finest first). Meshlet spheres lie inside
     the object sphere; cone axes are unit-random with cutoff U[-0.2, 0.9];
@@ -321,7 +321,7 @@ def vg_scene(n_objects: int = 8192, lod_counts=(384, 96, 32), config: int = 2, i
     n_m = n_objects * per_obj
     rot = _rotations(g, n_objects)
     scale = g.uniform(0.5, 2.0, n_objects)
-    pos = g.uniform(0.0, cube, (n_objects, 3))
+    pos = g.uniform(0.0, cube, (n_objects, 3)) if positions is None else np.asarray(positions, np.float64)
     obj_r = g.uniform(2.0, 15.0, n_objects)
 
     inst = np.zeros(n_objects, L.INSTANCE)
@@ -403,3 +403,38 @@ def benchmark_vg_case(object_count: int, meshlets_per_object: int):
     objs["local_bounds"] = [0.0, 0.0, -10.0, 1.0]
     objs["lod_meshlet_counts"][:, 0] = meshlets_per_object
     return cam, VgScene(ms, objs, inst)
+
+
+def frustum_points(g: np.random.Generator, view: View, n: int, dist=(8.0, 600.0), spread: float = 1.15,
+                   fov_deg: float = 60.0, aspect: float = 16 / 9) -> np.ndarray:
+    """n points in (and slightly around) a perspective view's frustum at
+    log-uniform distances: NDC-like x, y uniform in [-spread, spread]."""
+    view_m = view.camera["view"].reshape(4, 4).astype(np.float64)  # numpy row c = matrix column c
+    right = np.array([view_m[0][0], view_m[1][0], view_m[2][0]])
+    up = np.array([view_m[0][1], view_m[1][1], view_m[2][1]])
+    fwd = -np.array([view_m[0][2], view_m[1][2], view_m[2][2]])
+    ty = math.tan(math.radians(fov_deg) / 2)
+    tx = ty * aspect
+    x = g.uniform(-spread, spread, n) * tx
+    y = g.uniform(-spread, spread, n) * ty
+    d = np.exp(g.uniform(math.log(dist[0]), math.log(dist[1]), n))
+    dirs = fwd[None, :] + x[:, None] * right[None, :] + y[:, None] * up[None, :]
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    return view.position[None, :] + dirs * d[:, None]
+
+
+def vg_workload(n_objects: int = 8192, config: int = 2, index: int = 0, reversed_z: bool = True,
+                boxes: int = 2000, width: int = 1920, height: int = 1080, threads: int = 8):
+    """Config 2 recipe: a camera, `n_objects` VG objects placed in its frustum at
+    8-600 m (LOD varies with distance), and the depth of `boxes` occluders of
+    1-50 m placed in the same frustum at 15-800 m. Returns (view, VgScene, depth)."""
+    g = rng(config, 100 + index)
+    view = perspective_view(g, reversed_z=reversed_z)
+    pos = frustum_points(g, view, n_objects)
+    sc = vg_scene(n_objects, config=config, index=index, positions=pos)
+    centers = frustum_points(g, view, boxes, dist=(15.0, 800.0), spread=1.0)
+    sz = g.uniform(1.0, 50.0, (boxes, 3))
+    occ = np.empty((boxes, 6), np.float32)
+    occ[:, :3] = centers - sz / 2
+    occ[:, 3:] = centers + sz / 2
+    return view, sc, rasterize_depth(view, occ, width, height, reversed_z, threads)

@@ -115,26 +115,26 @@ def _compare(got, ref):
 @pytest.mark.parametrize("rev", [True, False])
 def test_vg_parity_with_hiz(ctx, rev):
     """Config-2 recipe at 1,024 objects x 512 meshlets: frustum + cone + Hi-Z + LOD."""
-    sc = synth.vg_scene(1024, config=2, index=1, cube=400.0)
-    view, depth, _ = synth.occluder_view_and_depth(config=2, reversed_z=rev, boxes=400)
+    view, sc, depth = synth.vg_workload(1024, index=1, reversed_z=rev, boxes=400)
     print(f"vg parity rev={rev}", flush=True)
     got, ref, _ = _vg_case(ctx, sc, view, depth, rev)
     _compare(got, ref)
     assert ref[2][0] > 1000, "too few visible meshlets to pin the path"
+    hist = ref[2][2:5]
+    assert (hist > 0).sum() >= 2, f"LOD selection not exercised: {hist}"
 
 
 def test_vg_parity_config2_full(ctx):
     """Config 2 at full size: 8,192 objects x 512 meshlets (4,194,304 descriptors)."""
-    sc = synth.vg_scene(8192, config=2)
-    view, depth, _ = synth.occluder_view_and_depth(config=2, reversed_z=True)
+    view, sc, depth = synth.vg_workload(8192)
     got, ref, _ = _vg_case(ctx, sc, view, depth, True)
     _compare(got, ref)
+    assert ref[2][0] > 100_000
 
 
 def test_vg_overflow_counts(ctx):
     """A publication budget below the visible count: counts exact, slots dense."""
-    sc = synth.vg_scene(512, config=2, index=2, cube=300.0)
-    view, depth, _ = synth.occluder_view_and_depth(config=2, reversed_z=True, boxes=50)
+    view, sc, depth = synth.vg_workload(512, index=2, boxes=50)
     got, ref, _ = _vg_case(ctx, sc, view, depth, True, publication_limit=257)
     assert ref[2][1] > 0
     _compare(got, ref)
