@@ -185,7 +185,7 @@ def occlusion_leg(ctx, steps: int) -> dict:
     return {
         "pyramid_ms": hz,
         "pyramid_gbs": 11_058_620 / (hz * 1e-3) / 1e9,
-        "cull_kernels_ms": {"classify": cls, "count_scan": float(np.median(k[:, 1])),
+        "cull_kernels_ms": {"classify": cls, "colscan": float(np.median(k[:, 1])),
                             "scatter": float(np.median(k[:, 2]))},
         "spheres_per_s": sc.n / (tot * 1e-3),
         "classify_gbs": sc.n * BYTES_PER_TEST / (cls * 1e-3) / 1e9,
@@ -217,8 +217,7 @@ def vg_leg(ctx, steps: int) -> dict:
         out["benchmark_cases"][name] = {"objects": objs, "meshlets_per_object": total // objs, "median_ms": med,
                                         "p95_ms": float(np.percentile(t, 95)), "meshlets_per_second": total / (med * 1e-3)}
         vg.close()
-    sc = synth.vg_scene(8192, config=2)
-    view, depth, _ = synth.occluder_view_and_depth(config=2, reversed_z=True, threads=min(16, os.cpu_count() or 1))
+    view, sc, depth = synth.vg_workload(8192)
     hiz = HiZBuildPass(ctx, 1920, 1080, L.HIZ_MIN)
     hiz.upload_depth(depth)
     hiz.execute()
@@ -307,7 +306,8 @@ def main() -> int:
     k_ms = np.array([batch.execute_timed() for _ in range(max(10, args.steps // 2))])
     k1 = float(k_ms[:, 0].mean())
     achieved = tests_per_step * BYTES_PER_TEST / (k1 * 1e-3) / 1e9
-    log(f"kernels ms: classify {k1:.4f} count {k_ms[:, 1].mean():.4f} scatter {k_ms[:, 2].mean():.4f}")
+    log(f"kernels ms: classify {k1:.4f} colscan {k_ms[:, 1].mean():.4f} scatter {k_ms[:, 2].mean():.4f}")
+    visible = sum(len(batch.read(i)[0]) for i in range(len(views)))
 
     # ---- plain per-view launch loop (same views, prepared once) ----
     passes = []
@@ -399,7 +399,8 @@ def main() -> int:
             "tests_per_step": tests_per_step,
             "lod_distances_m": [25, 75, 200],
             "buckets_per_view": sc.mesh_count * sc.lod_count,
-            "launch": "hipGraph replay (classify, count+scan, scatter)",
+            "launch": "hipGraph replay (classify, column scan, scatter)",
+            "visible_per_step": visible,
         },
         "roofline": {
             "bound": "hbm",
@@ -422,7 +423,7 @@ def main() -> int:
             "batched_latency_ms_p99": pct(lat_b, 99),
             "plain_loop_latency_ms_p50": pct(lat_p, 50),
             "plain_loop_latency_ms_p99": pct(lat_p, 99),
-            "kernel_ms": {"classify": k1, "count_scan": float(k_ms[:, 1].mean()),
+            "kernel_ms": {"classify": k1, "colscan": float(k_ms[:, 1].mean()),
                           "scatter": float(k_ms[:, 2].mean())},
         },
         "serving": serving,

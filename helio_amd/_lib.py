@@ -119,7 +119,7 @@ class ServerStats(C.Structure):
 
 
 class TuneResult(C.Structure):
-    _fields_ = [("wg", u32), ("ipt", u32), ("classify_ms", f32)]
+    _fields_ = [("wg", u32), ("ipt", u32), ("classify_ms", f32), ("total_ms", f32)]
 
 
 # name -> (restype, argtypes)

@@ -32,8 +32,9 @@ def tune_batch(batch: BatchCull, reps: int = 5) -> dict:
     res = (TuneResult * 16)()
     n, best = C.c_uint32(), TuneResult()
     call("hv_autotune_batch", batch.handle, reps, res, 16, C.byref(n), C.byref(best))
-    return {"best": {"wg": best.wg, "ipt": best.ipt, "classify_ms": best.classify_ms},
-            "table": [{"wg": r.wg, "ipt": r.ipt, "classify_ms": r.classify_ms} for r in res[: n.value]]}
+    return {"best": {"wg": best.wg, "ipt": best.ipt, "classify_ms": best.classify_ms, "total_ms": best.total_ms},
+            "table": [{"wg": r.wg, "ipt": r.ipt, "classify_ms": r.classify_ms, "total_ms": r.total_ms}
+                      for r in res[: n.value]]}
 
 
 def tune_hiz(hiz: HiZBuildPass, reps: int = 10) -> dict:

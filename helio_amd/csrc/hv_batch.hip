@@ -136,7 +136,7 @@ hv_status hv_autotune_batch(hv_batch* b, uint32_t reps, hv_tune_result* results,
     static const uint32_t kIpt[] = {1, 2, 4, 8};
     CullBatch* cb = b->batch;
     hipStream_t s = cb->ctx()->stream;
-    hv_tune_result top{0, 0, 1e30f};
+    hv_tune_result top{0, 0, 1e30f, 1e30f};
     uint32_t n = 0;
     for (uint32_t wg : kWg)
         for (uint32_t ipt : kIpt) {
@@ -144,16 +144,18 @@ hv_status hv_autotune_batch(hv_batch* b, uint32_t reps, hv_tune_result* results,
             HV_TRY(cb->reprepare());
             float ms[3];
             HV_TRY(cb->execute(s, ms));  // warm-up
-            std::vector<float> t;
+            std::vector<float> t1, tt;
             for (uint32_t r = 0; r < reps; ++r) {
                 HV_TRY(cb->execute(s, ms));
-                t.push_back(ms[0]);
+                t1.push_back(ms[0]);
+                tt.push_back(ms[0] + ms[1] + ms[2]);
             }
-            std::sort(t.begin(), t.end());
-            const hv_tune_result res{wg, ipt, t[t.size() / 2]};
+            std::sort(t1.begin(), t1.end());
+            std::sort(tt.begin(), tt.end());
+            const hv_tune_result res{wg, ipt, t1[t1.size() / 2], tt[tt.size() / 2]};
             if (results && n < capacity) results[n] = res;
             ++n;
-            if (res.classify_ms < top.classify_ms) top = res;
+            if (res.total_ms < top.total_ms) top = res;
         }
     HV_TRY(set_shape(cb, top.wg, top.ipt));
     HV_TRY(cb->reprepare());

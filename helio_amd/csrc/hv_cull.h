@@ -9,31 +9,36 @@
 
 namespace hv {
 
-constexpr uint32_t kMaxViews = 1024;      // views per batch launch
-constexpr uint32_t kMaxRows = 32;         // bucket rows per view (groups of classify chunks)
+constexpr uint32_t kMaxViews = 1024;      // views per batch launch (10 bits of a column-scan unit)
 constexpr uint32_t kMaxBuckets = 14336;   // mesh_count * lod_count per view (LDS histogram, < 64 KiB with statics)
 constexpr uint32_t kCounterStride = 16;   // u32 counters per view
+constexpr uint32_t kColscanThreads = 256; // threads of a column-scan block
+constexpr uint32_t kEntryShift = 12;      // stage entry = (bucket << 12) | index in chunk (chunk <= 4096)
 
-// Per-view results (u32, kCounterStride per view), written by the finalize kernel:
+// Per-view results (u32, kCounterStride per view), written by the column scan:
 //   [0..7] stats (layout of indirect_dispatch.wgsl:69-77), [8] visible ids.
 enum : uint32_t { kCtrVisible = 8 };
 
+// A batch is cut into work items of `chunk` consecutive candidates of one view.
+// Classify block b owns the items [b*N/G, (b+1)*N/G) (N items, G blocks); the
+// part of that range inside one view is a "row" of the view: one LDS bucket
+// histogram and one 8-word stats record. Rows of a view are numbered by block:
+// row r of view v belongs to block block_begin + r.
 struct alignas(16) CullView {
     float planes[24];        // extract_frustum_planes(view_proj)
     float vp[16];            // view_proj, column-major
     float p00, p11;          // proj[0][0], proj[1][1]
     float cam[3];            // camera position
     float lod_d2[7];         // squared-distance thresholds: sqrt_rn(s) >= t  <=>  s >= lod_d2
+    float pv_sign[18];       // per plane component: +1 if it is >= 0 (p-vertex takes the max corner), else -1
     uint32_t lod_count;      // 1..8
     uint32_t flags;          // HV_CULL_*
     uint32_t nb;             // buckets = mesh_count * lod_count
     uint32_t first, count;   // candidate range in the scene
-    uint32_t item_begin;     // first classify work item of this view
-    uint32_t n_items;        // classify work items (chunks) of this view
-    uint32_t row_begin;      // first row of this view in the batch's row numbering
-    uint32_t n_rows;         // rows of chunks (<= kMaxRows; 0 when the view is empty)
-    uint32_t items_per_row;  // chunks per row
-    uint32_t chunk;          // instances per chunk (= stage slots per chunk)
+    uint32_t item_begin;     // first work item of this view
+    uint32_t n_items;        // work items of this view
+    uint32_t block_begin;    // block owning the view's first item (its row 0)
+    uint32_t n_rows;         // blocks whose range meets the view (0 when the view is empty)
     uint32_t screen_h;       // occlusion: r_px scale
     uint32_t hiz_mip_count;  // CullParams.hiz_mip_count = mip_levels(screen)
     uint32_t hiz_w, hiz_h, hiz_levels;
@@ -44,21 +49,27 @@ struct alignas(16) CullView {
     const hv_mesh_lod* mesh_lods;
     const float* hiz0;
     const float* hiz_packed;
-    uint2* stage;                      // (instance id, bucket): chunk c owns [c*chunk, (c+1)*chunk)
     uint32_t* ids;                     // capacity count
     hv_draw_indexed_indirect* args;    // nb
     uint32_t* counters;                // kCounterStride
-    uint32_t* item_stats;              // n_items x 8: [0] survivors, [1..7] stats 1..7
-    uint32_t* row_stats;               // n_rows x 8: sums of the row's chunk records
-    uint32_t* block_hist;              // n_rows x nb: bucket counts, then scatter offsets
-    uint32_t _pad[1];
+    uint32_t* row_stats;               // n_rows x 8: [0] survivors, [1..7] stats 1..7
+    uint32_t* block_hist;              // n_rows x nb: bucket counts, then offsets within the bucket
+    uint32_t* totals;                  // nb: bucket totals
 };
 
-struct BatchHdr {
+struct alignas(16) BatchHdr {
     uint32_t n_views;
     uint32_t n_items;
     uint32_t n_rows;
-    uint32_t _pad;
+    uint32_t n_units;        // column-scan work units
+    uint32_t grid;           // G: blocks that own items (classify and scatter)
+    uint32_t chunk;          // candidates per work item (WG * IPT)
+    uint32_t sub;            // stage slots per (item, wave) = IPT * 64
+    uint32_t waves_log2;     // log2(WG / 64)
+    uint32_t* stage;         // n_items x chunk packed survivors; wave w of item i fills [(i*W+w)*sub, +count)
+    uint32_t* sub_cnt;       // n_items x W survivor counts
+    const uint32_t* units;   // n_units x (view | log2(slice) << 10 | first bucket << 14)
+    const uint32_t* block_view;  // G: view holding each block's first item
 };
 
 // One view of a batch, as described by the host.
@@ -78,11 +89,15 @@ class CullBatch {
     // Preallocate for up to `views` views, `candidates` candidates in total and
     // `buckets` buckets per view, so later prepares never reallocate.
     hv_status reserve(uint32_t views, uint64_t candidates, uint32_t buckets);
-    // Device bytes a batch of these views needs (stage, ids, args, stats, rows).
-    static uint64_t bytes_for(uint64_t candidates, uint32_t views, uint32_t buckets, uint32_t chunk);
+    // Device bytes a batch of these views needs (stage, ids, args, stats, rows)
+    // when at most `max_rows` rows are cut in total.
+    static uint64_t bytes_for(uint64_t candidates, uint32_t views, uint32_t buckets, uint32_t chunk,
+                              uint64_t max_rows);
+    // Upper bound of classify blocks that own items, for this context.
+    static uint32_t max_grid(const hv_ctx* ctx);
     // Eager launches on `stream` (the plain path). If `ms` is non-NULL the
     // kernels are bracketed by events and their durations returned
-    // (ms[0] classify, ms[1] rows + finalize, ms[2] scatter); this synchronises.
+    // (ms[0] classify, ms[1] column scan, ms[2] scatter); this synchronises.
     hv_status execute(hipStream_t stream, float* ms = nullptr);
     // The same three kernels captured once into a hipGraph and replayed; the
     // view table lives in device memory, so a new prepare() needs no re-capture.
@@ -101,21 +116,21 @@ class CullBatch {
 
   private:
     hv_status launch(hipStream_t stream, float* ms, bool fixed_grids);
-    uint32_t n_rows_ = 0;
     hv_status ensure_occupancy();
     hv_ctx* ctx_;
     hv_cull_config cfg_;
     uint32_t max_views_;
-    uint32_t n_views_ = 0, n_items_ = 0, max_nb_ = 0;
-    uint32_t grid1_ = 0, grid23_ = 0;
-    uint32_t resident1_ = 0, lds_cap_ = 0;   // capacity grids used by the captured graph
-    uint32_t occ_key_ = 0;
+    uint32_t n_views_ = 0, n_items_ = 0, n_rows_ = 0, n_units_ = 0, max_nb_ = 0;
+    uint32_t grid_ = 0;                       // G of the prepared table
+    uint32_t resident1_ = 0, lds_cap_ = 0;    // capacity grid / LDS of the captured graph
+    uint32_t occ_key_[3] = {0, 0, 0};
     std::vector<CullView> host_views_;
+    std::vector<uint32_t> host_units_, host_block_view_;
     std::vector<ViewDesc> descs_;
-    DevBuf d_table_, d_stage_, d_ids_, d_args_, d_counters_, d_item_stats_, d_row_stats_, d_block_hist_;
+    DevBuf d_table_, d_stage_, d_ids_, d_args_, d_counters_, d_sub_cnt_, d_row_stats_, d_block_hist_, d_totals_;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
-    uint32_t graph_key_[4] = {0, 0, 0, 0};
+    uint32_t graph_key_[5] = {0, 0, 0, 0, 0};
     hipEvent_t ev_[5] = {};
     hipEvent_t done_ = nullptr;    // recorded after every launch
     bool done_recorded_ = false;

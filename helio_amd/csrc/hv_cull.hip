@@ -1,4 +1,4 @@
-// Per-instance compacting cull for gfx950: classify -> rows -> finalize -> scatter.
+// Per-instance compacting cull for gfx950: classify -> column scan -> scatter.
 //
 // Predicate: indirect_dispatch.wgsl:79-137 applied per instance (frustum with
 // the AABB-or-sphere rule and the sub-pixel test), optionally followed by the
@@ -7,19 +7,22 @@
 // DrawIndexedIndirect per bucket whose first_instance is the exclusive prefix
 // of the bucket counts.
 //
-// Four launches, persistent grids, the view table in device memory, and no
-// global atomics or device-scope fences anywhere (on a multi-XCD CDNA part a
-// device-scope fence writes back L2, and single-address atomics serialise):
-//   k_classify  streams 20 B/instance (float4 bounds + u32 meta); each chunk of
-//               WG x IPT candidates compacts its survivors with wave ballots
-//               into its own stage region and writes one 8-word record
-//               (survivors, stats 1..7).
-//   k_rows      the chunks of a view form <= 32 rows; one block per row builds
-//               the row's bucket histogram in LDS and sums its chunk records.
-//   k_finalize  one 1024-thread block per view: stats, bucket totals over the
-//               rows, exclusive scan -> args, and per-row bucket offsets.
-//   k_scatter   one block per row: its offsets into LDS, each id placed with an
-//               LDS atomic.
+// Three launches, no global atomics and no device-scope fences (on a
+// multi-XCD CDNA part a device-scope fence writes back L2, and single-address
+// atomics serialise). A batch is cut into work items of `chunk` consecutive
+// candidates; classify block b owns a contiguous range of items, and the part
+// of that range inside one view is a row of the view.
+//   k_classify  streams 20 B/instance (float4 bounds + u32 meta). Each wave
+//               compacts its survivors with ballots into its own stage slots
+//               as 4-byte entries (bucket << 12 | index in chunk); the block
+//               keeps the row's bucket histogram in LDS and writes it, with
+//               the row's stats, once per row. No barrier inside a row.
+//   k_colscan   work units of (view, bucket slice): per bucket, the exclusive
+//               prefix of the counts over the view's rows, and the bucket
+//               total; unit 0 of a view also sums the stats.
+//   k_scatter   the same block -> item ranges as classify: per row, cursors =
+//               scan(totals) + row offsets in LDS, args from row 0, then a flat
+//               loop over the row's entries places each id with an LDS atomic.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -32,17 +35,12 @@ namespace dev {
 
 constexpr uint32_t kInvalid = 0xffffffffu;
 
-__device__ __forceinline__ Pyramid view_pyramid(const CullView* __restrict__ V) {
-    Pyramid p;
-    p.level0 = V->hiz0;
-    p.packed = V->hiz_packed;
-    p.w = V->hiz_w;
-    p.h = V->hiz_h;
-    p.levels = V->hiz_levels;
-#pragma unroll
-    for (int i = 0; i < 12; ++i) p.off[i] = V->hiz_off[i];
-    return p;
-}
+// Global-address-space views of pointers read from the view table (plain
+// pointers loaded from memory compile to flat accesses).
+typedef float F4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const F4v GlobalF4;
+typedef __attribute__((address_space(1))) const uint32_t GlobalU32;
+typedef __attribute__((address_space(1))) uint32_t GlobalU32Out;
 
 // occlusion_cull.wgsl:158-220 for one sphere (reversed-Z mirror when rev).
 __device__ __forceinline__ bool occluded(const CullView* __restrict__ V, const float* vp, float p00, float p11,
@@ -76,197 +74,372 @@ __device__ __forceinline__ bool occluded(const CullView* __restrict__ V, const f
     const float uv_half = ndc_r * 0.5f;
     const float u0 = wclamp(u - uv_half, 0.0f, 1.0f), v0 = wclamp(vv - uv_half, 0.0f, 1.0f);
     const float u1 = wclamp(u + uv_half, 0.0f, 1.0f), v1 = wclamp(vv + uv_half, 0.0f, 1.0f);
-    const Pyramid p = view_pyramid(V);
+    Pyramid p;
+    p.level0 = V->hiz0;
+    p.packed = V->hiz_packed;
+    p.w = V->hiz_w;
+    p.h = V->hiz_h;
+    p.levels = V->hiz_levels;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) p.off[i] = V->hiz_off[i];
     const float hz = hiz_reduce4(rev, hiz_tap(p, mip, u0, v0), hiz_tap(p, mip, u1, v0), hiz_tap(p, mip, u0, v1),
                                  hiz_tap(p, mip, u1, v1));
     constexpr float bias = 1.0f / 65536.0f;
     return rev ? (near_z < hz - bias) : (near_z > hz + bias);
 }
 
-// Flag bits produced per instance.
-enum : uint32_t { fValid = 1u, fVisible = 2u, fDraw = 4u, fOccl = 8u, fShadow = 16u };
+// Per-candidate outcome of the indirect-dispatch predicate (+ Hi-Z).
+struct Outcome {
+    bool visible;   // in the frustum (indirect_dispatch.wgsl:100-113)
+    bool draw;      // visible and not sub-pixel (:127-137)
+    bool occl;      // draw and occluded by the Hi-Z test (occlusion_cull.wgsl:158-220)
+};
 
-template <bool OCCL, bool AABB>
-__device__ __forceinline__ uint32_t classify(const CullView* __restrict__ V, const float* pl, const float* vp,
-                                             float p00, float p11, float camx, float camy, float camz,
-                                             const float* lod_d, uint32_t lod_count, float4 b, uint32_t m,
-                                             const hv_instance_aabb* ab, uint32_t& fl) {
-    const float cx = b.x, cy = b.y, cz = b.z, r = b.w;
-    float mnx, mny, mnz, mxx, mxy, mxz;
-    if (AABB) {
-        mnx = ab->min[0]; mny = ab->min[1]; mnz = ab->min[2];
-        mxx = ab->max[0]; mxy = ab->max[1]; mxz = ab->max[2];
-    } else {  // sphere_to_aabb (crates/helio/src/scene/helpers.rs:54-65)
-        mnx = cx - r; mny = cy - r; mnz = cz - r;
-        mxx = cx + r; mxy = cy + r; mxz = cz + r;
-    }
-    const bool degenerate = (mnx == mxx) && (mny == mxy) && (mnz == mxz);
-    bool visible = true;
-    if (degenerate) {  // sphere_in_frustum (indirect_dispatch.wgsl:79-86)
+// sphere_in_frustum (indirect_dispatch.wgsl:79-86).
+__device__ __forceinline__ bool sphere_visible(const float* pl, float cx, float cy, float cz, float r) {
+    bool v = true;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            const float dist = dot3(pl[4 * i], pl[4 * i + 1], pl[4 * i + 2], cx, cy, cz) + pl[4 * i + 3];
-            visible = visible && !(dist + r < 0.0f);
-        }
-    } else {  // aabb_in_frustum (:88-98)
+    for (int i = 0; i < 6; ++i) {
+        const float dist = dot3(pl[4 * i], pl[4 * i + 1], pl[4 * i + 2], cx, cy, cz) + pl[4 * i + 3];
+        v = v && !(dist + r < 0.0f);
+    }
+    return v;
+}
+
+// aabb_in_frustum (indirect_dispatch.wgsl:88-98) of the box sphere_to_aabb
+// (crates/helio/src/scene/helpers.rs:54-65) makes of (c, r). The p-vertex
+// component (n >= 0 ? c + r : c - r) is formed as fma(+-1, r, c): the product
+// is exact, so the single rounding equals that of c + r or c - r, and the
+// per-plane signs stay plain scalars instead of lane masks.
+//
+// The six results combine as !(minNum(dist_0..dist_5) < 0): minNum (fminf)
+// ignores quiet NaNs, so the minimum is negative exactly when some non-NaN
+// distance is, which is when some !(dist_i < 0) fails (a NaN distance passes
+// its plane in the reference too).
+__device__ __forceinline__ bool box_visible(const float* pl, const float* sg, float cx, float cy, float cz, float r) {
+    float d[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const float px = __builtin_fmaf(sg[3 * i], r, cx);
+        const float py = __builtin_fmaf(sg[3 * i + 1], r, cy);
+        const float pz = __builtin_fmaf(sg[3 * i + 2], r, cz);
+        d[i] = dot3(pl[4 * i], pl[4 * i + 1], pl[4 * i + 2], px, py, pz) + pl[4 * i + 3];
+    }
+    const float m = fminf(fminf(fminf(d[0], d[1]), d[2]), fminf(fminf(d[3], d[4]), d[5]));
+    return !(m < 0.0f);
+}
+
+// The frustum rule of indirect_dispatch.wgsl:100-113: a degenerate box
+// (min == max on every axis) is tested as a sphere, any other as a box.
+template <bool AABB>
+__device__ __forceinline__ bool frustum_visible(const float* pl, const float* sg, float cx, float cy, float cz, float r,
+                                                const hv_instance_aabb* ab) {
+    if (AABB) {
+        const float mnx = ab->min[0], mny = ab->min[1], mnz = ab->min[2];
+        const float mxx = ab->max[0], mxy = ab->max[1], mxz = ab->max[2];
+        if ((mnx == mxx) && (mny == mxy) && (mnz == mxz)) return sphere_visible(pl, cx, cy, cz, r);
+        bool v = true;
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
             const float px = pl[4 * i] >= 0.0f ? mxx : mnx;
             const float py = pl[4 * i + 1] >= 0.0f ? mxy : mny;
             const float pz = pl[4 * i + 2] >= 0.0f ? mxz : mnz;
             const float dist = dot3(pl[4 * i], pl[4 * i + 1], pl[4 * i + 2], px, py, pz) + pl[4 * i + 3];
-            visible = visible && !(dist < 0.0f);
+            v = v && !(dist < 0.0f);
         }
+        return v;
     }
-    bool draw = visible;
-    if (visible) {  // sub-pixel cull (:127-137)
-        const float w = mrow(vp, 3, cx, cy, cz);
-        if (w > 0.0f && fabsf(r * p11 / w) < 0.001f) draw = false;
+    bool v = box_visible(pl, sg, cx, cy, cz, r);
+    // c - r == c + r on an axis needs r <= 1 ulp of c there (or r == 0): a
+    // radius above 2^-22 of every |c| component makes every axis
+    // non-degenerate. Only a wave holding some other lane runs the exact test.
+    const float cmax = fmaxf(fmaxf(fabsf(cx), fabsf(cy)), fabsf(cz));
+    const bool sure_box = r > cmax * 0x1p-22f;
+    if (__ballot(!sure_box) != 0ull) {
+        // Empty volatile asm: keeps this rare block a real (uniform) branch
+        // instead of being speculated into every wave.
+        asm volatile("" ::: "memory");
+        const bool degenerate = (cx - r == cx + r) && (cy - r == cy + r) && (cz - r == cz + r);
+        if (degenerate) v = sphere_visible(pl, cx, cy, cz, r);
     }
-    fl = fValid | (visible ? fVisible : 0u) | (draw ? fDraw : 0u) | ((m >> 31) ? fShadow : 0u);
-    if (OCCL && draw && occluded(V, vp, p00, p11, camx, camy, camz, (V->flags & HV_CULL_REVERSED_Z) != 0u, cx, cy,
-                                 cz, r)) {
-        fl |= fOccl;
-        draw = false;
-    }
-    if (!draw) return kInvalid;
+    return v;
+}
+
+// Per-row constants of the classify loop. The frustum planes and p-vertex
+// signs (42 floats) are scalar loads from the table through a constant-space
+// pointer and stay in SGPRs; the remaining per-view floats are copied into LDS
+// once per row and read from there, so they sit in a few VGPRs instead of
+// pushing the scalar file into spills.
+struct RowLds {
+    float vp3[4];      // row 3 of view_proj (clip w)
+    float cam[3];
+    float p11;
+    float lod_d2[7];
+    uint32_t lod_count;
+};
+
+// Read of a view-table field through the constant address space. The table
+// is not written while the kernels run; saying so lets uniform reads compile
+// to scalar loads (a generic pointer read from memory yields vector loads, and
+// everything derived from it then lives in VGPRs).
+template <class T>
+__device__ __forceinline__ T ldc(const T* p) {
+    return *(const __attribute__((address_space(4))) T*)p;
+}
+
+// One candidate. Returns its bucket (mesh_id * lod_count + lod) when it is
+// drawn, kInvalid otherwise. Lanes with !valid report all-false.
+template <bool OCCL, bool AABB>
+__device__ __forceinline__ uint32_t classify(const CullView* __restrict__ V, const float* pl, const float* sg,
+                                             const RowLds& rc, bool valid, float4 b, uint32_t m,
+                                             const hv_instance_aabb* ab, Outcome& o) {
+    const float cx = b.x, cy = b.y, cz = b.z, r = b.w;
+    o.visible = valid && frustum_visible<AABB>(pl, sg, cx, cy, cz, r, ab);
+    // Sub-pixel cull, evaluated for every lane (bitwise &: no divergent
+    // branch around the division): w = ((vp[3] x + vp[7] y) + vp[11] z) + vp[15].
+    const float w = ((rc.vp3[0] * cx + rc.vp3[1] * cy) + rc.vp3[2] * cz) + rc.vp3[3];
+    const float q = fabsf(r * rc.p11 / w);
+    const bool tiny = (w > 0.0f) & (q < 0.001f);
+    o.draw = o.visible & !tiny;
+    o.occl = false;
+    const float camx = rc.cam[0], camy = rc.cam[1], camz = rc.cam[2];
+    if (OCCL && o.draw)
+        o.occl = occluded(V, V->vp, V->p00, rc.p11, camx, camy, camz, (V->flags & HV_CULL_REVERSED_Z) != 0u, cx, cy,
+                          cz, r);
     // lod = #{t : sqrt_rn(d2) >= lod_distance[t]}, evaluated without the
     // square root as d2 >= lod_d2[t] (the host picks lod_d2[t] as the least
     // float whose correctly rounded square root reaches the distance).
     const float dx = cx - camx, dy = cy - camy, dz = cz - camz;
     const float d2 = dot3(dx, dy, dz, dx, dy, dz);
-    uint32_t lod = 0;
-    for (uint32_t t = 0; t + 1 < lod_count; ++t) lod += (d2 >= lod_d[t]) ? 1u : 0u;
-    return (m & 0x7fffffffu) * lod_count + lod;
+    // Thresholds past lod_count - 1 are NaN (d2 >= NaN is false), so a fixed
+    // number of compares is exact: 3 for up to 4 LODs, else 7 (uniform branch).
+    const uint32_t lod_count = rc.lod_count;
+    uint32_t lod = (uint32_t)(d2 >= rc.lod_d2[0]) + (uint32_t)(d2 >= rc.lod_d2[1]) + (uint32_t)(d2 >= rc.lod_d2[2]);
+    if (lod_count > 4u)
+        lod += (uint32_t)(d2 >= rc.lod_d2[3]) + (uint32_t)(d2 >= rc.lod_d2[4]) + (uint32_t)(d2 >= rc.lod_d2[5]) +
+               (uint32_t)(d2 >= rc.lod_d2[6]);
+    // mesh_id < 2^14 here (buckets <= 14336), so a 24-bit multiply-add is exact.
+    const uint32_t code = __umul24(m & 0x7fffffffu, lod_count) + lod;
+    return (o.draw && !o.occl) ? code : kInvalid;
+}
+
+// Advance v to the view holding `item` (items are visited in order).
+__device__ __forceinline__ uint32_t advance_view(const CullView* __restrict__ views, uint32_t n_views, uint32_t v,
+                                                 uint32_t item) {
+    while (v + 1u < n_views && item >= ldc(&views[v].item_begin) + ldc(&views[v].n_items)) ++v;
+    return v;
+}
+
+// Item range [beg, end) of block b of G over N items.
+__device__ __forceinline__ void block_items(uint32_t b, uint32_t G, uint32_t N, uint32_t& beg, uint32_t& end) {
+    beg = (uint32_t)(((uint64_t)b * N) / G);
+    end = (uint32_t)(((uint64_t)(b + 1u) * N) / G);
+}
+
+// What the per-item helpers of k_classify need from the current row.
+struct ItemCtx {
+    const CullView* __restrict__ V;
+    const RowLds* rc;     // LDS copy of the per-view constants other than the planes
+    const GlobalF4* __restrict__ bounds;
+    const GlobalU32* __restrict__ meta;
+    const hv_instance_aabb* __restrict__ aabb;
+    GlobalU32Out* __restrict__ stage;
+    GlobalU32Out* __restrict__ sub_cnt;
+    uint32_t ibeg, vcount, vfirst;
+};
+
+// Loads the IPT candidates of work item `it` held by this thread.
+template <int WG, int IPT>
+__device__ __forceinline__ void load_item(const ItemCtx& c, uint32_t it, float4* bb, uint32_t* mm) {
+    constexpr uint32_t CHUNK = WG * IPT;
+    const uint32_t base_idx = (it - c.ibeg) * CHUNK;
+    const uint32_t rem = c.vcount - base_idx;
+    const uint32_t n = rem < CHUNK ? rem : CHUNK;
+    const uint32_t first = c.vfirst + base_idx;
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const uint32_t j = k * WG + threadIdx.x;
+        if (j < n) {
+            const F4v q = c.bounds[first + j];
+            bb[k] = make_float4(q.x, q.y, q.z, q.w);
+            mm[k] = c.meta[first + j];
+        } else {
+            bb[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            mm[k] = 0u;
+        }
+    }
+}
+
+// Classifies the loaded candidates of item `it`: wave-compacted survivors into
+// the wave's stage slots, the row histogram in LDS, stats in `st`.
+template <int WG, int IPT, bool OCCL, bool AABB>
+__device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, const float4* bb, const uint32_t* mm,
+                                             uint32_t* s_hist, uint32_t* st) {
+    constexpr uint32_t NW = WG / 64;
+    constexpr uint32_t CHUNK = WG * IPT, SUB = IPT * 64;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
+    const uint32_t base_idx = (it - c.ibeg) * CHUNK;
+    const uint32_t rem = c.vcount - base_idx;
+    const uint32_t n = rem < CHUNK ? rem : CHUNK;
+    const uint32_t first = c.vfirst + base_idx;
+    GlobalU32Out* __restrict__ out = c.stage + ((size_t)it * NW + wave) * SUB;
+    // The 42 plane/sign constants, as scalar loads per item. `it >> 31` is
+    // zero (items number below 2^31) but not provably so: the loads stay in
+    // the item instead of being hoisted out of the row loop, which would hold
+    // them, with everything else, beyond the scalar register file.
+    const CullView* __restrict__ Vk = c.V + (it >> 31);
+    float pl[24], sg[18];
+#pragma unroll
+    for (int i = 0; i < 24; ++i) pl[i] = ldc(&Vk->planes[i]);
+#pragma unroll
+    for (int i = 0; i < 18; ++i) sg[i] = ldc(&Vk->pv_sign[i]);
+    uint32_t pos = 0;
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const uint32_t j = k * WG + tid;
+        const bool valid = j < n;
+        Outcome o;
+        const uint32_t code = classify<OCCL, AABB>(c.V, pl, sg, *c.rc, valid, bb[k], mm[k],
+                                                   AABB ? c.aabb + first + (valid ? j : 0u) : nullptr, o);
+        const uint64_t keep = __ballot(code != kInvalid);
+        if (code != kInvalid) {
+            out[pos + mbcnt(keep)] = (code << kEntryShift) | j;
+            atomicAdd(&s_hist[code], 1u);
+        }
+        pos += (uint32_t)__popcll(keep);
+        // Stats from lane masks (layout of indirect_dispatch.wgsl:69-77).
+        const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
+        const uint32_t nvis = (uint32_t)__popcll(__ballot(o.visible));
+        const uint64_t mdraw = __ballot(o.draw), mocc = __ballot(o.occl);
+        const uint64_t msh = __ballot(valid && (mm[k] >> 31) != 0u);
+        const uint32_t ndraw = (uint32_t)__popcll(mdraw);
+        st[1] += nv - nvis;
+        st[2] += nvis - ndraw;
+        st[3] += ndraw;
+        st[4] += (uint32_t)__popcll(mocc);
+        st[5] += (uint32_t)__popcll(msh);
+        st[6] += (uint32_t)__popcll(msh & mdraw);
+        st[7] += (uint32_t)__popcll(msh & mocc);
+    }
+    st[0] += pos;
+    if (lane == 0) c.sub_cnt[(size_t)it * NW + wave] = pos;
 }
 
 template <int WG, int IPT, bool OCCL, bool AABB>
 __global__ __launch_bounds__(WG) void k_classify(const BatchHdr* __restrict__ hdr,
                                                  const CullView* __restrict__ views) {
-    constexpr int NW = WG / 64;
+    constexpr uint32_t NW = WG / 64;
     constexpr uint32_t CHUNK = WG * IPT;
-    __shared__ uint32_t s_cnt[NW];
+    static_assert(CHUNK <= (1u << kEntryShift), "chunk index must fit the entry");
+    extern __shared__ uint32_t s_hist[];
     __shared__ uint32_t s_st[NW][8];
-    __shared__ uint32_t s_base;  // survivors of the chunk
+    __shared__ RowLds s_rc;
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
-    const uint32_t n_views = hdr->n_views, n_items = hdr->n_items;
-    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-        uint32_t lo = 0, hi = n_views;
-        while (hi - lo > 1u) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (views[mid].item_begin <= item) lo = mid;
-            else hi = mid;
+    const uint32_t G = ldc(&hdr->grid);
+    if (blockIdx.x >= G) return;
+    const uint32_t n_views = ldc(&hdr->n_views);
+    uint32_t item, i_end;
+    block_items(blockIdx.x, G, ldc(&hdr->n_items), item, i_end);
+    GlobalU32Out* __restrict__ stage = (GlobalU32Out*)ldc(&hdr->stage);
+    GlobalU32Out* __restrict__ sub_cnt = (GlobalU32Out*)ldc(&hdr->sub_cnt);
+    uint32_t v = ldc(ldc(&hdr->block_view) + blockIdx.x);
+    while (item < i_end) {
+        v = advance_view(views, n_views, v, item);
+        const CullView* __restrict__ V = views + v;
+        if (tid == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) s_rc.vp3[i] = V->vp[4 * i + 3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) s_rc.cam[i] = V->cam[i];
+            s_rc.p11 = V->p11;
+#pragma unroll
+            for (int i = 0; i < 7; ++i) s_rc.lod_d2[i] = V->lod_d2[i];
+            s_rc.lod_count = V->lod_count;
         }
-        const CullView* __restrict__ V = views + lo;
-        float pl[24], vp[16], lod_d[7];
-#pragma unroll
-        for (int i = 0; i < 24; ++i) pl[i] = V->planes[i];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) vp[i] = V->vp[i];
-#pragma unroll
-        for (int i = 0; i < 7; ++i) lod_d[i] = V->lod_d2[i];
-        const float p00 = V->p00, p11 = V->p11, camx = V->cam[0], camy = V->cam[1], camz = V->cam[2];
-        const uint32_t lod_count = V->lod_count;
-        const uint32_t local_item = item - V->item_begin;
-        const uint32_t base_idx = local_item * CHUNK;
-        const uint32_t rem = V->count - base_idx;
-        const uint32_t n = rem < CHUNK ? rem : CHUNK;
-        const uint32_t first = V->first + base_idx;
-        const float4* __restrict__ bounds = V->bounds + first;
-        const uint32_t* __restrict__ meta = V->meta + first;
-
-        float4 b[IPT];
-        uint32_t mt[IPT];
-#pragma unroll
-        for (int k = 0; k < IPT; ++k) {
-            const uint32_t j = k * WG + tid;
-            if (j < n) {
-                b[k] = bounds[j];
-                mt[k] = meta[j];
-            } else {
-                b[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-                mt[k] = 0u;
-            }
-        }
-        uint32_t code[IPT], fl[IPT];
-#pragma unroll
-        for (int k = 0; k < IPT; ++k) {
-            const uint32_t j = k * WG + tid;
-            fl[k] = 0u;
-            code[k] = kInvalid;
-            if (j < n)
-                code[k] = classify<OCCL, AABB>(V, pl, vp, p00, p11, camx, camy, camz, lod_d, lod_count, b[k], mt[k],
-                                               AABB ? V->aabb + first + j : nullptr, fl[k]);
-        }
-        // Wave-level compaction masks and stats (ballots: scalar work).
-        uint64_t vis_mask[IPT];
+        const uint32_t nb = ldc(&V->nb), vcount = ldc(&V->count), vfirst = ldc(&V->first);
+        const uint32_t ibeg = ldc(&V->item_begin);
+        const uint32_t seg_end = min(i_end, ibeg + ldc(&V->n_items));
+        const GlobalF4* __restrict__ bounds = (const GlobalF4*)ldc(&V->bounds);
+        const GlobalU32* __restrict__ meta = (const GlobalU32*)ldc(&V->meta);
+        const hv_instance_aabb* __restrict__ aabb = ldc(&V->aabb);
+        for (uint32_t b = tid; b < nb; b += WG) s_hist[b] = 0u;
+        __syncthreads();
         uint32_t st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        uint32_t wave_vis = 0;
-#pragma unroll
-        for (int k = 0; k < IPT; ++k) {
-            const uint32_t f = fl[k];
-            vis_mask[k] = __ballot(code[k] != kInvalid);
-            wave_vis += (uint32_t)__popcll(vis_mask[k]);
-            st[1] += (uint32_t)__popcll(__ballot((f & fValid) && !(f & fVisible)));
-            st[2] += (uint32_t)__popcll(__ballot((f & fVisible) && !(f & fDraw)));
-            st[3] += (uint32_t)__popcll(__ballot((f & fDraw) != 0u));
-            st[4] += (uint32_t)__popcll(__ballot((f & fOccl) != 0u));
-            st[5] += (uint32_t)__popcll(__ballot((f & fShadow) != 0u));
-            st[6] += (uint32_t)__popcll(__ballot((f & fShadow) && (f & fDraw)));
-            st[7] += (uint32_t)__popcll(__ballot((f & fShadow) && (f & fOccl)));
+        // Software pipeline over two register sets: the loads of the next
+        // item are in flight while this one is classified (no register copies,
+        // so the compiler's wait counts cover only the older loads). On gfx9
+        // the vector-memory counter also counts stores, whose number per item
+        // varies with the data; an explicit vmcnt(0) before each prefetch (it
+        // covers loads issued one item earlier) keeps the compiler from placing
+        // that full wait after the prefetch instead.
+        constexpr int kWaitVm0 = 0x0F70;  // s_waitcnt vmcnt(0), expcnt/lgkmcnt untouched
+        const ItemCtx ic{V, &s_rc, bounds, meta, aabb, stage, sub_cnt, ibeg, vcount, vfirst};
+        float4 b0[IPT], b1[IPT];
+        uint32_t m0[IPT], m1[IPT];
+        if (item < seg_end) load_item<WG, IPT>(ic, item, b0, m0);
+        while (item < seg_end) {
+            __builtin_amdgcn_s_waitcnt(kWaitVm0);
+            if (item + 1u < seg_end) load_item<WG, IPT>(ic, item + 1u, b1, m1);
+            process_item<WG, IPT, OCCL, AABB>(ic, item, b0, m0, s_hist, st);
+            if (++item >= seg_end) break;
+            __builtin_amdgcn_s_waitcnt(kWaitVm0);
+            if (item + 1u < seg_end) load_item<WG, IPT>(ic, item + 1u, b0, m0);
+            process_item<WG, IPT, OCCL, AABB>(ic, item, b1, m1, s_hist, st);
+            ++item;
         }
         if (lane == 0) {
-            s_cnt[wave] = wave_vis;
 #pragma unroll
-            for (int i = 1; i < 8; ++i) s_st[wave][i] = st[i];
+            for (int i = 0; i < 8; ++i) s_st[wave][i] = st[i];
         }
         __syncthreads();
-        if (tid == 0) {
-            uint32_t tot = 0;
-            for (int w = 0; w < NW; ++w) {
-                const uint32_t c = s_cnt[w];
-                s_cnt[w] = tot;
-                tot += c;
-            }
-            s_base = tot;
-        }
-        __syncthreads();
+        const uint32_t r = blockIdx.x - ldc(&V->block_begin);
+        uint32_t* __restrict__ hrow = ldc(&V->block_hist) + (size_t)r * nb;
+        for (uint32_t b = tid; b < nb; b += WG) hrow[b] = s_hist[b];
         if (tid < 8) {
             uint32_t sum = 0;
-            if (tid == 0) sum = s_base;  // survivors of this chunk
-            else
-                for (int w = 0; w < NW; ++w) sum += s_st[w][tid];
-            V->item_stats[(size_t)local_item * 8 + tid] = sum;
-        }
-        // Survivors go to this chunk's own stage region, in wave order.
-        uint2* __restrict__ out = V->stage + (size_t)local_item * CHUNK;
-        uint32_t pos = s_cnt[wave];
 #pragma unroll
-        for (int k = 0; k < IPT; ++k) {
-            if (code[k] != kInvalid) out[pos + mbcnt(vis_mask[k])] = make_uint2(first + k * WG + tid, code[k]);
-            pos += (uint32_t)__popcll(vis_mask[k]);
+            for (uint32_t w = 0; w < NW; ++w) sum += s_st[w][tid];
+            ldc(&V->row_stats)[(size_t)r * 8 + tid] = sum;
         }
         __syncthreads();
     }
 }
 
-// Exclusive scan of s[0..n) in place by a block of NT threads (s_tmp: NT words).
+// Exclusive scan of one value per thread across an NT-thread block. `s_w`
+// holds NT / 64 words; the caller synchronises before reusing it.
 template <int NT>
-__device__ void block_excl_scan(uint32_t* s, uint32_t n, uint32_t* s_tmp) {
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* s_w, uint32_t& total) {
+    constexpr uint32_t NW = NT / 64;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    uint32_t inc = x;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += y;
+    }
+    if (lane == 63u) s_w[w] = inc;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < NW; ++i) {
+        const uint32_t t = s_w[i];
+        base += (i < w) ? t : 0u;
+        tot += t;
+    }
+    total = tot;
+    return base + inc - x;
+}
+
+// In-place exclusive scan of s[0..n) by an NT-thread block (s_w: NT / 64 words).
+template <int NT>
+__device__ void block_excl_scan_lds(uint32_t* s, uint32_t n, uint32_t* s_w) {
     const uint32_t tid = threadIdx.x;
     const uint32_t per = (n + NT - 1u) / NT;
     const uint32_t beg = min(n, tid * per), end = min(n, beg + per);
     uint32_t local = 0;
     for (uint32_t i = beg; i < end; ++i) local += s[i];
-    s_tmp[tid] = local;
-    __syncthreads();
-    for (uint32_t off = 1; off < (uint32_t)NT; off <<= 1) {
-        const uint32_t v = tid >= off ? s_tmp[tid - off] : 0u;
-        __syncthreads();
-        s_tmp[tid] += v;
-        __syncthreads();
-    }
-    uint32_t run = s_tmp[tid] - local;
+    uint32_t total;
+    uint32_t run = block_excl_scan<NT>(local, s_w, total);
     for (uint32_t i = beg; i < end; ++i) {
         const uint32_t t = s[i];
         s[i] = run;
@@ -275,125 +448,214 @@ __device__ void block_excl_scan(uint32_t* s, uint32_t n, uint32_t* s_tmp) {
     __syncthreads();
 }
 
-// View owning global row `row` (largest v with row_begin <= row and rows > 0).
-__device__ __forceinline__ uint32_t view_of_row(const CullView* __restrict__ views, uint32_t n_views, uint32_t row) {
-    uint32_t lo = 0, hi = n_views;
-    while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (views[mid].row_begin <= row) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// Row histogram: the bucket counts of one row of chunks, and the sum of their
-// 8-word records.
-__global__ __launch_bounds__(256) void k_rows(const BatchHdr* __restrict__ hdr, const CullView* __restrict__ views) {
-    extern __shared__ uint32_t s_bins[];
+// Column scan. A unit covers buckets [b0, b0 + 2^lg) of one view; its 256
+// threads form 256 >> lg row groups per bucket, each owning a contiguous run
+// of rows (the host picks lg so that a run is at most 8 rows).
+__global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __restrict__ hdr,
+                                                             const CullView* __restrict__ views) {
+    __shared__ uint32_t s_part[kColscanThreads];
     const uint32_t tid = threadIdx.x;
-    const uint32_t n_views = hdr->n_views, n_rows = hdr->n_rows;
-    for (uint32_t row = blockIdx.x; row < n_rows; row += gridDim.x) {
-        const CullView* __restrict__ V = views + view_of_row(views, n_views, row);
-        const uint32_t r = row - V->row_begin;
-        const uint32_t nb = V->nb, chunk = V->chunk;
-        const uint32_t i0 = r * V->items_per_row, i1 = min(V->n_items, i0 + V->items_per_row);
-        for (uint32_t b = tid; b < nb; b += 256u) s_bins[b] = 0u;
-        __syncthreads();
-        for (uint32_t it = i0; it < i1; ++it) {
-            const uint32_t cnt = V->item_stats[(size_t)it * 8];
-            const uint2* __restrict__ st = V->stage + (size_t)it * chunk;
-            for (uint32_t e = tid; e < cnt; e += 256u) atomicAdd(&s_bins[st[e].y], 1u);
-        }
-        if (tid < 8) {
-            uint32_t sum = 0;
-            for (uint32_t it = i0; it < i1; ++it) sum += V->item_stats[(size_t)it * 8 + tid];
-            V->row_stats[(size_t)r * 8 + tid] = sum;
-        }
-        __syncthreads();
-        uint32_t* __restrict__ out = V->block_hist + (size_t)r * nb;
-        for (uint32_t b = tid; b < nb; b += 256u) out[b] = s_bins[b];
-        __syncthreads();
-    }
-}
-
-// Per view: stats, visible count, bucket totals over the rows, exclusive scan
-// into the args, and each row's starting offset per bucket.
-__global__ __launch_bounds__(1024) void k_finalize(const BatchHdr* __restrict__ hdr,
-                                                   const CullView* __restrict__ views) {
-    extern __shared__ uint32_t s_bins[];
-    __shared__ uint32_t s_tmp[1024];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t n_views = hdr->n_views;
-    for (uint32_t v = blockIdx.x; v < n_views; v += gridDim.x) {
+    const uint32_t n_units = ldc(&hdr->n_units);
+    const uint32_t* __restrict__ units = ldc(&hdr->units);
+    for (uint32_t u = blockIdx.x; u < n_units; u += gridDim.x) {
+        const uint32_t code = ldc(units + u);
+        const uint32_t v = code & 1023u, lg = (code >> 10) & 15u, b0 = code >> 14;
         const CullView* __restrict__ V = views + v;
-        const uint32_t nb = V->nb, rows = V->n_rows;
-        if (tid < 8) {
-            uint32_t sum = 0;
-            for (uint32_t r = 0; r < rows; ++r) sum += V->row_stats[(size_t)r * 8 + tid];
-            if (tid == 0) {
-                V->counters[kCtrVisible] = sum;  // survivors
-                V->counters[0] = V->count;       // stats[0]: instances tested
-            } else {
-                V->counters[tid] = sum;
+        const uint32_t R = ldc(&V->n_rows), nb = ldc(&V->nb);
+        const uint32_t col = tid & ((1u << lg) - 1u), grp = tid >> lg, RG = kColscanThreads >> lg;
+        const uint32_t b = b0 + col;
+        const bool live = b < nb;
+        const uint32_t rpg = (R + RG - 1u) / RG;
+        const uint32_t r0 = min(R, grp * rpg), r1 = min(R, r0 + rpg);
+        uint32_t* __restrict__ hist = ldc(&V->block_hist);
+        uint32_t x[8];
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+            x[k] = (live && r0 + k < r1) ? hist[(size_t)(r0 + k) * nb + b] : 0u;
+            sum += x[k];
+        }
+        for (uint32_t r = r0 + 8u; r < r1; r += 8u) {
+            uint32_t y[8];
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) y[k] = (live && r + k < r1) ? hist[(size_t)(r + k) * nb + b] : 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) sum += y[k];
+        }
+        // Inclusive scan over the row groups of each bucket column.
+        s_part[tid] = sum;
+        __syncthreads();
+        uint32_t inc = sum;
+        for (uint32_t off = 1; off < RG; off <<= 1) {
+            const uint32_t y = grp >= off ? s_part[tid - (off << lg)] : 0u;
+            __syncthreads();
+            inc += y;
+            s_part[tid] = inc;
+            __syncthreads();
+        }
+        if (live && grp == RG - 1u) V->totals[b] = inc;
+        uint32_t run = inc - sum;
+        if (r1 - r0 <= 8u) {
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k)
+                if (live && r0 + k < r1) {
+                    hist[(size_t)(r0 + k) * nb + b] = run;
+                    run += x[k];
+                }
+        } else {
+            for (uint32_t r = r0; r < r1; r += 8u) {
+                uint32_t y[8];
+#pragma unroll
+                for (uint32_t k = 0; k < 8; ++k) y[k] = (live && r + k < r1) ? hist[(size_t)(r + k) * nb + b] : 0u;
+#pragma unroll
+                for (uint32_t k = 0; k < 8; ++k)
+                    if (live && r + k < r1) {
+                        hist[(size_t)(r + k) * nb + b] = run;
+                        run += y[k];
+                    }
             }
         }
-        const uint32_t* __restrict__ hist = V->block_hist;
-        for (uint32_t b = tid; b < nb; b += 1024u) {
-            uint32_t c[kMaxRows];
-#pragma unroll
-            for (uint32_t r = 0; r < kMaxRows; ++r) c[r] = r < rows ? hist[(size_t)r * nb + b] : 0u;
-            uint32_t t = 0;
-#pragma unroll
-            for (uint32_t r = 0; r < kMaxRows; ++r) t += c[r];
-            s_bins[b] = t;
-        }
-        __syncthreads();
-        block_excl_scan<1024>(s_bins, nb, s_tmp);
-        for (uint32_t b = tid; b < nb; b += 1024u) {
-            uint32_t run = s_bins[b];
+        // An empty view has no row 0 in the scatter: its args are written here.
+        if (R == 0u && live && grp == 0u) {
             const hv_mesh_lod ml = V->mesh_lods[b];
             hv_draw_indexed_indirect a;
             a.index_count = ml.index_count;
+            a.instance_count = 0u;
             a.first_index = ml.first_index;
             a.base_vertex = ml.base_vertex;
-            a.first_instance = run;
-            uint32_t* cell = V->block_hist + b;
-            for (uint32_t r = 0; r < rows; ++r, cell += nb) {
-                const uint32_t t = *cell;
-                *cell = run;
-                run += t;
-            }
-            a.instance_count = run - a.first_instance;
+            a.first_instance = 0u;
             V->args[b] = a;
+        }
+        if (b0 == 0u) {  // stats of the view: 32 row groups x 8 words
+            __syncthreads();
+            const uint32_t word = tid & 7u, g = tid >> 3;
+            const uint32_t* __restrict__ rs = V->row_stats;
+            uint32_t acc = 0;
+            for (uint32_t r = g; r < R; r += 128u) {
+                uint32_t y[4];
+#pragma unroll
+                for (uint32_t k = 0; k < 4; ++k) y[k] = (r + 32u * k < R) ? rs[(size_t)(r + 32u * k) * 8 + word] : 0u;
+#pragma unroll
+                for (uint32_t k = 0; k < 4; ++k) acc += y[k];
+            }
+            s_part[tid] = acc;
+            __syncthreads();
+            if (tid < 8) {
+                uint32_t t = 0;
+                for (uint32_t q = 0; q < 32; ++q) t += s_part[q * 8 + tid];
+                if (tid == 0) {
+                    V->counters[kCtrVisible] = t;  // survivors
+                    V->counters[0] = V->count;     // stats[0]: instances tested
+                } else {
+                    V->counters[tid] = t;
+                }
+            }
         }
         __syncthreads();
     }
 }
 
-// Scatter: each row places its survivors at its bucket offsets.
-__global__ __launch_bounds__(256) void k_scatter(const BatchHdr* __restrict__ hdr,
-                                                 const CullView* __restrict__ views) {
+// Scatter: block b revisits the items it classified, row by row. NT threads
+// (the classify block size when that is 512, else 256: one scatter block per
+// classify block).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_scatter(const BatchHdr* __restrict__ hdr, const CullView* __restrict__ views) {
+    constexpr uint32_t KR = 1024u / NT;  // offsets held in registers: the first 1024 buckets
     extern __shared__ uint32_t s_cur[];
+    __shared__ uint32_t s_pre[NT];
+    __shared__ uint32_t s_w[NT / 64];
     const uint32_t tid = threadIdx.x;
-    const uint32_t n_views = hdr->n_views, n_rows = hdr->n_rows;
-    for (uint32_t row = blockIdx.x; row < n_rows; row += gridDim.x) {
-        const CullView* __restrict__ V = views + view_of_row(views, n_views, row);
-        const uint32_t r = row - V->row_begin;
-        const uint32_t nb = V->nb, chunk = V->chunk;
-        const uint32_t i0 = r * V->items_per_row, i1 = min(V->n_items, i0 + V->items_per_row);
-        const uint32_t* __restrict__ offs = V->block_hist + (size_t)r * nb;
-        for (uint32_t b = tid; b < nb; b += 256u) s_cur[b] = offs[b];
-        __syncthreads();
-        uint32_t* __restrict__ ids = V->ids;
-        for (uint32_t it = i0; it < i1; ++it) {
-            const uint32_t cnt = V->item_stats[(size_t)it * 8];
-            const uint2* __restrict__ st = V->stage + (size_t)it * chunk;
-            for (uint32_t e = tid; e < cnt; e += 256u) {
-                const uint2 en = st[e];
-                ids[atomicAdd(&s_cur[en.y], 1u)] = en.x;
-            }
+    const uint32_t G = ldc(&hdr->grid);
+    if (blockIdx.x >= G) return;
+    const uint32_t n_views = ldc(&hdr->n_views), chunk = ldc(&hdr->chunk), sub = ldc(&hdr->sub);
+    const uint32_t wl = ldc(&hdr->waves_log2);
+    const uint32_t* __restrict__ stage = ldc(&hdr->stage);
+    const uint32_t* __restrict__ sub_cnt = ldc(&hdr->sub_cnt);
+    uint32_t item, i_end;
+    block_items(blockIdx.x, G, ldc(&hdr->n_items), item, i_end);
+    uint32_t v = ldc(ldc(&hdr->block_view) + blockIdx.x);
+    while (item < i_end) {
+        v = advance_view(views, n_views, v, item);
+        const CullView* __restrict__ V = views + v;
+        const uint32_t nb = ldc(&V->nb), ibeg = ldc(&V->item_begin), vfirst = ldc(&V->first);
+        const uint32_t seg_end = min(i_end, ibeg + ldc(&V->n_items));
+        const uint32_t r = blockIdx.x - ldc(&V->block_begin);
+        const uint32_t* __restrict__ totals = ldc(&V->totals);
+        const uint32_t* __restrict__ offs = ldc(&V->block_hist) + (size_t)r * nb;
+        // Independent loads issued together: the first sub-chunk counts, the
+        // bucket totals and this row's offsets (the first 1024 buckets' in
+        // registers; larger tables reread their offsets below).
+        const size_t gs0 = (size_t)item << wl, gs1 = (size_t)seg_end << wl;
+        const uint32_t c0 = (gs0 + tid < gs1) ? sub_cnt[gs0 + tid] : 0u;
+        uint32_t offr[KR];
+#pragma unroll
+        for (uint32_t k = 0; k < KR; ++k) {
+            const uint32_t bb = tid + NT * k;
+            offr[k] = bb < nb ? offs[bb] : 0u;
+            if (bb < nb) s_cur[bb] = totals[bb];
         }
+        for (uint32_t bb = tid + NT * KR; bb < nb; bb += NT) s_cur[bb] = totals[bb];
         __syncthreads();
+        block_excl_scan_lds<NT>(s_cur, nb, s_w);
+        // Cursors: bucket base (scan of the totals) + this row's offsets; row 0
+        // writes the args.
+        const hv_mesh_lod* __restrict__ mesh_lods = ldc(&V->mesh_lods);
+        hv_draw_indexed_indirect* __restrict__ args = ldc(&V->args);
+        auto cursor = [&](uint32_t bb, uint32_t off) {
+            const uint32_t base = s_cur[bb];
+            if (r == 0u) {
+                const hv_mesh_lod ml = mesh_lods[bb];
+                hv_draw_indexed_indirect a;
+                a.index_count = ml.index_count;
+                a.instance_count = totals[bb];
+                a.first_index = ml.first_index;
+                a.base_vertex = ml.base_vertex;
+                a.first_instance = base;
+                args[bb] = a;
+            }
+            s_cur[bb] = base + off;
+        };
+#pragma unroll
+        for (uint32_t k = 0; k < KR; ++k)
+            if (tid + NT * k < nb) cursor(tid + NT * k, offr[k]);
+        for (uint32_t bb = tid + NT * KR; bb < nb; bb += NT) cursor(bb, offs[bb]);
+        __syncthreads();
+        uint32_t* __restrict__ ids = ldc(&V->ids);
+        for (size_t s0 = gs0; s0 < gs1; s0 += NT) {
+            const uint32_t ns = (uint32_t)min((size_t)NT, gs1 - s0);
+            const uint32_t c = s0 == gs0 ? c0 : (tid < ns ? sub_cnt[s0 + tid] : 0u);
+            uint32_t total;
+            s_pre[tid] = block_excl_scan<NT>(c, s_w, total);
+            __syncthreads();
+            for (uint32_t e0 = 0; e0 < total; e0 += 4u * NT) {
+                uint32_t ent[4], sidx[4];
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q) {
+                    const uint32_t e = e0 + q * NT + tid;
+                    sidx[q] = kInvalid;
+                    ent[q] = 0u;
+                    if (e < total) {
+                        uint32_t lo = 0, hi = ns;  // largest i with s_pre[i] <= e
+                        while (hi - lo > 1u) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (s_pre[mid] <= e) lo = mid;
+                            else hi = mid;
+                        }
+                        sidx[q] = lo;
+                        ent[q] = stage[(s0 + lo) * sub + (e - s_pre[lo])];
+                    }
+                }
+#pragma unroll
+                for (uint32_t q = 0; q < 4; ++q)
+                    if (sidx[q] != kInvalid) {
+                        const uint32_t it = (uint32_t)((s0 + sidx[q]) >> wl);
+                        const uint32_t id = vfirst + (it - ibeg) * chunk + (ent[q] & ((1u << kEntryShift) - 1u));
+                        ids[atomicAdd(&s_cur[ent[q] >> kEntryShift], 1u)] = id;
+                    }
+            }
+            __syncthreads();
+        }
+        item = seg_end;
     }
 }
 
@@ -466,55 +728,84 @@ CullBatch::~CullBatch() {
     for (auto& e : ev_)
         if (e) (void)hipEventDestroy(e);
     if (done_) (void)hipEventDestroy(done_);
-    d_table_.release();
-    d_stage_.release();
-    d_ids_.release();
-    d_args_.release();
-    d_counters_.release();
-    d_item_stats_.release();
-    d_block_hist_.release();
+    for (DevBuf* b : {&d_table_, &d_stage_, &d_ids_, &d_args_, &d_counters_, &d_sub_cnt_, &d_row_stats_, &d_block_hist_,
+                      &d_totals_})
+        b->release();
 }
 
 uint64_t CullBatch::device_bytes() const {
-    return d_table_.bytes + d_stage_.bytes + d_ids_.bytes + d_args_.bytes + d_counters_.bytes +
-           d_item_stats_.bytes + d_block_hist_.bytes;
+    uint64_t t = 0;
+    for (const DevBuf* b : {&d_table_, &d_stage_, &d_ids_, &d_args_, &d_counters_, &d_sub_cnt_, &d_row_stats_,
+                            &d_block_hist_, &d_totals_})
+        t += b->bytes;
+    return t;
 }
 
-uint64_t CullBatch::bytes_for(uint64_t candidates, uint32_t views, uint32_t buckets, uint32_t chunk) {
+uint32_t CullBatch::max_grid(const hv_ctx* ctx) { return std::max(1u, ctx->cu_count) * 32u; }
+
+namespace {
+
+// Upper bound of column-scan units for `views` views of at most `buckets`
+// buckets with `rows` rows in total (see unit_lg: a slice is at least
+// min(256, 1024 / R) buckets wide).
+uint64_t unit_bound(uint64_t views, uint64_t buckets, uint64_t rows) {
+    return views * (buckets / 128 + 2) + (buckets * rows) / 1024 + 1;
+}
+
+// log2 of the bucket slice of a column-scan unit for a view of R rows: the
+// widest power of two (<= 256) that leaves each of the 256 / slice row groups
+// at most 8 rows.
+uint32_t unit_lg(uint32_t R) {
+    if (R == 0) return 8;
+    uint32_t lg = 8;
+    while (lg > 0 && (R + (kColscanThreads >> lg) - 1) / (kColscanThreads >> lg) > 8) --lg;
+    return lg;
+}
+
+}  // namespace
+
+uint64_t CullBatch::bytes_for(uint64_t candidates, uint32_t views, uint32_t buckets, uint32_t chunk,
+                              uint64_t max_rows) {
     const uint64_t items = candidates / chunk + views;
-    return (items * chunk) * 8 + candidates * 4 + items * 32 +
-           (uint64_t)views * (buckets * 20ull + kCounterStride * 4 + sizeof(CullView) + kMaxRows * (buckets + 8ull) * 4);
+    const uint64_t rows = std::min(items, max_rows + views);
+    return items * chunk * 4 + items * 8 * 4 + candidates * 4 + rows * (buckets + 8ull) * 4 +
+           (uint64_t)views * (buckets * 24ull + kCounterStride * 4 + sizeof(CullView)) +
+           unit_bound(views, buckets, rows) * 4 + max_rows * 4;
 }
 
 hv_status CullBatch::reserve(uint32_t views, uint64_t candidates, uint32_t buckets) {
     HV_HIP(hipSetDevice(ctx_->device));
     if (done_recorded_) HV_HIP(hipEventSynchronize(done_));
     const hipStream_t st = ctx_->stream;
-    const uint32_t chunk = shape.chunk();
+    const uint32_t chunk = shape.chunk(), nw = shape.wg / 64;
     const uint64_t items = candidates / chunk + views + 1;
-    HV_TRY(d_stage_.reserve(items * chunk * 8, false, st));
+    const uint64_t rows = std::min<uint64_t>(items, (uint64_t)max_grid(ctx_) + views);
+    const uint64_t nv = std::max(views, 1u);
+    HV_TRY(d_stage_.reserve(items * chunk * 4, false, st));
+    HV_TRY(d_sub_cnt_.reserve(items * nw * 4, false, st));
     HV_TRY(d_ids_.reserve(std::max<uint64_t>(candidates, 1) * 4, false, st));
-    HV_TRY(d_args_.reserve(std::max<uint64_t>((uint64_t)views * buckets, 1) * sizeof(hv_draw_indexed_indirect), false,
-                           st));
-    HV_TRY(d_item_stats_.reserve(items * 32, false, st));
-    HV_TRY(d_row_stats_.reserve((uint64_t)std::max(views, 1u) * kMaxRows * 32, false, st));
-    HV_TRY(d_block_hist_.reserve((uint64_t)std::max(views, 1u) * kMaxRows * buckets * 4, false, st));
-    HV_TRY(d_counters_.reserve((uint64_t)std::max(views, 1u) * kCounterStride * 4, false, st));
-    HV_TRY(d_table_.reserve(sizeof(BatchHdr) + (uint64_t)std::max(views, 1u) * sizeof(CullView), false, st));
+    HV_TRY(d_args_.reserve(nv * buckets * sizeof(hv_draw_indexed_indirect), false, st));
+    HV_TRY(d_totals_.reserve(nv * buckets * 4, false, st));
+    HV_TRY(d_row_stats_.reserve(rows * 32, false, st));
+    HV_TRY(d_block_hist_.reserve(std::max<uint64_t>(rows * buckets, 1) * 4, false, st));
+    HV_TRY(d_counters_.reserve(nv * kCounterStride * 4, false, st));
+    HV_TRY(d_table_.reserve(sizeof(BatchHdr) + nv * sizeof(CullView) + unit_bound(nv, buckets, rows) * 4 +
+                                (uint64_t)max_grid(ctx_) * 4,
+                            false, st));
     lds_cap_ = std::max(lds_cap_, buckets * 4u);
     return HV_OK;
 }
 
 hv_status CullBatch::ensure_occupancy() {
-    const uint32_t key = shape.wg * 16u + shape.ipt + ((cfg_.flags & HV_CULL_OCCLUSION) ? 1024u : 0u);
-    if (key == occ_key_ && resident1_) return HV_OK;
+    const uint32_t key[3] = {shape.wg * 16u + shape.ipt, (cfg_.flags & HV_CULL_OCCLUSION) ? 1u : 0u, lds_cap_};
+    if (resident1_ && std::memcmp(key, occ_key_, sizeof key) == 0) return HV_OK;
     const bool occl = (cfg_.flags & HV_CULL_OCCLUSION) != 0;
     int occ = 0;
     // The AABB variant uses at least as many registers; size the grid for it.
     HV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(classify_fn(shape, occl, true)),
-                                                        (int)shape.wg, 0));
+                                                        (int)shape.wg, (size_t)lds_cap_));
     resident1_ = std::max(1u, ctx_->cu_count * (uint32_t)std::max(occ, 1));
-    occ_key_ = key;
+    std::memcpy(occ_key_, key, sizeof key);
     return HV_OK;
 }
 
@@ -525,14 +816,14 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     }
     if (n > max_views_ || n > kMaxViews)
         return set_error(HV_ELIMIT, "prepare: %u views exceed the batch capacity %u", n, std::min(max_views_, kMaxViews));
-    if (!classify_fn(shape, false, false))
+    if (!classify_fn(shape, false, false) || shape.chunk() > (1u << kEntryShift))
         return set_error(HV_EINVAL, "prepare: unsupported launch shape wg=%u ipt=%u", shape.wg, shape.ipt);
     HV_HIP(hipSetDevice(ctx_->device));
     // The previous launch may still read the table and buffers rewritten below.
     if (done_recorded_) HV_HIP(hipEventSynchronize(done_));
     const bool occl = (cfg_.flags & HV_CULL_OCCLUSION) != 0;
-    const uint32_t chunk = shape.chunk();
-    uint64_t stage_total = 0, args_total = 0, items_total = 0, hist_total = 0;
+    const uint32_t chunk = shape.chunk(), nw = shape.wg / 64;
+    uint64_t ids_total = 0, args_total = 0, items_total = 0;
     uint32_t max_nb = 1;
     for (uint32_t v = 0; v < n; ++v) {
         const ViewDesc& d = views[v];
@@ -557,55 +848,78 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
         }
         const uint32_t nb = d.scene->mesh_count * d.scene->lod_count;
         max_nb = std::max(max_nb, nb);
-        stage_total += d.count;
+        ids_total += d.count;
         args_total += nb;
         items_total += (d.count + chunk - 1) / chunk;
-        hist_total += (uint64_t)std::min((d.count + chunk - 1) / chunk, kMaxRows) * nb;
     }
-    if (items_total > 0xffffffffull) return set_error(HV_ELIMIT, "prepare: too many work items");
-    const hipStream_t st = ctx_->stream;
-    uint64_t rows_total = 0;
+    if (items_total >= 0xffffffffull / nw) return set_error(HV_ELIMIT, "prepare: too many work items");
+    const uint32_t N = (uint32_t)items_total;
+    // Launch capacity first: the item -> block mapping depends on the grid.
+    lds_cap_ = std::max(lds_cap_, max_nb * 4u);
+    HV_TRY(ensure_occupancy());
+    const uint32_t G = N ? std::min(resident1_, N) : 0u;
+    auto block_of = [&](uint32_t i) { return (uint32_t)(((uint64_t)(i + 1) * G - 1) / N); };
+
+    host_views_.assign(n, CullView{});
+    host_units_.clear();
+    uint64_t rows_total = 0, hist_total = 0;
+    uint32_t item = 0;
     for (uint32_t v = 0; v < n; ++v) {
-        const uint32_t items = (views[v].count + chunk - 1) / chunk;
-        rows_total += std::min(items, kMaxRows);
+        CullView& cv = host_views_[v];
+        const uint32_t nb = views[v].scene->mesh_count * views[v].scene->lod_count;
+        cv.n_items = (views[v].count + chunk - 1) / chunk;
+        cv.item_begin = item;
+        if (cv.n_items) {
+            cv.block_begin = block_of(item);
+            cv.n_rows = block_of(item + cv.n_items - 1) - cv.block_begin + 1;
+        }
+        item += cv.n_items;
+        rows_total += cv.n_rows;
+        hist_total += (uint64_t)cv.n_rows * nb;
+        const uint32_t lg = unit_lg(cv.n_rows);
+        for (uint32_t b0 = 0; b0 < nb; b0 += (1u << lg)) host_units_.push_back(v | (lg << 10) | (b0 << 14));
     }
-    HV_TRY(d_stage_.reserve(std::max<uint64_t>(items_total, 1) * chunk * 8, false, st));
-    HV_TRY(d_ids_.reserve(std::max<uint64_t>(stage_total, 1) * 4, false, st));
+    const hipStream_t st = ctx_->stream;
+    HV_TRY(d_stage_.reserve(std::max<uint64_t>(items_total, 1) * chunk * 4, false, st));
+    HV_TRY(d_sub_cnt_.reserve(std::max<uint64_t>(items_total, 1) * nw * 4, false, st));
+    HV_TRY(d_ids_.reserve(std::max<uint64_t>(ids_total, 1) * 4, false, st));
     HV_TRY(d_args_.reserve(std::max<uint64_t>(args_total, 1) * sizeof(hv_draw_indexed_indirect), false, st));
-    HV_TRY(d_item_stats_.reserve(std::max<uint64_t>(items_total, 1) * 32, false, st));
+    HV_TRY(d_totals_.reserve(std::max<uint64_t>(args_total, 1) * 4, false, st));
     HV_TRY(d_row_stats_.reserve(std::max<uint64_t>(rows_total, 1) * 32, false, st));
     HV_TRY(d_block_hist_.reserve(std::max<uint64_t>(hist_total, 1) * 4, false, st));
     HV_TRY(d_counters_.reserve((uint64_t)std::max(n, 1u) * kCounterStride * 4, false, st));
-    const uint64_t table_bytes = sizeof(BatchHdr) + (uint64_t)std::max(n, 1u) * sizeof(CullView);
+    // Block b's first view (the view holding item b*N/G).
+    host_block_view_.assign(std::max(G, 1u), 0u);
+    for (uint32_t b = 0, v = 0; b < G; ++b) {
+        const uint32_t it = (uint32_t)(((uint64_t)b * N) / G);
+        while (v + 1 < n && it >= host_views_[v].item_begin + host_views_[v].n_items) ++v;
+        host_block_view_[b] = v;
+    }
+    const uint64_t units_off = sizeof(BatchHdr) + (uint64_t)std::max(n, 1u) * sizeof(CullView);
+    const uint64_t bview_off = units_off + std::max<uint64_t>(host_units_.size(), 1) * 4;
+    const uint64_t table_bytes = bview_off + (uint64_t)host_block_view_.size() * 4;
     HV_TRY(d_table_.reserve(table_bytes, false, st));
 
-    host_views_.assign(n, CullView{});
-    uint64_t so = 0, ao = 0, io = 0, ho = 0, ro = 0;
-    uint32_t item = 0, row = 0;
+    uint64_t so = 0, ao = 0, ho = 0, ro = 0;
     for (uint32_t v = 0; v < n; ++v) {
         const ViewDesc& d = views[v];
         CullView& cv = host_views_[v];
-        std::memset(&cv, 0, sizeof cv);
         HV_TRY(hv_extract_frustum_planes(d.camera->view_proj, cv.planes));
+        for (int i = 0; i < 6; ++i)
+            for (int a = 0; a < 3; ++a) cv.pv_sign[3 * i + a] = cv.planes[4 * i + a] >= 0.0f ? 1.0f : -1.0f;
         std::memcpy(cv.vp, d.camera->view_proj, 64);
         cv.p00 = d.camera->proj[0];
         cv.p11 = d.camera->proj[5];
         std::memcpy(cv.cam, d.camera->position_near, 12);
-        for (int i = 0; i < 7; ++i) cv.lod_d2[i] = sqrt_threshold(cfg_.lod_distances[i]);
+        // Thresholds past lod_count - 1 are NaN: d2 >= NaN is false, so the
+        // kernel may compare against all of them.
+        for (uint32_t i = 0; i < 7; ++i)
+            cv.lod_d2[i] = i + 1 < cfg_.lod_count ? sqrt_threshold(cfg_.lod_distances[i]) : std::nanf("");
         cv.lod_count = cfg_.lod_count;
         cv.flags = cfg_.flags;
         cv.nb = d.scene->mesh_count * d.scene->lod_count;
         cv.first = d.first;
         cv.count = d.count;
-        cv.item_begin = item;
-        cv.n_items = (d.count + chunk - 1) / chunk;
-        cv.chunk = chunk;
-        cv.n_rows = std::min(cv.n_items, kMaxRows);
-        cv.items_per_row = cv.n_rows ? (cv.n_items + cv.n_rows - 1) / cv.n_rows : 0;
-        if (cv.n_rows) cv.n_rows = (cv.n_items + cv.items_per_row - 1) / cv.items_per_row;
-        cv.row_begin = row;
-        item += cv.n_items;
-        row += cv.n_rows;
         cv.screen_h = cfg_.screen_height;
         cv.hiz_mip_count = hv_mip_levels(cfg_.screen_width, cfg_.screen_height);
         cv.bounds = d.scene->bounds.as<float4>();
@@ -620,36 +934,45 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
             cv.hiz0 = d.hiz->level0;
             cv.hiz_packed = d.hiz->packed.as<float>();
         }
-        cv.stage = d_stage_.as<uint2>() + io * chunk;
         cv.ids = d_ids_.as<uint32_t>() + so;
         cv.args = d_args_.as<hv_draw_indexed_indirect>() + ao;
+        cv.totals = d_totals_.as<uint32_t>() + ao;
         cv.counters = d_counters_.as<uint32_t>() + (uint64_t)v * kCounterStride;
-        cv.item_stats = d_item_stats_.as<uint32_t>() + io * 8;
         cv.row_stats = d_row_stats_.as<uint32_t>() + ro * 8;
         cv.block_hist = d_block_hist_.as<uint32_t>() + ho;
         so += d.count;
         ao += cv.nb;
-        io += cv.n_items;
         ro += cv.n_rows;
         ho += (uint64_t)cv.n_rows * cv.nb;
     }
     n_views_ = n;
-    n_items_ = item;
-    n_rows_ = row;
+    n_items_ = N;
+    n_rows_ = (uint32_t)rows_total;
+    n_units_ = (uint32_t)host_units_.size();
+    grid_ = G;
     descs_.assign(views, views + n);
     max_nb_ = max_nb;
     // Upload the table (ordered before any later launch on the context stream).
     std::vector<uint8_t> buf(table_bytes, 0);
-    BatchHdr hdr{n, item, row, 0};
+    BatchHdr hdr{};
+    hdr.n_views = n;
+    hdr.n_items = N;
+    hdr.n_rows = n_rows_;
+    hdr.n_units = n_units_;
+    hdr.grid = G;
+    hdr.chunk = chunk;
+    hdr.sub = shape.ipt * 64u;
+    hdr.waves_log2 = nw == 1 ? 0u : nw == 2 ? 1u : nw == 4 ? 2u : 3u;
+    hdr.stage = d_stage_.as<uint32_t>();
+    hdr.sub_cnt = d_sub_cnt_.as<uint32_t>();
+    hdr.units = reinterpret_cast<const uint32_t*>(d_table_.as<uint8_t>() + units_off);
+    hdr.block_view = reinterpret_cast<const uint32_t*>(d_table_.as<uint8_t>() + bview_off);
     std::memcpy(buf.data(), &hdr, sizeof hdr);
     if (n) std::memcpy(buf.data() + sizeof hdr, host_views_.data(), (size_t)n * sizeof(CullView));
+    if (n_units_) std::memcpy(buf.data() + units_off, host_units_.data(), (size_t)n_units_ * 4);
+    std::memcpy(buf.data() + bview_off, host_block_view_.data(), host_block_view_.size() * 4);
     HV_HIP(hipMemcpyAsync(d_table_.ptr, buf.data(), table_bytes, hipMemcpyHostToDevice, st));
     HV_HIP(hipStreamSynchronize(st));
-    // Persistent grids.
-    HV_TRY(ensure_occupancy());
-    grid1_ = std::max(1u, std::min(resident1_, std::max(n_items_, 1u)));
-    grid23_ = std::max(1u, std::min(n_rows_, 2u * ctx_->cu_count));
-    lds_cap_ = std::max(lds_cap_, max_nb_ * 4u);
     return HV_OK;
 }
 
@@ -659,21 +982,23 @@ hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed) {
     ClassifyFn fn = classify_fn(shape, occl, aabb_any);
     const BatchHdr* hdr = d_table_.as<BatchHdr>();
     const CullView* views = reinterpret_cast<const CullView*>(d_table_.as<uint8_t>() + sizeof(BatchHdr));
-    // Fixed (capacity) grids make one captured graph valid for any later table.
+    // Fixed (capacity) grids make one captured graph valid for any later table:
+    // blocks beyond the table's G (or past the last unit) exit at once.
     const size_t lds = fixed ? (size_t)lds_cap_ : (size_t)max_nb_ * 4;
-    const uint32_t g1 = fixed ? resident1_ : grid1_;
-    const uint32_t g23 = fixed ? 2u * ctx_->cu_count : grid23_;
-    const uint32_t gf = fixed ? 2u * ctx_->cu_count : std::max(1u, std::min(n_views_, 2u * ctx_->cu_count));
+    const uint32_t colscan_cap = 8u * ctx_->cu_count;
+    const uint32_t g1 = fixed ? resident1_ : std::max(1u, grid_);
+    const uint32_t g2 = fixed ? colscan_cap : std::max(1u, std::min(n_units_, colscan_cap));
     if (ms) HV_HIP(hipEventRecord(ev_[0], s));
-    hipLaunchKernelGGL(fn, dim3(g1), dim3(shape.wg), 0, s, hdr, views);
+    hipLaunchKernelGGL(fn, dim3(g1), dim3(shape.wg), lds, s, hdr, views);
     HV_HIP(hipGetLastError());
     if (ms) HV_HIP(hipEventRecord(ev_[1], s));
-    hipLaunchKernelGGL(dev::k_rows, dim3(g23), dim3(256), lds, s, hdr, views);
-    HV_HIP(hipGetLastError());
-    hipLaunchKernelGGL(dev::k_finalize, dim3(gf), dim3(1024), lds, s, hdr, views);
+    hipLaunchKernelGGL(dev::k_colscan, dim3(g2), dim3(kColscanThreads), 0, s, hdr, views);
     HV_HIP(hipGetLastError());
     if (ms) HV_HIP(hipEventRecord(ev_[2], s));
-    hipLaunchKernelGGL(dev::k_scatter, dim3(g23), dim3(256), lds, s, hdr, views);
+    if (shape.wg >= 512)
+        hipLaunchKernelGGL(dev::k_scatter<512>, dim3(g1), dim3(512), lds, s, hdr, views);
+    else
+        hipLaunchKernelGGL(dev::k_scatter<256>, dim3(g1), dim3(256), lds, s, hdr, views);
     HV_HIP(hipGetLastError());
     HV_HIP(hipEventRecord(done_, s));
     done_recorded_ = true;
@@ -699,7 +1024,7 @@ hv_status CullBatch::execute_graph(hipStream_t s) {
     // The classify variant (occlusion, explicit AABBs) and the LDS size are
     // baked into the graph; grids are the capacity grids.
     const bool aabb_any = std::any_of(host_views_.begin(), host_views_.end(), [](const CullView& c) { return c.aabb; });
-    const uint32_t key[4] = {resident1_, lds_cap_, aabb_any ? 1u : 0u, shape.wg * 16u + shape.ipt};
+    const uint32_t key[5] = {resident1_, lds_cap_, aabb_any ? 1u : 0u, shape.wg * 16u + shape.ipt, ctx_->cu_count};
     if (!graph_exec_ || std::memcmp(key, graph_key_, sizeof key) != 0) {
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
         if (graph_) (void)hipGraphDestroy(graph_);

@@ -53,7 +53,12 @@ __device__ __forceinline__ float hiz_tap(const Pyramid& p, uint32_t m, float u, 
     const float fx = floorf(u * (float)wm), fy = floorf(v * (float)hm);
     const int x = (fx >= 0.0f) ? (fx < (float)wm ? (int)fx : (int)wm - 1) : 0;
     const int y = (fy >= 0.0f) ? (fy < (float)hm ? (int)fy : (int)hm - 1) : 0;
-    const float* base = (m == 0u) ? p.level0 : p.packed + p.off[m];
+    // Select the level offset with a compare chain: a dynamically indexed
+    // array would be placed in scratch memory.
+    uint32_t o = 0u;
+#pragma unroll
+    for (uint32_t i = 1; i < 12; ++i) o = (m == i) ? p.off[i] : o;
+    const float* base = (m == 0u) ? p.level0 : p.packed + o;
     return base[(size_t)y * wm + (uint32_t)x];
 }
 

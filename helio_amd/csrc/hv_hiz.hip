@@ -41,9 +41,20 @@ __device__ __forceinline__ float reduce4(float s00, float s10, float s01, float 
 
 __device__ __forceinline__ uint32_t lvl(uint32_t d, uint32_t k) { return max(d >> k, 1u); }
 
-__device__ __forceinline__ void put(const HizArgs& a, uint32_t k, uint32_t x, uint32_t y, float v) {
-    const uint32_t dw = lvl(a.w, k), dh = lvl(a.h, k);
-    if (x < dw && y < dh) a.packed[a.off[k] + (size_t)y * dw + x] = v;
+// Offset of level m in `packed`, by a compare chain (a dynamically indexed
+// kernel-argument array would be copied to scratch memory).
+__device__ __forceinline__ uint32_t level_off(const HizArgs& a, uint32_t m) {
+    uint32_t o = 0u;
+#pragma unroll
+    for (uint32_t i = 1; i < 16; ++i) o = (m == i) ? a.off[i] : o;
+    return o;
+}
+
+// Store texel (x, y) of level K (a compile-time level: its offset is a scalar).
+template <uint32_t K>
+__device__ __forceinline__ void put(const HizArgs& a, uint32_t x, uint32_t y, float v) {
+    const uint32_t dw = lvl(a.w, K), dh = lvl(a.h, K);
+    if (x < dw && y < dh) a.packed[a.off[K] + (size_t)y * dw + x] = v;
 }
 
 // Reduce a whole level held in LDS (sw x sh texels) into the next one, with
@@ -54,6 +65,29 @@ __device__ __forceinline__ float lds_reduce(const float* src, uint32_t pitch, ui
     const uint32_t x0 = 2 * x, y0 = 2 * y;
     const uint32_t x1 = min(x0 + 1, sw - 1), y1 = min(y0 + 1, sh - 1);
     return reduce4<MIN>(src[y0 * pitch + x0], src[y0 * pitch + x1], src[y1 * pitch + x0], src[y1 * pitch + x1]);
+}
+
+// Tile levels K .. TL-1 -> K+1 .. TL through LDS (level K is in `src`, EDGE
+// texels per axis). Unrolled at compile time so every level offset is a scalar.
+template <uint32_t K, uint32_t TL, bool MIN, uint32_t EDGE = 16>
+__device__ __forceinline__ void tile_lds_levels(const HizArgs& a, float* src, float* dst, uint32_t tx, uint32_t ty,
+                                                uint32_t top) {
+    if constexpr (K < TL) {
+        constexpr uint32_t ne = EDGE / 2;
+        const uint32_t tid = threadIdx.x;
+        if (tid < ne * ne) {
+            const uint32_t x = tid % ne, y = tid / ne;
+            const uint32_t sw = lvl(a.w, K), sh = lvl(a.h, K);
+            // In-tile clamp: a level one texel wide stays on its texel.
+            const uint32_t x1 = (sw == 1u) ? 2 * x : 2 * x + 1, y1 = (sh == 1u) ? 2 * y : 2 * y + 1;
+            const float v = reduce4<MIN>(src[2 * y * EDGE + 2 * x], src[2 * y * EDGE + x1], src[y1 * EDGE + 2 * x],
+                                         src[y1 * EDGE + x1]);
+            if (K + 1 <= top) put<K + 1>(a, tx * ne + x, ty * ne + y, v);
+            dst[y * ne + x] = v;
+        }
+        __syncthreads();
+        tile_lds_levels<K + 1, TL, MIN, ne>(a, dst, src, tx, ty, top);
+    }
 }
 
 template <int TILE, bool MIN, bool VEC4>
@@ -94,44 +128,26 @@ __global__ __launch_bounds__(256) void k_hiz(HizArgs a) {
         for (uint32_t c = 0; c < P1; ++c) {
             const uint32_t cx1 = (sw0 == 1u) ? 2 * c : 2 * c + 1, ry1 = (sh0 == 1u) ? 2 * r : 2 * r + 1;
             v1[r][c] = reduce4<MIN>(v0[2 * r][2 * c], v0[2 * r][cx1], v0[ry1][2 * c], v0[ry1][cx1]);
-            if (top >= 1u) put(a, 1, tx * (TILE / 2) + lx * P1 + c, ty * (TILE / 2) + ly * P1 + r, v1[r][c]);
+            if (top >= 1u) put<1>(a, tx * (TILE / 2) + lx * P1 + c, ty * (TILE / 2) + ly * P1 + r, v1[r][c]);
         }
     // ---- level 2 in registers (TILE 64) -> LDS 16x16 ----
-    uint32_t k;  // level held in s_a
+    constexpr uint32_t K0 = (PER == 4) ? 2 : 1;  // level held in s_a
     if (PER == 4) {
         const uint32_t sw1 = lvl(a.w, 1), sh1 = lvl(a.h, 1);
-        const uint32_t c1 = (sw1 == 1u) ? 0 : 1, r1 = (sh1 == 1u) ? 0 : 1;
-        const float v2 = reduce4<MIN>(v1[0][0], v1[0][c1 % P1], v1[r1 % P1][0], v1[r1 % P1][c1 % P1]);
-        if (top >= 2u) put(a, 2, tx * 16 + lx, ty * 16 + ly, v2);
+        // Clamped neighbours by selects (a runtime index into v1 would put it in scratch).
+        const bool c1 = sw1 != 1u, r1 = sh1 != 1u;
+        const float t01 = c1 ? v1[0][P1 - 1] : v1[0][0];
+        const float t10 = r1 ? v1[P1 - 1][0] : v1[0][0];
+        const float t11 = r1 ? (c1 ? v1[P1 - 1][P1 - 1] : v1[P1 - 1][0]) : t01;
+        const float v2 = reduce4<MIN>(v1[0][0], t01, t10, t11);
+        if (top >= 2u) put<2>(a, tx * 16 + lx, ty * 16 + ly, v2);
         s_a[ly * 16 + lx] = v2;
-        k = 2;
     } else {
         s_a[ly * 16 + lx] = v1[0][0];
-        k = 1;
     }
     __syncthreads();
     // ---- remaining tile levels through LDS: 16 -> 8 -> 4 -> 2 -> 1 ----
-    float* src = s_a;
-    float* dst = s_b;
-    uint32_t edge = 16;  // texels per axis of the tile at level k
-    for (; k < TL; ++k) {
-        const uint32_t ne = edge / 2;
-        if (tid < ne * ne) {
-            const uint32_t x = tid % ne, y = tid / ne;
-            const uint32_t sw = lvl(a.w, k), sh = lvl(a.h, k);
-            // In-tile clamp: a level one texel wide stays on its texel.
-            const uint32_t x1 = (sw == 1u) ? 2 * x : 2 * x + 1, y1 = (sh == 1u) ? 2 * y : 2 * y + 1;
-            const float v = reduce4<MIN>(src[2 * y * edge + 2 * x], src[2 * y * edge + x1], src[y1 * edge + 2 * x],
-                                         src[y1 * edge + x1]);
-            if (k + 1 <= top) put(a, k + 1, tx * ne + x, ty * ne + y, v);
-            dst[y * ne + x] = v;
-        }
-        __syncthreads();
-        float* t = src;
-        src = dst;
-        dst = t;
-        edge = ne;
-    }
+    tile_lds_levels<K0, TL, MIN>(a, s_a, s_b, tx, ty, top);
 }
 
 // Levels TL+1 .. top from level TL (<= 64 x 64 texels), one workgroup.
@@ -142,7 +158,7 @@ __global__ __launch_bounds__(256) void k_hiz_tail(HizArgs a) {
     const uint32_t tid = threadIdx.x;
     const uint32_t top = a.levels - 1u, TL = a.first_tail - 1u;
     uint32_t sw = lvl(a.w, TL), sh = lvl(a.h, TL);
-    const float* lv = a.packed + a.off[TL];
+    const float* lv = a.packed + level_off(a, TL);
     for (uint32_t i = tid; i < sw * sh; i += 256u) s_tail_a[i] = lv[i];
     __syncthreads();
     // Ping-pong: level TL in the 64x64 buffer, TL+1 (<= 32x32) in the other, ...
@@ -154,7 +170,7 @@ __global__ __launch_bounds__(256) void k_hiz_tail(HizArgs a) {
         for (uint32_t i = tid; i < dw * dh; i += 256u) {
             const uint32_t x = i % dw, y = i / dw;
             const float v = lds_reduce<MIN>(ts, sw, x, y, sw, sh);
-            a.packed[a.off[kk + 1] + i] = v;
+            a.packed[level_off(a, kk + 1) + i] = v;
             td[i] = v;
         }
         __syncthreads();

@@ -74,8 +74,8 @@ struct DevBuf {
 // Launch shape of the instance-cull kernels (chosen by the autotuner).
 // ---------------------------------------------------------------------------
 struct CullShape {
-    uint32_t wg = 256;   // threads per workgroup: 64, 128, 256, 512
-    uint32_t ipt = 4;    // instances per thread per work item: 1, 2, 4, 8
+    uint32_t wg = 512;   // threads per workgroup: 64, 128, 256, 512
+    uint32_t ipt = 2;    // instances per thread per work item: 1, 2, 4, 8
     uint32_t chunk() const { return wg * ipt; }
 };
 

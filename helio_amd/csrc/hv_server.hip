@@ -302,7 +302,7 @@ hv_status hv_server_submit(hv_server* s, const hv_server_request* r, uint64_t* t
     q->cam = *v.camera;
     q->r.view.camera = &q->cam;
     q->nb = nb;
-    q->bytes = CullBatch::bytes_for(v.count, 1, nb, 2048);
+    q->bytes = CullBatch::bytes_for(v.count, 1, nb, 2048, CullBatch::max_grid(s->ctx));
     std::lock_guard<std::mutex> lk(s->mu);
     if (v.count > s->cfg.max_batch_candidates || q->bytes > s->cfg.device_budget_bytes) {
         ++s->n_rejected;

@@ -391,7 +391,7 @@ hv_status hv_batch_prepare(hv_batch* batch, const hv_view_request* views, uint32
 /* use_graph != 0: replay the captured graph (captured on first use). */
 hv_status hv_batch_execute(hv_batch* batch, void* stream, uint32_t use_graph);
 /* Eager execute with HIP events around the three kernels; synchronises and
- * returns ms[0] classify, ms[1] bucket count + scan, ms[2] scatter. */
+ * returns ms[0] classify, ms[1] column scan (bucket offsets), ms[2] scatter. */
 hv_status hv_batch_execute_timed(hv_batch* batch, void* stream, float ms[3]);
 hv_status hv_batch_outputs(const hv_batch* batch, uint32_t view, hv_cull_outputs* out);
 hv_status hv_batch_read(hv_batch* batch, uint32_t view, uint32_t* ids, uint32_t ids_capacity, uint32_t* id_count,
@@ -573,13 +573,15 @@ typedef struct hv_tune_result {
     uint32_t wg;          /* classify workgroup size */
     uint32_t ipt;         /* instances per thread per work item */
     float classify_ms;    /* median classify-kernel time on the tuned batch */
+    float total_ms;       /* median of classify + column scan + scatter (what the tuner minimises) */
 } hv_tune_result;
 
 hv_status hv_ctx_set_shapes(hv_ctx* ctx, uint32_t wg, uint32_t ipt, uint32_t hiz_tile);
 hv_status hv_ctx_get_shapes(hv_ctx* ctx, uint32_t* wg, uint32_t* ipt, uint32_t* hiz_tile);
 /* Times every supported (wg, ipt) on the views of a prepared batch (`reps`
- * timed runs each, median), leaves the batch prepared with the fastest and
- * makes it the context default. results may be NULL. */
+ * timed runs each, median of the three kernels' sum), leaves the batch
+ * prepared with the fastest and makes it the context default. results may be
+ * NULL. */
 hv_status hv_autotune_batch(hv_batch* batch, uint32_t reps, hv_tune_result* results, uint32_t capacity,
                             uint32_t* count, hv_tune_result* best);
 /* Times the pyramid build with 32x32 and 64x64 tiles (ms[0], ms[1]) and makes
