@@ -14,7 +14,9 @@ visible ids and DrawIndexedIndirect args.
   plain          the same 64 views as a per-view launch loop (prepared once,
                  3 eager launches per view), views/s and p50/p99.
   roofline       the classify kernel: algorithmic bytes (20 B per test) /
-                 its average duration from HIP events on its stream.
+                 its average duration from HIP events on its stream; traffic
+                 = its measured HBM bytes per launch when the counter summary
+                 profiles/traffic_k_classify_multiview.json is present.
   cpu_baseline   the CPU oracle (the reference WGSL restated in C++, the
                  wgpu/lavapipe baseline being unavailable) on the host cores.
 
@@ -38,6 +40,20 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 N_INSTANCES = 8_388_608
 BYTES_PER_TEST = 20  # float4 bounds + u32 meta
+CALIBRATION_COPY_BYTES = 256 << 20  # device-to-device copy used to scale the HBM counters
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_k_classify_multiview.json"
+
+
+def measured_traffic(tests_per_step: int):
+    """HBM bytes per classify launch from the committed counter summary, if it
+    was taken on this workload (FETCH_SIZE + WRITE_SIZE scaled by a copy)."""
+    try:
+        t = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    if int(t.get("tests_per_step", -1)) != tests_per_step:
+        return None
+    return t.get("traffic_bytes_per_launch")
 
 
 def log(msg: str) -> None:
@@ -256,7 +272,11 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip the occlusion and VG legs")
     ap.add_argument("--serving-rates", default="2000,10000,50000")
     ap.add_argument("--serving-seconds", type=float, default=0.5)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched leg and one calibration copy (for counter runs)")
     args = ap.parse_args()
+    if args.batched_only:
+        args.no_serving = args.no_extra = args.no_cpu_baseline = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the visibility path is "
               "single-device; see the sharding notes in DESIGN.md)", file=sys.stderr)
@@ -308,6 +328,19 @@ def main() -> int:
     achieved = tests_per_step * BYTES_PER_TEST / (k1 * 1e-3) / 1e9
     log(f"kernels ms: classify {k1:.4f} colscan {k_ms[:, 1].mean():.4f} scatter {k_ms[:, 2].mean():.4f}")
     visible = sum(len(batch.read(i)[0]) for i in range(len(views)))
+
+    if args.batched_only:
+        # A device-to-device copy of known size: the reference point that
+        # scales the HBM byte counters of the classify kernel.
+        from helio_amd.passes import DeviceBuffer
+        src, dst = DeviceBuffer(ctx, CALIBRATION_COPY_BYTES), DeviceBuffer(ctx, CALIBRATION_COPY_BYTES)
+        dst.copy_from(src)
+        print(json.dumps({"metric": "culled instances/s", "value": tests_per_step * args.steps / dt_batched,
+                          "unit": "instances/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": ms_batched, "mode": "batched-only", "classify_ms": k1,
+                          "kernel_ms": k_ms.mean(0).tolist(), "calibration_copy_bytes": CALIBRATION_COPY_BYTES}),
+              flush=True)
+        return 0
 
     # ---- plain per-view launch loop (same views, prepared once) ----
     passes = []
@@ -408,7 +441,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(tests_per_step),
             "kernel": "k_classify",
             "kernel_ms": k1,
             "algorithmic_bytes_per_launch": tests_per_step * BYTES_PER_TEST,

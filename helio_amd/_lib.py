@@ -133,6 +133,7 @@ _SIGS: dict[str, tuple] = {
     "hv_device_alloc": (i32, [vp, u64, P(vp)]),
     "hv_device_free": (i32, [vp, vp]),
     "hv_memcpy_h2d": (i32, [vp, vp, vp, u64, vp]),
+    "hv_memcpy_d2d": (i32, [vp, vp, vp, u64, vp]),
     "hv_memcpy_d2h": (i32, [vp, vp, vp, u64, vp]),
     "hv_memset_d32": (i32, [vp, vp, u32, u64, vp]),
     "hv_timer_record": (i32, [vp, u32, vp]),

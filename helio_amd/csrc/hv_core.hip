@@ -162,6 +162,14 @@ hv_status hv_memcpy_d2h(hv_ctx* ctx, void* dst, const void* src, uint64_t bytes,
     return HV_OK;
 }
 
+hv_status hv_memcpy_d2d(hv_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return set_error(HV_EINVAL, "hv_memcpy_d2d: NULL argument");
+    hipStream_t s = pick(ctx, stream);
+    HV_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+    HV_HIP(hipStreamSynchronize(s));
+    return HV_OK;
+}
+
 hv_status hv_memset_d32(hv_ctx* ctx, void* dst, uint32_t value, uint64_t count, void* stream) {
     if (!ctx || (!dst && count)) return set_error(HV_EINVAL, "hv_memset_d32: NULL argument");
     HV_HIP(hipMemsetD32Async(static_cast<hipDeviceptr_t>(dst), (int)value, count, pick(ctx, stream)));

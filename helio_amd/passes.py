@@ -445,6 +445,11 @@ class DeviceBuffer:
         call("hv_memcpy_d2h", self.ctx.handle, ptr(out), self.ptr, out.nbytes, None)
         return out
 
+    def copy_from(self, src: "DeviceBuffer", nbytes: int | None = None) -> None:
+        n = min(self.nbytes, src.nbytes) if nbytes is None else int(nbytes)
+        assert n <= self.nbytes and n <= src.nbytes
+        call("hv_memcpy_d2d", self.ctx.handle, self.ptr, src.ptr, n, None)
+
     def fill_u32(self, value: int = 0) -> None:
         call("hv_memset_d32", self.ctx.handle, self.ptr, value, self.nbytes // 4, None)
         self.ctx.synchronize()

@@ -234,6 +234,9 @@ hv_status hv_device_free(hv_ctx* ctx, void* dptr);
 hv_status hv_memcpy_h2d(hv_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream);
 hv_status hv_memcpy_d2h(hv_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream);
 hv_status hv_memset_d32(hv_ctx* ctx, void* dst, uint32_t value, uint64_t count, void* stream);
+/* Device-to-device copy (CommandEncoder::copy_buffer_to_buffer, as used at
+ * crates/helio-pass-virtual-geometry/src/rendering.rs:1168); synchronises. */
+hv_status hv_memcpy_d2d(hv_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream);
 
 /* Stream-ordered timing: record a HIP event on `stream` into slot `slot`
  * (0..63) and read the elapsed ms between two recorded slots. */
