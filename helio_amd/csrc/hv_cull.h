@@ -31,6 +31,9 @@ struct alignas(16) CullView {
     float cam[3];            // camera position
     float lod_d2[7];         // squared-distance thresholds: sqrt_rn(s) >= t  <=>  s >= lod_d2
     float pv_sign[18];       // per plane component: +1 if it is >= 0 (p-vertex takes the max corner), else -1
+    float plane_l1[6];       // |nx| + |ny| + |nz| per plane
+    float sure_kl, sure_kd;  // frustum decision margin E = fma(sure_kl, max|c| + r, sure_kd)
+    uint32_t sure_ok;        // planes finite and bounded: the margin-based decisions apply
     uint32_t lod_count;      // 1..8
     uint32_t flags;          // HV_CULL_*
     uint32_t nb;             // buckets = mesh_count * lod_count
@@ -116,13 +119,14 @@ class CullBatch {
 
   private:
     hv_status launch(hipStream_t stream, float* ms, bool fixed_grids);
-    hv_status ensure_occupancy();
+    hv_status ensure_occupancy(bool aabb);
     hv_ctx* ctx_;
     hv_cull_config cfg_;
     uint32_t max_views_;
     uint32_t n_views_ = 0, n_items_ = 0, n_rows_ = 0, n_units_ = 0, max_nb_ = 0;
     uint32_t grid_ = 0;                       // G of the prepared table
     uint32_t resident1_ = 0, lds_cap_ = 0;    // capacity grid / LDS of the captured graph
+    uint32_t resident3_ = 0;                  // resident scatter blocks (its grid-stride capacity)
     uint32_t occ_key_[3] = {0, 0, 0};
     std::vector<CullView> host_views_;
     std::vector<uint32_t> host_units_, host_block_view_;
@@ -130,10 +134,14 @@ class CullBatch {
     DevBuf d_table_, d_stage_, d_ids_, d_args_, d_counters_, d_sub_cnt_, d_row_stats_, d_block_hist_, d_totals_;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
-    uint32_t graph_key_[5] = {0, 0, 0, 0, 0};
+    uint32_t graph_key_[6] = {0, 0, 0, 0, 0, 0};
     hipEvent_t ev_[5] = {};
     hipEvent_t done_ = nullptr;    // recorded after every launch
     bool done_recorded_ = false;
+    bool pending_ = false;                  // a replay was launched and done_ not yet recorded after it
+    hipStream_t pending_stream_ = nullptr;  // the stream of that replay
+    // Waits for the last launch (recording done_ on a replay's stream first).
+    hv_status sync_last();
 };
 
 }  // namespace hv

@@ -176,6 +176,8 @@ struct RowLds {
     float p11;
     float lod_d2[7];
     uint32_t lod_count;
+    float sure_kl, sure_kd;
+    uint32_t sure_ok;
 };
 
 // Read of a view-table field through the constant address space. The table
@@ -187,19 +189,73 @@ __device__ __forceinline__ T ldc(const T* p) {
     return *(const __attribute__((address_space(4))) T*)p;
 }
 
+// Margin-based decisions (no change to any result, only to how fast it is
+// reached). Frustum, sphere-derived box: the reference distance of plane i is
+// D_i = n.c + L1_i r + d_i up to rounding, |ref_i - D_i| <= 5.1 u T and the
+// FMA chain s_i = fma(nx,cx, fma(ny,cy, fma(nz,cz, fma(L1_i,r,d_i)))) has
+// |s_i - D_i| <= 6.1 u T (u = 2^-24, T = sum |n||c| + L1 r + |d| <=
+// L1max (max|c| + r) + |d|max). With E = 2^-20 (L1max (max|c| + r) + |d|max)
+// + 2^-140 (16 u T, plus slack for the denormal range): min_i s_i >= E proves
+// every ref_i >= 0 (visible), min_i s_i < -E proves some ref_i < 0 (culled).
+// Lanes with neither, degenerate boxes (the sphere rule), r <= 0 or non-finite
+// values take the exact reference arithmetic, in a wave-uniform branch.
+// Sub-pixel: |fl(a / w)| < 0.001 with a = fl(r P11) and w in [2^-64, 2^62) is
+// decided from q = |a| rcp(w) (v_rcp_f32: within 1 ulp, so q is within
+// 1.5 * 2^-23 of |a| / w): q < 0.001 (1 - 2^-20) proves it, q > 0.001 (1 +
+// 2^-20) disproves it; anything else divides exactly.
+constexpr float kTinyT = 0.001f;
+constexpr float kTinyLo = 0.001f * (1.0f - 0x1p-20f);
+constexpr float kTinyHi = 0.001f * (1.0f + 0x1p-20f);
+
 // One candidate. Returns its bucket (mesh_id * lod_count + lod) when it is
 // drawn, kInvalid otherwise. Lanes with !valid report all-false.
 template <bool OCCL, bool AABB>
-__device__ __forceinline__ uint32_t classify(const CullView* __restrict__ V, const float* pl, const float* sg,
-                                             const RowLds& rc, bool valid, float4 b, uint32_t m,
-                                             const hv_instance_aabb* ab, Outcome& o) {
+__device__ __forceinline__ uint32_t classify(const CullView* __restrict__ V, const CullView* __restrict__ Vk,
+                                             const float* pl, const float* l1s, const RowLds& rc, bool valid, float4 b,
+                                             uint32_t m, const hv_instance_aabb* ab, Outcome& o) {
     const float cx = b.x, cy = b.y, cz = b.z, r = b.w;
-    o.visible = valid && frustum_visible<AABB>(pl, sg, cx, cy, cz, r, ab);
-    // Sub-pixel cull, evaluated for every lane (bitwise &: no divergent
-    // branch around the division): w = ((vp[3] x + vp[7] y) + vp[11] z) + vp[15].
+    if (AABB) {
+        o.visible = valid && frustum_visible<true>(pl, nullptr, cx, cy, cz, r, ab);
+    } else {
+        const float cmax = fmaxf(fmaxf(fabsf(cx), fabsf(cy)), fabsf(cz));
+        const float bb = cmax + r;
+        // r > 2^-22 max|c| also rules out a degenerate box (see frustum_visible).
+        const bool lane_ok = (r > cmax * 0x1p-22f) & (bb < 0x1p100f);
+        float s[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            const float l1 = l1s[i];
+            s[i] = __builtin_fmaf(pl[4 * i], cx,
+                                  __builtin_fmaf(pl[4 * i + 1], cy,
+                                                 __builtin_fmaf(pl[4 * i + 2], cz, __builtin_fmaf(l1, r, pl[4 * i + 3]))));
+        }
+        const float smin = fminf(fminf(fminf(s[0], s[1]), s[2]), fminf(fminf(s[3], s[4]), s[5]));
+        const float e = __builtin_fmaf(rc.sure_kl, bb, rc.sure_kd);
+        bool vis = smin >= e;
+        const bool unsure = valid & ((rc.sure_ok == 0u) | !(lane_ok & ((smin >= e) | (smin < -e))));
+        if (__ballot(unsure) != 0ull) {
+            asm volatile("" ::: "memory");
+            // The p-vertex signs are needed only here: loaded on demand.
+            float sg[18];
+#pragma unroll
+            for (int i = 0; i < 18; ++i) sg[i] = ldc(&Vk->pv_sign[i]);
+            const bool ve = frustum_visible<false>(pl, sg, cx, cy, cz, r, nullptr);
+            vis = unsure ? ve : vis;
+        }
+        o.visible = valid & vis;
+    }
+    // Sub-pixel cull: w = ((vp[3] x + vp[7] y) + vp[11] z) + vp[15].
     const float w = ((rc.vp3[0] * cx + rc.vp3[1] * cy) + rc.vp3[2] * cz) + rc.vp3[3];
-    const float q = fabsf(r * rc.p11 / w);
-    const bool tiny = (w > 0.0f) & (q < 0.001f);
+    const float a = r * rc.p11;
+    const bool w_ok = (__float_as_uint(w) - 0x1F800000u) < 0x3F000000u;  // w in [2^-64, 2^62)
+    const float q = fabsf(a) * __builtin_amdgcn_rcpf(w);
+    bool tiny = w_ok & (q < kTinyLo);
+    const bool unsure_t = o.visible & (w > 0.0f) & !(w_ok & ((q < kTinyLo) | (q > kTinyHi)));
+    if (__ballot(unsure_t) != 0ull) {
+        asm volatile("" ::: "memory");
+        const bool te = fabsf(a / w) < kTinyT;
+        tiny = unsure_t ? te : tiny;
+    }
     o.draw = o.visible & !tiny;
     o.occl = false;
     const float camx = rc.cam[0], camy = rc.cam[1], camz = rc.cam[2];
@@ -270,11 +326,18 @@ __device__ __forceinline__ void load_item(const ItemCtx& c, uint32_t it, float4*
     }
 }
 
+// Per-lane stat counters of a row: tests seen as visible / drawn / occluded,
+// shadow casters, shadow & drawn, shadow & occluded (reduced once per row).
+struct LaneStats {
+    uint32_t vis = 0, draw = 0, occl = 0, sh = 0, sh_draw = 0, sh_occl = 0;
+};
+
 // Classifies the loaded candidates of item `it`: wave-compacted survivors into
-// the wave's stage slots, the row histogram in LDS, stats in `st`.
+// the wave's stage slots, the row histogram in LDS; `valid` and `kept` count
+// this wave's valid candidates and survivors.
 template <int WG, int IPT, bool OCCL, bool AABB>
 __device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, const float4* bb, const uint32_t* mm,
-                                             uint32_t* s_hist, uint32_t* st) {
+                                             uint32_t* s_hist, LaneStats& ls, uint32_t& n_valid, uint32_t& kept) {
     constexpr uint32_t NW = WG / 64;
     constexpr uint32_t CHUNK = WG * IPT, SUB = IPT * 64;
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
@@ -283,23 +346,23 @@ __device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, cons
     const uint32_t n = rem < CHUNK ? rem : CHUNK;
     const uint32_t first = c.vfirst + base_idx;
     GlobalU32Out* __restrict__ out = c.stage + ((size_t)it * NW + wave) * SUB;
-    // The 42 plane/sign constants, as scalar loads per item. `it >> 31` is
-    // zero (items number below 2^31) but not provably so: the loads stay in
-    // the item instead of being hoisted out of the row loop, which would hold
+    // The plane constants, as scalar loads per item. `it >> 31` is zero
+    // (items number below 2^31) but not provably so: the loads stay in the
+    // item instead of being hoisted out of the row loop, which would hold
     // them, with everything else, beyond the scalar register file.
     const CullView* __restrict__ Vk = c.V + (it >> 31);
-    float pl[24], sg[18];
+    float pl[24], l1[6];
 #pragma unroll
     for (int i = 0; i < 24; ++i) pl[i] = ldc(&Vk->planes[i]);
 #pragma unroll
-    for (int i = 0; i < 18; ++i) sg[i] = ldc(&Vk->pv_sign[i]);
+    for (int i = 0; i < 6; ++i) l1[i] = ldc(&Vk->plane_l1[i]);
     uint32_t pos = 0;
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
         const uint32_t j = k * WG + tid;
         const bool valid = j < n;
         Outcome o;
-        const uint32_t code = classify<OCCL, AABB>(c.V, pl, sg, *c.rc, valid, bb[k], mm[k],
+        const uint32_t code = classify<OCCL, AABB>(c.V, Vk, pl, l1, *c.rc, valid, bb[k], mm[k],
                                                    AABB ? c.aabb + first + (valid ? j : 0u) : nullptr, o);
         const uint64_t keep = __ballot(code != kInvalid);
         if (code != kInvalid) {
@@ -307,22 +370,28 @@ __device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, cons
             atomicAdd(&s_hist[code], 1u);
         }
         pos += (uint32_t)__popcll(keep);
-        // Stats from lane masks (layout of indirect_dispatch.wgsl:69-77).
-        const uint32_t nv = (uint32_t)__popcll(__ballot(valid));
-        const uint32_t nvis = (uint32_t)__popcll(__ballot(o.visible));
-        const uint64_t mdraw = __ballot(o.draw), mocc = __ballot(o.occl);
-        const uint64_t msh = __ballot(valid && (mm[k] >> 31) != 0u);
-        const uint32_t ndraw = (uint32_t)__popcll(mdraw);
-        st[1] += nv - nvis;
-        st[2] += nvis - ndraw;
-        st[3] += ndraw;
-        st[4] += (uint32_t)__popcll(mocc);
-        st[5] += (uint32_t)__popcll(msh);
-        st[6] += (uint32_t)__popcll(msh & mdraw);
-        st[7] += (uint32_t)__popcll(msh & mocc);
+        // Valid lanes of this wave in round k: n - (k*WG + wave*64), clamped.
+        const uint32_t w0 = k * WG + wave * 64u;
+        n_valid += n > w0 ? min(n - w0, 64u) : 0u;
+        const bool sh = valid & ((mm[k] >> 31) != 0u);
+        ls.vis += o.visible ? 1u : 0u;
+        ls.draw += o.draw ? 1u : 0u;
+        ls.sh += sh ? 1u : 0u;
+        ls.sh_draw += (sh & o.draw) ? 1u : 0u;
+        if (OCCL) {
+            ls.occl += o.occl ? 1u : 0u;
+            ls.sh_occl += (sh & o.occl) ? 1u : 0u;
+        }
     }
-    st[0] += pos;
+    kept += pos;
     if (lane == 0) c.sub_cnt[(size_t)it * NW + wave] = pos;
+}
+
+// Sum of a lane value over the wave.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+    for (uint32_t off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
 }
 
 template <int WG, int IPT, bool OCCL, bool AABB>
@@ -355,6 +424,9 @@ __global__ __launch_bounds__(WG) void k_classify(const BatchHdr* __restrict__ hd
 #pragma unroll
             for (int i = 0; i < 7; ++i) s_rc.lod_d2[i] = V->lod_d2[i];
             s_rc.lod_count = V->lod_count;
+            s_rc.sure_kl = V->sure_kl;
+            s_rc.sure_kd = V->sure_kd;
+            s_rc.sure_ok = V->sure_ok;
         }
         const uint32_t nb = ldc(&V->nb), vcount = ldc(&V->count), vfirst = ldc(&V->first);
         const uint32_t ibeg = ldc(&V->item_begin);
@@ -364,32 +436,33 @@ __global__ __launch_bounds__(WG) void k_classify(const BatchHdr* __restrict__ hd
         const hv_instance_aabb* __restrict__ aabb = ldc(&V->aabb);
         for (uint32_t b = tid; b < nb; b += WG) s_hist[b] = 0u;
         __syncthreads();
-        uint32_t st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        // Software pipeline over two register sets: the loads of the next
-        // item are in flight while this one is classified (no register copies,
-        // so the compiler's wait counts cover only the older loads). On gfx9
-        // the vector-memory counter also counts stores, whose number per item
-        // varies with the data; an explicit vmcnt(0) before each prefetch (it
-        // covers loads issued one item earlier) keeps the compiler from placing
-        // that full wait after the prefetch instead.
-        constexpr int kWaitVm0 = 0x0F70;  // s_waitcnt vmcnt(0), expcnt/lgkmcnt untouched
+        LaneStats ls;
+        uint32_t n_valid = 0, kept = 0;
+        // One item at a time: latency is covered by the other resident waves
+        // (a two-deep register pipeline measured no faster and cost registers).
         const ItemCtx ic{V, &s_rc, bounds, meta, aabb, stage, sub_cnt, ibeg, vcount, vfirst};
-        float4 b0[IPT], b1[IPT];
-        uint32_t m0[IPT], m1[IPT];
-        if (item < seg_end) load_item<WG, IPT>(ic, item, b0, m0);
-        while (item < seg_end) {
-            __builtin_amdgcn_s_waitcnt(kWaitVm0);
-            if (item + 1u < seg_end) load_item<WG, IPT>(ic, item + 1u, b1, m1);
-            process_item<WG, IPT, OCCL, AABB>(ic, item, b0, m0, s_hist, st);
-            if (++item >= seg_end) break;
-            __builtin_amdgcn_s_waitcnt(kWaitVm0);
-            if (item + 1u < seg_end) load_item<WG, IPT>(ic, item + 1u, b0, m0);
-            process_item<WG, IPT, OCCL, AABB>(ic, item, b1, m1, s_hist, st);
-            ++item;
+        for (; item < seg_end; ++item) {
+            float4 bb[IPT];
+            uint32_t mm[IPT];
+            load_item<WG, IPT>(ic, item, bb, mm);
+            process_item<WG, IPT, OCCL, AABB>(ic, item, bb, mm, s_hist, ls, n_valid, kept);
         }
+        // Row stats (layout of indirect_dispatch.wgsl:69-77): [0] survivors,
+        // [1] invalid (not visible), [2] visible but sub-pixel, [3] drawn,
+        // [4] occluded, [5] shadow casters, [6] shadow & drawn, [7] shadow &
+        // occluded.
+        const uint32_t nvis = wave_sum(ls.vis), ndraw = wave_sum(ls.draw), nsh = wave_sum(ls.sh);
+        const uint32_t nshd = wave_sum(ls.sh_draw);
+        const uint32_t nocc = OCCL ? wave_sum(ls.occl) : 0u, nsho = OCCL ? wave_sum(ls.sh_occl) : 0u;
         if (lane == 0) {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) s_st[wave][i] = st[i];
+            s_st[wave][0] = kept;
+            s_st[wave][1] = n_valid - nvis;
+            s_st[wave][2] = nvis - ndraw;
+            s_st[wave][3] = ndraw;
+            s_st[wave][4] = nocc;
+            s_st[wave][5] = nsh;
+            s_st[wave][6] = nshd;
+            s_st[wave][7] = nsho;
         }
         __syncthreads();
         const uint32_t r = blockIdx.x - ldc(&V->block_begin);
@@ -557,36 +630,56 @@ __global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __r
 
 // Scatter: block b revisits the items it classified, row by row. NT threads
 // (the classify block size when that is 512, else 256: one scatter block per
-// classify block).
-template <int NT>
+// classify block). A sub-chunk is one classify wave's survivors of one item
+// (at most IPT x 64 stage entries); wave w of the block takes sub-chunks w,
+// w + NT/64, ... of the row, KB at a time. The first batch is loaded before
+// the cursor set-up, so its latency overlaps the scan of the bucket totals.
+template <int NT, int IPT>
+__device__ __forceinline__ void scatter_load(const uint32_t* __restrict__ stage, const uint32_t* __restrict__ sub_cnt,
+                                             size_t gs, size_t gs1, uint32_t sub, uint32_t& cnt, uint32_t* ent) {
+    const uint32_t lane = threadIdx.x & 63u;
+    cnt = gs < gs1 ? ldc(sub_cnt + gs) : 0u;
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+        const uint32_t e = lane + 64u * q;
+        ent[q] = e < cnt ? stage[gs * sub + e] : 0u;
+    }
+}
+
+template <int NT, int IPT>
 __global__ __launch_bounds__(NT) void k_scatter(const BatchHdr* __restrict__ hdr, const CullView* __restrict__ views) {
     constexpr uint32_t KR = 1024u / NT;  // offsets held in registers: the first 1024 buckets
+    constexpr uint32_t NWS = NT / 64;     // scatter waves
+    constexpr uint32_t KB = IPT >= 4 ? 1u : 4u / IPT;  // sub-chunks per batch: 4 entries per lane (IPT for IPT > 4)
     extern __shared__ uint32_t s_cur[];
-    __shared__ uint32_t s_pre[NT];
-    __shared__ uint32_t s_w[NT / 64];
-    const uint32_t tid = threadIdx.x;
+    __shared__ uint32_t s_w[NWS];
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
     const uint32_t G = ldc(&hdr->grid);
-    if (blockIdx.x >= G) return;
     const uint32_t n_views = ldc(&hdr->n_views), chunk = ldc(&hdr->chunk), sub = ldc(&hdr->sub);
-    const uint32_t wl = ldc(&hdr->waves_log2);
+    const uint32_t wl = ldc(&hdr->waves_log2), n_items = ldc(&hdr->n_items);
     const uint32_t* __restrict__ stage = ldc(&hdr->stage);
     const uint32_t* __restrict__ sub_cnt = ldc(&hdr->sub_cnt);
+    // Classify block cb's rows, for cb = blockIdx.x, blockIdx.x + gridDim.x, ...
+    for (uint32_t cb = blockIdx.x; cb < G; cb += gridDim.x) {
     uint32_t item, i_end;
-    block_items(blockIdx.x, G, ldc(&hdr->n_items), item, i_end);
-    uint32_t v = ldc(ldc(&hdr->block_view) + blockIdx.x);
+    block_items(cb, G, n_items, item, i_end);
+    uint32_t v = ldc(ldc(&hdr->block_view) + cb);
     while (item < i_end) {
         v = advance_view(views, n_views, v, item);
         const CullView* __restrict__ V = views + v;
         const uint32_t nb = ldc(&V->nb), ibeg = ldc(&V->item_begin), vfirst = ldc(&V->first);
         const uint32_t seg_end = min(i_end, ibeg + ldc(&V->n_items));
-        const uint32_t r = blockIdx.x - ldc(&V->block_begin);
+        const uint32_t r = cb - ldc(&V->block_begin);
         const uint32_t* __restrict__ totals = ldc(&V->totals);
         const uint32_t* __restrict__ offs = ldc(&V->block_hist) + (size_t)r * nb;
-        // Independent loads issued together: the first sub-chunk counts, the
-        // bucket totals and this row's offsets (the first 1024 buckets' in
-        // registers; larger tables reread their offsets below).
+        uint32_t* __restrict__ ids = ldc(&V->ids);
         const size_t gs0 = (size_t)item << wl, gs1 = (size_t)seg_end << wl;
-        const uint32_t c0 = (gs0 + tid < gs1) ? sub_cnt[gs0 + tid] : 0u;
+        // First batch of this wave's sub-chunks (independent of the cursors).
+        uint32_t cA[KB], eA[KB][IPT], cB[KB], eB[KB][IPT];
+#pragma unroll
+        for (uint32_t k = 0; k < KB; ++k) scatter_load<NT, IPT>(stage, sub_cnt, gs0 + wave + k * NWS, gs1, sub, cA[k], eA[k]);
+        // Bucket totals and this row's offsets (the first 1024 buckets' offsets
+        // in registers; larger tables reread theirs below).
         uint32_t offr[KR];
 #pragma unroll
         for (uint32_t k = 0; k < KR; ++k) {
@@ -620,42 +713,41 @@ __global__ __launch_bounds__(NT) void k_scatter(const BatchHdr* __restrict__ hdr
             if (tid + NT * k < nb) cursor(tid + NT * k, offr[k]);
         for (uint32_t bb = tid + NT * KR; bb < nb; bb += NT) cursor(bb, offs[bb]);
         __syncthreads();
-        uint32_t* __restrict__ ids = ldc(&V->ids);
-        for (size_t s0 = gs0; s0 < gs1; s0 += NT) {
-            const uint32_t ns = (uint32_t)min((size_t)NT, gs1 - s0);
-            const uint32_t c = s0 == gs0 ? c0 : (tid < ns ? sub_cnt[s0 + tid] : 0u);
-            uint32_t total;
-            s_pre[tid] = block_excl_scan<NT>(c, s_w, total);
-            __syncthreads();
-            for (uint32_t e0 = 0; e0 < total; e0 += 4u * NT) {
-                uint32_t ent[4], sidx[4];
+        // Place the ids, batch by batch; the next batch's loads are in flight
+        // while the current one is placed (two register sets, no copies).
+        auto place = [&](size_t g, const uint32_t* cnt, const uint32_t (*ent)[IPT]) {
 #pragma unroll
-                for (uint32_t q = 0; q < 4; ++q) {
-                    const uint32_t e = e0 + q * NT + tid;
-                    sidx[q] = kInvalid;
-                    ent[q] = 0u;
-                    if (e < total) {
-                        uint32_t lo = 0, hi = ns;  // largest i with s_pre[i] <= e
-                        while (hi - lo > 1u) {
-                            const uint32_t mid = (lo + hi) >> 1;
-                            if (s_pre[mid] <= e) lo = mid;
-                            else hi = mid;
-                        }
-                        sidx[q] = lo;
-                        ent[q] = stage[(s0 + lo) * sub + (e - s_pre[lo])];
-                    }
-                }
+            for (uint32_t k = 0; k < KB; ++k) {
+                const uint32_t it = (uint32_t)((g + k * NWS) >> wl);
+                const uint32_t id0 = vfirst + (it - ibeg) * chunk;
 #pragma unroll
-                for (uint32_t q = 0; q < 4; ++q)
-                    if (sidx[q] != kInvalid) {
-                        const uint32_t it = (uint32_t)((s0 + sidx[q]) >> wl);
-                        const uint32_t id = vfirst + (it - ibeg) * chunk + (ent[q] & ((1u << kEntryShift) - 1u));
-                        ids[atomicAdd(&s_cur[ent[q] >> kEntryShift], 1u)] = id;
+                for (int q = 0; q < IPT; ++q)
+                    if (lane + 64u * q < cnt[k]) {
+                        const uint32_t en = ent[k][q];
+                        ids[atomicAdd(&s_cur[en >> kEntryShift], 1u)] = id0 + (en & ((1u << kEntryShift) - 1u));
                     }
             }
-            __syncthreads();
+        };
+        constexpr uint32_t STEP = KB * NWS;
+        for (size_t g = gs0 + wave;; g += 2 * STEP) {
+            const bool more_a = g + STEP < gs1;
+            if (more_a)
+#pragma unroll
+                for (uint32_t k = 0; k < KB; ++k)
+                    scatter_load<NT, IPT>(stage, sub_cnt, g + STEP + k * NWS, gs1, sub, cB[k], eB[k]);
+            place(g, cA, eA);
+            if (!more_a) break;
+            const bool more_b = g + 2 * STEP < gs1;
+            if (more_b)
+#pragma unroll
+                for (uint32_t k = 0; k < KB; ++k)
+                    scatter_load<NT, IPT>(stage, sub_cnt, g + 2 * STEP + k * NWS, gs1, sub, cA[k], eA[k]);
+            place(g + STEP, cB, eB);
+            if (!more_b) break;
         }
+        __syncthreads();
         item = seg_end;
+    }
     }
 }
 
@@ -684,6 +776,24 @@ ClassifyFn pick_ipt(uint32_t ipt, bool occl, bool aabb) {
         default: return nullptr;
     }
 }
+
+using ScatterFn = void (*)(const BatchHdr*, const CullView*);
+
+template <int NT>
+ScatterFn scatter_ipt(uint32_t ipt) {
+    switch (ipt) {
+        case 1: return dev::k_scatter<NT, 1>;
+        case 2: return dev::k_scatter<NT, 2>;
+        case 4: return dev::k_scatter<NT, 4>;
+        default: return dev::k_scatter<NT, 8>;
+    }
+}
+
+// Scatter variant for a classify shape: 512 threads beside a 512-wide
+// classify (one scatter block per classify block), else 256; IPT entries per
+// lane per sub-chunk.
+ScatterFn scatter_fn(const CullShape& s) { return s.wg >= 512 ? scatter_ipt<512>(s.ipt) : scatter_ipt<256>(s.ipt); }
+uint32_t scatter_threads(const CullShape& s) { return s.wg >= 512 ? 512u : 256u; }
 
 ClassifyFn classify_fn(const CullShape& s, bool occl, bool aabb) {
     switch (s.wg) {
@@ -775,7 +885,7 @@ uint64_t CullBatch::bytes_for(uint64_t candidates, uint32_t views, uint32_t buck
 
 hv_status CullBatch::reserve(uint32_t views, uint64_t candidates, uint32_t buckets) {
     HV_HIP(hipSetDevice(ctx_->device));
-    if (done_recorded_) HV_HIP(hipEventSynchronize(done_));
+    HV_TRY(sync_last());
     const hipStream_t st = ctx_->stream;
     const uint32_t chunk = shape.chunk(), nw = shape.wg / 64;
     const uint64_t items = candidates / chunk + views + 1;
@@ -796,15 +906,19 @@ hv_status CullBatch::reserve(uint32_t views, uint64_t candidates, uint32_t bucke
     return HV_OK;
 }
 
-hv_status CullBatch::ensure_occupancy() {
-    const uint32_t key[3] = {shape.wg * 16u + shape.ipt, (cfg_.flags & HV_CULL_OCCLUSION) ? 1u : 0u, lds_cap_};
+hv_status CullBatch::ensure_occupancy(bool aabb) {
+    const uint32_t key[3] = {shape.wg * 16u + shape.ipt,
+                             ((cfg_.flags & HV_CULL_OCCLUSION) ? 1u : 0u) | (aabb ? 2u : 0u), lds_cap_};
     if (resident1_ && std::memcmp(key, occ_key_, sizeof key) == 0) return HV_OK;
     const bool occl = (cfg_.flags & HV_CULL_OCCLUSION) != 0;
     int occ = 0;
-    // The AABB variant uses at least as many registers; size the grid for it.
-    HV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(classify_fn(shape, occl, true)),
+    // Sized for the variant that will run (G, and with it the rows, follow).
+    HV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(classify_fn(shape, occl, aabb)),
                                                         (int)shape.wg, (size_t)lds_cap_));
     resident1_ = std::max(1u, ctx_->cu_count * (uint32_t)std::max(occ, 1));
+    HV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(scatter_fn(shape)),
+                                                        (int)scatter_threads(shape), (size_t)lds_cap_));
+    resident3_ = std::max(1u, ctx_->cu_count * (uint32_t)std::max(occ, 1));
     std::memcpy(occ_key_, key, sizeof key);
     return HV_OK;
 }
@@ -820,7 +934,7 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
         return set_error(HV_EINVAL, "prepare: unsupported launch shape wg=%u ipt=%u", shape.wg, shape.ipt);
     HV_HIP(hipSetDevice(ctx_->device));
     // The previous launch may still read the table and buffers rewritten below.
-    if (done_recorded_) HV_HIP(hipEventSynchronize(done_));
+    HV_TRY(sync_last());
     const bool occl = (cfg_.flags & HV_CULL_OCCLUSION) != 0;
     const uint32_t chunk = shape.chunk(), nw = shape.wg / 64;
     uint64_t ids_total = 0, args_total = 0, items_total = 0;
@@ -856,7 +970,9 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     const uint32_t N = (uint32_t)items_total;
     // Launch capacity first: the item -> block mapping depends on the grid.
     lds_cap_ = std::max(lds_cap_, max_nb * 4u);
-    HV_TRY(ensure_occupancy());
+    bool aabb_any = false;
+    for (uint32_t v = 0; v < n; ++v) aabb_any = aabb_any || views[v].scene->explicit_aabb;
+    HV_TRY(ensure_occupancy(aabb_any));
     const uint32_t G = N ? std::min(resident1_, N) : 0u;
     auto block_of = [&](uint32_t i) { return (uint32_t)(((uint64_t)(i + 1) * G - 1) / N); };
 
@@ -907,6 +1023,23 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
         HV_TRY(hv_extract_frustum_planes(d.camera->view_proj, cv.planes));
         for (int i = 0; i < 6; ++i)
             for (int a = 0; a < 3; ++a) cv.pv_sign[3 * i + a] = cv.planes[4 * i + a] >= 0.0f ? 1.0f : -1.0f;
+        {
+            // Margin of the frustum decisions (see the kernel): 2^-20 (L1max B + |d|max) + 2^-140.
+            double l1max = 0.0, dmax = 0.0;
+            bool finite = true;
+            for (int i = 0; i < 6; ++i) {
+                cv.plane_l1[i] = (std::fabs(cv.planes[4 * i]) + std::fabs(cv.planes[4 * i + 1])) + std::fabs(cv.planes[4 * i + 2]);
+                const double l1 = std::fabs((double)cv.planes[4 * i]) + std::fabs((double)cv.planes[4 * i + 1]) +
+                                  std::fabs((double)cv.planes[4 * i + 2]);
+                l1max = std::max(l1max, l1);
+                dmax = std::max(dmax, std::fabs((double)cv.planes[4 * i + 3]));
+                for (int k = 0; k < 4; ++k) finite = finite && std::isfinite(cv.planes[4 * i + k]);
+            }
+            // Rounded up, so the kernel's margin never falls below the bound.
+            cv.sure_kl = std::nextafter((float)(l1max * 0x1p-20), INFINITY);
+            cv.sure_kd = std::nextafter((float)(dmax * 0x1p-20 + 0x1p-140), INFINITY);
+            cv.sure_ok = (finite && l1max <= 0x1p20 && dmax <= 0x1p100) ? 1u : 0u;
+        }
         std::memcpy(cv.vp, d.camera->view_proj, 64);
         cv.p00 = d.camera->proj[0];
         cv.p11 = d.camera->proj[5];
@@ -988,6 +1121,7 @@ hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed) {
     const uint32_t colscan_cap = 8u * ctx_->cu_count;
     const uint32_t g1 = fixed ? resident1_ : std::max(1u, grid_);
     const uint32_t g2 = fixed ? colscan_cap : std::max(1u, std::min(n_units_, colscan_cap));
+    const uint32_t g3 = fixed ? resident3_ : std::max(1u, std::min(grid_, resident3_));
     if (ms) HV_HIP(hipEventRecord(ev_[0], s));
     hipLaunchKernelGGL(fn, dim3(g1), dim3(shape.wg), lds, s, hdr, views);
     HV_HIP(hipGetLastError());
@@ -995,13 +1129,13 @@ hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed) {
     hipLaunchKernelGGL(dev::k_colscan, dim3(g2), dim3(kColscanThreads), 0, s, hdr, views);
     HV_HIP(hipGetLastError());
     if (ms) HV_HIP(hipEventRecord(ev_[2], s));
-    if (shape.wg >= 512)
-        hipLaunchKernelGGL(dev::k_scatter<512>, dim3(g1), dim3(512), lds, s, hdr, views);
-    else
-        hipLaunchKernelGGL(dev::k_scatter<256>, dim3(g1), dim3(256), lds, s, hdr, views);
+    hipLaunchKernelGGL(scatter_fn(shape), dim3(g3), dim3(scatter_threads(shape)), lds, s, hdr, views);
     HV_HIP(hipGetLastError());
-    HV_HIP(hipEventRecord(done_, s));
-    done_recorded_ = true;
+    if (!fixed) {  // eager: record completion now (a replay records it lazily)
+        HV_HIP(hipEventRecord(done_, s));
+        done_recorded_ = true;
+        pending_ = false;
+    }
     if (ms) {
         HV_HIP(hipEventRecord(ev_[3], s));
         HV_HIP(hipEventSynchronize(ev_[3]));
@@ -1024,7 +1158,8 @@ hv_status CullBatch::execute_graph(hipStream_t s) {
     // The classify variant (occlusion, explicit AABBs) and the LDS size are
     // baked into the graph; grids are the capacity grids.
     const bool aabb_any = std::any_of(host_views_.begin(), host_views_.end(), [](const CullView& c) { return c.aabb; });
-    const uint32_t key[5] = {resident1_, lds_cap_, aabb_any ? 1u : 0u, shape.wg * 16u + shape.ipt, ctx_->cu_count};
+    const uint32_t key[6] = {resident1_, lds_cap_, aabb_any ? 1u : 0u, shape.wg * 16u + shape.ipt, ctx_->cu_count,
+                             resident3_};
     if (!graph_exec_ || std::memcmp(key, graph_key_, sizeof key) != 0) {
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
         if (graph_) (void)hipGraphDestroy(graph_);
@@ -1048,15 +1183,28 @@ hv_status CullBatch::execute_graph(hipStream_t s) {
         HV_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
         std::memcpy(graph_key_, key, sizeof key);
     }
+    // No completion event per replay (it costs a queue packet each time): the
+    // stream is remembered and sync_last() records the event on it when a
+    // later call needs the launch finished.
     HV_HIP(hipGraphLaunch(graph_exec_, s));
-    HV_HIP(hipEventRecord(done_, s));
-    done_recorded_ = true;
+    pending_ = true;
+    pending_stream_ = s;
+    return HV_OK;
+}
+
+hv_status CullBatch::sync_last() {
+    if (pending_) {
+        HV_HIP(hipEventRecord(done_, pending_stream_));
+        done_recorded_ = true;
+        pending_ = false;
+    }
+    if (done_recorded_) HV_HIP(hipEventSynchronize(done_));
     return HV_OK;
 }
 
 hv_status CullBatch::wait() {
     HV_HIP(hipSetDevice(ctx_->device));
-    if (done_recorded_) HV_HIP(hipEventSynchronize(done_));
+    HV_TRY(sync_last());
     return HV_OK;
 }
 
@@ -1075,7 +1223,7 @@ hv_status CullBatch::read(uint32_t v, uint32_t* ids, uint32_t ids_capacity, uint
                           hv_draw_indexed_indirect* args, uint32_t args_capacity, uint32_t* stats) {
     if (v >= n_views_) return set_error(HV_EINVAL, "read: view %u of %u", v, n_views_);
     HV_HIP(hipSetDevice(ctx_->device));
-    if (done_recorded_) HV_HIP(hipEventSynchronize(done_));
+    HV_TRY(sync_last());
     const CullView& cv = host_views_[v];
     uint32_t ctr[kCounterStride];
     HV_HIP(hipMemcpy(ctr, cv.counters, sizeof ctr, hipMemcpyDeviceToHost));

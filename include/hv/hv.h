@@ -391,7 +391,9 @@ void hv_batch_destroy(hv_batch* batch);
 /* Launch shape of the classify kernel: wg in {64,128,256,512}, ipt in {1,2,4,8}. */
 hv_status hv_batch_set_shape(hv_batch* batch, uint32_t wg, uint32_t ipt);
 hv_status hv_batch_prepare(hv_batch* batch, const hv_view_request* views, uint32_t count);
-/* use_graph != 0: replay the captured graph (captured on first use). */
+/* use_graph != 0: replay the captured graph (captured on first use). A replay
+ * enqueues the graph only; the next hv_batch_wait / _read / _prepare records
+ * completion on `stream`, which must stay valid until then. */
 hv_status hv_batch_execute(hv_batch* batch, void* stream, uint32_t use_graph);
 /* Eager execute with HIP events around the three kernels; synchronises and
  * returns ms[0] classify, ms[1] column scan (bucket offsets), ms[2] scatter. */

@@ -1,0 +1,104 @@
+"""Batched multi-view cull (graph replay) against the CPU oracle, view by view,
+and the exact fallback of the kernel's margin decisions at the boundaries."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from helio_amd import layouts as L
+from helio_amd import synth
+from helio_amd.passes import BatchCull, InstanceCullPass, Scene, extract_frustum_planes
+from tests.conftest import args_as_u32, canonical_ids
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(ctx, sc):
+    s = Scene(ctx, sc.n)
+    s.upload_soa(sc.bounds, sc.mesh_ids, sc.flags)
+    s.set_mesh_lods(sc.mesh_lods, sc.mesh_count, sc.lod_count)
+    return s
+
+
+def _oracle(sc, cam, first, count):
+    ov = oracle.view_from_camera(cam, mesh_lods=sc.mesh_lods, mesh_count=sc.mesh_count)
+    return oracle.cull_view(ov, sc.bounds, sc.meta(), first, count, threads=8)
+
+
+def _check(got, ref):
+    ids, args, stats = got
+    rids, rargs, rstats = ref
+    assert np.array_equal(stats, rstats), (stats, rstats)
+    assert np.array_equal(args_as_u32(args), rargs)
+    assert np.array_equal(canonical_ids(ids, args), rids)
+
+
+def test_batch_graph_replay_matches_oracle(ctx):
+    """The 64-view recipe (16 cameras, 6x4 cascades, 4x6 cube faces) over
+    262,144 instances: two graph replays, then every view read at once (the
+    read must wait for the replay) and compared with the oracle."""
+    sc = synth.instance_scene(262_144, config=3)
+    scene = _scene(ctx, sc)
+    views = synth.multiview_views(sc.n, max_count=120_000)
+    b = BatchCull(ctx, len(views))
+    b.prepare([(scene, v.camera, f, c) for v, f, c in views])
+    print(f"batch of {len(views)} views, {sum(c for _, _, c in views)} tests", flush=True)
+    b.execute(graph=True)
+    b.execute(graph=True)
+    for i, (v, f, c) in enumerate(views):
+        _check(b.read(i), _oracle(sc, v.camera, f, c))
+
+
+def test_batch_reprepare_after_replay(ctx):
+    """prepare() right after a replay waits for it: a second table of other
+    ranges gives the oracle's results."""
+    sc = synth.instance_scene(200_000, config=3, index=1)
+    scene = _scene(ctx, sc)
+    views = synth.multiview_views(sc.n, max_count=100_000)
+    b = BatchCull(ctx, len(views))
+    b.prepare([(scene, v.camera, f, c) for v, f, c in views])
+    b.execute(graph=True)
+    g = np.random.default_rng(5)
+    views2 = [(v, int(g.integers(0, sc.n - 1000)), 1000 + int(g.integers(0, 20_000))) for v, _, _ in views[:24]]
+    views2 = [(v, f, min(c, sc.n - f)) for v, f, c in views2]
+    b.prepare([(scene, v.camera, f, c) for v, f, c in views2])
+    b.execute(graph=True)
+    for i, (v, f, c) in enumerate(views2):
+        _check(b.read(i), _oracle(sc, v.camera, f, c))
+
+
+def test_frustum_and_subpixel_boundaries(ctx):
+    """Instances placed on the frustum planes (the p-vertex distance at zero,
+    give or take an ulp) and radii at the sub-pixel threshold: the kernel's
+    margin decisions must fall back to the exact arithmetic and agree with the
+    oracle bit for bit."""
+    g = np.random.default_rng(21)
+    view = synth.perspective_view(synth.rng(0, 77), position=(500, 500, 500), direction=(0.3, -0.2, 1.0))
+    cam = view.camera
+    planes = extract_frustum_planes(cam["view_proj"].reshape(16)).astype(np.float64)
+    n = 120_000
+    sc = synth.instance_scene(n, config=0, index=21, cube=1000.0)
+    r = np.exp(g.uniform(np.log(0.05), np.log(20.0), n))
+    c = sc.bounds[:, :3].astype(np.float64)
+    k = g.integers(0, 6, n)
+    nk, dk = planes[k, :3], planes[k, 3]
+    off = (nk * c).sum(1) + np.abs(nk).sum(1) * r + dk
+    c = c - nk * (off / (nk * nk).sum(1))[:, None]
+    c *= 1.0 + g.uniform(-3e-7, 3e-7, (n, 1))
+    # A quarter of the instances: radius at the sub-pixel threshold |r P11 / w| = 0.001.
+    vp = cam["view_proj"].reshape(4, 4).T.astype(np.float64)  # row-major
+    p11 = float(cam["proj"].reshape(16)[5])
+    w = c @ vp[3, :3] + vp[3, 3]
+    sel = (g.random(n) < 0.25) & (w > 1.0)
+    r[sel] = 0.001 * w[sel] / p11 * (1.0 + g.uniform(-2e-7, 2e-7, sel.sum()))
+    sc.bounds[:, :3] = c.astype(np.float32)
+    sc.bounds[:, 3] = r.astype(np.float32)
+    scene = _scene(ctx, sc)
+    p = InstanceCullPass(ctx)
+    p.prepare(scene, cam)
+    p.execute()
+    got = p.read()
+    ref = _oracle(sc, cam, 0, n)
+    assert ref[2][1] > 1000 and ref[2][3] > 1000, ref[2]  # both sides of the planes are populated
+    _check(got, ref)
