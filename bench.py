@@ -302,6 +302,12 @@ def main() -> int:
     # ---- batched: 64 views, one graph replay per step ----
     batch = BatchCull(ctx, len(views))
     batch.prepare([(scene, v.camera, f, c) for v, f, c in views])
+    # Launch shape: the autotuner times the 16 classify shapes on this batch
+    # (sum of the three kernels) and leaves the batch and the context on the
+    # fastest; every later pass of this run uses it.
+    from helio_amd import autotune
+    tuned = autotune.tune_batch(batch, reps=3)["best"]
+    log(f"launch shape wg={tuned['wg']} ipt={tuned['ipt']} ({tuned['total_ms'] * 1e3:.1f} us kernels)")
     for _ in range(args.warmup):
         batch.execute(graph=True)
     batch.wait()
@@ -338,6 +344,7 @@ def main() -> int:
         print(json.dumps({"metric": "culled instances/s", "value": tests_per_step * args.steps / dt_batched,
                           "unit": "instances/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                           "ms_per_step": ms_batched, "mode": "batched-only", "classify_ms": k1,
+                          "launch_shape": {"wg": tuned["wg"], "ipt": tuned["ipt"]},
                           "kernel_ms": k_ms.mean(0).tolist(), "calibration_copy_bytes": CALIBRATION_COPY_BYTES}),
               flush=True)
         return 0
@@ -433,6 +440,7 @@ def main() -> int:
             "lod_distances_m": [25, 75, 200],
             "buckets_per_view": sc.mesh_count * sc.lod_count,
             "launch": "hipGraph replay (classify, column scan, scatter)",
+            "launch_shape": {"wg": tuned["wg"], "ipt": tuned["ipt"]},
             "visible_per_step": visible,
         },
         "roofline": {

@@ -122,12 +122,18 @@ __global__ __launch_bounds__(256) void k_hiz(HizArgs a) {
     constexpr uint32_t P1 = PER / 2;
     float v1[P1][P1];
     const uint32_t sw0 = a.w, sh0 = a.h;
+    // Clamped neighbours by selecting values (an index chosen at run time
+    // would make the compiler move v0 out of registers).
+    const bool cx = sw0 != 1u, ry = sh0 != 1u;
 #pragma unroll
     for (uint32_t r = 0; r < P1; ++r)
 #pragma unroll
         for (uint32_t c = 0; c < P1; ++c) {
-            const uint32_t cx1 = (sw0 == 1u) ? 2 * c : 2 * c + 1, ry1 = (sh0 == 1u) ? 2 * r : 2 * r + 1;
-            v1[r][c] = reduce4<MIN>(v0[2 * r][2 * c], v0[2 * r][cx1], v0[ry1][2 * c], v0[ry1][cx1]);
+            const float t00 = v0[2 * r][2 * c];
+            const float t01 = cx ? v0[2 * r][2 * c + 1] : t00;
+            const float t10 = ry ? v0[2 * r + 1][2 * c] : t00;
+            const float t11 = ry ? (cx ? v0[2 * r + 1][2 * c + 1] : t10) : t01;
+            v1[r][c] = reduce4<MIN>(t00, t01, t10, t11);
             if (top >= 1u) put<1>(a, tx * (TILE / 2) + lx * P1 + c, ty * (TILE / 2) + ly * P1 + r, v1[r][c]);
         }
     // ---- level 2 in registers (TILE 64) -> LDS 16x16 ----
