@@ -44,16 +44,18 @@ CALIBRATION_COPY_BYTES = 256 << 20  # device-to-device copy used to scale the HB
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_k_classify_multiview.json"
 
 
-def measured_traffic(tests_per_step: int):
-    """HBM bytes per classify launch from the committed counter summary, if it
-    was taken on this workload (FETCH_SIZE + WRITE_SIZE scaled by a copy)."""
+def measured_traffic(tests_per_step: int, wg: int, ipt: int):
+    """HBM bytes per classify launch of this launch shape from the committed
+    counter summary, if it was taken on this workload (FETCH_SIZE + WRITE_SIZE
+    scaled by a device copy of known size)."""
     try:
         t = json.loads(TRAFFIC_FILE.read_text())
     except (OSError, ValueError):
         return None
     if int(t.get("tests_per_step", -1)) != tests_per_step:
         return None
-    return t.get("traffic_bytes_per_launch")
+    rec = t.get("k_classify_by_launch_shape", {}).get(f"{wg}x{ipt}")
+    return None if rec is None else rec.get("traffic_bytes")
 
 
 def log(msg: str) -> None:
@@ -272,6 +274,8 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip the occlusion and VG legs")
     ap.add_argument("--serving-rates", default="2000,10000,50000")
     ap.add_argument("--serving-seconds", type=float, default=0.5)
+    ap.add_argument("--launch-shape", default=None, metavar="WGxIPT",
+                    help="classify launch shape, e.g. 512x4 (default: autotuned on the batch)")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched leg and one calibration copy (for counter runs)")
     args = ap.parse_args()
@@ -300,14 +304,21 @@ def main() -> int:
         f"setup {time.perf_counter() - t_setup:.1f} s")
 
     # ---- batched: 64 views, one graph replay per step ----
+    from helio_amd import autotune
+    if args.launch_shape:
+        wg, ipt = (int(x) for x in args.launch_shape.lower().split("x"))
+        autotune.apply(ctx, dict(autotune.get_shapes(ctx), wg=wg, ipt=ipt))
     batch = BatchCull(ctx, len(views))
     batch.prepare([(scene, v.camera, f, c) for v, f, c in views])
-    # Launch shape: the autotuner times the 16 classify shapes on this batch
-    # (sum of the three kernels) and leaves the batch and the context on the
-    # fastest; every later pass of this run uses it.
-    from helio_amd import autotune
-    tuned = autotune.tune_batch(batch, reps=3)["best"]
-    log(f"launch shape wg={tuned['wg']} ipt={tuned['ipt']} ({tuned['total_ms'] * 1e3:.1f} us kernels)")
+    if args.launch_shape:
+        tuned = {"wg": wg, "ipt": ipt}
+        log(f"launch shape wg={wg} ipt={ipt} (fixed)")
+    else:
+        # The autotuner times the 16 classify shapes on this batch (median of
+        # the sum of the three kernels) and leaves the batch and the context on
+        # the fastest; every later pass of this run uses it.
+        tuned = autotune.tune_batch(batch, reps=5)["best"]
+        log(f"launch shape wg={tuned['wg']} ipt={tuned['ipt']} ({tuned['total_ms'] * 1e3:.1f} us kernels)")
     for _ in range(args.warmup):
         batch.execute(graph=True)
     batch.wait()
@@ -344,6 +355,7 @@ def main() -> int:
         print(json.dumps({"metric": "culled instances/s", "value": tests_per_step * args.steps / dt_batched,
                           "unit": "instances/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
                           "ms_per_step": ms_batched, "mode": "batched-only", "classify_ms": k1,
+                          "tests_per_step": tests_per_step,
                           "launch_shape": {"wg": tuned["wg"], "ipt": tuned["ipt"]},
                           "kernel_ms": k_ms.mean(0).tolist(), "calibration_copy_bytes": CALIBRATION_COPY_BYTES}),
               flush=True)
@@ -449,8 +461,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(tests_per_step),
-            "kernel": "k_classify",
+            "traffic": measured_traffic(tests_per_step, tuned["wg"], tuned["ipt"]),
+            "kernel": f"k_classify<{tuned['wg']}, {tuned['ipt']}, false, false>",
             "kernel_ms": k1,
             "algorithmic_bytes_per_launch": tests_per_step * BYTES_PER_TEST,
             "unique_bytes_per_launch": unique * BYTES_PER_TEST,
