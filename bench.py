@@ -254,8 +254,8 @@ def vg_leg(ctx, steps: int) -> dict:
         "objects_selected": int((sel > 0).sum()),
         "meshlets_tested": tested,
         "published": int(ctr[0]),
-        "select_ms": float(np.median(t[:, 0])),
-        "cull_ms": cull,
+        "clear_ms": float(np.median(t[:, 0])),
+        "select_cull_ms": cull,
         "meshlets_per_s": tested / (float(np.median(t.sum(1))) * 1e-3),
         "cull_gbs": (tested * 64 + int(ctr[0]) * 36) / (cull * 1e-3) / 1e9,
     }

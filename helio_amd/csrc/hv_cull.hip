@@ -82,8 +82,7 @@ __device__ __forceinline__ bool occluded(const CullView* __restrict__ V, const f
     p.levels = V->hiz_levels;
 #pragma unroll
     for (int i = 0; i < 12; ++i) p.off[i] = V->hiz_off[i];
-    const float hz = hiz_reduce4(rev, hiz_tap(p, mip, u0, v0), hiz_tap(p, mip, u1, v0), hiz_tap(p, mip, u0, v1),
-                                 hiz_tap(p, mip, u1, v1));
+    const float hz = hiz_rect4(hiz_level(p, mip), rev, u0, v0, u1, v1);
     constexpr float bias = 1.0f / 65536.0f;
     return rev ? (near_z < hz - bias) : (near_z > hz + bias);
 }

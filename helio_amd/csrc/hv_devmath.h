@@ -45,25 +45,50 @@ struct Pyramid {
     uint32_t off[12];  // float offset of level m >= 1 in packed
 };
 
-// Nearest tap (wgpu sampler: nearest, clamp-to-edge, level clamped to the
-// last one): texel = clamp(floor(u * w_m), 0, w_m - 1).
-__device__ __forceinline__ float hiz_tap(const Pyramid& p, uint32_t m, float u, float v) {
+// One pyramid level as the taps address it.
+struct HizLevel {
+    const float* base;
+    uint32_t w, h;
+};
+
+// Level m of p, m < p.levels, with its offset in the packed levels `off_m`
+// resolved by the caller.
+__device__ __forceinline__ HizLevel hiz_level_at(const Pyramid& p, uint32_t m, uint32_t off_m) {
+    HizLevel l;
+    l.w = max(p.w >> m, 1u);
+    l.h = max(p.h >> m, 1u);
+    l.base = (m == 0u) ? p.level0 : p.packed + off_m;
+    return l;
+}
+
+// Level m of p (clamped to the last level); the packed offset is picked with a
+// compare chain, since a dynamically indexed array would live in scratch.
+__device__ __forceinline__ HizLevel hiz_level(const Pyramid& p, uint32_t m) {
     if (m >= p.levels) m = p.levels - 1u;
-    const uint32_t wm = max(p.w >> m, 1u), hm = max(p.h >> m, 1u);
-    const float fx = floorf(u * (float)wm), fy = floorf(v * (float)hm);
-    const int x = (fx >= 0.0f) ? (fx < (float)wm ? (int)fx : (int)wm - 1) : 0;
-    const int y = (fy >= 0.0f) ? (fy < (float)hm ? (int)fy : (int)hm - 1) : 0;
-    // Select the level offset with a compare chain: a dynamically indexed
-    // array would be placed in scratch memory.
     uint32_t o = 0u;
 #pragma unroll
     for (uint32_t i = 1; i < 12; ++i) o = (m == i) ? p.off[i] : o;
-    const float* base = (m == 0u) ? p.level0 : p.packed + o;
-    return base[(size_t)y * wm + (uint32_t)x];
+    return hiz_level_at(p, m, o);
 }
 
-// Four taps reduced in the reference order op(op(a, b), op(c, d)).
-__device__ __forceinline__ float hiz_reduce4(bool rev, float a, float b, float c, float d) {
+// Nearest texel index along one axis (wgpu sampler: nearest, clamp-to-edge):
+// clamp(floor(u * n), 0, n - 1); NaN -> 0.
+__device__ __forceinline__ uint32_t hiz_texel(float u, uint32_t n) {
+    const float f = floorf(u * (float)n);
+    return (f >= 0.0f) ? (f < (float)n ? (uint32_t)(int)f : n - 1u) : 0u;
+}
+
+// The four nearest taps of the rect (u0, v0)-(u1, v1) on one level, reduced
+// in the reference order op(op(t(u0,v0), t(u1,v0)), op(t(u0,v1), t(u1,v1))).
+// The two columns and two rows are computed once for the four taps.
+// The pyramid is device memory: the taps are global loads even when the level
+// pointer itself was read from LDS.
+__device__ __forceinline__ float hiz_rect4(const HizLevel& l, bool rev, float u0, float v0, float u1, float v1) {
+    typedef __attribute__((address_space(1))) const float GlobalF;
+    const GlobalF* base = (const GlobalF*)l.base;
+    const uint32_t x0 = hiz_texel(u0, l.w), x1 = hiz_texel(u1, l.w);
+    const size_t r0 = (size_t)hiz_texel(v0, l.h) * l.w, r1 = (size_t)hiz_texel(v1, l.h) * l.w;
+    const float a = base[r0 + x0], b = base[r0 + x1], c = base[r1 + x0], d = base[r1 + x1];
     return rev ? wmin(wmin(a, b), wmin(c, d)) : wmax(wmax(a, b), wmax(c, d));
 }
 

@@ -114,8 +114,7 @@ __global__ __launch_bounds__(64) void k_occlusion_draws(hv_occl_cull_params prm,
                 const float uh = ndc_r * 0.5f;
                 const float u0 = wclamp(u - uh, 0.0f, 1.0f), v0 = wclamp(vv - uh, 0.0f, 1.0f);
                 const float u1 = wclamp(u + uh, 0.0f, 1.0f), v1 = wclamp(vv + uh, 0.0f, 1.0f);
-                const float h = hiz_reduce4(rev != 0u, hiz_tap(hz, mip, u0, v0), hiz_tap(hz, mip, u1, v0),
-                                            hiz_tap(hz, mip, u0, v1), hiz_tap(hz, mip, u1, v1));
+                const float h = hiz_rect4(hiz_level(hz, mip), rev != 0u, u0, v0, u1, v1);
                 constexpr float bias = 1.0f / 65536.0f;
                 occ = rev ? (near_z < h - bias) : (near_z > h + bias);
             }

@@ -1,19 +1,25 @@
 // Virtual-geometry culling for gfx950 (vg_cull.wgsl:126-405).
 //
-//   k_vg_clear   zeroes the 11 draw counters (a kernel, so the sequence can be
-//                captured in a graph).
-//   k_vg_select  one lane per object: world sphere, frustum, projected-error
-//                LOD (cs_select_objects, :282-352); the selected-LOD histogram
-//                is reduced in LDS and published with <= 9 atomics per block.
-//   k_vg_cull    one WAVE per 64-meshlet work item (a reference workgroup of
-//                64 is exactly one wave64): the work item, object, transform
-//                and InstanceCullData are wave-uniform (scalar loads), each
-//                lane loads its 64-byte descriptor with four 16-byte loads and
-//                runs cull_meshlet (:135-264). A workgroup handles 8 rounds of
-//                4 spans and keeps its survivors' outputs in registers, so ONE
-//                slot reservation (a single-address atomic, which serialises
-//                device-wide) covers 32 spans instead of one per meshlet; the
-//                capacity-overflow count is derived from the same reservation.
+//   k_vg_clear  zeroes the 11 draw counters (a kernel, so the sequence can be
+//               captured in a graph).
+//   k_vg        object selection and meshlet culling in ONE launch:
+//               - select workgroups, one lane per object (cs_select_objects,
+//                 :282-352): selected_lod_plus_one, and the selected-LOD
+//                 histogram reduced in LDS and published with <= 9 atomics;
+//               - cull workgroups, 64 work items each (cs_cull_meshlets,
+//                 :354-405): wave 0 decodes the 64 items at once (one lane
+//                 each, recomputing the object's LOD so no cull waits for a
+//                 select workgroup), compacts the live spans into LDS with the
+//                 instance transforms, then all 256 threads cull the flattened
+//                 meshlets (cull_meshlet, :135-264) with four 64-byte
+//                 descriptors in flight per thread. Survivors are staged in LDS
+//                 and ONE slot reservation (a single-address atomic, which
+//                 serialises device-wide) covers the workgroup; the
+//                 capacity-overflow count is derived from the same reservation.
+// A meshlet's work is therefore two dependent rounds of loads (work item ->
+// object and transform, then descriptor -> Hi-Z taps) instead of a chain per
+// 64-meshlet span, and partial spans (LODs smaller than max_meshlet_count, or
+// objects of fewer than 64 meshlets) share waves instead of idling lanes.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -24,6 +30,20 @@
 namespace hv {
 namespace {
 
+// The per-view uniforms (vg_cull.wgsl CullUniforms + camera). They live in a
+// small device buffer written at prepare, and each workgroup copies them to
+// LDS: as kernel arguments they would occupy ~70 SGPRs next to the buffer
+// pointers, and the spills cost a VALU readlane at every use.
+struct VgView {
+    float planes[24];  // publish_frustum_planes (vg_cull.wgsl:266-280)
+    float vp[16];
+    float p00, p11, camx, camy, camz, near, threshold;
+    uint32_t screen_h, hiz_mip_count, reversed, has_hiz;
+    dev::Pyramid hiz;
+};
+constexpr uint32_t kVgViewWords = sizeof(VgView) / 4;
+static_assert(sizeof(VgView) % 4 == 0 && kVgViewWords <= 256, "VgView is copied by one word per thread");
+
 struct VgArgs {
     const hv_meshlet_entry* meshlets;
     hv_vg_object* objects;
@@ -33,16 +53,12 @@ struct VgArgs {
     hv_draw_indexed_indirect* indirect;
     hv_vg_draw* metadata;
     uint32_t* counters;
+    const VgView* view;
     uint32_t meshlet_count, object_count_buf, instance_count, icd_count, work_item_buf_count;
     uint32_t capacity, counter_len;
-    uint32_t object_count, screen_h, hiz_mip_count, work_item_count;
-    float threshold;
-    float planes[24];  // publish_frustum_planes (vg_cull.wgsl:266-280)
-    float vp[16];
-    float p00, p11, camx, camy, camz, near;
-    uint32_t reversed;
-    uint32_t has_hiz;
-    dev::Pyramid hiz;
+    uint32_t object_count, work_item_count;
+    uint32_t select_blocks;    // leading workgroups of k_vg that select objects
+    uint32_t spans_per_block;  // work items per cull workgroup (8..64, a power of two)
 };
 
 using namespace dev;
@@ -59,76 +75,86 @@ __global__ __launch_bounds__(64) void k_vg_clear(uint32_t* counters, uint32_t n)
     if (threadIdx.x < n) counters[threadIdx.x] = 0u;
 }
 
-__global__ __launch_bounds__(256) void k_vg_select(VgArgs a) {
-    __shared__ uint32_t s_hist[8];
-    __shared__ uint32_t s_max;
-    const uint32_t tid = threadIdx.x;
-    if (tid < 8) s_hist[tid] = 0u;
-    if (tid == 0) s_max = 0u;
-    __syncthreads();
-    const uint32_t i = blockIdx.x * 256u + tid;
-    if (i < a.object_count && i < a.object_count_buf) {
-        hv_vg_object* o = a.objects + i;
-        const uint32_t ii = o->instance_index;
-        const uint32_t lod_count = min(o->lod_count, 8u);
-        uint32_t sel_plus_one = 0u;
-        if (lod_count > 0u && ii < a.instance_count && ii < a.icd_count) {
-            const hv_instance_cull_data d = a.icd[ii];
-            if (d.valid_transform != 0u) {
-                const float* M = a.instances[ii].model;
-                const float4 lb = *reinterpret_cast<const float4*>(o->local_bounds);
-                const float cx = mrow(M, 0, lb.x, lb.y, lb.z), cy = mrow(M, 1, lb.x, lb.y, lb.z),
-                            cz = mrow(M, 2, lb.x, lb.y, lb.z);
-                const float wr = wmax(lb.w * d.max_scale, 0.0f);
-                if (sphere_visible(a.planes, cx, cy, cz, wr)) {
-                    const float dx = cx - a.camx, dy = cy - a.camy, dz = cz - a.camz;
-                    const float dist = sqrtf(dot3(dx, dy, dz, dx, dy, dz));
-                    const float closest = wmax(dist - wr, wmax(a.near, 1.0e-4f));
-                    const float focal = fabsf(a.p11) * (float)a.screen_h * 0.5f;
-                    uint32_t sel = 0u;
-                    for (uint32_t level = 1u; level < lod_count; ++level) {
-                        const float err = o->lod_errors[level] * d.max_scale * focal / closest;
-                        if (err <= a.threshold) sel = level;
-                        else break;
-                    }
-                    sel_plus_one = sel + 1u;
-                    if (a.counter_len >= 11u) {
-                        atomicAdd(&s_hist[sel], 1u);
-                        atomicMax(&s_max, lod_count - 1u);
-                    }
-                }
-            }
-        }
-        o->selected_lod_plus_one = sel_plus_one;
+// LOD selection of one object (the body of cs_select_objects,
+// vg_cull.wgsl:300-351): selected_lod + 1, or 0 when the object has no LODs,
+// an invalid transform or a world sphere outside the frustum.
+// *lod_count_out receives min(lod_count, 8) for the histogram counters.
+__device__ __forceinline__ uint32_t select_lod(const VgArgs& a, const VgView& v, const hv_vg_object* o,
+                                              uint32_t* lod_count_out) {
+    const uint32_t ii = o->instance_index;
+    const uint32_t lod_count = min(o->lod_count, 8u);
+    *lod_count_out = lod_count;
+    if (lod_count == 0u || ii >= a.instance_count || ii >= a.icd_count) return 0u;
+    const hv_instance_cull_data d = a.icd[ii];
+    if (d.valid_transform == 0u) return 0u;
+    const float* M = a.instances[ii].model;
+    const float4 lb = *reinterpret_cast<const float4*>(o->local_bounds);
+    const float cx = mrow(M, 0, lb.x, lb.y, lb.z), cy = mrow(M, 1, lb.x, lb.y, lb.z), cz = mrow(M, 2, lb.x, lb.y, lb.z);
+    const float wr = wmax(lb.w * d.max_scale, 0.0f);
+    if (!sphere_visible(v.planes, cx, cy, cz, wr)) return 0u;
+    const float dx = cx - v.camx, dy = cy - v.camy, dz = cz - v.camz;
+    const float dist = sqrtf(dot3(dx, dy, dz, dx, dy, dz));
+    const float closest = wmax(dist - wr, wmax(v.near, 1.0e-4f));
+    const float focal = fabsf(v.p11) * (float)v.screen_h * 0.5f;
+    uint32_t sel = 0u;
+    for (uint32_t level = 1u; level < lod_count; ++level) {
+        const float err = o->lod_errors[level] * d.max_scale * focal / closest;
+        if (err <= v.threshold) sel = level;
+        else break;
     }
-    __syncthreads();
-    if (a.counter_len >= 11u) {
-        if (tid < 8 && s_hist[tid]) atomicAdd(&a.counters[2 + tid], s_hist[tid]);
-        if (tid == 8 && s_max) atomicMax(&a.counters[10], s_max);
-    }
+    return sel + 1u;
+}
+
+// Row r (0..2) of the instance's affine transform applied to (x, y, z, 1),
+// from the compact per-span record (model columns 0..3, rows 0..2).
+__device__ __forceinline__ float xrow(const float* xf, int r, float x, float y, float z) {
+    return ((xf[r] * x + xf[3 + r] * y) + xf[6 + r] * z) + xf[9 + r] * 1.0f;
+}
+
+// Words of a span record in LDS: model rows 0..2 of columns 0..3 (12),
+// normal matrix rows 0..2 of columns 0..2 (9), max_scale, cone_cull_enabled,
+// first meshlet index of the span.
+constexpr uint32_t kXfWords = 24;
+constexpr uint32_t kXfScale = 21, kXfCone = 22, kXfFirst = 23;
+
+// Copies N uniform words out of the LDS view.
+template <int N>
+__device__ __forceinline__ void vload(const float* src, float* dst) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) dst[i] = src[i];
 }
 
 // cull_meshlet (vg_cull.wgsl:135-264) for one lane; true = publish.
-__device__ __forceinline__ bool cull_meshlet(const VgArgs& a, uint32_t mi, uint32_t ii, const float* M,
-                                             const float* N, float max_scale, uint32_t cone_enabled, float4 q0,
-                                             float4 q1, float4 q2) {
-    const float cx = mrow(M, 0, q0.x, q0.y, q0.z), cy = mrow(M, 1, q0.x, q0.y, q0.z),
-                cz = mrow(M, 2, q0.x, q0.y, q0.z);
-    const float wr = wmax(q0.w * max_scale, 0.0f);
-    const float tcx = cx - a.camx, tcy = cy - a.camy, tcz = cz - a.camz;
-    if (!sphere_visible(a.planes, cx, cy, cz, wr)) return false;
+// The view `a` is the workgroup's LDS copy. Compiler-only fences (asm with a
+// memory clobber) keep each section's LDS reads after the previous section
+// and inside the caller's loop, so the uniforms are re-read where they are
+// used (broadcast LDS reads) instead of ~60 of them staying live in registers
+// across the workgroup loop.
+__device__ __forceinline__ bool cull_meshlet(const VgView& a, const float* xf, float4 q0, float4 q1, float4 q2) {
+    asm volatile("" ::: "memory");
+    const float cx = xrow(xf, 0, q0.x, q0.y, q0.z), cy = xrow(xf, 1, q0.x, q0.y, q0.z),
+                cz = xrow(xf, 2, q0.x, q0.y, q0.z);
+    const float wr = wmax(q0.w * xf[kXfScale], 0.0f);
+    {
+        float pl[24];
+        vload<24>(a.planes, pl);
+        if (!sphere_visible(pl, cx, cy, cz, wr)) return false;
+    }
+    asm volatile("" ::: "memory");
+    const float camx = a.camx, camy = a.camy, camz = a.camz;
+    const float tcx = cx - camx, tcy = cy - camy, tcz = cz - camz;
     // Perspective cone test (meshoptimizer form).
     const float cam_dist_sq = dot3(tcx, tcy, tcz, tcx, tcy, tcz);
     const float guard = wr * 1.5f;
-    if (cam_dist_sq > guard * guard && q1.w <= 1.0f && cone_enabled != 0u) {
-        const float ax = (N[0] * q2.x + N[4] * q2.y) + N[8] * q2.z;
-        const float ay = (N[1] * q2.x + N[5] * q2.y) + N[9] * q2.z;
-        const float az = (N[2] * q2.x + N[6] * q2.y) + N[10] * q2.z;
+    if (cam_dist_sq > guard * guard && q1.w <= 1.0f && __float_as_uint(xf[kXfCone]) != 0u) {
+        const float ax = (xf[12] * q2.x + xf[15] * q2.y) + xf[18] * q2.z;
+        const float ay = (xf[13] * q2.x + xf[16] * q2.y) + xf[19] * q2.z;
+        const float az = (xf[14] * q2.x + xf[17] * q2.y) + xf[20] * q2.z;
         const float alen = sqrtf(dot3(ax, ay, az, ax, ay, az));
         const float nax = ax / alen, nay = ay / alen, naz = az / alen;
-        const float px = mrow(M, 0, q1.x, q1.y, q1.z), py = mrow(M, 1, q1.x, q1.y, q1.z),
-                    pz = mrow(M, 2, q1.x, q1.y, q1.z);
-        const float tax = px - a.camx, tay = py - a.camy, taz = pz - a.camz;
+        const float px = xrow(xf, 0, q1.x, q1.y, q1.z), py = xrow(xf, 1, q1.x, q1.y, q1.z),
+                    pz = xrow(xf, 2, q1.x, q1.y, q1.z);
+        const float tax = px - camx, tay = py - camy, taz = pz - camz;
         const float ad2 = dot3(tax, tay, taz, tax, tay, taz);
         if (ad2 > 1.0e-12f) {
             const float s = sqrtf(ad2);
@@ -136,9 +162,13 @@ __device__ __forceinline__ bool cull_meshlet(const VgArgs& a, uint32_t mi, uint3
         }
     }
     // Conservative Hi-Z test.
-    const float w = mrow(a.vp, 3, cx, cy, cz);
-    if (a.has_hiz && w > 0.0f) {
-        const float ndx = mrow(a.vp, 0, cx, cy, cz) / w, ndy = mrow(a.vp, 1, cx, cy, cz) / w;
+    asm volatile("" ::: "memory");
+    if (a.has_hiz == 0u) return true;
+    float vp[16];
+    vload<16>(a.vp, vp);
+    const float w = mrow(vp, 3, cx, cy, cz);
+    if (w > 0.0f) {
+        const float ndx = mrow(vp, 0, cx, cy, cz) / w, ndy = mrow(vp, 1, cx, cy, cz) / w;
         const float cu = ndx * 0.5f + 0.5f, cv = ndy * -0.5f + 0.5f;
         const float nvd = w - wr;
         if (nvd > a.near) {
@@ -151,131 +181,262 @@ __device__ __forceinline__ bool cull_meshlet(const VgArgs& a, uint32_t mi, uint3
                 if (cam_dist_sq > wr * wr) {
                     const float s = sqrtf(cam_dist_sq);
                     const float nx = cx - (tcx / s) * wr, ny = cy - (tcy / s) * wr, nz = cz - (tcz / s) * wr;
-                    const float nw = mrow(a.vp, 3, nx, ny, nz);
-                    if (nw > 0.0f) near_z = wclamp(mrow(a.vp, 2, nx, ny, nz) / nw, 0.0f, 1.0f);
+                    const float nw = mrow(vp, 3, nx, ny, nz);
+                    if (nw > 0.0f) near_z = wclamp(mrow(vp, 2, nx, ny, nz) / nw, 0.0f, 1.0f);
                 }
                 const float half_h = (float)a.screen_h * 0.5f;
                 const float diameter = wmax(ndc_r * half_h * 2.0f, 1.0f);
                 uint32_t mip = ceil_log2(diameter);
                 const uint32_t mc = a.hiz_mip_count - 1u;
                 mip = mip < mc ? mip : mc;
-                const float hz = hiz_reduce4(rev, hiz_tap(a.hiz, mip, umin, vmin), hiz_tap(a.hiz, mip, umax, vmin),
-                                             hiz_tap(a.hiz, mip, umin, vmax), hiz_tap(a.hiz, mip, umax, vmax));
+                const uint32_t levels = a.hiz.levels;
+                if (mip >= levels) mip = levels - 1u;
+                HizLevel l;
+                l.w = max(a.hiz.w >> mip, 1u);
+                l.h = max(a.hiz.h >> mip, 1u);
+                l.base = (mip == 0u) ? a.hiz.level0 : a.hiz.packed + a.hiz.off[mip];
+                const float hz = hiz_rect4(l, rev, umin, vmin, umax, vmax);
                 constexpr float bias = 1.0f / 65536.0f;
                 if (rev ? (near_z < hz - bias) : (near_z > hz + bias)) return false;
             }
         }
     }
-    (void)mi;
-    (void)ii;
     return true;
 }
 
-// Spans handled per workgroup: 4 waves x kVgRounds rounds. Survivors' outputs
-// stay in registers until the workgroup's single slot reservation.
-constexpr uint32_t kVgRounds = 8;
+// At most kVgSpans work items (64-meshlet spans) are decoded per cull
+// workgroup, one per lane of wave 0; kVgUnroll flattened meshlets are in
+// flight per thread. The host lowers the spans per workgroup (down to
+// kVgMinSpans) until the grid has kVgTargetBlocks cull workgroups, so short
+// work lists still spread over every CU.
+constexpr uint32_t kVgSpans = 64;
+constexpr uint32_t kVgMinSpans = 8;
+constexpr uint32_t kVgTargetBlocks = 1024;
+constexpr uint32_t kVgUnroll = 4;
 
-__global__ __launch_bounds__(256) void k_vg_cull(VgArgs a) {
-    __shared__ uint32_t s_cnt[4];
-    __shared__ uint32_t s_base;
+struct VgLds {
+    float xf[kVgSpans][kXfWords];
+    uint32_t ii[kVgSpans];
+    uint32_t lod[kVgSpans];
+    uint32_t pref[kVgSpans + 1];      // exclusive prefix of the spans' meshlet counts; [K] = total
+    uint16_t stage[kVgSpans * 64];    // survivors: span << 6 | meshlet within the span
+    VgView v;                         // the per-view uniforms
+    uint32_t hist[8];
+    uint32_t nspans, nsurv, base, max_lod;
+};
+
+// One select workgroup: 256 objects (cs_select_objects). Writes
+// selected_lod_plus_one and the histogram counters [2..10].
+__device__ __forceinline__ void vg_select_block(const VgArgs& a, uint32_t block, VgLds& L) {
+    const uint32_t tid = threadIdx.x;
+    if (tid < 8) L.hist[tid] = 0u;
+    if (tid == 0) L.max_lod = 0u;
+    __syncthreads();
+    const VgView& v = L.v;
+    const uint32_t i = block * 256u + tid;
+    if (i < a.object_count && i < a.object_count_buf) {
+        hv_vg_object* o = a.objects + i;
+        uint32_t lc = 0;
+        const uint32_t sp1 = select_lod(a, v, o, &lc);
+        if (sp1 != 0u && a.counter_len >= 11u) {
+            atomicAdd(&L.hist[sp1 - 1u], 1u);
+            atomicMax(&L.max_lod, lc - 1u);
+        }
+        o->selected_lod_plus_one = sp1;
+    }
+    __syncthreads();
+    if (a.counter_len >= 11u) {
+        if (tid < 8 && L.hist[tid]) atomicAdd(&a.counters[2 + tid], L.hist[tid]);
+        if (tid == 8 && L.max_lod) atomicMax(&a.counters[10], L.max_lod);
+    }
+}
+
+// One cull workgroup: kVgSpans work items (cs_cull_meshlets for each).
+//   A. wave 0 decodes one work item per lane: the object's LOD (recomputed
+//      with select_lod, so the cull does not wait for the select workgroups),
+//      the span of the selected LOD, the instance's transform; live spans are
+//      compacted into LDS with their meshlet-count prefix;
+//   B. the 256 threads walk the flattened meshlets of the live spans,
+//      kVgUnroll descriptors in flight per thread, and stage survivors in LDS;
+//   C. one slot reservation for the workgroup, then the survivors' commands and
+//      metadata are written to consecutive slots.
+__device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, VgLds& L) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    uint64_t mask[kVgRounds];
-    uint32_t o_mi[kVgRounds], o_fi[kVgRounds], o_ic[kVgRounds], o_vo[kVgRounds];
-    uint32_t u_ii[kVgRounds], u_lod[kVgRounds];
-    uint32_t wave_total = 0;
-#pragma unroll
-    for (uint32_t round = 0; round < kVgRounds; ++round) {
-        const uint32_t work = (blockIdx.x * kVgRounds + round) * 4u + wave;
-        // Wave-uniform work-item decode (the lane-0 prologue of cs_cull_meshlets).
-        uint32_t first = 0, count = 0, ii = 0, lod = 0, base = 0;
-        bool item_ok = false;
-        if (work < a.work_item_count && work < a.work_item_buf_count) {
+    const VgView& v = L.v;
+    if (tid < 64u) {
+        const uint32_t work = block * a.spans_per_block + lane;
+        uint32_t first = 0, n = 0, ii = 0, lod = 0;
+        bool ok = false;
+        hv_instance_cull_data d{};
+        if (lane < a.spans_per_block && work < a.work_item_count && work < a.work_item_buf_count) {
             const hv_vg_work_item item = a.work_items[work];
             if (item.object_index < a.object_count_buf) {
                 const hv_vg_object* o = a.objects + item.object_index;
-                const uint32_t sel = o->selected_lod_plus_one;
+                uint32_t lc = 0;
+                const uint32_t sel = select_lod(a, v, o, &lc);
                 if (sel != 0u) {
                     lod = sel - 1u;
                     const uint32_t cnt = o->lod_meshlet_counts[lod];
                     if (item.local_meshlet_base < cnt) {
-                        first = o->lod_first_meshlets[lod];
-                        count = cnt;
                         ii = o->instance_index;
-                        base = item.local_meshlet_base;
-                        item_ok = true;
+                        first = o->lod_first_meshlets[lod] + item.local_meshlet_base;
+                        n = min(cnt - item.local_meshlet_base, 64u);
+                        d = a.icd[ii];  // select_lod checked ii against both arrays
+                        ok = d.valid_transform != 0u;
                     }
                 }
             }
         }
-        bool visible = false;
-        uint32_t mi = 0;
-        uint4 q3 = make_uint4(0, 0, 0, 0);
-        if (item_ok && base + lane < count) {
-            mi = first + base + lane;
-            if (mi < a.meshlet_count && ii < a.instance_count && ii < a.icd_count) {
-                const hv_instance_cull_data d = a.icd[ii];
-                if (d.valid_transform != 0u) {
-                    const float4* mp = reinterpret_cast<const float4*>(a.meshlets + mi);
-                    const float4 q0 = mp[0], q1 = mp[1], q2 = mp[2];
-                    q3 = reinterpret_cast<const uint4*>(mp)[3];
-                    const hv_instance_data& inst = a.instances[ii];
-                    visible = cull_meshlet(a, mi, ii, inst.model, inst.normal_mat, d.max_scale, d.cone_cull_enabled,
-                                           q0, q1, q2);
-                }
+        const uint64_t m = __ballot(ok);
+        const uint32_t k = mbcnt(m);
+        uint32_t v = ok ? n : 0u;
+#pragma unroll
+        for (uint32_t off = 1; off < 64u; off <<= 1) {
+            const uint32_t t = __shfl_up(v, off);
+            if (lane >= off) v += t;
+        }
+        if (ok) {
+            L.pref[k] = v - n;
+            const float* M = a.instances[ii].model;
+            const float* N = a.instances[ii].normal_mat;
+            float* x = L.xf[k];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) x[3 * c + r] = M[4 * c + r];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) x[12 + 3 * c + r] = N[4 * c + r];
+            x[kXfScale] = d.max_scale;
+            x[kXfCone] = __uint_as_float(d.cone_cull_enabled);
+            x[kXfFirst] = __uint_as_float(first);
+            L.ii[k] = ii;
+            L.lod[k] = lod;
+        }
+        if (lane == 63u) {
+            const uint32_t K = (uint32_t)__popcll(m);
+            L.nspans = K;
+            L.pref[K] = v;
+            L.nsurv = 0u;
+        }
+    }
+    __syncthreads();
+    const uint32_t K = L.nspans, T = L.pref[K];
+    // Flattened meshlet f -> (span, meshlet within the span, meshlet index),
+    // and the descriptor fetched one step ahead of the cull that uses it.
+    auto locate = [&](uint32_t f, uint32_t& k, uint32_t& li) -> uint32_t {
+        k = 0;
+#pragma unroll
+        for (uint32_t step = 32; step; step >>= 1)
+            if (k + step < K && L.pref[k + step] <= f) k += step;  // largest k with pref[k] <= f
+        li = f - L.pref[k];
+        return __float_as_uint(L.xf[k][kXfFirst]) + li;
+    };
+    uint32_t nk = 0, nli = 0;
+    bool nact = false;
+    float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0, n2 = n0;
+    if (tid < T) {
+        const uint32_t mi = locate(tid, nk, nli);
+        nact = mi < a.meshlet_count;
+        if (nact) {
+            const float4* mp = reinterpret_cast<const float4*>(a.meshlets + mi);
+            n0 = mp[0];
+            n1 = mp[1];
+            n2 = mp[2];
+        }
+    }
+#pragma unroll 1
+    for (uint32_t f0 = 0; f0 < T; f0 += 256u) {
+        const uint32_t ck = nk, cli = nli;
+        const bool cact = nact;
+        const float4 c0 = n0, c1 = n1, c2 = n2;
+        const uint32_t f = f0 + 256u + tid;
+        nact = false;
+        if (f < T) {
+            const uint32_t mi = locate(f, nk, nli);
+            nact = mi < a.meshlet_count;
+            if (nact) {
+                const float4* mp = reinterpret_cast<const float4*>(a.meshlets + mi);
+                n0 = mp[0];
+                n1 = mp[1];
+                n2 = mp[2];
             }
         }
-        mask[round] = __ballot(visible);
-        wave_total += (uint32_t)__popcll(mask[round]);
-        o_mi[round] = mi;
-        o_fi[round] = q3.x;
-        o_ic[round] = q3.y;
-        o_vo[round] = q3.z;
-        u_ii[round] = ii;
-        u_lod[round] = lod;
+        const bool vis = cact && cull_meshlet(v, L.xf[ck], c0, c1, c2);
+        const uint64_t m = __ballot(vis);
+        if (m) {
+            uint32_t wb = 0;
+            if (lane == 0) wb = atomicAdd(&L.nsurv, (uint32_t)__popcll(m));
+            wb = __shfl(wb, 0);
+            if (vis) L.stage[wb + mbcnt(m)] = (uint16_t)((ck << 6) | cli);
+        }
     }
-    if (lane == 0) s_cnt[wave] = wave_total;
     __syncthreads();
     if (tid == 0) {
-        uint32_t tot = 0;
-        for (int w = 0; w < 4; ++w) {
-            const uint32_t c = s_cnt[w];
-            s_cnt[w] = tot;
-            tot += c;
-        }
+        const uint32_t tot = L.nsurv;
         uint32_t b = 0;
         if (tot) {
             b = atomicAdd(&a.counters[0], tot);
             const uint32_t end = b + tot, keep_from = max(b, a.capacity);
             if (end > keep_from) atomicAdd(&a.counters[1], end - keep_from);
         }
-        s_base = b;
+        L.base = b;
     }
     __syncthreads();
-    uint32_t slot0 = s_base + s_cnt[wave];
+    const uint32_t tot = L.nsurv, b = L.base;
+    // Survivors below the capacity: the descriptors' draw fields are loaded
+    // kVgUnroll at a time before any store, so the reloads overlap.
+    const uint32_t keep = b < a.capacity ? min(tot, a.capacity - b) : 0u;
+    for (uint32_t e0 = 0; e0 < keep; e0 += 256u * kVgUnroll) {
+        uint4 q3[kVgUnroll];
+        uint32_t mi[kVgUnroll], kk[kVgUnroll];
 #pragma unroll
-    for (uint32_t round = 0; round < kVgRounds; ++round) {
-        const uint64_t m = mask[round];
-        if ((m >> lane) & 1ull) {
-            const uint32_t slot = slot0 + mbcnt(m);
-            if (slot < a.capacity) {
+        for (uint32_t u = 0; u < kVgUnroll; ++u) {
+            const uint32_t e = e0 + u * 256u + tid;
+            kk[u] = 0;
+            mi[u] = 0;
+            q3[u] = make_uint4(0, 0, 0, 0);
+            if (e < keep) {
+                const uint32_t en = L.stage[e];
+                kk[u] = en >> 6;
+                mi[u] = __float_as_uint(L.xf[kk[u]][kXfFirst]) + (en & 63u);
+                q3[u] = reinterpret_cast<const uint4*>(a.meshlets + mi[u])[3];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kVgUnroll; ++u) {
+            const uint32_t e = e0 + u * 256u + tid;
+            if (e < keep) {
+                const uint32_t slot = b + e;
                 hv_draw_indexed_indirect cmd;
-                cmd.index_count = o_ic[round];
+                cmd.index_count = q3[u].y;
                 cmd.instance_count = 1u;
-                cmd.first_index = o_fi[round];
-                cmd.base_vertex = (int32_t)o_vo[round];
+                cmd.first_index = q3[u].x;
+                cmd.base_vertex = (int32_t)q3[u].z;
                 cmd.first_instance = slot;
                 a.indirect[slot] = cmd;
                 hv_vg_draw md;
-                md.instance_index = u_ii[round];
-                md.meshlet_index = o_mi[round];
-                md.lod_level = u_lod[round];
+                md.instance_index = L.ii[kk[u]];
+                md.meshlet_index = mi[u];
+                md.lod_level = L.lod[kk[u]];
                 md.reserved = 0u;
                 a.metadata[slot] = md;
             }
         }
-        slot0 += (uint32_t)__popcll(m);
     }
+}
+
+// Select and cull in one launch: blocks [0, select_blocks) select objects,
+// the rest cull work items. The cull recomputes each object's LOD with the
+// same select_lod, so the two halves do not depend on each other.
+__global__ __launch_bounds__(256) void k_vg(VgArgs a) {
+    __shared__ VgLds L;
+    if (threadIdx.x < kVgViewWords)
+        reinterpret_cast<uint32_t*>(&L.v)[threadIdx.x] = reinterpret_cast<const uint32_t*>(a.view)[threadIdx.x];
+    __syncthreads();
+    if (blockIdx.x < a.select_blocks) vg_select_block(a, blockIdx.x, L);
+    else vg_cull_block(a, blockIdx.x - a.select_blocks, L);
 }
 
 }  // namespace
@@ -286,7 +447,7 @@ using namespace hv;
 struct hv_vg_pass {
     hv_ctx* ctx = nullptr;
     uint32_t publication_limit = 0;
-    DevBuf meshlets, objects, instances, icd, work_items, indirect, metadata, counters;
+    DevBuf meshlets, objects, instances, icd, work_items, indirect, metadata, counters, view;
     uint32_t meshlet_cap = 0, object_cap = 0, instance_cap = 0, work_cap = 0, indirect_cap = 0;
     uint64_t last_version = ~0ull, last_instance_version = ~0ull;
     uint32_t meshlet_count = 0, object_count = 0, work_item_count = 0, max_draw = 0;
@@ -338,7 +499,7 @@ void hv_vg_destroy(hv_vg_pass* p) {
         if (e) (void)hipEventDestroy(e);
     if (p->done) (void)hipEventDestroy(p->done);
     for (DevBuf* b : {&p->meshlets, &p->objects, &p->instances, &p->icd, &p->work_items, &p->indirect, &p->metadata,
-                      &p->counters})
+                      &p->counters, &p->view})
         b->release();
     delete p;
 }
@@ -427,10 +588,11 @@ hv_status hv_vg_prepare(hv_vg_pass* p, const hv_vg_frame_data* vg, const hv_came
     a.capacity = std::min(p->max_draw, p->indirect_cap);
     a.counter_len = HV_VG_COUNTER_COUNT;
     a.object_count = p->object_count;
-    a.screen_h = height;
-    a.hiz_mip_count = hv_mip_levels(width, height);
     a.work_item_count = p->work_item_count;
-    a.threshold = threshold;
+    VgView v{};
+    v.screen_h = height;
+    v.hiz_mip_count = hv_mip_levels(width, height);
+    v.threshold = threshold;
     // publish_frustum_planes (vg_cull.wgsl:266-280): rows of view_proj, p / length(p.xyz), no guard.
     const float* m = cam->view_proj;
     const float pl[6][4] = {
@@ -443,25 +605,32 @@ hv_status hv_vg_prepare(hv_vg_pass* p, const hv_vg_frame_data* vg, const hv_came
     };
     for (int i = 0; i < 6; ++i) {
         const float len = std::sqrt((pl[i][0] * pl[i][0] + pl[i][1] * pl[i][1]) + pl[i][2] * pl[i][2]);
-        for (int c = 0; c < 4; ++c) a.planes[4 * i + c] = pl[i][c] / len;
+        for (int c = 0; c < 4; ++c) v.planes[4 * i + c] = pl[i][c] / len;
     }
-    std::memcpy(a.vp, cam->view_proj, 64);
-    a.p00 = cam->proj[0];
-    a.p11 = cam->proj[5];
-    a.camx = cam->position_near[0];
-    a.camy = cam->position_near[1];
-    a.camz = cam->position_near[2];
-    a.near = cam->position_near[3];
-    a.reversed = (flags & HV_CULL_REVERSED_Z) ? 1u : 0u;
+    std::memcpy(v.vp, cam->view_proj, 64);
+    v.p00 = cam->proj[0];
+    v.p11 = cam->proj[5];
+    v.camx = cam->position_near[0];
+    v.camy = cam->position_near[1];
+    v.camz = cam->position_near[2];
+    v.near = cam->position_near[3];
+    v.reversed = (flags & HV_CULL_REVERSED_Z) ? 1u : 0u;
     if (hiz) {
-        a.has_hiz = 1u;
-        a.hiz.level0 = hiz->level0;
-        a.hiz.packed = hiz->packed.as<float>();
-        a.hiz.w = hiz->width;
-        a.hiz.h = hiz->height;
-        a.hiz.levels = hiz->levels;
-        for (uint32_t k = 1; k < hiz->levels && k < 12; ++k) a.hiz.off[k] = (uint32_t)hiz->level_off[k];
+        v.has_hiz = 1u;
+        v.hiz.level0 = hiz->level0;
+        v.hiz.packed = hiz->packed.as<float>();
+        v.hiz.w = hiz->width;
+        v.hiz.h = hiz->height;
+        v.hiz.levels = hiz->levels;
+        for (uint32_t k = 1; k < hiz->levels && k < 12; ++k) v.hiz.off[k] = (uint32_t)hiz->level_off[k];
     }
+    // The previous execute has finished (done event above), so the uniforms
+    // can be replaced; the copy completes before prepare returns, so an
+    // execute on any stream sees them.
+    HV_TRY(p->view.reserve(sizeof(VgView), false, st));
+    HV_HIP(hipMemcpyAsync(p->view.ptr, &v, sizeof(VgView), hipMemcpyHostToDevice, st));
+    HV_HIP(hipStreamSynchronize(st));
+    a.view = p->view.as<VgView>();
     p->prepared = true;
     return HV_OK;
 }
@@ -476,17 +645,20 @@ static hv_status vg_launch(hv_vg_pass* p, hipStream_t s, float* ms) {
     }
     if (ms && !p->ev[0])
         for (auto& e : p->ev) HV_HIP(hipEventCreate(&e));
+    if (ms) HV_HIP(hipEventRecord(p->ev[0], s));
     hipLaunchKernelGGL(k_vg_clear, dim3(1), dim3(64), 0, s, a.counters, a.counter_len);
     HV_HIP(hipGetLastError());
-    if (ms) HV_HIP(hipEventRecord(p->ev[0], s));
-    if (p->object_count) {
-        hipLaunchKernelGGL(k_vg_select, dim3((p->object_count + 255) / 256), dim3(256), 0, s, a);
-        HV_HIP(hipGetLastError());
-    }
     if (ms) HV_HIP(hipEventRecord(p->ev[1], s));
-    if (p->work_item_count) {
-        const uint32_t per_block = 4u * kVgRounds;
-        hipLaunchKernelGGL(k_vg_cull, dim3((p->work_item_count + per_block - 1) / per_block), dim3(256), 0, s, a);
+    {
+        a.select_blocks = (p->object_count + 255u) / 256u;
+        uint32_t spans = kVgSpans;
+        while (spans > kVgMinSpans && ((uint64_t)p->work_item_count + spans - 1) / spans < kVgTargetBlocks) spans >>= 1;
+        a.spans_per_block = spans;
+        const uint64_t cull_blocks = ((uint64_t)p->work_item_count + spans - 1) / spans;
+        const uint64_t grid = a.select_blocks + cull_blocks;
+        if (grid > 0x7fffffffull) return set_error(HV_ELIMIT, "hv_vg_execute: %llu workgroups exceed the grid limit",
+                                                   (unsigned long long)grid);
+        hipLaunchKernelGGL(k_vg, dim3((uint32_t)grid), dim3(256), 0, s, a);
         HV_HIP(hipGetLastError());
     }
     HV_HIP(hipEventRecord(p->done, s));

@@ -455,9 +455,10 @@ hv_status hv_vg_prepare(hv_vg_pass* pass, const hv_vg_frame_data* vg, const hv_c
                         uint32_t width, uint32_t height, float lod_error_threshold_px, const hv_hiz* hiz,
                         uint32_t flags);
 /* VirtualGeometryPass::execute (rendering.rs:1031-1165): clear counters,
- * select objects, cull meshlets. */
+ * select objects, cull meshlets (selection and cull share one launch). */
 hv_status hv_vg_execute(hv_vg_pass* pass, void* stream);
-/* Execute with HIP events: ms[0] object select, ms[1] meshlet cull. Synchronises. */
+/* Execute with HIP events: ms[0] counter clear, ms[1] object select + meshlet
+ * cull (one launch). Synchronises. */
 hv_status hv_vg_execute_timed(hv_vg_pass* pass, void* stream, float ms[2]);
 hv_status hv_vg_outputs(const hv_vg_pass* pass, hv_vg_buffers* out);
 /* Blocking readback; any pointer may be NULL. selected: one u32 per object
