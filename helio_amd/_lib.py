@@ -118,6 +118,20 @@ class ServerStats(C.Structure):
     ]
 
 
+class ShadowMatrixBuffers(C.Structure):
+    _fields_ = [
+        ("lights", vp), ("shadow_matrices", vp), ("camera", vp), ("shadow_dirty", vp), ("shadow_hashes", vp),
+        ("light_count", u32), ("matrix_capacity", u32), ("slot_capacity", u32),
+    ]
+
+
+class ShadowCullBuffers(C.Structure):
+    _fields_ = [
+        ("shadow_matrices", vp), ("instances", vp), ("src_indirect", vp), ("face_dirty", vp),
+        ("draw_count", u32), ("face_count", u32), ("instance_capacity", u32),
+    ]
+
+
 class TuneResult(C.Structure):
     _fields_ = [("wg", u32), ("ipt", u32), ("classify_ms", f32), ("total_ms", f32)]
 
@@ -192,6 +206,14 @@ _SIGS: dict[str, tuple] = {
     "hv_occlusion_cull_prepare": (i32, [vp, vp, u32, u64]),
     "hv_occlusion_cull_params": (i32, [vp, vp]),
     "hv_occlusion_cull_execute": (i32, [vp, P(DrawBuffers), vp, vp]),
+    "hv_shadow_matrix_create": (i32, [vp, u32, P(vp)]),
+    "hv_shadow_matrix_destroy": (None, [vp]),
+    "hv_shadow_matrix_execute": (i32, [vp, P(ShadowMatrixBuffers), vp]),
+    "hv_shadow_cull_create": (i32, [vp, u32, u32, P(vp)]),
+    "hv_shadow_cull_destroy": (None, [vp]),
+    "hv_shadow_cull_execute": (i32, [vp, P(ShadowCullBuffers), vp]),
+    "hv_shadow_cull_outputs": (i32, [vp, P(vp), P(vp)]),
+    "hv_shadow_cull_read": (i32, [vp, vp, vp]),
     "hv_server_create": (i32, [vp, P(ServerConfig), P(vp)]),
     "hv_server_destroy": (None, [vp]),
     "hv_server_submit": (i32, [vp, P(ServerRequest), P(u64)]),

@@ -131,6 +131,23 @@ HIZ_UNIFORMS = np.dtype([("src_size", "<u4", 2), ("dst_size", "<u4", 2)])
 # hv_mesh_lod: (index_count, first_index, base_vertex) of one (mesh, LOD) bucket.
 MESH_LOD = np.dtype([("index_count", "<u4"), ("first_index", "<u4"), ("base_vertex", "<i4")])
 
+# GpuLight, crates/libhelio/src/light.rs:29-45 (64 bytes).
+LIGHT = np.dtype([
+    ("position_range", "<f4", 4),
+    ("direction_outer", "<f4", 4),
+    ("color_intensity", "<f4", 4),
+    ("shadow_index", "<u4"),
+    ("light_type", "<u4"),
+    ("inner_angle", "<f4"),
+    ("_pad", "<u4"),
+])
+LIGHT_DIRECTIONAL, LIGHT_POINT, LIGHT_SPOT = 0, 1, 2
+SHADOW_NO_SHADOW = 0xFFFFFFFF
+SHADOW_MAX_FACES = 256
+SHADOW_MAX_DRAWS_PER_FACE = 4096
+SHADOW_PLANES_REFERENCE = 0  # shadow_cull.wgsl's column vectors
+SHADOW_PLANES_ROWS = 1       # Gribb-Hartmann rows
+
 STAT_NAMES = (
     "total", "frustum_culled", "subpixel_culled", "frustum_visible",
     "occlusion_culled", "shadow_total", "shadow_frustum_visible", "shadow_occlusion_culled",

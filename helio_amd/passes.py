@@ -12,8 +12,8 @@ import ctypes as C
 import numpy as np
 
 from . import layouts as L
-from ._lib import (CullConfig, CullOutputs, DrawBuffers, ServerConfig, ServerRequest, ServerStats, VgFrameData,
-                   VgOutputs, ViewRequest, call, load, ptr)
+from ._lib import (CullConfig, CullOutputs, DrawBuffers, ServerConfig, ServerRequest, ServerStats, ShadowCullBuffers,
+                   ShadowMatrixBuffers, VgFrameData, VgOutputs, ViewRequest, call, load, ptr)
 
 
 class Context:
@@ -617,3 +617,82 @@ class CullServer:
         call("hv_server_run_trace", self.handle, arr, a.ctypes.data_as(C.POINTER(C.c_double)), n,
              lat.ctypes.data_as(C.POINTER(C.c_double)), C.byref(wall))
         return lat[:n], wall.value
+
+
+class ShadowMatrixPass:
+    """ShadowMatrixPass (crates/helio-pass-shadow-matrix/src/lib.rs): light-space
+    view-projections of every shadow-casting light, written on the device."""
+
+    def __init__(self, ctx: Context, shadow_atlas_size: int = 4096):
+        self.ctx = ctx
+        h = C.c_void_p()
+        call("hv_shadow_matrix_create", ctx.handle, shadow_atlas_size, C.byref(h))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            load().hv_shadow_matrix_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def execute(self, lights: DeviceBuffer, light_count: int, matrices: DeviceBuffer, camera: DeviceBuffer,
+                dirty: DeviceBuffer, hashes: DeviceBuffer, stream: int | None = None) -> None:
+        b = ShadowMatrixBuffers()
+        b.lights, b.shadow_matrices, b.camera = lights.ptr, matrices.ptr, camera.ptr
+        b.shadow_dirty, b.shadow_hashes = dirty.ptr, hashes.ptr
+        b.light_count = light_count
+        b.matrix_capacity = matrices.nbytes // 64
+        b.slot_capacity = min(dirty.nbytes, hashes.nbytes) // 4
+        call("hv_shadow_matrix_execute", self.handle, C.byref(b), stream)
+
+
+class ShadowCullPass:
+    """ShadowCullPass (crates/helio-pass-shadow-cull/src/lib.rs): per dirty shadow
+    face, the movable draws whose instance sphere passes the face's frustum,
+    appended to that face's indirect list."""
+
+    def __init__(self, ctx: Context, max_draws_per_face: int = L.SHADOW_MAX_DRAWS_PER_FACE,
+                 plane_mode: int = L.SHADOW_PLANES_REFERENCE):
+        self.ctx = ctx
+        self.max_draws_per_face = max_draws_per_face
+        h = C.c_void_p()
+        call("hv_shadow_cull_create", ctx.handle, max_draws_per_face, plane_mode, C.byref(h))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            load().hv_shadow_cull_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def execute(self, matrices: DeviceBuffer, instances: DeviceBuffer, instance_capacity: int,
+                src_indirect: DeviceBuffer, draw_count: int, face_dirty: DeviceBuffer, face_count: int,
+                stream: int | None = None) -> None:
+        b = ShadowCullBuffers()
+        b.shadow_matrices, b.instances, b.src_indirect, b.face_dirty = (
+            matrices.ptr, instances.ptr, src_indirect.ptr, face_dirty.ptr)
+        b.draw_count, b.face_count, b.instance_capacity = draw_count, face_count, instance_capacity
+        call("hv_shadow_cull_execute", self.handle, C.byref(b), stream)
+
+    def outputs(self) -> tuple[int, int]:
+        ind, cnt = C.c_void_p(), C.c_void_p()
+        call("hv_shadow_cull_outputs", self.handle, C.byref(ind), C.byref(cnt))
+        return ind.value, cnt.value
+
+    def read(self, with_commands: bool = True):
+        """(counts[256], commands[256, max_draws_per_face] or None), blocking."""
+        counts = np.zeros(L.SHADOW_MAX_FACES, np.uint32)
+        cmds = np.zeros((L.SHADOW_MAX_FACES, self.max_draws_per_face), L.DRAW_INDEXED_INDIRECT) if with_commands \
+            else None
+        call("hv_shadow_cull_read", self.handle, ptr(counts), None if cmds is None else ptr(cmds))
+        return counts, cmds

@@ -438,3 +438,76 @@ def vg_workload(n_objects: int = 8192, config: int = 2, index: int = 0, reversed
     occ[:, :3] = centers - sz / 2
     occ[:, 3:] = centers + sz / 2
     return view, sc, rasterize_depth(view, occ, width, height, reversed_z, threads)
+
+
+# ---------------------------------------------------------------------------
+# Shadow-casting lights and movable shadow draws (multi-view producers).
+# ---------------------------------------------------------------------------
+
+
+def shadow_lights(g: np.random.Generator, n_point: int = 4, n_spot: int = 4, n_directional: int = 1,
+                  n_unshadowed: int = 1) -> np.ndarray:
+    """GpuLight records: point lights (range 5-60 m), spot lights (outer cone
+    10-85 deg, range 10-80 m), directional lights (random sun directions) in
+    the 1 km cube, each shadowed light owning 6 consecutive shadow matrices;
+    plus lights without a shadow. Light order is shuffled."""
+    kinds = [L.LIGHT_POINT] * n_point + [L.LIGHT_SPOT] * n_spot + [L.LIGHT_DIRECTIONAL] * n_directional
+    kinds += [None] * n_unshadowed
+    order = g.permutation(len(kinds))
+    out = np.zeros(len(kinds), L.LIGHT)
+    slot = 0
+    for i, k in enumerate(order):
+        kind = kinds[k]
+        d = g.normal(size=3)
+        d /= np.linalg.norm(d)
+        rec = out[i]
+        rec["color_intensity"] = [1.0, 0.9, 0.8, float(g.uniform(1, 100))]
+        if kind is None:
+            rec["light_type"] = L.LIGHT_POINT
+            rec["shadow_index"] = L.SHADOW_NO_SHADOW
+            rec["position_range"] = [*g.uniform(0, CUBE, 3), 10.0]
+            continue
+        rec["light_type"] = kind
+        rec["shadow_index"] = slot * 6
+        slot += 1
+        if kind == L.LIGHT_DIRECTIONAL:
+            d[1] = -abs(d[1]) - 0.2
+            d /= np.linalg.norm(d)
+            rec["position_range"] = [0.0, 0.0, 0.0, 1e30]
+            rec["direction_outer"] = [*d, 0.0]
+        elif kind == L.LIGHT_POINT:
+            rec["position_range"] = [*g.uniform(0, CUBE, 3), g.uniform(5, 60)]
+            rec["direction_outer"] = [0.0, -1.0, 0.0, 0.0]
+        else:
+            outer = math.radians(g.uniform(10, 85))
+            rec["position_range"] = [*g.uniform(0, CUBE, 3), g.uniform(10, 80)]
+            rec["direction_outer"] = [*d, math.cos(outer)]
+            rec["inner_angle"] = math.cos(outer * 0.8)
+    return out
+
+
+def shadow_draws(g: np.random.Generator, n_instances: int, n_draws: int, centers=None):
+    """Instances (GpuInstanceData, bounds = seeded spheres in the cube or around
+    `centers`) and movable shadow draws (DrawIndexedIndirect whose
+    first_instance picks the instance)."""
+    inst = np.zeros(n_instances, L.INSTANCE)
+    if centers is None:
+        c = g.uniform(0, CUBE, (n_instances, 3))
+    else:
+        centers = np.asarray(centers, np.float64)
+        c = centers[g.integers(0, len(centers), n_instances)] + g.normal(0, 25.0, (n_instances, 3))
+    r = np.exp(g.uniform(math.log(0.1), math.log(10.0), n_instances))
+    inst["bounds"][:, :3] = c
+    inst["bounds"][:, 3] = r
+    m = np.zeros(16, np.float32)
+    m[[0, 5, 10, 15]] = 1
+    inst["model"] = m
+    inst["mesh_id"] = g.integers(0, 256, n_instances)
+    inst["flags"] = 1
+    draws = np.zeros(n_draws, L.DRAW_INDEXED_INDIRECT)
+    draws["index_count"] = g.integers(3, 30000, n_draws)
+    draws["instance_count"] = 1
+    draws["first_index"] = g.integers(0, 1 << 24, n_draws)
+    draws["base_vertex"] = g.integers(-1000, 1 << 20, n_draws)
+    draws["first_instance"] = g.integers(0, n_instances, n_draws)
+    return inst, draws

@@ -522,6 +522,97 @@ hv_status hv_occlusion_cull_execute(hv_occlusion_cull_pass* pass, const hv_draw_
                                     void* stream);
 
 /* ------------------------------------------------------------------------- */
+/* Multi-view producers: light-space view-projections for shadow faces       */
+/* (ShadowMatrixPass) and per-face culling of the movable shadow draws       */
+/* (ShadowCullPass), over device buffers owned by the caller.                */
+/* ------------------------------------------------------------------------- */
+
+/* GpuLight, crates/libhelio/src/light.rs:29-45. 64 bytes. */
+typedef struct hv_light {
+    float position_range[4];  /* @0  xyz position, w range */
+    float direction_outer[4]; /* @16 xyz direction, w cos(outer angle) */
+    float color_intensity[4]; /* @32 */
+    uint32_t shadow_index;    /* @48 first of 6 shadow matrices; 0xFFFFFFFF = no shadow */
+    uint32_t light_type;      /* @52 HV_LIGHT_* */
+    float inner_angle;        /* @56 */
+    uint32_t _pad;            /* @60 */
+} hv_light;
+
+#define HV_LIGHT_DIRECTIONAL 0u
+#define HV_LIGHT_POINT 1u
+#define HV_LIGHT_SPOT 2u
+#define HV_SHADOW_NO_SHADOW 0xFFFFFFFFu
+#define HV_SHADOW_FACES_PER_LIGHT 6u
+#define HV_SHADOW_MAX_FACES 256u           /* MAX_FACES, shadow_cull.wgsl:5 */
+#define HV_SHADOW_MAX_DRAWS_PER_FACE 4096u /* crates/helio-pass-shadow-cull/src/lib.rs:41 */
+
+typedef struct hv_shadow_matrix_buffers {
+    const hv_light* lights;            /* light_count entries */
+    float* shadow_matrices;            /* GpuShadowMatrix (float[16] column-major) x matrix_capacity */
+    const hv_camera_uniforms* camera;  /* the main camera (CSM cascades) */
+    uint32_t* shadow_dirty;            /* one flag per caster slot (shadow_index / 6) */
+    uint32_t* shadow_hashes;           /* one FNV-1a hash per caster slot */
+    uint32_t light_count;              /* lights processed (the reference's movable light count) */
+    uint32_t matrix_capacity;          /* matrices in shadow_matrices */
+    uint32_t slot_capacity;            /* entries in shadow_dirty and shadow_hashes */
+} hv_shadow_matrix_buffers;
+
+typedef struct hv_shadow_matrix_pass hv_shadow_matrix_pass;
+
+/* ShadowMatrixPass (crates/helio-pass-shadow-matrix/src/lib.rs; shadow_matrices.wgsl:262-315):
+ * one lane per light writes 6 cube-face view-projections (point), 4 CSM
+ * cascades + 2 identities (directional) or 1 perspective (spot) at
+ * shadow_index, hashes the light's 6 matrices (FNV-1a over their bytes) and
+ * sets the caster slot's dirty flag when the hash changed. Lights whose
+ * matrices or slot fall outside the given capacities are skipped (the
+ * reference would address past its buffers). shadow_atlas_size 0 is taken as 1
+ * (ShadowMatrixPass::new). tan and acos are this library's canonical
+ * binary32 definitions (WGSL leaves their precision to the implementation). */
+hv_status hv_shadow_matrix_create(hv_ctx* ctx, uint32_t shadow_atlas_size, hv_shadow_matrix_pass** out);
+void hv_shadow_matrix_destroy(hv_shadow_matrix_pass* pass);
+/* ShadowMatrixPass::execute: no dispatch when light_count is 0. */
+hv_status hv_shadow_matrix_execute(hv_shadow_matrix_pass* pass, const hv_shadow_matrix_buffers* buffers,
+                                   void* stream);
+
+/* Plane rule of the per-face sphere test. */
+#define HV_SHADOW_PLANES_REFERENCE 0u /* shadow_cull.wgsl:61-73 as written: vp[3] +/- vp[i] are COLUMNS
+                                         of the column-major matrix (the planes of its transpose) */
+#define HV_SHADOW_PLANES_ROWS 1u      /* Gribb-Hartmann rows, as extract_frustum_planes
+                                         (crates/helio-pass-indirect-dispatch/src/lib.rs:271-300) */
+
+typedef struct hv_shadow_cull_buffers {
+    const float* shadow_matrices;              /* HV_SHADOW_MAX_FACES float[16] view-projections */
+    const hv_instance_data* instances;         /* indexed by draw.first_instance */
+    const hv_draw_indexed_indirect* src_indirect; /* the movable shadow draws */
+    const uint32_t* face_dirty;                /* HV_SHADOW_MAX_FACES flags; 0 skips the face */
+    uint32_t draw_count;                       /* the reference's shadow_movable_draw_count */
+    uint32_t face_count;                       /* the reference's shadow_count; 0 = no dispatch */
+    uint32_t instance_capacity;                /* a first_instance beyond it reads zeros (radius 0: skipped) */
+} hv_shadow_cull_buffers;
+
+typedef struct hv_shadow_cull_pass hv_shadow_cull_pass;
+
+/* ShadowCullPass (crates/helio-pass-shadow-cull/src/lib.rs; shadow_cull.wgsl:75-96):
+ * face_counts cleared, then every draw whose instance sphere (r > 0) passes a
+ * dirty face's frustum is appended to that face's list
+ * (face_indirect[face * max_draws_per_face + slot]); face_counts[face] counts
+ * every visible draw, also past max_draws_per_face (those are not written).
+ * Slot order inside a face is unspecified, as with the reference's atomic. */
+hv_status hv_shadow_cull_create(hv_ctx* ctx, uint32_t max_draws_per_face, uint32_t plane_mode,
+                                hv_shadow_cull_pass** out);
+void hv_shadow_cull_destroy(hv_shadow_cull_pass* pass);
+hv_status hv_shadow_cull_execute(hv_shadow_cull_pass* pass, const hv_shadow_cull_buffers* buffers, void* stream);
+/* Device outputs: HV_SHADOW_MAX_FACES * max_draws_per_face commands and
+ * HV_SHADOW_MAX_FACES counts (directly usable as multi_draw_indexed_indirect_count
+ * count buffers, one per face). */
+hv_status hv_shadow_cull_outputs(const hv_shadow_cull_pass* pass, hv_draw_indexed_indirect** face_indirect,
+                                 uint32_t** face_counts);
+/* Blocking readback of the counts and of each face's written commands
+ * (face_indirect: HV_SHADOW_MAX_FACES * max_draws_per_face entries, either may be NULL). */
+hv_status hv_shadow_cull_read(hv_shadow_cull_pass* pass, uint32_t counts[HV_SHADOW_MAX_FACES],
+                              hv_draw_indexed_indirect* face_indirect);
+
+/* ------------------------------------------------------------------------- */
 /* Serving loop: resident scenes, cull requests admitted by device memory,   */
 /* continuously batched into graph-replayed launches on several streams.     */
 /* No reference counterpart (the reference renders one frame at a time); it  */

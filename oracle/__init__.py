@@ -76,6 +76,11 @@ _SIGS = {
     "or_instance_cull_data": (None, [vp, C.POINTER(f32), C.POINTER(f32), C.POINTER(u32), C.POINTER(u32)]),
     "or_select_object_lod": (u32, [vp, u32, f32, f32, f32, f32]),
     "or_vg_cull": (None, [C.POINTER(OrVgInputs), vp, vp, vp]),
+    "or_tanf": (f32, [f32]),
+    "or_acosf": (f32, [f32]),
+    "or_fnv_mat": (u32, [vp]),
+    "or_shadow_matrices": (None, [vp, u32, vp, u32, vp, u32, vp, vp, u32]),
+    "or_shadow_cull": (None, [vp, vp, u32, vp, u32, vp, u32, u32, vp, vp]),
 }
 
 _lib = None
@@ -174,3 +179,30 @@ def classify(view: OrView, bounds4, meta, first: int, count: int, pyr=None, aabb
     lib().or_classify(C.byref(view), _p(b), _p(m), _p(a), first, count,
                       None if pyr is None else C.addressof(pyr), _p(code), _p(stats))
     return code[:count], stats
+
+
+def shadow_matrices(lights: np.ndarray, camera: np.ndarray, atlas_size: int, mats: np.ndarray,
+                    dirty: np.ndarray, hashes: np.ndarray):
+    """compute_shadow_matrices over `lights` (in place on copies): (mats, dirty, hashes)."""
+    li = np.ascontiguousarray(lights)
+    cam = np.ascontiguousarray(camera).reshape(-1)[:1]
+    m = np.ascontiguousarray(mats, np.float32).copy()
+    d = np.ascontiguousarray(dirty, np.uint32).copy()
+    h = np.ascontiguousarray(hashes, np.uint32).copy()
+    lib().or_shadow_matrices(_p(li), len(li), _p(cam), atlas_size, _p(m), m.size // 16, _p(d), _p(h),
+                             min(d.size, h.size))
+    return m, d, h
+
+
+def shadow_cull(mats: np.ndarray, instances: np.ndarray, instance_capacity: int, src_indirect: np.ndarray,
+                face_dirty: np.ndarray, max_per_face: int, plane_mode: int):
+    """shadow_cull.wgsl main over every draw in order: (counts[256], dst[256, max_per_face, 5])."""
+    m = np.ascontiguousarray(mats, np.float32)
+    inst = np.ascontiguousarray(instances)
+    src = np.ascontiguousarray(src_indirect).view(np.uint32).reshape(-1)
+    fd = np.ascontiguousarray(face_dirty, np.uint32)
+    counts = np.zeros(256, np.uint32)
+    dst = np.zeros((256, max_per_face, 5), np.uint32)
+    lib().or_shadow_cull(_p(m), _p(inst), instance_capacity, _p(src), src.size // 5, _p(fd), max_per_face,
+                         plane_mode, _p(counts), _p(dst))
+    return counts, dst

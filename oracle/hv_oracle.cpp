@@ -642,3 +642,302 @@ void or_vg_cull(const or_vg_inputs* in, uint32_t* indirect, uint32_t* metadata, 
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Multi-view producers: shadow_matrices.wgsl and shadow_cull.wgsl.
+// ===========================================================================
+namespace {
+
+// Canonical tan: reduce by k = rint(x * 2/pi) with pi/2 split in three
+// binary32 parts (C1 has 12 significant bits, so k * C1 is exact for |k| <
+// 2^11), evaluate tan(r) = r + r * (z * P(z)), z = r^2, and take -1/tan(r) for
+// odd k. P: degree-6 least-squares fit of (tan(r)/r - 1)/r^2 on |r| <= pi/4.
+const float kTanP[7] = {0x1.555562p-2f, 0x1.110d5ep-3f, 0x1.bae360p-5f, 0x1.5c69e4p-6f,
+                        0x1.645f6cp-7f, 0x1.64f368p-15f, 0x1.203adep-8f};
+// asin on |s| <= 0.5: s + s * (z * Q(z)), Q a degree-5 fit of (asin(s)/s - 1)/s^2.
+const float kAsinQ[6] = {0x1.55554cp-3f, 0x1.33387cp-4f, 0x1.6caf8cp-5f,
+                         0x1.048952p-5f, 0x1.d9b21ep-7f, 0x1.331dc6p-5f};
+
+float horner(const float* c, int n, float z) {
+    float p = c[n - 1];
+    for (int i = n - 2; i >= 0; --i) p = p * z + c[i];
+    return p;
+}
+
+float asin_small(float s) {
+    const float z = s * s;
+    return s + s * (z * horner(kAsinQ, 6, z));
+}
+
+}  // namespace
+
+extern "C" {
+
+float or_tanf(float x) {
+    if (!(std::fabs(x) < 2048.0f)) return std::nanf("");
+    const float k = std::nearbyint(x * 0x1.45f306p-1f);  // round half to even
+    float r = x - k * 0x1.922p+0f;
+    r = r - k * -0x1.2aeef4p-18f;
+    r = r - k * -0x1.74p-43f;
+    const float z = r * r;
+    const float t = r + r * (z * horner(kTanP, 7, z));
+    return (static_cast<int>(k) % 2 != 0) ? -1.0f / t : t;
+}
+
+float or_acosf(float x) {
+    if (std::isnan(x)) return x;
+    if (x < -1.0f || x > 1.0f) return std::nanf("");
+    const float pio2_hi = 0x1.921fb6p+0f, pio2_lo = -0x1.777a5cp-25f;  // pi/2 = hi + lo
+    if (std::fabs(x) <= 0.5f) return (pio2_hi - asin_small(x)) + pio2_lo;
+    if (x > 0.0f) return 2.0f * asin_small(std::sqrt((1.0f - x) * 0.5f));
+    return (2.0f * pio2_hi - 2.0f * asin_small(std::sqrt((1.0f + x) * 0.5f))) + 2.0f * pio2_lo;
+}
+
+uint32_t or_fnv_mat(const float m[16]) {
+    uint32_t h = 2166136261u;
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(m);
+    for (int i = 0; i < 64; ++i) {  // little-endian: byte 0 of each float is its low byte
+        h ^= b[i];
+        h *= 16777619u;
+    }
+    return h;
+}
+
+}  // extern "C"
+
+namespace {
+
+// WGSL vec3 / mat4x4 helpers restated from shadow_matrices.wgsl.
+struct Vec3 {
+    float v[3];
+};
+Vec3 mk(float x, float y, float z) { return Vec3{{x, y, z}}; }
+Vec3 add3(const Vec3& a, const Vec3& b) { return mk(a.v[0] + b.v[0], a.v[1] + b.v[1], a.v[2] + b.v[2]); }
+Vec3 sub3(const Vec3& a, const Vec3& b) { return mk(a.v[0] - b.v[0], a.v[1] - b.v[1], a.v[2] - b.v[2]); }
+Vec3 scale3(const Vec3& a, float s) { return mk(a.v[0] * s, a.v[1] * s, a.v[2] * s); }
+float dot3v(const Vec3& a, const Vec3& b) { return (a.v[0] * b.v[0] + a.v[1] * b.v[1]) + a.v[2] * b.v[2]; }
+float len3(const Vec3& a) { return std::sqrt(dot3v(a, a)); }
+Vec3 norm3(const Vec3& a) {
+    const float l = len3(a);
+    return mk(a.v[0] / l, a.v[1] / l, a.v[2] / l);
+}
+Vec3 cross3(const Vec3& a, const Vec3& b) {
+    return mk(a.v[1] * b.v[2] - a.v[2] * b.v[1], a.v[2] * b.v[0] - a.v[0] * b.v[2], a.v[0] * b.v[1] - a.v[1] * b.v[0]);
+}
+// WGSL mix(a, b, t) = a * (1 - t) + b * t.
+Vec3 mix3(const Vec3& a, const Vec3& b, float t) {
+    Vec3 o;
+    for (int i = 0; i < 3; ++i) o.v[i] = a.v[i] * (1.0f - t) + b.v[i] * t;
+    return o;
+}
+
+struct Mat4 {
+    float c[4][4];  // c[column][row], as WGSL indexes mat4x4f
+};
+// m * v: sum over columns of column * component, left to right.
+void matvec(const Mat4& m, const float v[4], float out[4]) {
+    for (int r = 0; r < 4; ++r) {
+        float acc = m.c[0][r] * v[0];
+        for (int k = 1; k < 4; ++k) acc = acc + m.c[k][r] * v[k];
+        out[r] = acc;
+    }
+}
+Mat4 matmul(const Mat4& a, const Mat4& b) {
+    Mat4 o;
+    for (int col = 0; col < 4; ++col) matvec(a, b.c[col], o.c[col]);
+    return o;
+}
+Mat4 from_cols(const float (&v)[16]) {
+    Mat4 m;
+    for (int i = 0; i < 16; ++i) m.c[i / 4][i % 4] = v[i];
+    return m;
+}
+Mat4 perspective(float fovy, float aspect, float near, float far) {  // :76-85
+    const float f = 1.0f / or_tanf(fovy * 0.5f);
+    const float nf = 1.0f / (near - far);
+    return from_cols({f / aspect, 0, 0, 0, 0, f, 0, 0, 0, 0, far * nf, -1.0f, 0, 0, near * far * nf, 0});
+}
+Mat4 orthographic(float left, float right, float bottom, float top, float near, float far) {  // :88-98
+    const float rml = 1.0f / (right - left);
+    const float tmb = 1.0f / (top - bottom);
+    const float fmn = 1.0f / (far - near);
+    return from_cols({2.0f * rml, 0, 0, 0, 0, 2.0f * tmb, 0, 0, 0, 0, fmn, 0, -(right + left) * rml,
+                      -(top + bottom) * tmb, -near * fmn, 1.0f});
+}
+Mat4 look_at(const Vec3& eye, const Vec3& center, const Vec3& up) {  // :101-111
+    const Vec3 f = norm3(sub3(center, eye));
+    const Vec3 s = norm3(cross3(f, up));
+    const Vec3 u = cross3(s, f);
+    return from_cols({s.v[0], u.v[0], -f.v[0], 0, s.v[1], u.v[1], -f.v[1], 0, s.v[2], u.v[2], -f.v[2], 0,
+                      -dot3v(s, eye), -dot3v(u, eye), dot3v(f, eye), 1.0f});
+}
+void put(float* mats, uint32_t i, const Mat4& m) {
+    for (int k = 0; k < 16; ++k) mats[16ull * i + k] = m.c[k / 4][k % 4];
+}
+
+const float kPiF = 3.14159265359f, kFracPi2F = 1.57079632679f, kSceneDepthF = 4000.0f;
+
+struct Light {
+    float pos[4], dir[4], color[4];
+    uint32_t shadow_index, light_type;
+    float inner;
+    uint32_t pad;
+};
+static_assert(sizeof(Light) == 64, "GpuLight");
+
+void point_light(float* mats, uint32_t base, const Vec3& p, float range) {  // :115-136
+    const float far_plane = wmax(range, 0.1f) * 2.5f;
+    const Mat4 proj = perspective(kFracPi2F, 1.0f, 0.05f, far_plane);
+    const Vec3 fwd[6] = {mk(1, 0, 0), mk(-1, 0, 0), mk(0, 1, 0), mk(0, -1, 0), mk(0, 0, 1), mk(0, 0, -1)};
+    const Vec3 up[6] = {mk(0, -1, 0), mk(0, -1, 0), mk(0, 0, 1), mk(0, 0, -1), mk(0, -1, 0), mk(0, -1, 0)};
+    for (int i = 0; i < 6; ++i) put(mats, base + i, matmul(proj, look_at(p, add3(p, fwd[i]), up[i])));
+}
+
+void spot_light(float* mats, uint32_t base, const Vec3& p, const Vec3& direction, float range, float cos_outer) {
+    const Vec3 dir = norm3(direction);  // :140-154
+    const float outer = or_acosf(cos_outer);
+    const float fov = wclamp(outer * 2.0f, kPiF * 0.25f, kPiF - 0.01f);
+    const Vec3 up = std::fabs(dot3v(dir, mk(0, 1, 0))) < 0.99f ? mk(0, 1, 0) : mk(0, 0, 1);
+    const Mat4 view = look_at(p, add3(p, dir), up);
+    const Mat4 proj = perspective(fov, 1.0f, 0.05f, wmax(range, 0.1f));
+    put(mats, base, matmul(proj, view));
+}
+
+void directional_light(float* mats, uint32_t base, const Vec3& direction, const float* camera, uint32_t atlas) {
+    const Vec3 dir = norm3(direction);  // :158-232
+    const Vec3 up = std::fabs(dot3v(dir, mk(0, 1, 0))) > 0.99f ? mk(0, 0, 1) : mk(0, 1, 0);
+    Mat4 ivp;
+    for (int i = 0; i < 16; ++i) ivp.c[i / 4][i % 4] = camera[48 + i];  // inv_view_proj @192
+    const Vec3 cam = mk(camera[kPosNear], camera[kPosNear + 1], camera[kPosNear + 2]);
+    const float ndc[8][4] = {{-1, -1, 0, 1}, {1, -1, 0, 1}, {-1, 1, 0, 1}, {1, 1, 0, 1},
+                             {-1, -1, 1, 1}, {1, -1, 1, 1}, {-1, 1, 1, 1}, {1, 1, 1, 1}};
+    Vec3 world[8];
+    for (int i = 0; i < 8; ++i) {
+        float v[4];
+        matvec(ivp, ndc[i], v);
+        world[i] = mk(v[0] / v[3], v[1] / v[3], v[2] / v[3]);
+    }
+    float near_dist = 0.0f, far_dist = 0.0f;
+    for (int i = 0; i < 4; ++i) {
+        near_dist += len3(sub3(world[i], cam));
+        far_dist += len3(sub3(world[i + 4], cam));
+    }
+    near_dist /= 4.0f;
+    far_dist /= 4.0f;
+    const float depth = wmax(far_dist - near_dist, 1.0f);
+    const float splits[4] = {16.0f, 80.0f, 300.0f, 1400.0f};
+    const float prev_d[4] = {0.0f, splits[0], splits[1], splits[2]};
+    for (int ci = 0; ci < 4; ++ci) {
+        const float t0 = wclamp((prev_d[ci] - near_dist) / depth, 0.0f, 1.0f);
+        const float t1 = wclamp((splits[ci] - near_dist) / depth, 0.0f, 1.0f);
+        Vec3 cc[8];
+        for (int j = 0; j < 4; ++j) {
+            cc[j * 2] = mix3(world[j], world[j + 4], t0);
+            cc[j * 2 + 1] = mix3(world[j], world[j + 4], t1);
+        }
+        Vec3 centroid = mk(0, 0, 0);
+        for (int i = 0; i < 8; ++i) centroid = add3(centroid, cc[i]);
+        centroid = mk(centroid.v[0] / 8.0f, centroid.v[1] / 8.0f, centroid.v[2] / 8.0f);
+        float radius = 0.0f;
+        for (int i = 0; i < 8; ++i) radius = wmax(radius, len3(sub3(cc[i], centroid)));
+        const float texel_size = (2.0f * radius) / static_cast<float>(atlas ? atlas : 1u);
+        const float radius_snap = std::ceil(radius / texel_size) * texel_size;
+        const Mat4 raw = look_at(sub3(centroid, scale3(dir, kSceneDepthF)), centroid, up);
+        const float c4[4] = {centroid.v[0], centroid.v[1], centroid.v[2], 1.0f};
+        float ls[4];
+        matvec(raw, c4, ls);
+        const Vec3 cls = mk(ls[0] / ls[3], ls[1] / ls[3], ls[2] / ls[3]);
+        const float snap = texel_size;
+        const float sx = std::nearbyint(cls.v[0] / snap) * snap;
+        const float sy = std::nearbyint(cls.v[1] / snap) * snap;
+        const Vec3 right_ws = norm3(mk(raw.c[0][0], raw.c[1][0], raw.c[2][0]));
+        const Vec3 up_ws = norm3(mk(raw.c[0][1], raw.c[1][1], raw.c[2][1]));
+        const Vec3 off = add3(scale3(right_ws, sx - cls.v[0]), scale3(up_ws, sy - cls.v[1]));
+        const Vec3 stable = add3(centroid, off);
+        const Mat4 view = look_at(sub3(stable, scale3(dir, kSceneDepthF)), stable, up);
+        const Mat4 proj = orthographic(-radius_snap, radius_snap, -radius_snap, radius_snap, 0.1f, kSceneDepthF * 2.0f);
+        put(mats, base + ci, matmul(proj, view));
+    }
+    const Mat4 id = from_cols({1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1});
+    put(mats, base + 4, id);
+    put(mats, base + 5, id);
+}
+
+// normalize_plane (shadow_cull.wgsl:51-57).
+void norm_plane(const float p[4], float out[4]) {
+    const float len = std::sqrt((p[0] * p[0] + p[1] * p[1]) + p[2] * p[2]);
+    for (int i = 0; i < 4; ++i) out[i] = len > 1e-10f ? p[i] / len : p[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+void or_shadow_matrices(const void* lights, uint32_t light_count, const float* camera, uint32_t atlas_size,
+                        float* mats, uint32_t matrix_capacity, uint32_t* dirty, uint32_t* hashes,
+                        uint32_t slot_capacity) {
+    const Light* ls = static_cast<const Light*>(lights);
+    for (uint32_t li = 0; li < light_count; ++li) {
+        const Light& l = ls[li];
+        if (l.shadow_index == 0xFFFFFFFFu) continue;
+        const uint32_t base = l.shadow_index, slot = base / 6u;
+        if (static_cast<uint64_t>(base) + 6u > matrix_capacity || slot >= slot_capacity) continue;
+        const Vec3 pos = mk(l.pos[0], l.pos[1], l.pos[2]);
+        const Vec3 dir = mk(l.dir[0], l.dir[1], l.dir[2]);
+        if (l.light_type == 1u) point_light(mats, base, pos, l.pos[3]);
+        else if (l.light_type == 0u) directional_light(mats, base, dir, camera, atlas_size);
+        else if (l.light_type == 2u) spot_light(mats, base, pos, dir, l.pos[3], l.dir[3]);
+        // fnv_hash_mats_6: FNV-1a over the 4 little-endian bytes of each matrix hash.
+        uint32_t h = 2166136261u;
+        for (uint32_t i = 0; i < 6; ++i) {
+            const uint32_t mh = or_fnv_mat(mats + 16ull * (base + i));
+            for (int b = 0; b < 4; ++b) {
+                h ^= (mh >> (8 * b)) & 0xFFu;
+                h *= 16777619u;
+            }
+        }
+        if (h != hashes[slot]) {
+            hashes[slot] = h;
+            dirty[slot] = 1u;
+        }
+    }
+}
+
+void or_shadow_cull(const float* mats, const void* instances, uint32_t instance_capacity,
+                    const uint32_t* src_indirect, uint32_t draw_count, const uint32_t* face_dirty,
+                    uint32_t max_per_face, uint32_t plane_mode, uint32_t* counts, uint32_t* dst) {
+    for (uint32_t f = 0; f < 256; ++f) counts[f] = 0;
+    const unsigned char* inst = static_cast<const unsigned char*>(instances);
+    for (uint32_t d = 0; d < draw_count; ++d) {
+        const uint32_t* draw = src_indirect + 5ull * d;
+        float b[4] = {0, 0, 0, 0};
+        if (draw[4] < instance_capacity) std::memcpy(b, inst + 144ull * draw[4] + 112, 16);
+        if (b[3] <= 0.0f) continue;
+        for (uint32_t face = 0; face < 256; ++face) {
+            if (face_dirty[face] == 0u) continue;
+            const float* m = mats + 16ull * face;
+            // vector i component j: column i of the matrix (the shader's vp[i]) or row i.
+            auto vec = [&](int i, int j) { return plane_mode == 1u ? m[i + 4 * j] : m[4 * i + j]; };
+            float raw[6][4];
+            for (int j = 0; j < 4; ++j) {
+                raw[0][j] = vec(3, j) + vec(0, j);
+                raw[1][j] = vec(3, j) - vec(0, j);
+                raw[2][j] = vec(3, j) + vec(1, j);
+                raw[3][j] = vec(3, j) - vec(1, j);
+                raw[4][j] = vec(2, j);
+                raw[5][j] = vec(3, j) - vec(2, j);
+            }
+            bool visible = true;
+            for (int i = 0; i < 6 && visible; ++i) {
+                float p[4];
+                norm_plane(raw[i], p);
+                if (((p[0] * b[0] + p[1] * b[1]) + p[2] * b[2]) + p[3] + b[3] < 0.0f) visible = false;
+            }
+            if (!visible) continue;
+            const uint32_t slot = counts[face]++;
+            if (slot < max_per_face) std::memcpy(dst + 5ull * (static_cast<uint64_t>(face) * max_per_face + slot), draw, 20);
+        }
+    }
+}
+
+}  // extern "C"

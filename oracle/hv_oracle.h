@@ -102,6 +102,30 @@ typedef struct or_vg_inputs {
 void or_vg_cull(const or_vg_inputs* in, uint32_t* indirect /* 5 u32 per slot */,
                 uint32_t* metadata /* 4 u32 per slot */, uint32_t* counters);
 
+/* --- multi-view producers ---------------------------------------------------
+ * Canonical binary32 tan and acos of the library (WGSL leaves their precision
+ * to the implementation): polynomial fits evaluated with + - * / and sqrt in a
+ * fixed order, so the oracle and the HIP kernels agree bit for bit. */
+float or_tanf(float x);
+float or_acosf(float x);
+/* FNV-1a (32-bit) of one matrix's 64 bytes in memory order (shadow_matrices.wgsl:238-252). */
+uint32_t or_fnv_mat(const float m[16]);
+/* compute_shadow_matrices (shadow_matrices.wgsl:262-315) for lights 0..n-1 in
+ * order. lights: 64-byte GpuLight records; camera: 92 floats; mats:
+ * matrix_capacity float[16]; dirty/hashes: slot_capacity u32. Lights whose six
+ * matrices or caster slot fall outside the capacities are skipped. */
+void or_shadow_matrices(const void* lights, uint32_t light_count, const float* camera, uint32_t atlas_size,
+                        float* mats, uint32_t matrix_capacity, uint32_t* dirty, uint32_t* hashes,
+                        uint32_t slot_capacity);
+/* shadow_cull.wgsl main (:75-96) over draws 0..n-1 in order: counts[256]
+ * cleared first, visible draws appended per face in draw order to
+ * dst[(face * max_per_face + slot) * 5]. plane_mode 0 = the shader's column
+ * vectors, 1 = Gribb-Hartmann rows. A first_instance >= instance_capacity
+ * reads a zero sphere. */
+void or_shadow_cull(const float* mats, const void* instances, uint32_t instance_capacity,
+                    const uint32_t* src_indirect, uint32_t draw_count, const uint32_t* face_dirty,
+                    uint32_t max_per_face, uint32_t plane_mode, uint32_t* counts, uint32_t* dst);
+
 #ifdef __cplusplus
 }
 #endif
