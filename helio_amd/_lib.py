@@ -206,6 +206,7 @@ _SIGS: dict[str, tuple] = {
     "hv_occlusion_cull_prepare": (i32, [vp, vp, u32, u64]),
     "hv_occlusion_cull_params": (i32, [vp, vp]),
     "hv_occlusion_cull_execute": (i32, [vp, P(DrawBuffers), vp, vp]),
+    "hv_compact_draws": (i32, [vp, vp, u32, vp, vp, vp]),
     "hv_shadow_matrix_create": (i32, [vp, u32, P(vp)]),
     "hv_shadow_matrix_destroy": (None, [vp]),
     "hv_shadow_matrix_execute": (i32, [vp, P(ShadowMatrixBuffers), vp]),

@@ -75,6 +75,13 @@ __global__ __launch_bounds__(64) void k_vg_clear(uint32_t* counters, uint32_t n)
     if (threadIdx.x < n) counters[threadIdx.x] = 0u;
 }
 
+// Zeroes n words (the indirect buffer when the consumer draws every slot).
+__global__ __launch_bounds__(256) void k_vg_zero(uint4* p, uint64_t n4, uint32_t* tail, uint32_t n_tail) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * 256u)
+        p[i] = make_uint4(0, 0, 0, 0);
+    if (blockIdx.x == 0 && threadIdx.x < n_tail) tail[threadIdx.x] = 0u;
+}
+
 // LOD selection of one object (the body of cs_select_objects,
 // vg_cull.wgsl:300-351): selected_lod + 1, or 0 when the object has no LODs,
 // an invalid transform or a world sphere outside the frustum.
@@ -453,6 +460,7 @@ struct hv_vg_pass {
     uint32_t meshlet_count = 0, object_count = 0, work_item_count = 0, max_draw = 0;
     std::vector<hv_instance_cull_data> icd_scratch;
     VgArgs args{};
+    bool clear_indirect = false;
     bool prepared = false;
     hipEvent_t ev[3] = {};
     hipEvent_t done = nullptr;
@@ -507,7 +515,7 @@ void hv_vg_destroy(hv_vg_pass* p) {
 hv_status hv_vg_prepare(hv_vg_pass* p, const hv_vg_frame_data* vg, const hv_camera_uniforms* cam, uint32_t width,
                         uint32_t height, float threshold, const hv_hiz* hiz, uint32_t flags) {
     if (!p || !vg || !cam) return set_error(HV_EINVAL, "hv_vg_prepare: NULL argument");
-    if (flags & ~HV_CULL_REVERSED_Z) return set_error(HV_EINVAL, "hv_vg_prepare: unknown flags");
+    if (flags & ~(HV_CULL_REVERSED_Z | HV_VG_CLEAR_INDIRECT)) return set_error(HV_EINVAL, "hv_vg_prepare: unknown flags");
     if (hiz && (hiz->mode == HV_HIZ_MIN) != ((flags & HV_CULL_REVERSED_Z) != 0))
         return set_error(HV_EINVAL, "hv_vg_prepare: pyramid mode does not match HV_CULL_REVERSED_Z");
     if (hiz && !hiz->level0) return set_error(HV_ESTATE, "hv_vg_prepare: pyramid has no depth bound");
@@ -631,6 +639,7 @@ hv_status hv_vg_prepare(hv_vg_pass* p, const hv_vg_frame_data* vg, const hv_came
     HV_HIP(hipMemcpyAsync(p->view.ptr, &v, sizeof(VgView), hipMemcpyHostToDevice, st));
     HV_HIP(hipStreamSynchronize(st));
     a.view = p->view.as<VgView>();
+    p->clear_indirect = (flags & HV_VG_CLEAR_INDIRECT) != 0;
     p->prepared = true;
     return HV_OK;
 }
@@ -648,6 +657,14 @@ static hv_status vg_launch(hv_vg_pass* p, hipStream_t s, float* ms) {
     if (ms) HV_HIP(hipEventRecord(p->ev[0], s));
     hipLaunchKernelGGL(k_vg_clear, dim3(1), dim3(64), 0, s, a.counters, a.counter_len);
     HV_HIP(hipGetLastError());
+    if (p->clear_indirect && p->indirect_cap) {
+        // clear_buffer(indirect_buf) of the non-count path: the whole buffer.
+        const uint64_t words = (uint64_t)p->indirect_cap * 5u, n4 = words / 4u;
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>((n4 + 255u) / 256u + 1u, 4096u);
+        hipLaunchKernelGGL(k_vg_zero, dim3(blocks), dim3(256), 0, s, reinterpret_cast<uint4*>(a.indirect), n4,
+                           reinterpret_cast<uint32_t*>(a.indirect) + 4u * n4, (uint32_t)(words - 4u * n4));
+        HV_HIP(hipGetLastError());
+    }
     if (ms) HV_HIP(hipEventRecord(p->ev[1], s));
     {
         a.select_blocks = (p->object_count + 255u) / 256u;

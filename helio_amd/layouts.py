@@ -157,3 +157,4 @@ HIZ_MAX = 0
 HIZ_MIN = 1
 CULL_OCCLUSION = 1
 CULL_REVERSED_Z = 2
+VG_CLEAR_INDIRECT = 0x100

@@ -377,7 +377,9 @@ class VirtualGeometryPass:
         self._dirty = (start, len(instances))
 
     def prepare(self, camera: np.ndarray, width: int = 1920, height: int = 1080, threshold_px: float = 0.5,
-                hiz: HiZBuildPass | None = None, reversed_z: bool = False) -> None:
+                hiz: HiZBuildPass | None = None, reversed_z: bool = False, clear_indirect: bool = False) -> None:
+        """clear_indirect: zero every indirect slot each execute (a consumer
+        without multi_draw_indexed_indirect_count draws them all)."""
         m, o, i, w, mx = self._frame
         fd = VgFrameData()
         fd.meshlets, fd.objects, fd.instances, fd.work_items = ptr(m), ptr(o), ptr(i), ptr(w)
@@ -388,8 +390,9 @@ class VirtualGeometryPass:
         fd.instance_dirty_start, fd.instance_dirty_count = self._dirty
         cam = np.ascontiguousarray(camera, dtype=L.CAMERA).reshape(1)
         self._cam = cam
+        flags = (L.CULL_REVERSED_Z if reversed_z else 0) | (L.VG_CLEAR_INDIRECT if clear_indirect else 0)
         call("hv_vg_prepare", self.handle, C.byref(fd), ptr(cam), width, height, threshold_px,
-             None if hiz is None else hiz.handle, L.CULL_REVERSED_Z if reversed_z else 0)
+             None if hiz is None else hiz.handle, flags)
 
     def execute(self, stream: int | None = None) -> None:
         call("hv_vg_execute", self.handle, stream)
@@ -617,6 +620,15 @@ class CullServer:
         call("hv_server_run_trace", self.handle, arr, a.ctypes.data_as(C.POINTER(C.c_double)), n,
              lat.ctypes.data_as(C.POINTER(C.c_double)), C.byref(wall))
         return lat[:n], wall.value
+
+
+def compact_draws(ctx: Context, args_ptr: int, n: int, out: "DeviceBuffer", count: "DeviceBuffer",
+                  stream: int | None = None) -> None:
+    """The non-empty draws of a device args array (n records), in order, into
+    `out`, and their number into count[0] (multi_draw_indexed_indirect_count
+    wire format)."""
+    assert out.nbytes >= 20 * n and count.nbytes >= 4
+    call("hv_compact_draws", ctx.handle, args_ptr, n, out.ptr, count.ptr, stream)
 
 
 class ShadowMatrixPass:

@@ -449,8 +449,12 @@ hv_status hv_vg_create(hv_ctx* ctx, uint32_t publication_limit, hv_vg_pass** out
 void hv_vg_destroy(hv_vg_pass* pass);
 /* VirtualGeometryPass::prepare (rendering.rs:708-955): uploads on a new
  * buffer_version, a dirty instance range on a new instance_version, derives
- * InstanceCullData on the host, writes the cull uniforms. flags: 0 or
- * HV_CULL_REVERSED_Z (the pyramid must then be HV_HIZ_MIN). */
+ * InstanceCullData on the host, writes the cull uniforms. flags: any of
+ * HV_CULL_REVERSED_Z (the pyramid must then be HV_HIZ_MIN) and
+ * HV_VG_CLEAR_INDIRECT: also zero the whole indirect buffer each execute, for
+ * a consumer without multi_draw_indexed_indirect_count that draws every slot
+ * (the !use_count_indirect path, rendering.rs:1127-1130,1203-1204). */
+#define HV_VG_CLEAR_INDIRECT 0x100u
 hv_status hv_vg_prepare(hv_vg_pass* pass, const hv_vg_frame_data* vg, const hv_camera_uniforms* camera,
                         uint32_t width, uint32_t height, float lod_error_threshold_px, const hv_hiz* hiz,
                         uint32_t flags);
@@ -520,6 +524,19 @@ hv_status hv_occlusion_cull_prepare(hv_occlusion_cull_pass* pass, const hv_camer
 hv_status hv_occlusion_cull_params(const hv_occlusion_cull_pass* pass, hv_occl_cull_params* out);
 hv_status hv_occlusion_cull_execute(hv_occlusion_cull_pass* pass, const hv_draw_buffers* buffers, const hv_hiz* hiz,
                                     void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* Indirect-args wire format. The per-instance pass writes one args record   */
+/* per (mesh, LOD) bucket, empty buckets included (instance_count 0), which  */
+/* multi_draw_indexed_indirect consumes as is. For                           */
+/* multi_draw_indexed_indirect_count (the VG draw, rendering.rs:1194-1201,   */
+/* and the shadow faces) this copies the non-empty draws, in order, to `out` */
+/* and writes their number to count[0] (count at byte 0, as the VG draw      */
+/* counter buffer keeps it, rendering.rs:648-661). Device pointers; out has  */
+/* room for n records; n <= 2^20. Enqueues on `stream` only.                 */
+/* ------------------------------------------------------------------------- */
+hv_status hv_compact_draws(hv_ctx* ctx, const hv_draw_indexed_indirect* args, uint32_t n,
+                           hv_draw_indexed_indirect* out, uint32_t* count, void* stream);
 
 /* ------------------------------------------------------------------------- */
 /* Multi-view producers: light-space view-projections for shadow faces       */
