@@ -97,6 +97,19 @@ class ServerConfig(C.Structure):
         ("max_buckets", u32),
         ("use_graph", u32),
         ("copy_ids", u32),
+        ("max_vg_draws", u32),
+    ]
+
+
+class VgRequest(C.Structure):
+    _fields_ = [
+        ("scene", vp),
+        ("width", u32), ("height", u32),
+        ("lod_error_threshold_px", f32),
+        ("flags", u32),
+        ("hiz", vp),
+        ("capacity", u32),
+        ("indirect_out", vp), ("metadata_out", vp), ("counters_out", vp),
     ]
 
 
@@ -106,6 +119,7 @@ class ServerRequest(C.Structure):
         ("ids_out", vp),
         ("args_out", vp),
         ("stats_out", vp),
+        ("vg", VgRequest),
     ]
 
 
@@ -196,6 +210,8 @@ _SIGS: dict[str, tuple] = {
     "hv_vg_outputs": (i32, [vp, P(VgOutputs)]),
     "hv_vg_read": (i32, [vp, vp, vp, u32, vp, vp, u32]),
     "hv_vg_build_work_items": (i32, [vp, u32, vp, u32, P(u32)]),
+    "hv_vg_scene_create": (i32, [vp, P(VgFrameData), P(vp)]),
+    "hv_vg_scene_destroy": (None, [vp]),
     "hv_indirect_dispatch_create": (i32, [vp, P(vp)]),
     "hv_indirect_dispatch_destroy": (None, [vp]),
     "hv_indirect_dispatch_prepare": (i32, [vp, vp, u32]),

@@ -14,6 +14,10 @@
 //      slot's stream and records the slot's completion event.
 // Requests that arrive while batches are on the GPU are therefore admitted as
 // soon as a slot and memory free up, and their batch overlaps the others.
+// A request may carry a meshlet part (an hv_vg_scene culled from the same
+// camera): the batch's meshlet parts run in ONE k_vg_multi launch after the
+// instance kernels, on the slot's stream, each into its own slice of the
+// slot's VG outputs.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -26,6 +30,7 @@
 #include <vector>
 
 #include "hv_cull.h"
+#include "hv_vg.h"
 
 using namespace hv;
 using Clock = std::chrono::steady_clock;
@@ -50,10 +55,19 @@ struct Slot {
     hipEvent_t done = nullptr;
     std::vector<Req*> reqs;
     std::vector<uint64_t> args_off, ids_off;
+    std::vector<int32_t> inst_view;  // request -> view in the instance batch (-1: none)
+    std::vector<int32_t> vg_req;     // request -> meshlet request of the launch (-1: none)
+    std::vector<uint32_t> vg_off;    // meshlet request -> first slot in the VG outputs
     bool busy = false;
     uint32_t* h_ctr = nullptr;
     hv_draw_indexed_indirect* h_args = nullptr;
     uint32_t* h_ids = nullptr;
+    // Meshlet parts: request records, uniforms, workgroup prefix, outputs.
+    DevBuf d_vg_args, d_vg_views, d_vg_begin, d_vg_ind, d_vg_md, d_vg_ctr;
+    VgArgs* h_vg_args = nullptr;
+    VgView* h_vg_views = nullptr;
+    uint32_t* h_vg_begin = nullptr;
+    uint32_t* h_vg_ctr = nullptr;
 };
 
 }  // namespace
@@ -77,17 +91,37 @@ struct hv_server {
 
 static void deliver(hv_server* s, Slot& slot) {
     const uint32_t n = (uint32_t)slot.reqs.size();
-    for (uint32_t v = 0; v < n; ++v) {
-        Req* q = slot.reqs[v];
-        const uint32_t* ctr = slot.h_ctr + (size_t)v * kCounterStride;
-        q->visible = ctr[kCtrVisible];
-        if (q->r.stats_out) {
-            std::memcpy(q->r.stats_out, ctr, HV_STAT_COUNT * 4);
-            q->r.stats_out[HV_STAT_COUNT] = q->visible;
+    for (uint32_t i = 0; i < n; ++i) {
+        Req* q = slot.reqs[i];
+        q->visible = 0;
+        const int32_t v = slot.inst_view[i];
+        if (v >= 0) {
+            const uint32_t* ctr = slot.h_ctr + (size_t)v * kCounterStride;
+            q->visible = ctr[kCtrVisible];
+            if (q->r.stats_out) {
+                std::memcpy(q->r.stats_out, ctr, HV_STAT_COUNT * 4);
+                q->r.stats_out[HV_STAT_COUNT] = q->visible;
+            }
+            if (q->r.args_out) std::memcpy(q->r.args_out, slot.h_args + slot.args_off[v], (size_t)q->nb * 20);
+            if (q->r.ids_out && s->cfg.copy_ids && slot.h_ids)
+                std::memcpy(q->r.ids_out, slot.h_ids + slot.ids_off[v], (size_t)q->visible * 4);
         }
-        if (q->r.args_out) std::memcpy(q->r.args_out, slot.h_args + slot.args_off[v], (size_t)q->nb * 20);
-        if (q->r.ids_out && s->cfg.copy_ids && slot.h_ids)
-            std::memcpy(q->r.ids_out, slot.h_ids + slot.ids_off[v], (size_t)q->visible * 4);
+        const int32_t k = slot.vg_req[i];
+        if (k >= 0) {
+            const uint32_t* ctr = slot.h_vg_ctr + (size_t)k * HV_VG_COUNTER_COUNT;
+            const uint32_t published = std::min(ctr[0], q->r.vg.capacity);
+            q->visible += published;
+            if (q->r.vg.counters_out) std::memcpy(q->r.vg.counters_out, ctr, HV_VG_COUNTER_COUNT * 4);
+            // Exactly the published commands, now that the count is known.
+            hipError_t e = hipSuccess;
+            if (q->r.vg.indirect_out && published)
+                e = hipMemcpy(q->r.vg.indirect_out, slot.d_vg_ind.as<hv_draw_indexed_indirect>() + slot.vg_off[k],
+                              (size_t)published * 20, hipMemcpyDeviceToHost);
+            if (e == hipSuccess && q->r.vg.metadata_out && published)
+                e = hipMemcpy(q->r.vg.metadata_out, slot.d_vg_md.as<hv_vg_draw>() + slot.vg_off[k],
+                              (size_t)published * 16, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) q->status = hip_error(e, "hv_server (meshlet results)", __FILE__, __LINE__);
+        }
     }
     const Clock::time_point now = Clock::now();
     std::lock_guard<std::mutex> lk(s->mu);
@@ -105,37 +139,102 @@ static void deliver(hv_server* s, Slot& slot) {
 
 static hv_status launch_slot(hv_server* s, Slot& slot) {
     const uint32_t n = (uint32_t)slot.reqs.size();
-    std::vector<ViewDesc> vd(n);
-    slot.args_off.assign(n, 0);
-    slot.ids_off.assign(n, 0);
+    std::vector<ViewDesc> vd;
+    slot.args_off.clear();
+    slot.ids_off.clear();
+    slot.inst_view.assign(n, -1);
+    slot.vg_req.assign(n, -1);
+    slot.vg_off.clear();
     uint64_t ao = 0, io = 0;
-    for (uint32_t v = 0; v < n; ++v) {
-        const Req* q = slot.reqs[v];
-        vd[v].scene = q->r.view.scene;
-        vd[v].camera = &q->cam;
-        vd[v].first = q->r.view.first;
-        vd[v].count = q->r.view.count;
-        vd[v].hiz = q->r.view.hiz;
-        slot.args_off[v] = ao;
-        slot.ids_off[v] = io;
+    for (uint32_t i = 0; i < n; ++i) {
+        const Req* q = slot.reqs[i];
+        if (!q->r.view.scene) continue;
+        slot.inst_view[i] = (int32_t)vd.size();
+        ViewDesc d;
+        d.scene = q->r.view.scene;
+        d.camera = &q->cam;
+        d.first = q->r.view.first;
+        d.count = q->r.view.count;
+        d.hiz = q->r.view.hiz;
+        vd.push_back(d);
+        slot.args_off.push_back(ao);
+        slot.ids_off.push_back(io);
         ao += q->nb;
         io += q->r.view.count;
     }
-    HV_TRY(slot.batch->prepare(vd.data(), n));
-    HV_TRY(s->cfg.use_graph ? slot.batch->execute_graph(slot.stream) : slot.batch->execute(slot.stream));
-    // Result copies ride on the same stream, into pinned staging.
-    hv_cull_outputs o0;
-    HV_TRY(slot.batch->outputs(0, &o0));
-    HV_HIP(hipMemcpyAsync(slot.h_ctr, o0.stats, (size_t)n * kCounterStride * 4, hipMemcpyDeviceToHost, slot.stream));
-    HV_HIP(hipMemcpyAsync(slot.h_args, o0.args, (size_t)ao * 20, hipMemcpyDeviceToHost, slot.stream));
-    if (s->cfg.copy_ids && slot.h_ids)
-        for (uint32_t v = 0; v < n; ++v)
-            if (slot.reqs[v]->r.ids_out && slot.reqs[v]->r.view.count) {
-                hv_cull_outputs o;
-                HV_TRY(slot.batch->outputs(v, &o));
-                HV_HIP(hipMemcpyAsync(slot.h_ids + slot.ids_off[v], o.ids, (size_t)slot.reqs[v]->r.view.count * 4,
-                                      hipMemcpyDeviceToHost, slot.stream));
+    const uint32_t nv = (uint32_t)vd.size();
+    if (nv) {
+        HV_TRY(slot.batch->prepare(vd.data(), nv));
+        HV_TRY(s->cfg.use_graph ? slot.batch->execute_graph(slot.stream) : slot.batch->execute(slot.stream));
+        // Result copies ride on the same stream, into pinned staging.
+        hv_cull_outputs o0;
+        HV_TRY(slot.batch->outputs(0, &o0));
+        HV_HIP(hipMemcpyAsync(slot.h_ctr, o0.stats, (size_t)nv * kCounterStride * 4, hipMemcpyDeviceToHost,
+                              slot.stream));
+        HV_HIP(hipMemcpyAsync(slot.h_args, o0.args, (size_t)ao * 20, hipMemcpyDeviceToHost, slot.stream));
+        if (s->cfg.copy_ids && slot.h_ids)
+            for (uint32_t i = 0; i < n; ++i) {
+                const int32_t v = slot.inst_view[i];
+                if (v >= 0 && slot.reqs[i]->r.ids_out && slot.reqs[i]->r.view.count) {
+                    hv_cull_outputs o;
+                    HV_TRY(slot.batch->outputs((uint32_t)v, &o));
+                    HV_HIP(hipMemcpyAsync(slot.h_ids + slot.ids_off[v], o.ids, (size_t)slot.reqs[i]->r.view.count * 4,
+                                          hipMemcpyDeviceToHost, slot.stream));
+                }
             }
+    }
+    // Meshlet parts: one record per request (its slice of the VG outputs),
+    // staged in pinned memory, then one launch for all of them.
+    uint32_t nk = 0, out_off = 0;
+    uint64_t blocks = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const Req* q = slot.reqs[i];
+        if (!q->r.vg.scene) continue;
+        const hv_vg_scene* sc = q->r.vg.scene;
+        const hv_vg_request& r = q->r.vg;
+        VgView& v = slot.h_vg_views[nk];
+        vg_build_view(v, &q->cam, r.width, r.height, r.lod_error_threshold_px, r.hiz, r.flags);
+        VgArgs& a = slot.h_vg_args[nk];
+        a = VgArgs{};
+        a.meshlets = sc->meshlets.as<hv_meshlet_entry>();
+        a.objects = sc->objects.as<hv_vg_object>();
+        a.instances = sc->instances.as<hv_instance_data>();
+        a.icd = sc->icd.as<hv_instance_cull_data>();
+        a.work_items = sc->work_items.as<hv_vg_work_item>();
+        a.indirect = slot.d_vg_ind.as<hv_draw_indexed_indirect>() + out_off;
+        a.metadata = slot.d_vg_md.as<hv_vg_draw>() + out_off;
+        a.counters = slot.d_vg_ctr.as<uint32_t>() + (size_t)nk * HV_VG_COUNTER_COUNT;
+        a.view = slot.d_vg_views.as<VgView>() + nk;
+        a.meshlet_count = sc->meshlet_count;
+        a.object_count_buf = a.instance_count = a.icd_count = a.object_count = sc->object_count;
+        a.work_item_buf_count = a.work_item_count = sc->work_item_count;
+        a.capacity = r.capacity;
+        a.counter_len = HV_VG_COUNTER_COUNT;
+        a.write_selection = 0u;  // concurrent requests may share the scene
+        uint64_t b = 0;
+        if (sc->object_count && sc->work_item_count) vg_grid(sc->object_count, sc->work_item_count, a.select_blocks,
+                                                             a.spans_per_block, b);
+        else a.spans_per_block = 64u;  // nothing to cull: zero workgroups, the counters stay cleared
+        slot.h_vg_begin[nk] = (uint32_t)blocks;
+        blocks += b;
+        slot.vg_req[i] = (int32_t)nk;
+        slot.vg_off.push_back(out_off);
+        out_off += r.capacity;
+        ++nk;
+    }
+    if (nk) {
+        slot.h_vg_begin[nk] = (uint32_t)blocks;
+        HV_HIP(hipMemcpyAsync(slot.d_vg_views.ptr, slot.h_vg_views, (size_t)nk * sizeof(VgView), hipMemcpyHostToDevice,
+                              slot.stream));
+        HV_HIP(hipMemcpyAsync(slot.d_vg_args.ptr, slot.h_vg_args, (size_t)nk * sizeof(VgArgs), hipMemcpyHostToDevice,
+                              slot.stream));
+        HV_HIP(hipMemcpyAsync(slot.d_vg_begin.ptr, slot.h_vg_begin, (size_t)(nk + 1) * 4, hipMemcpyHostToDevice,
+                              slot.stream));
+        HV_TRY(vg_launch_multi(slot.d_vg_args.as<VgArgs>(), slot.d_vg_begin.as<uint32_t>(), slot.d_vg_ctr.as<uint32_t>(),
+                               nk, blocks, slot.stream));
+        HV_HIP(hipMemcpyAsync(slot.h_vg_ctr, slot.d_vg_ctr.ptr, (size_t)nk * HV_VG_COUNTER_COUNT * 4,
+                              hipMemcpyDeviceToHost, slot.stream));
+    }
     HV_HIP(hipEventRecord(slot.done, slot.stream));
     return HV_OK;
 }
@@ -171,15 +270,17 @@ static void worker_main(hv_server* s) {
         bool launched = false;
         for (Slot& sl : s->slots) {
             if (sl.busy || s->queue.empty()) continue;
-            uint64_t cand = 0;
+            uint64_t cand = 0, vg_draws = 0;
             while (!s->queue.empty() && sl.reqs.size() < s->cfg.max_batch_views) {
                 Req* q = s->queue.front();
                 if (cand + q->r.view.count > s->cfg.max_batch_candidates) break;
+                if (q->r.vg.scene && vg_draws + q->r.vg.capacity > s->cfg.max_vg_draws) break;
                 if (s->admitted_bytes + q->bytes > s->cfg.device_budget_bytes) break;
                 s->queue.pop_front();
                 s->admitted_bytes += q->bytes;
                 s->peak_bytes = std::max(s->peak_bytes, s->admitted_bytes);
                 cand += q->r.view.count;
+                if (q->r.vg.scene) vg_draws += q->r.vg.capacity;
                 sl.reqs.push_back(q);
             }
             if (sl.reqs.empty()) break;  // head request waits for memory
@@ -247,6 +348,24 @@ hv_status hv_server_create(hv_ctx* ctx, const hv_server_config* cfg, hv_server**
                               hipHostMallocDefault);
         if (e == hipSuccess && cfg->copy_ids)
             e = hipHostMalloc((void**)&sl.h_ids, (size_t)cfg->max_batch_candidates * 4, hipHostMallocDefault);
+        if (e == hipSuccess && cfg->max_vg_draws) {
+            const uint32_t nvw = cfg->max_batch_views;
+            e = hipHostMalloc((void**)&sl.h_vg_args, (size_t)nvw * sizeof(VgArgs), hipHostMallocDefault);
+            if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_vg_views, (size_t)nvw * sizeof(VgView), hipHostMallocDefault);
+            if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_vg_begin, (size_t)(nvw + 1) * 4, hipHostMallocDefault);
+            if (e == hipSuccess)
+                e = hipHostMalloc((void**)&sl.h_vg_ctr, (size_t)nvw * HV_VG_COUNTER_COUNT * 4, hipHostMallocDefault);
+            if (e == hipSuccess) {
+                st = sl.d_vg_args.reserve((uint64_t)nvw * sizeof(VgArgs), false, ctx->stream);
+                if (st == HV_OK) st = sl.d_vg_views.reserve((uint64_t)nvw * sizeof(VgView), false, ctx->stream);
+                if (st == HV_OK) st = sl.d_vg_begin.reserve((uint64_t)(nvw + 1) * 4, false, ctx->stream);
+                if (st == HV_OK) st = sl.d_vg_ctr.reserve((uint64_t)nvw * HV_VG_COUNTER_COUNT * 4, false, ctx->stream);
+                if (st == HV_OK) st = sl.d_vg_ind.reserve((uint64_t)cfg->max_vg_draws * 20, false, ctx->stream);
+                if (st == HV_OK) st = sl.d_vg_md.reserve((uint64_t)cfg->max_vg_draws * 16, false, ctx->stream);
+                if (st != HV_OK) break;
+                s->slot_bytes += (uint64_t)cfg->max_vg_draws * 36 + (uint64_t)nvw * (sizeof(VgArgs) + sizeof(VgView) + 48);
+            }
+        }
         if (e != hipSuccess) {
             st = hip_error(e, "hv_server_create", __FILE__, __LINE__);
             break;
@@ -278,6 +397,12 @@ void hv_server_destroy(hv_server* s) {
         if (sl.h_ctr) (void)hipHostFree(sl.h_ctr);
         if (sl.h_args) (void)hipHostFree(sl.h_args);
         if (sl.h_ids) (void)hipHostFree(sl.h_ids);
+        if (sl.h_vg_args) (void)hipHostFree(sl.h_vg_args);
+        if (sl.h_vg_views) (void)hipHostFree(sl.h_vg_views);
+        if (sl.h_vg_begin) (void)hipHostFree(sl.h_vg_begin);
+        if (sl.h_vg_ctr) (void)hipHostFree(sl.h_vg_ctr);
+        for (DevBuf* b : {&sl.d_vg_args, &sl.d_vg_views, &sl.d_vg_begin, &sl.d_vg_ind, &sl.d_vg_md, &sl.d_vg_ctr})
+            b->release();
     }
     for (auto& kv : s->tickets) delete kv.second;
     delete s;
@@ -286,23 +411,43 @@ void hv_server_destroy(hv_server* s) {
 hv_status hv_server_submit(hv_server* s, const hv_server_request* r, uint64_t* ticket) {
     if (!s || !r || !ticket) return set_error(HV_EINVAL, "hv_server_submit: NULL argument");
     const hv_view_request& v = r->view;
-    if (!v.scene || !v.camera) return set_error(HV_EINVAL, "hv_server_submit: request without scene or camera");
-    if ((uint64_t)v.first + v.count > v.scene->count)
-        return set_error(HV_EINVAL, "hv_server_submit: range exceeds the scene");
-    if (!v.scene->mesh_count || v.scene->lod_count != s->cfg.cull.lod_count)
-        return set_error(HV_EINVAL, "hv_server_submit: scene LOD table does not match the server");
-    if (v.scene->count && v.scene->max_mesh_id >= v.scene->mesh_count)
-        return set_error(HV_EINVAL, "hv_server_submit: scene mesh ids exceed its LOD table");
-    const uint32_t nb = v.scene->mesh_count * v.scene->lod_count;
-    if (nb > s->cfg.max_buckets) return set_error(HV_ELIMIT, "hv_server_submit: %u buckets > max_buckets", nb);
-    if ((s->cfg.cull.flags & HV_CULL_OCCLUSION) && !v.hiz)
-        return set_error(HV_EINVAL, "hv_server_submit: occlusion needs a pyramid");
+    const hv_vg_request& g = r->vg;
+    if (!v.camera) return set_error(HV_EINVAL, "hv_server_submit: request without a camera");
+    if (!v.scene && !g.scene) return set_error(HV_EINVAL, "hv_server_submit: request without a scene");
+    uint32_t nb = 0;
+    if (v.scene) {
+        if ((uint64_t)v.first + v.count > v.scene->count)
+            return set_error(HV_EINVAL, "hv_server_submit: range exceeds the scene");
+        if (!v.scene->mesh_count || v.scene->lod_count != s->cfg.cull.lod_count)
+            return set_error(HV_EINVAL, "hv_server_submit: scene LOD table does not match the server");
+        if (v.scene->count && v.scene->max_mesh_id >= v.scene->mesh_count)
+            return set_error(HV_EINVAL, "hv_server_submit: scene mesh ids exceed its LOD table");
+        nb = v.scene->mesh_count * v.scene->lod_count;
+        if (nb > s->cfg.max_buckets) return set_error(HV_ELIMIT, "hv_server_submit: %u buckets > max_buckets", nb);
+        if ((s->cfg.cull.flags & HV_CULL_OCCLUSION) && !v.hiz)
+            return set_error(HV_EINVAL, "hv_server_submit: occlusion needs a pyramid");
+    } else if (v.count) {
+        return set_error(HV_EINVAL, "hv_server_submit: candidates without an instance scene");
+    }
+    if (g.scene) {
+        if (!s->cfg.max_vg_draws) return set_error(HV_EINVAL, "hv_server_submit: the server takes no meshlet requests");
+        if (!g.capacity || g.capacity > s->cfg.max_vg_draws)
+            return set_error(HV_ELIMIT, "hv_server_submit: meshlet capacity %u not in 1..%u", g.capacity,
+                             s->cfg.max_vg_draws);
+        if (g.flags & ~HV_CULL_REVERSED_Z) return set_error(HV_EINVAL, "hv_server_submit: unknown meshlet flags");
+        if (!g.width || !g.height) return set_error(HV_EINVAL, "hv_server_submit: empty meshlet screen");
+        if (g.hiz && (g.hiz->mode == HV_HIZ_MIN) != ((g.flags & HV_CULL_REVERSED_Z) != 0))
+            return set_error(HV_EINVAL, "hv_server_submit: pyramid mode does not match HV_CULL_REVERSED_Z");
+        if (g.hiz && !g.hiz->level0) return set_error(HV_ESTATE, "hv_server_submit: pyramid has no depth bound");
+        if (g.scene->ctx != s->ctx) return set_error(HV_EINVAL, "hv_server_submit: meshlet scene of another context");
+    }
     Req* q = new Req();
     q->r = *r;
     q->cam = *v.camera;
     q->r.view.camera = &q->cam;
     q->nb = nb;
-    q->bytes = CullBatch::bytes_for(v.count, 1, nb, 2048, CullBatch::max_grid(s->ctx));
+    q->bytes = v.scene ? CullBatch::bytes_for(v.count, 1, nb, 2048, CullBatch::max_grid(s->ctx)) : 0;
+    if (g.scene) q->bytes += (uint64_t)g.capacity * 36 + HV_VG_COUNTER_COUNT * 4;
     std::lock_guard<std::mutex> lk(s->mu);
     if (v.count > s->cfg.max_batch_candidates || q->bytes > s->cfg.device_budget_bytes) {
         ++s->n_rejected;

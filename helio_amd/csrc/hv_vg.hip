@@ -24,42 +24,10 @@
 #include <cstring>
 #include <vector>
 
-#include "hv_devmath.h"
-#include "hv_internal.h"
+#include "hv_vg.h"
 
 namespace hv {
 namespace {
-
-// The per-view uniforms (vg_cull.wgsl CullUniforms + camera). They live in a
-// small device buffer written at prepare, and each workgroup copies them to
-// LDS: as kernel arguments they would occupy ~70 SGPRs next to the buffer
-// pointers, and the spills cost a VALU readlane at every use.
-struct VgView {
-    float planes[24];  // publish_frustum_planes (vg_cull.wgsl:266-280)
-    float vp[16];
-    float p00, p11, camx, camy, camz, near, threshold;
-    uint32_t screen_h, hiz_mip_count, reversed, has_hiz;
-    dev::Pyramid hiz;
-};
-constexpr uint32_t kVgViewWords = sizeof(VgView) / 4;
-static_assert(sizeof(VgView) % 4 == 0 && kVgViewWords <= 256, "VgView is copied by one word per thread");
-
-struct VgArgs {
-    const hv_meshlet_entry* meshlets;
-    hv_vg_object* objects;
-    const hv_instance_data* instances;
-    const hv_instance_cull_data* icd;
-    const hv_vg_work_item* work_items;
-    hv_draw_indexed_indirect* indirect;
-    hv_vg_draw* metadata;
-    uint32_t* counters;
-    const VgView* view;
-    uint32_t meshlet_count, object_count_buf, instance_count, icd_count, work_item_buf_count;
-    uint32_t capacity, counter_len;
-    uint32_t object_count, work_item_count;
-    uint32_t select_blocks;    // leading workgroups of k_vg that select objects
-    uint32_t spans_per_block;  // work items per cull workgroup (8..64, a power of two)
-};
 
 using namespace dev;
 
@@ -249,7 +217,7 @@ __device__ __forceinline__ void vg_select_block(const VgArgs& a, uint32_t block,
             atomicAdd(&L.hist[sp1 - 1u], 1u);
             atomicMax(&L.max_lod, lc - 1u);
         }
-        o->selected_lod_plus_one = sp1;
+        if (a.write_selection) o->selected_lod_plus_one = sp1;
     }
     __syncthreads();
     if (a.counter_len >= 11u) {
@@ -437,13 +405,46 @@ __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, V
 // Select and cull in one launch: blocks [0, select_blocks) select objects,
 // the rest cull work items. The cull recomputes each object's LOD with the
 // same select_lod, so the two halves do not depend on each other.
-__global__ __launch_bounds__(256) void k_vg(VgArgs a) {
-    __shared__ VgLds L;
+__device__ __forceinline__ void vg_block(const VgArgs& a, uint32_t block, VgLds& L) {
     if (threadIdx.x < kVgViewWords)
         reinterpret_cast<uint32_t*>(&L.v)[threadIdx.x] = reinterpret_cast<const uint32_t*>(a.view)[threadIdx.x];
     __syncthreads();
-    if (blockIdx.x < a.select_blocks) vg_select_block(a, blockIdx.x, L);
-    else vg_cull_block(a, blockIdx.x - a.select_blocks, L);
+    if (block < a.select_blocks) vg_select_block(a, block, L);
+    else vg_cull_block(a, block - a.select_blocks, L);
+}
+
+__global__ __launch_bounds__(256) void k_vg(VgArgs a) {
+    __shared__ VgLds L;
+    vg_block(a, blockIdx.x, L);
+}
+
+// Scalar (constant address space) load of a request record: it is not
+// written while the kernel runs and the index is uniform per workgroup.
+template <class T>
+__device__ __forceinline__ T ldc(const T* p) {
+    return *(const __attribute__((address_space(4))) T*)p;
+}
+
+// Several requests in one launch: workgroup b belongs to request r with
+// begin[r] <= b < begin[r + 1].
+__global__ __launch_bounds__(256) void k_vg_multi(const VgArgs* __restrict__ reqs, const uint32_t* __restrict__ begin,
+                                                  uint32_t n) {
+    __shared__ VgLds L;
+    uint32_t r = 0;
+#pragma unroll
+    for (uint32_t step = 512; step; step >>= 1)
+        if (r + step < n && ldc(&begin[r + step]) <= blockIdx.x) r += step;
+    VgArgs a;
+    static_assert(sizeof(VgArgs) % 4 == 0, "VgArgs is loaded word by word");
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(reqs + r);
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(VgArgs) / 4; ++i) reinterpret_cast<uint32_t*>(&a)[i] = ldc(src + i);
+    vg_block(a, blockIdx.x - ldc(&begin[r]), L);
+}
+
+__global__ __launch_bounds__(256) void k_vg_clear_many(uint32_t* counters, uint32_t words) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < words) counters[i] = 0u;
 }
 
 }  // namespace
@@ -466,6 +467,74 @@ struct hv_vg_pass {
     hipEvent_t done = nullptr;
     bool done_recorded = false;
 };
+
+namespace hv {
+
+void vg_build_view(VgView& v, const hv_camera_uniforms* cam, uint32_t width, uint32_t height, float threshold,
+                   const hv_hiz* hiz, uint32_t flags) {
+    v = VgView{};
+    v.screen_h = height;
+    v.hiz_mip_count = hv_mip_levels(width, height);
+    v.threshold = threshold;
+    // publish_frustum_planes (vg_cull.wgsl:266-280): rows of view_proj, p / length(p.xyz), no guard.
+    const float* m = cam->view_proj;
+    const float pl[6][4] = {
+        {m[3] + m[0], m[7] + m[4], m[11] + m[8], m[15] + m[12]},
+        {m[3] - m[0], m[7] - m[4], m[11] - m[8], m[15] - m[12]},
+        {m[3] + m[1], m[7] + m[5], m[11] + m[9], m[15] + m[13]},
+        {m[3] - m[1], m[7] - m[5], m[11] - m[9], m[15] - m[13]},
+        {m[2], m[6], m[10], m[14]},
+        {m[3] - m[2], m[7] - m[6], m[11] - m[10], m[15] - m[14]},
+    };
+    for (int i = 0; i < 6; ++i) {
+        const float len = std::sqrt((pl[i][0] * pl[i][0] + pl[i][1] * pl[i][1]) + pl[i][2] * pl[i][2]);
+        for (int c = 0; c < 4; ++c) v.planes[4 * i + c] = pl[i][c] / len;
+    }
+    std::memcpy(v.vp, cam->view_proj, 64);
+    v.p00 = cam->proj[0];
+    v.p11 = cam->proj[5];
+    v.camx = cam->position_near[0];
+    v.camy = cam->position_near[1];
+    v.camz = cam->position_near[2];
+    v.near = cam->position_near[3];
+    v.reversed = (flags & HV_CULL_REVERSED_Z) ? 1u : 0u;
+    if (hiz) {
+        v.has_hiz = 1u;
+        v.hiz.level0 = hiz->level0;
+        v.hiz.packed = hiz->packed.as<float>();
+        v.hiz.w = hiz->width;
+        v.hiz.h = hiz->height;
+        v.hiz.levels = hiz->levels;
+        for (uint32_t k = 1; k < hiz->levels && k < 12; ++k) v.hiz.off[k] = (uint32_t)hiz->level_off[k];
+    }
+}
+
+void vg_grid(uint32_t object_count, uint32_t work_item_count, uint32_t& select_blocks, uint32_t& spans,
+             uint64_t& blocks) {
+    select_blocks = (object_count + 255u) / 256u;
+    spans = kVgSpans;
+    while (spans > kVgMinSpans && ((uint64_t)work_item_count + spans - 1) / spans < kVgTargetBlocks) spans >>= 1;
+    blocks = select_blocks + ((uint64_t)work_item_count + spans - 1) / spans;
+}
+
+hv_status vg_launch_multi(const VgArgs* d_reqs, const uint32_t* d_begin, uint32_t* counters, uint32_t n,
+                          uint64_t total_blocks, hipStream_t s) {
+    if (!n) return HV_OK;
+    if (n > 1024u) return set_error(HV_ELIMIT, "vg_launch_multi: %u requests exceed 1024", n);
+    if (total_blocks > 0x7fffffffull)
+        return set_error(HV_ELIMIT, "vg_launch_multi: %llu workgroups exceed the grid limit",
+                         (unsigned long long)total_blocks);
+    const uint32_t words = n * HV_VG_COUNTER_COUNT;
+    hipLaunchKernelGGL(k_vg_clear_many, dim3((words + 255) / 256), dim3(256), 0, s, counters, words);
+    HV_HIP(hipGetLastError());
+    if (total_blocks) {
+        hipLaunchKernelGGL(k_vg_multi, dim3((uint32_t)total_blocks), dim3(256), 0, s, d_reqs, d_begin, n);
+        HV_HIP(hipGetLastError());
+    }
+    return HV_OK;
+}
+
+}  // namespace hv
 
 static hv_status vg_grow(DevBuf& b, uint32_t& cap, uint64_t need, uint64_t elem, hipStream_t s) {
     if (need <= cap) return HV_OK;
@@ -597,41 +666,9 @@ hv_status hv_vg_prepare(hv_vg_pass* p, const hv_vg_frame_data* vg, const hv_came
     a.counter_len = HV_VG_COUNTER_COUNT;
     a.object_count = p->object_count;
     a.work_item_count = p->work_item_count;
+    a.write_selection = 1u;
     VgView v{};
-    v.screen_h = height;
-    v.hiz_mip_count = hv_mip_levels(width, height);
-    v.threshold = threshold;
-    // publish_frustum_planes (vg_cull.wgsl:266-280): rows of view_proj, p / length(p.xyz), no guard.
-    const float* m = cam->view_proj;
-    const float pl[6][4] = {
-        {m[3] + m[0], m[7] + m[4], m[11] + m[8], m[15] + m[12]},
-        {m[3] - m[0], m[7] - m[4], m[11] - m[8], m[15] - m[12]},
-        {m[3] + m[1], m[7] + m[5], m[11] + m[9], m[15] + m[13]},
-        {m[3] - m[1], m[7] - m[5], m[11] - m[9], m[15] - m[13]},
-        {m[2], m[6], m[10], m[14]},
-        {m[3] - m[2], m[7] - m[6], m[11] - m[10], m[15] - m[14]},
-    };
-    for (int i = 0; i < 6; ++i) {
-        const float len = std::sqrt((pl[i][0] * pl[i][0] + pl[i][1] * pl[i][1]) + pl[i][2] * pl[i][2]);
-        for (int c = 0; c < 4; ++c) v.planes[4 * i + c] = pl[i][c] / len;
-    }
-    std::memcpy(v.vp, cam->view_proj, 64);
-    v.p00 = cam->proj[0];
-    v.p11 = cam->proj[5];
-    v.camx = cam->position_near[0];
-    v.camy = cam->position_near[1];
-    v.camz = cam->position_near[2];
-    v.near = cam->position_near[3];
-    v.reversed = (flags & HV_CULL_REVERSED_Z) ? 1u : 0u;
-    if (hiz) {
-        v.has_hiz = 1u;
-        v.hiz.level0 = hiz->level0;
-        v.hiz.packed = hiz->packed.as<float>();
-        v.hiz.w = hiz->width;
-        v.hiz.h = hiz->height;
-        v.hiz.levels = hiz->levels;
-        for (uint32_t k = 1; k < hiz->levels && k < 12; ++k) v.hiz.off[k] = (uint32_t)hiz->level_off[k];
-    }
+    vg_build_view(v, cam, width, height, threshold, hiz, flags);
     // The previous execute has finished (done event above), so the uniforms
     // can be replaced; the copy completes before prepare returns, so an
     // execute on any stream sees them.
@@ -667,12 +704,8 @@ static hv_status vg_launch(hv_vg_pass* p, hipStream_t s, float* ms) {
     }
     if (ms) HV_HIP(hipEventRecord(p->ev[1], s));
     {
-        a.select_blocks = (p->object_count + 255u) / 256u;
-        uint32_t spans = kVgSpans;
-        while (spans > kVgMinSpans && ((uint64_t)p->work_item_count + spans - 1) / spans < kVgTargetBlocks) spans >>= 1;
-        a.spans_per_block = spans;
-        const uint64_t cull_blocks = ((uint64_t)p->work_item_count + spans - 1) / spans;
-        const uint64_t grid = a.select_blocks + cull_blocks;
+        uint64_t grid = 0;
+        vg_grid(p->object_count, p->work_item_count, a.select_blocks, a.spans_per_block, grid);
         if (grid > 0x7fffffffull) return set_error(HV_ELIMIT, "hv_vg_execute: %llu workgroups exceed the grid limit",
                                                    (unsigned long long)grid);
         hipLaunchKernelGGL(k_vg, dim3((uint32_t)grid), dim3(256), 0, s, a);
@@ -731,6 +764,58 @@ hv_status hv_vg_read(hv_vg_pass* p, hv_draw_indexed_indirect* indirect, hv_vg_dr
         for (uint32_t i = 0; i < p->object_count; ++i) selected[i] = objs[i].selected_lod_plus_one;
     }
     return HV_OK;
+}
+
+hv_status hv_vg_scene_create(hv_ctx* ctx, const hv_vg_frame_data* f, hv_vg_scene** out) {
+    if (!ctx || !f || !out) return set_error(HV_EINVAL, "hv_vg_scene_create: NULL argument");
+    if ((f->meshlet_count && !f->meshlets) || (f->object_count && (!f->objects || !f->instances)) ||
+        (f->work_item_count && !f->work_items))
+        return set_error(HV_EINVAL, "hv_vg_scene_create: NULL slice with a non-zero count");
+    HV_HIP(hipSetDevice(ctx->device));
+    hv_vg_scene* sc = new hv_vg_scene();
+    sc->ctx = ctx;
+    const hipStream_t st = ctx->stream;
+    auto fail = [&](hv_status e) {
+        hv_vg_scene_destroy(sc);
+        return e;
+    };
+    hv_status e = sc->meshlets.reserve(std::max<uint64_t>((uint64_t)f->meshlet_count * 64, 64), false, st);
+    if (e == HV_OK) e = sc->objects.reserve(std::max<uint64_t>((uint64_t)f->object_count * 128, 128), false, st);
+    if (e == HV_OK) e = sc->instances.reserve(std::max<uint64_t>((uint64_t)f->object_count * 144, 144), false, st);
+    if (e == HV_OK) e = sc->icd.reserve(std::max<uint64_t>((uint64_t)f->object_count * 16, 16), false, st);
+    if (e == HV_OK) e = sc->work_items.reserve(std::max<uint64_t>((uint64_t)f->work_item_count * 8, 8), false, st);
+    if (e != HV_OK) return fail(e);
+    std::vector<hv_instance_cull_data> icd(f->object_count);
+    if (f->object_count && (e = hv_instance_cull_data_from_instances(f->instances, f->object_count, icd.data())) != HV_OK)
+        return fail(e);
+    hipError_t he = hipSuccess;
+    if (f->meshlet_count)
+        he = hipMemcpyAsync(sc->meshlets.ptr, f->meshlets, (size_t)f->meshlet_count * 64, hipMemcpyHostToDevice, st);
+    if (he == hipSuccess && f->object_count) {
+        he = hipMemcpyAsync(sc->objects.ptr, f->objects, (size_t)f->object_count * 128, hipMemcpyHostToDevice, st);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(sc->instances.ptr, f->instances, (size_t)f->object_count * 144, hipMemcpyHostToDevice,
+                                st);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(sc->icd.ptr, icd.data(), (size_t)f->object_count * 16, hipMemcpyHostToDevice, st);
+    }
+    if (he == hipSuccess && f->work_item_count)
+        he = hipMemcpyAsync(sc->work_items.ptr, f->work_items, (size_t)f->work_item_count * 8, hipMemcpyHostToDevice, st);
+    if (he == hipSuccess) he = hipStreamSynchronize(st);
+    if (he != hipSuccess) return fail(hip_error(he, "hv_vg_scene_create", __FILE__, __LINE__));
+    sc->meshlet_count = f->meshlet_count;
+    sc->object_count = f->object_count;
+    sc->work_item_count = f->work_item_count;
+    *out = sc;
+    return HV_OK;
+}
+
+void hv_vg_scene_destroy(hv_vg_scene* sc) {
+    if (!sc) return;
+    (void)hipSetDevice(sc->ctx->device);
+    (void)hipDeviceSynchronize();
+    for (DevBuf* b : {&sc->meshlets, &sc->objects, &sc->instances, &sc->icd, &sc->work_items}) b->release();
+    delete sc;
 }
 
 hv_status hv_vg_build_work_items(const hv_vg_object* objects, uint32_t object_count, hv_vg_work_item* out,

@@ -13,7 +13,7 @@ import numpy as np
 
 from . import layouts as L
 from ._lib import (CullConfig, CullOutputs, DrawBuffers, ServerConfig, ServerRequest, ServerStats, ShadowCullBuffers,
-                   ShadowMatrixBuffers, VgFrameData, VgOutputs, ViewRequest, call, load, ptr)
+                   ShadowMatrixBuffers, VgFrameData, VgOutputs, VgRequest, ViewRequest, call, load, ptr)
 
 
 class Context:
@@ -544,19 +544,52 @@ class OcclusionCullPass:
         call("hv_occlusion_cull_execute", self.handle, C.byref(buffers), hiz.handle, stream)
 
 
+class VgScene:
+    """A device-resident virtual-geometry scene for the serving loop (uploaded
+    once; requests only read it)."""
+
+    def __init__(self, ctx: Context, meshlets: np.ndarray, objects: np.ndarray, instances: np.ndarray,
+                 work_items: np.ndarray | None = None):
+        self.ctx = ctx
+        wi = build_vg_work_items(objects) if work_items is None else work_items
+        keep = [np.ascontiguousarray(a) for a in (meshlets, objects, instances, wi)]
+        fd = VgFrameData()
+        fd.meshlets, fd.objects, fd.instances, fd.work_items = (ptr(a) for a in keep)
+        fd.meshlet_count, fd.object_count, fd.work_item_count = len(keep[0]), len(keep[1]), len(keep[3])
+        fd.max_draw_count = len(keep[0])
+        h = C.c_void_p()
+        call("hv_vg_scene_create", ctx.handle, C.byref(fd), C.byref(h))
+        self.handle = h
+        self.meshlet_count = len(keep[0])
+
+    def close(self) -> None:
+        if self.handle:
+            load().hv_vg_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class CullServer:
     """Serving loop: requests admitted by device memory, continuously batched
-    into graph-replayed launches on several streams (libhv.so worker thread)."""
+    into graph-replayed launches on several streams (libhv.so worker thread).
+    A request may also carry a meshlet cull of a resident VgScene."""
 
     def __init__(self, ctx: Context, max_batch_views: int = 64, max_inflight: int = 2,
                  max_batch_candidates: int = 1 << 24, device_budget_bytes: int = 1 << 30, max_buckets: int = 1024,
-                 lod_distances=(25.0, 75.0, 200.0), use_graph: bool = True, copy_ids: bool = True):
+                 lod_distances=(25.0, 75.0, 200.0), use_graph: bool = True, copy_ids: bool = True,
+                 max_vg_draws: int = 0):
         self.ctx = ctx
         c = ServerConfig()
         c.cull = cull_config(lod_distances)
         c.max_batch_views, c.max_inflight = max_batch_views, max_inflight
         c.max_batch_candidates, c.device_budget_bytes = max_batch_candidates, device_budget_bytes
         c.max_buckets, c.use_graph, c.copy_ids = max_buckets, int(use_graph), int(copy_ids)
+        c.max_vg_draws = max_vg_draws
         h = C.c_void_p()
         call("hv_server_create", ctx.handle, C.byref(c), C.byref(h))
         self.handle = h
@@ -574,33 +607,65 @@ class CullServer:
             pass
 
     @staticmethod
-    def request(scene: Scene, camera: np.ndarray, first: int, count: int, want_ids: bool = True):
-        """Build a request plus the host buffers it writes into."""
+    def request(scene: Scene | None, camera: np.ndarray, first: int = 0, count: int = 0, want_ids: bool = True,
+                vg: VgScene | None = None, vg_capacity: int = 0, screen=(1920, 1080), threshold_px: float = 0.5,
+                hiz: "HiZBuildPass | None" = None, reversed_z: bool = False):
+        """Build a request plus the host buffers it writes into. `vg` adds a
+        meshlet cull of that resident scene from the same camera."""
         cam = np.ascontiguousarray(camera, dtype=L.CAMERA).reshape(1)
-        nb = scene.info()["meshes"] * scene.info()["lods"]
-        ids = np.zeros(max(count, 1), np.uint32) if want_ids else None
-        args = np.zeros(max(nb, 1), L.DRAW_INDEXED_INDIRECT)
-        stats = np.zeros(9, np.uint32)
         r = ServerRequest()
-        r.view.scene, r.view.camera = scene.handle.value, cam.ctypes.data
-        r.view.first, r.view.count = first, count
-        r.ids_out = None if ids is None else ids.ctypes.data
-        r.args_out, r.stats_out = args.ctypes.data, stats.ctypes.data
-        return r, (cam, ids, args, stats)
+        r.view.camera = cam.ctypes.data
+        ids = args = None
+        stats = np.zeros(9, np.uint32)
+        if scene is not None:
+            nb = scene.info()["meshes"] * scene.info()["lods"]
+            ids = np.zeros(max(count, 1), np.uint32) if want_ids else None
+            args = np.zeros(max(nb, 1), L.DRAW_INDEXED_INDIRECT)
+            r.view.scene = scene.handle.value
+            r.view.first, r.view.count = first, count
+            r.ids_out = None if ids is None else ids.ctypes.data
+            r.args_out, r.stats_out = args.ctypes.data, stats.ctypes.data
+        vgo = None
+        if vg is not None:
+            cap = vg_capacity or vg.meshlet_count
+            vgo = (np.zeros(cap, L.DRAW_INDEXED_INDIRECT), np.zeros(cap, L.VG_DRAW), np.zeros(11, np.uint32))
+            g = r.vg
+            g.scene = vg.handle.value
+            g.width, g.height = screen
+            g.lod_error_threshold_px = threshold_px
+            g.flags = L.CULL_REVERSED_Z if reversed_z else 0
+            g.hiz = None if hiz is None else hiz.handle.value
+            g.capacity = cap
+            g.indirect_out, g.metadata_out, g.counters_out = (a.ctypes.data for a in vgo)
+        return r, (cam, ids, args, stats, vgo)
 
-    def submit(self, scene: Scene, camera: np.ndarray, first: int, count: int, want_ids: bool = True) -> int:
-        r, keep = self.request(scene, camera, first, count, want_ids)
+    def submit(self, scene: Scene | None, camera: np.ndarray, first: int = 0, count: int = 0, want_ids: bool = True,
+               **vg) -> int:
+        r, keep = self.request(scene, camera, first, count, want_ids, **vg)
         t = C.c_uint64()
         call("hv_server_submit", self.handle, C.byref(r), C.byref(t))
         self._live[t.value] = keep
         return t.value
 
     def wait(self, ticket: int, timeout_ms: int = -1):
-        """(ids, args, stats[8]) of a finished request."""
+        """(ids, args, stats[8]) of a finished instance request."""
+        res = self.wait_all(ticket, timeout_ms)
+        return res["ids"], res["args"], res["stats"]
+
+    def wait_all(self, ticket: int, timeout_ms: int = -1) -> dict:
+        """Every result of a finished request: ids, args, stats, and for a
+        meshlet part vg_indirect / vg_metadata (the published slots) and
+        vg_counters."""
         vis = C.c_uint32()
         call("hv_server_wait", self.handle, C.c_uint64(ticket), timeout_ms, C.byref(vis))
-        cam, ids, args, stats = self._live.pop(ticket)
-        return (None if ids is None else ids[: vis.value].copy()), args, stats[:8].copy()
+        cam, ids, args, stats, vgo = self._live.pop(ticket)
+        out = {"visible": vis.value, "args": args, "stats": stats[:8].copy(), "ids": None}
+        if ids is not None:
+            out["ids"] = ids[: int(stats[8])].copy()
+        if vgo is not None:
+            n = min(int(vgo[2][0]), len(vgo[0]))
+            out["vg_indirect"], out["vg_metadata"], out["vg_counters"] = vgo[0][:n].copy(), vgo[1][:n].copy(), vgo[2]
+        return out
 
     def stats(self) -> dict:
         st = ServerStats()

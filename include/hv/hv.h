@@ -470,6 +470,14 @@ hv_status hv_vg_outputs(const hv_vg_pass* pass, hv_vg_buffers* out);
 hv_status hv_vg_read(hv_vg_pass* pass, hv_draw_indexed_indirect* indirect, hv_vg_draw* metadata,
                      uint32_t capacity, uint32_t counters[HV_VG_COUNTER_COUNT], uint32_t* selected,
                      uint32_t selected_capacity);
+/* A device-resident virtual-geometry scene for the serving loop (its
+ * meshlet scenes): the frame data uploaded once, InstanceCullData derived
+ * at upload (InstanceCullData::from_instance, lib.rs:131-168). Requests on
+ * one scene may run concurrently: they never write to it. */
+typedef struct hv_vg_scene hv_vg_scene;
+hv_status hv_vg_scene_create(hv_ctx* ctx, const hv_vg_frame_data* frame, hv_vg_scene** out);
+void hv_vg_scene_destroy(hv_vg_scene* scene);
+
 /* append_work_items for every object (crates/helio/src/scene/virtual_geometry/rebuild.rs:45-58):
  * one record per 64-meshlet span of max_meshlet_count. */
 hv_status hv_vg_build_work_items(const hv_vg_object* objects, uint32_t object_count, hv_vg_work_item* out,
@@ -644,13 +652,32 @@ typedef struct hv_server_config {
     uint32_t max_buckets;           /* mesh_count * lod_count per request (sizes args buffers) */
     uint32_t use_graph;             /* replay each slot's captured hipGraph */
     uint32_t copy_ids;              /* copy visible ids back for requests that give ids_out */
+    uint32_t max_vg_draws;          /* meshlet draws per launch (sizes each slot's VG outputs);
+                                       0: requests with a meshlet part are refused */
 } hv_server_config;
 
+/* Meshlet part of a server request: VirtualGeometryPass semantics
+ * (vg_cull.wgsl select + cull) from the request's camera against a resident
+ * hv_vg_scene. */
+typedef struct hv_vg_request {
+    const hv_vg_scene* scene;            /* NULL: no meshlet cull */
+    uint32_t width, height;              /* screen: LOD focal length and Hi-Z mip count */
+    float lod_error_threshold_px;
+    uint32_t flags;                      /* 0 or HV_CULL_REVERSED_Z */
+    const hv_hiz* hiz;                   /* optional pyramid (HV_HIZ_MIN with HV_CULL_REVERSED_Z) */
+    uint32_t capacity;                   /* publication capacity (max_draw_count), >= 1 */
+    hv_draw_indexed_indirect* indirect_out; /* optional host buffer, >= capacity entries */
+    hv_vg_draw* metadata_out;            /* optional host buffer, >= capacity entries */
+    uint32_t* counters_out;              /* optional host buffer, HV_VG_COUNTER_COUNT entries */
+} hv_vg_request;
+
 typedef struct hv_server_request {
-    hv_view_request view;               /* the camera is copied at submit */
+    hv_view_request view;               /* the camera is copied at submit; scene may be NULL
+                                           (count 0) when the request has a meshlet part */
     uint32_t* ids_out;                  /* optional host buffer, >= view.count entries */
     hv_draw_indexed_indirect* args_out; /* optional host buffer, >= mesh_count * lod_count */
     uint32_t* stats_out;                /* optional host buffer, HV_STAT_COUNT + 1 (visible ids at [8]) */
+    hv_vg_request vg;                   /* optional meshlet cull of the same camera */
 } hv_server_request;
 
 typedef struct hv_server_stats {
@@ -670,7 +697,8 @@ void hv_server_destroy(hv_server* server);
  * slot, HV_EINVAL for a bad request. *ticket identifies it. */
 hv_status hv_server_submit(hv_server* server, const hv_server_request* request, uint64_t* ticket);
 /* Wait up to timeout_ms (< 0: forever) for a ticket; HV_EAGAIN on time-out.
- * On success the request's output buffers are filled and *visible is set. */
+ * On success the request's output buffers are filled and *visible is set:
+ * visible instance ids plus published meshlet draws (min(attempted, capacity)). */
 hv_status hv_server_wait(hv_server* server, uint64_t ticket, int32_t timeout_ms, uint32_t* visible);
 hv_status hv_server_stats_get(hv_server* server, hv_server_stats* out);
 hv_status hv_server_stats_reset(hv_server* server);

@@ -106,3 +106,94 @@ def test_run_trace_poisson(ctx):
         assert np.array_equal(got[2], ref[2])
         assert np.array_equal(args_as_u32(got[1]), args_as_u32(ref[1]))
     srv.close()
+
+
+# ---- meshlet requests -------------------------------------------------------
+
+def _vg_rows(ind, md):
+    """Published (instance, meshlet, lod, index_count, instance_count, first_index, base_vertex) rows, sorted."""
+    i5 = args_as_u32(ind)
+    m4 = np.ascontiguousarray(md).view(np.uint32).reshape(-1, 4)
+    rows = np.concatenate([m4[:, :3], i5[:, :4]], axis=1)
+    return rows[np.lexsort(rows.T[::-1])]
+
+
+def _vg_pass_result(ctx, sc, cam, capacity, hiz=None, rev=False):
+    from helio_amd.passes import VirtualGeometryPass
+    vg = VirtualGeometryPass(ctx, publication_limit=capacity)
+    vg.set_frame(sc.meshlets, sc.objects, sc.instances, max_draw_count=capacity)
+    vg.prepare(cam, 1920, 1080, 0.5, hiz, reversed_z=rev)
+    vg.execute()
+    ind, md, ctr, _ = vg.read()
+    vg.close()
+    return ind, md, ctr
+
+
+def test_meshlet_requests_alone_in_mix_and_pass(ctx):
+    """A meshlet part gives the VirtualGeometryPass result for its camera,
+    whether it runs alone, next to other meshlet requests, or beside instance
+    views in one launch (counters exact, published multiset exact)."""
+    from helio_amd import layouts as L
+    from helio_amd.passes import HiZBuildPass, VgScene
+    view, sc, depth = synth.vg_workload(512)
+    hiz = HiZBuildPass(ctx, 1920, 1080, L.HIZ_MIN)
+    hiz.upload_depth(depth)
+    hiz.execute()
+    vgs = VgScene(ctx, sc.meshlets, sc.objects, sc.instances)
+    scenes = _scenes(ctx)[:2]
+    g = synth.rng(4, 77)
+    fwd = np.asarray(view.camera["forward_far"][0][:3], np.float64)
+    cams = [view.camera] + [synth.perspective_view(g, position=view.position + g.normal(0, 20, 3),
+                                                   direction=fwd + g.normal(0, 0.05, 3), reversed_z=True).camera
+                            for _ in range(5)]
+    cap = 40_000
+    refs = [_vg_pass_result(ctx, sc, c, cap, hiz, True) for c in cams]
+    srv = CullServer(ctx, max_batch_views=16, max_inflight=2, max_vg_draws=6 * cap, copy_ids=True)
+    inst = _requests(scenes, len(cams), seed=3)
+    tickets = []
+    for k, c in enumerate(cams):
+        kw = dict(vg=vgs, vg_capacity=cap, hiz=hiz, reversed_z=True)
+        if k % 2:  # odd: instance view + meshlet part; even: meshlet part only
+            s, icam, f, n = inst[k]
+            tickets.append(srv.submit(s, c, f, n, **kw))
+        else:
+            tickets.append(srv.submit(None, c, **kw))
+    mixed = [srv.wait_all(t, 60_000) for t in tickets]
+    for k, (res, (ind, md, ctr)) in enumerate(zip(mixed, refs)):
+        assert np.array_equal(res["vg_counters"], ctr), k
+        assert ctr[0] > 0, k
+        assert np.array_equal(_vg_rows(res["vg_indirect"], res["vg_metadata"]), _vg_rows(ind, md)), k
+        assert np.array_equal(args_as_u32(res["vg_indirect"])[:, 4], np.arange(len(ind), dtype=np.uint32))
+        if k % 2:
+            s, icam, f, n = inst[k]
+            _same((res["ids"], res["args"], res["stats"]), _plain(ctx, (s, cams[k], f, n)))
+            assert res["visible"] == len(res["ids"]) + len(ind)
+        else:
+            assert res["visible"] == len(ind) and res["args"] is None
+    alone = srv.wait_all(srv.submit(None, cams[3], vg=vgs, vg_capacity=cap, hiz=hiz, reversed_z=True), 60_000)
+    assert np.array_equal(_vg_rows(alone["vg_indirect"], alone["vg_metadata"]), _vg_rows(*refs[3][:2]))
+    # Capacity: a request past the per-launch meshlet budget is refused at submit.
+    with pytest.raises(HvError):
+        srv.submit(None, cams[0], vg=vgs, vg_capacity=6 * cap + 1)
+    srv.close()
+    plain = CullServer(ctx, max_batch_views=4)
+    with pytest.raises(HvError):  # no meshlet budget configured
+        plain.submit(None, cams[0], vg=vgs, vg_capacity=10)
+    plain.close()
+    vgs.close()
+
+
+def test_meshlet_overflow_in_server(ctx):
+    """Publication past a request's capacity is counted (counter 1) and the
+    published slots are the first `capacity` ones, as in the pass."""
+    from helio_amd.passes import VgScene
+    cam, sc = synth.benchmark_vg_case(64, 1024)
+    vgs = VgScene(ctx, sc.meshlets, sc.objects, sc.instances)
+    srv = CullServer(ctx, max_batch_views=8, max_vg_draws=1 << 17)
+    res = srv.wait_all(srv.submit(None, cam, vg=vgs, vg_capacity=50_000, threshold_px=2.0), 60_000)
+    assert res["vg_counters"][0] == 65536 and res["vg_counters"][1] == 65536 - 50_000
+    assert len(res["vg_indirect"]) == 50_000 and res["visible"] == 50_000
+    keys = np.ascontiguousarray(res["vg_metadata"]).view(np.uint32).reshape(-1, 4)
+    assert len(np.unique(keys[:, 0].astype(np.uint64) * 1024 + keys[:, 1])) == 50_000
+    srv.close()
+    vgs.close()
