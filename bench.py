@@ -108,12 +108,15 @@ def cpu_baseline(sc, views, budget_s: float, threads: int) -> dict:
     }
 
 
-def serving_leg(ctx, rates, seconds: float) -> dict:
-    """Config 4: 8 resident scenes (10k..8M instances, log-spaced), one view per
-    request (perspective or ortho, whole scene), Poisson arrivals, batched
-    server vs the plain per-request path (1 view per launch, no graph)."""
+def serving_leg(ctx, rates, seconds: float, meshlet_fraction: float = 0.25) -> dict:
+    """Config 4: 8 resident scenes (10k..8M instances, log-spaced) and a
+    1,048,576-meshlet scene (2,048 objects x 512 meshlets); one view per
+    request (perspective or ortho, whole scene), a quarter of the requests
+    also culling the meshlet scene from the same camera; Poisson arrivals,
+    batched server vs the plain per-request path (1 request per launch, no
+    graph)."""
     from helio_amd import synth
-    from helio_amd.passes import CullServer, Scene
+    from helio_amd.passes import CullServer, Scene, VgScene
 
     sizes = np.unique(np.round(np.exp(np.linspace(np.log(10_000), np.log(8_000_000), 8))).astype(int))
     scenes = []
@@ -123,6 +126,9 @@ def serving_leg(ctx, rates, seconds: float) -> dict:
         s.upload_soa(sc.bounds, sc.mesh_ids, sc.flags)
         s.set_mesh_lods(sc.mesh_lods, sc.mesh_count, sc.lod_count)
         scenes.append(s)
+    vgsc = synth.vg_scene(2048, lod_counts=(384, 96, 32), config=4, index=100)
+    vg = VgScene(ctx, vgsc.meshlets, vgsc.objects, vgsc.instances)
+    vg_cap = 1 << 18
     g = synth.rng(4, 1000)
 
     def trace(n):
@@ -133,14 +139,17 @@ def serving_leg(ctx, rates, seconds: float) -> dict:
                 v = synth.ortho_view(g, g.uniform(0, 1000, 3), float(g.uniform(50, 400)), 2000.0, g.normal(size=3))
             else:
                 v = synth.perspective_view(g)
-            reqs.append(CullServer.request(scenes[si], v.camera, 0, int(sizes[si]), want_ids=False))
+            kw = dict(vg=vg, vg_capacity=vg_cap, vg_outputs=False) if g.random() < meshlet_fraction else {}
+            reqs.append((CullServer.request(scenes[si], v.camera, 0, int(sizes[si]), want_ids=False, **kw),
+                         int(sizes[si]), bool(kw)))
         return reqs
 
-    out = {"scenes": [int(x) for x in sizes], "rates": {}}
+    out = {"scenes": [int(x) for x in sizes], "meshlet_scene": int(vg.meshlet_count),
+           "meshlet_fraction": meshlet_fraction, "rates": {}}
     batched = CullServer(ctx, max_batch_views=64, max_inflight=3, max_batch_candidates=64 << 20,
-                         device_budget_bytes=16 << 30, copy_ids=False, use_graph=True)
+                         device_budget_bytes=16 << 30, copy_ids=False, use_graph=True, max_vg_draws=16 * vg_cap)
     plain = CullServer(ctx, max_batch_views=1, max_inflight=1, max_batch_candidates=8 << 20,
-                       device_budget_bytes=16 << 30, copy_ids=False, use_graph=False)
+                       device_budget_bytes=16 << 30, copy_ids=False, use_graph=False, max_vg_draws=vg_cap)
     for rate in rates:
         n = max(200, int(rate * seconds))
         row = {}
@@ -148,15 +157,19 @@ def serving_leg(ctx, rates, seconds: float) -> dict:
             reqs = trace(n)
             arr = np.cumsum(np.random.default_rng(rate).exponential(1.0 / rate, n))
             srv.reset_stats()
-            lat, wall = srv.run_trace([r for r, _ in reqs], arr)
+            lat, wall = srv.run_trace([r for (r, _), _, _ in reqs], arr)
             st = srv.stats()
+            with_vg = np.array([m for _, _, m in reqs])
             row[name] = {
                 "requests": n,
                 "views_per_s": n / wall,
                 "latency_ms_p50": float(np.percentile(lat, 50) * 1e3),
                 "latency_ms_p99": float(np.percentile(lat, 99) * 1e3),
+                "meshlet_requests": int(with_vg.sum()),
+                "meshlet_latency_ms_p99": float(np.percentile(lat[with_vg], 99) * 1e3) if with_vg.any() else None,
                 "mean_batch_views": st["mean_batch_views"],
-                "instances_per_s": float(sum(r.view.count for r, _ in reqs) / wall),
+                "instances_per_s": float(sum(c for _, c, _ in reqs) / wall),
+                "meshlets_per_s": float(with_vg.sum() * vg.meshlet_count / wall),
             }
             log(f"serving {name} @ {rate}/s: {row[name]['views_per_s']:.0f} views/s, "
                 f"p50 {row[name]['latency_ms_p50']:.3f} ms, p99 {row[name]['latency_ms_p99']:.3f} ms, "
@@ -164,7 +177,52 @@ def serving_leg(ctx, rates, seconds: float) -> dict:
         out["rates"][str(rate)] = row
     batched.close()
     plain.close()
+    vg.close()
     return out
+
+
+def shadow_leg(ctx, steps: int) -> dict:
+    """Multi-view producers: the matrices of 42 shadowed lights (36 point x 6
+    cube faces, 4 spot, 2 directional x 4 cascades) and the per-face cull of
+    65,536 movable draws against every dirty face."""
+    from helio_amd import layouts as L
+    from helio_amd import synth
+    from helio_amd.passes import DeviceBuffer, ShadowCullPass, ShadowMatrixPass
+    from helio_amd._lib import call
+    import ctypes as C
+
+    g = synth.rng(6, 1)
+    lights = synth.shadow_lights(g, n_point=36, n_spot=4, n_directional=2, n_unshadowed=0)
+    cam = synth.perspective_view(g).camera
+    inst, draws = synth.shadow_draws(g, 65536, 65536, centers=lights["position_range"][:, :3])
+    d_l, d_c = DeviceBuffer.from_array(ctx, lights), DeviceBuffer.from_array(ctx, cam)
+    d_m = DeviceBuffer.from_array(ctx, np.zeros((L.SHADOW_MAX_FACES, 16), np.float32))
+    d_d, d_h = DeviceBuffer.from_array(ctx, np.zeros(42, np.uint32)), DeviceBuffer.from_array(ctx, np.zeros(42, np.uint32))
+    d_i, d_dr = DeviceBuffer.from_array(ctx, inst), DeviceBuffer.from_array(ctx, draws)
+    d_fd = DeviceBuffer.from_array(ctx, np.ones(L.SHADOW_MAX_FACES, np.uint32))
+    mp = ShadowMatrixPass(ctx, 4096)
+    cp = ShadowCullPass(ctx, 4096, L.SHADOW_PLANES_ROWS)
+
+    def timed(fn):
+        ts = []
+        for i in range(steps + 3):
+            call("hv_timer_record", ctx.handle, 0, None)
+            fn()
+            call("hv_timer_record", ctx.handle, 1, None)
+            ms = C.c_float()
+            call("hv_timer_elapsed_ms", ctx.handle, 0, 1, C.byref(ms))
+            if i >= 3:
+                ts.append(ms.value)
+        return float(np.median(ts))
+
+    mat_ms = timed(lambda: mp.execute(d_l, len(lights), d_m, d_c, d_d, d_h))
+    cull_ms = timed(lambda: cp.execute(d_m, d_i, len(inst), d_dr, len(draws), d_fd, face_count=256))
+    counts, _ = cp.read(with_commands=False)
+    mp.close()
+    cp.close()
+    return {"lights": int(len(lights)), "faces": 256, "draws": int(len(draws)), "matrices_ms": mat_ms,
+            "cull_ms": cull_ms, "draw_face_tests_per_s": len(draws) * 256 / (cull_ms * 1e-3),
+            "visible_draw_faces": int(counts.sum())}
 
 
 def occlusion_leg(ctx, steps: int) -> dict:
@@ -410,7 +468,7 @@ def main() -> int:
         ctx.synchronize()
         lat0.append((time.perf_counter() - t) * 1e3)
 
-    occl = vgres = None
+    occl = vgres = shadow = None
     if not args.no_extra:
         log("occlusion leg")
         occl = occlusion_leg(ctx, max(10, args.steps // 2))
@@ -418,6 +476,9 @@ def main() -> int:
         log("vg leg")
         vgres = vg_leg(ctx, max(10, args.steps // 2))
         log(f"vg: {vgres}")
+        log("shadow leg")
+        shadow = shadow_leg(ctx, max(10, args.steps // 2))
+        log(f"shadow: {shadow}")
 
     serving = None
     if not args.no_serving:
@@ -482,6 +543,7 @@ def main() -> int:
         "serving": serving,
         "occlusion_1m": occl,
         "virtual_geometry": vgres,
+        "shadow_producers": shadow,
         "single_view_65536": {
             "latency_ms_p50": pct(lat0, 50),
             "latency_ms_p99": pct(lat0, 99),

@@ -9,10 +9,11 @@
 //                      six normalised planes of every dirty face ONCE
 //                      (shadow_cull.wgsl re-derives them per draw and face) and
 //                      the compacted list of dirty faces.
-//   k_shadow_cull      one lane per draw: the instance sphere against every
-//                      dirty face (planes from LDS); per face, a wave ballot
-//                      and one atomic per wave reserve the visible draws'
-//                      slots (the reference takes one atomic per draw and face).
+//   k_shadow_cull      one lane per draw and one workgroup per (256 draws,
+//                      16 dirty faces): the instance sphere against each face
+//                      of the group (planes from LDS); per face one atomic per
+//                      workgroup reserves the visible draws' slots (the
+//                      reference takes one atomic per visible draw and face).
 //
 // Float semantics follow the rest of the library: binary32, no contraction,
 // dot products and matrix products summed left to right, WGSL max/min as
@@ -366,14 +367,21 @@ __global__ __launch_bounds__(256) void k_shadow_planes(CullArgs a) {
     if (f == 255u) a.table->n_active = base + (uint32_t)__popcll(m);
 }
 
+// Dirty faces per workgroup of k_shadow_cull (blockIdx.y picks the group).
+constexpr uint32_t kFaceGroup = 16;
+constexpr uint32_t kFaceGroups = HV_SHADOW_MAX_FACES / kFaceGroup;
+
 __global__ __launch_bounds__(256) void k_shadow_cull(CullArgs a) {
-    __shared__ float s_pl[HV_SHADOW_MAX_FACES][24];
-    __shared__ uint32_t s_face[HV_SHADOW_MAX_FACES];
-    const uint32_t tid = threadIdx.x, lane = tid & 63u;
-    const uint32_t n = a.table->n_active;
-    for (uint32_t i = tid; i < n * 24u; i += 256u) (&s_pl[0][0])[i] = (&a.table->planes[0][0])[i];
-    for (uint32_t i = tid; i < n; i += 256u) s_face[i] = a.table->face[i];
-    __syncthreads();
+    __shared__ float s_pl[kFaceGroup][24];
+    __shared__ uint32_t s_face[kFaceGroup];
+    __shared__ uint64_t s_mask[4][kFaceGroup];  // per wave and face: visible lanes
+    __shared__ uint32_t s_off[4][kFaceGroup];   // per wave and face: first slot of the wave's draws
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t g0 = blockIdx.y * kFaceGroup, n = a.table->n_active;
+    if (g0 >= n) return;  // the grid covers all 256 faces; only the dirty ones have work
+    const uint32_t nf = min(kFaceGroup, n - g0);
+    for (uint32_t i = tid; i < nf * 24u; i += 256u) (&s_pl[0][0])[i] = (&a.table->planes[g0][0])[i];
+    if (tid < nf) s_face[tid] = a.table->face[g0 + tid];
     const uint32_t d = blockIdx.x * 256u + tid;
     bool live = false;
     uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0;  // the draw's five words (index_count .. first_instance)
@@ -394,23 +402,42 @@ __global__ __launch_bounds__(256) void k_shadow_cull(CullArgs a) {
         }
         live = !(r <= 0.0f);  // shadow_cull.wgsl:84 (NaN radii stay live, as there)
     }
-    if (__ballot(live) == 0ull) return;
-    for (uint32_t k = 0; k < n; ++k) {
-        const float* p = s_pl[k];
+    __syncthreads();
+    // Every wave records its visible lanes per face in LDS, one thread per
+    // face reserves the workgroup's slots with ONE atomic, then the waves
+    // write their draws (the reference takes one atomic per visible draw and
+    // face).
+    for (uint32_t j = 0; j < nf; ++j) {
+        const float* p = s_pl[j];
         bool vis = live;
 #pragma unroll
         for (int i = 0; i < 6; ++i)
             vis = vis && !(((p[4 * i] * cx + p[4 * i + 1] * cy) + p[4 * i + 2] * cz) + p[4 * i + 3] + r < 0.0f);
         const uint64_t m = __ballot(vis);
-        if (m == 0ull) continue;
-        const uint32_t face = s_face[k];
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&a.counts[face], (uint32_t)__popcll(m));
-        base = __shfl(base, 0);
-        if (vis) {
-            const uint32_t slot = base + dev::mbcnt(m);
+        if (lane == 0) s_mask[wave][j] = m;
+    }
+    __syncthreads();
+    if (tid < nf) {
+        uint32_t c[4], tot = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            c[w] = (uint32_t)__popcll(s_mask[w][tid]);
+            tot += c[w];
+        }
+        uint32_t base = tot ? atomicAdd(&a.counts[s_face[tid]], tot) : 0u;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            s_off[w][tid] = base;
+            base += c[w];
+        }
+    }
+    __syncthreads();
+    for (uint32_t j = 0; j < nf; ++j) {
+        const uint64_t m = s_mask[wave][j];
+        if ((m >> lane) & 1ull) {
+            const uint32_t slot = s_off[wave][j] + dev::mbcnt(m);
             if (slot < a.max_per_face) {
-                uint32_t* dw = reinterpret_cast<uint32_t*>(a.dst + ((size_t)face * a.max_per_face + slot));
+                uint32_t* dw = reinterpret_cast<uint32_t*>(a.dst + ((size_t)s_face[j] * a.max_per_face + slot));
                 dw[0] = w0;
                 dw[1] = w1;
                 dw[2] = w2;
@@ -522,7 +549,7 @@ hv_status hv_shadow_cull_execute(hv_shadow_cull_pass* p, const hv_shadow_cull_bu
     const hipStream_t s = stream ? (hipStream_t)stream : p->ctx->stream;
     hipLaunchKernelGGL(k_shadow_planes, dim3(1), dim3(HV_SHADOW_MAX_FACES), 0, s, a);
     HV_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_shadow_cull, dim3((b->draw_count + 255) / 256), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_shadow_cull, dim3((b->draw_count + 255) / 256, kFaceGroups), dim3(256), 0, s, a);
     HV_HIP(hipGetLastError());
     HV_HIP(hipEventRecord(p->done, s));
     p->done_recorded = true;

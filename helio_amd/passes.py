@@ -609,9 +609,11 @@ class CullServer:
     @staticmethod
     def request(scene: Scene | None, camera: np.ndarray, first: int = 0, count: int = 0, want_ids: bool = True,
                 vg: VgScene | None = None, vg_capacity: int = 0, screen=(1920, 1080), threshold_px: float = 0.5,
-                hiz: "HiZBuildPass | None" = None, reversed_z: bool = False):
+                hiz: "HiZBuildPass | None" = None, reversed_z: bool = False, vg_outputs: bool = True):
         """Build a request plus the host buffers it writes into. `vg` adds a
-        meshlet cull of that resident scene from the same camera."""
+        meshlet cull of that resident scene from the same camera (its
+        published commands are copied back unless vg_outputs is False; the
+        counters always are)."""
         cam = np.ascontiguousarray(camera, dtype=L.CAMERA).reshape(1)
         r = ServerRequest()
         r.view.camera = cam.ctypes.data
@@ -628,7 +630,8 @@ class CullServer:
         vgo = None
         if vg is not None:
             cap = vg_capacity or vg.meshlet_count
-            vgo = (np.zeros(cap, L.DRAW_INDEXED_INDIRECT), np.zeros(cap, L.VG_DRAW), np.zeros(11, np.uint32))
+            n_out = cap if vg_outputs else 0
+            vgo = (np.zeros(n_out, L.DRAW_INDEXED_INDIRECT), np.zeros(n_out, L.VG_DRAW), np.zeros(11, np.uint32))
             g = r.vg
             g.scene = vg.handle.value
             g.width, g.height = screen
@@ -636,7 +639,8 @@ class CullServer:
             g.flags = L.CULL_REVERSED_Z if reversed_z else 0
             g.hiz = None if hiz is None else hiz.handle.value
             g.capacity = cap
-            g.indirect_out, g.metadata_out, g.counters_out = (a.ctypes.data for a in vgo)
+            g.indirect_out, g.metadata_out = (vgo[0].ctypes.data, vgo[1].ctypes.data) if vg_outputs else (None, None)
+            g.counters_out = vgo[2].ctypes.data
         return r, (cam, ids, args, stats, vgo)
 
     def submit(self, scene: Scene | None, camera: np.ndarray, first: int = 0, count: int = 0, want_ids: bool = True,
