@@ -10,9 +10,10 @@
 //                 :354-405): wave 0 decodes the 64 items at once (one lane
 //                 each, recomputing the object's LOD so no cull waits for a
 //                 select workgroup), compacts the live spans into LDS with the
-//                 instance transforms, then all 256 threads cull the flattened
-//                 meshlets (cull_meshlet, :135-264) with four 64-byte
-//                 descriptors in flight per thread. Survivors are staged in LDS
+//                 instance transforms, then all 512 threads cull the flattened
+//                 meshlets (cull_meshlet, :135-264), each thread loading its
+//                 next descriptor while it tests the current one (the per-view
+//                 uniforms are read from an LDS copy). Survivors are staged in LDS
 //                 and ONE slot reservation (a single-address atomic, which
 //                 serialises device-wide) covers the workgroup; the
 //                 capacity-overflow count is derived from the same reservation.
@@ -180,13 +181,16 @@ __device__ __forceinline__ bool cull_meshlet(const VgView& a, const float* xf, f
 }
 
 // At most kVgSpans work items (64-meshlet spans) are decoded per cull
-// workgroup, one per lane of wave 0; kVgUnroll flattened meshlets are in
-// flight per thread. The host lowers the spans per workgroup (down to
-// kVgMinSpans) until the grid has kVgTargetBlocks cull workgroups, so short
-// work lists still spread over every CU.
+// workgroup, one per lane of wave 0; the survivors' output writes load
+// kVgUnroll descriptors per thread ahead of their stores. The host lowers the
+// spans per workgroup (down to kVgMinSpans) until the grid has
+// kVgTargetBlocks cull workgroups, so short work lists still spread over every
+// CU. Fewer, larger workgroups win here: each reserves its slots with one
+// single-address atomic, and those serialise device-wide.
+constexpr uint32_t kVgThreads = 512;  // threads of a k_vg / k_vg_multi workgroup
 constexpr uint32_t kVgSpans = 64;
 constexpr uint32_t kVgMinSpans = 8;
-constexpr uint32_t kVgTargetBlocks = 1024;
+constexpr uint32_t kVgTargetBlocks = 512;
 constexpr uint32_t kVgUnroll = 4;
 
 struct VgLds {
@@ -200,7 +204,7 @@ struct VgLds {
     uint32_t nspans, nsurv, base, max_lod;
 };
 
-// One select workgroup: 256 objects (cs_select_objects). Writes
+// One select workgroup: kVgThreads objects (cs_select_objects). Writes
 // selected_lod_plus_one and the histogram counters [2..10].
 __device__ __forceinline__ void vg_select_block(const VgArgs& a, uint32_t block, VgLds& L) {
     const uint32_t tid = threadIdx.x;
@@ -208,7 +212,7 @@ __device__ __forceinline__ void vg_select_block(const VgArgs& a, uint32_t block,
     if (tid == 0) L.max_lod = 0u;
     __syncthreads();
     const VgView& v = L.v;
-    const uint32_t i = block * 256u + tid;
+    const uint32_t i = block * kVgThreads + tid;
     if (i < a.object_count && i < a.object_count_buf) {
         hv_vg_object* o = a.objects + i;
         uint32_t lc = 0;
@@ -231,8 +235,9 @@ __device__ __forceinline__ void vg_select_block(const VgArgs& a, uint32_t block,
 //      with select_lod, so the cull does not wait for the select workgroups),
 //      the span of the selected LOD, the instance's transform; live spans are
 //      compacted into LDS with their meshlet-count prefix;
-//   B. the 256 threads walk the flattened meshlets of the live spans,
-//      kVgUnroll descriptors in flight per thread, and stage survivors in LDS;
+//   B. the kVgThreads threads walk the flattened meshlets of the live spans,
+//      the next descriptor loading while the current one is tested, and
+//      stage survivors in LDS;
 //   C. one slot reservation for the workgroup, then the survivors' commands and
 //      metadata are written to consecutive slots.
 __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, VgLds& L) {
@@ -322,11 +327,11 @@ __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, V
         }
     }
 #pragma unroll 1
-    for (uint32_t f0 = 0; f0 < T; f0 += 256u) {
+    for (uint32_t f0 = 0; f0 < T; f0 += kVgThreads) {
         const uint32_t ck = nk, cli = nli;
         const bool cact = nact;
         const float4 c0 = n0, c1 = n1, c2 = n2;
-        const uint32_t f = f0 + 256u + tid;
+        const uint32_t f = f0 + kVgThreads + tid;
         nact = false;
         if (f < T) {
             const uint32_t mi = locate(f, nk, nli);
@@ -363,12 +368,12 @@ __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, V
     // Survivors below the capacity: the descriptors' draw fields are loaded
     // kVgUnroll at a time before any store, so the reloads overlap.
     const uint32_t keep = b < a.capacity ? min(tot, a.capacity - b) : 0u;
-    for (uint32_t e0 = 0; e0 < keep; e0 += 256u * kVgUnroll) {
+    for (uint32_t e0 = 0; e0 < keep; e0 += kVgThreads * kVgUnroll) {
         uint4 q3[kVgUnroll];
         uint32_t mi[kVgUnroll], kk[kVgUnroll];
 #pragma unroll
         for (uint32_t u = 0; u < kVgUnroll; ++u) {
-            const uint32_t e = e0 + u * 256u + tid;
+            const uint32_t e = e0 + u * kVgThreads + tid;
             kk[u] = 0;
             mi[u] = 0;
             q3[u] = make_uint4(0, 0, 0, 0);
@@ -381,7 +386,7 @@ __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, V
         }
 #pragma unroll
         for (uint32_t u = 0; u < kVgUnroll; ++u) {
-            const uint32_t e = e0 + u * 256u + tid;
+            const uint32_t e = e0 + u * kVgThreads + tid;
             if (e < keep) {
                 const uint32_t slot = b + e;
                 hv_draw_indexed_indirect cmd;
@@ -413,7 +418,7 @@ __device__ __forceinline__ void vg_block(const VgArgs& a, uint32_t block, VgLds&
     else vg_cull_block(a, block - a.select_blocks, L);
 }
 
-__global__ __launch_bounds__(256) void k_vg(VgArgs a) {
+__global__ __launch_bounds__(kVgThreads) void k_vg(VgArgs a) {
     __shared__ VgLds L;
     vg_block(a, blockIdx.x, L);
 }
@@ -427,7 +432,7 @@ __device__ __forceinline__ T ldc(const T* p) {
 
 // Several requests in one launch: workgroup b belongs to request r with
 // begin[r] <= b < begin[r + 1].
-__global__ __launch_bounds__(256) void k_vg_multi(const VgArgs* __restrict__ reqs, const uint32_t* __restrict__ begin,
+__global__ __launch_bounds__(kVgThreads) void k_vg_multi(const VgArgs* __restrict__ reqs, const uint32_t* __restrict__ begin,
                                                   uint32_t n) {
     __shared__ VgLds L;
     uint32_t r = 0;
@@ -511,7 +516,7 @@ void vg_build_view(VgView& v, const hv_camera_uniforms* cam, uint32_t width, uin
 
 void vg_grid(uint32_t object_count, uint32_t work_item_count, uint32_t& select_blocks, uint32_t& spans,
              uint64_t& blocks) {
-    select_blocks = (object_count + 255u) / 256u;
+    select_blocks = (object_count + kVgThreads - 1) / kVgThreads;
     spans = kVgSpans;
     while (spans > kVgMinSpans && ((uint64_t)work_item_count + spans - 1) / spans < kVgTargetBlocks) spans >>= 1;
     blocks = select_blocks + ((uint64_t)work_item_count + spans - 1) / spans;
@@ -528,7 +533,7 @@ hv_status vg_launch_multi(const VgArgs* d_reqs, const uint32_t* d_begin, uint32_
     hipLaunchKernelGGL(k_vg_clear_many, dim3((words + 255) / 256), dim3(256), 0, s, counters, words);
     HV_HIP(hipGetLastError());
     if (total_blocks) {
-        hipLaunchKernelGGL(k_vg_multi, dim3((uint32_t)total_blocks), dim3(256), 0, s, d_reqs, d_begin, n);
+        hipLaunchKernelGGL(k_vg_multi, dim3((uint32_t)total_blocks), dim3(kVgThreads), 0, s, d_reqs, d_begin, n);
         HV_HIP(hipGetLastError());
     }
     return HV_OK;
@@ -708,7 +713,7 @@ static hv_status vg_launch(hv_vg_pass* p, hipStream_t s, float* ms) {
         vg_grid(p->object_count, p->work_item_count, a.select_blocks, a.spans_per_block, grid);
         if (grid > 0x7fffffffull) return set_error(HV_ELIMIT, "hv_vg_execute: %llu workgroups exceed the grid limit",
                                                    (unsigned long long)grid);
-        hipLaunchKernelGGL(k_vg, dim3((uint32_t)grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_vg, dim3((uint32_t)grid), dim3(kVgThreads), 0, s, a);
         HV_HIP(hipGetLastError());
     }
     HV_HIP(hipEventRecord(p->done, s));
