@@ -143,3 +143,19 @@ def test_occlusion_parity(ctx, rev):
     ref = _oracle(sc, view.camera, pyr=pyr, flags=flags)
     _check(gpu, ref)
     assert ref[2][4] > 0, "no instance was occluded: the test would not pin the Hi-Z path"
+
+
+def test_occlusion_parity_config1_full(ctx):
+    """Config 1 at its full size: 1,048,576 spheres against the 11-level min
+    pyramid of 2,000 rasterised occluder boxes (reversed Z)."""
+    view, depth, _ = synth.occluder_view_and_depth(config=1, threads=16)
+    sc = synth.instance_scene(1_048_576, config=1)
+    print("config 1 full: 1,048,576 spheres", flush=True)
+    hiz = HiZBuildPass(ctx, 1920, 1080, L.HIZ_MIN)
+    hiz.upload_depth(depth)
+    hiz.execute()
+    assert np.array_equal(hiz.read_packed(), oracle.hiz_build(depth, L.HIZ_MIN))
+    gpu = _run(ctx, sc, view.camera, hiz=hiz, occl=True, rev=True)
+    ref = _oracle(sc, view.camera, pyr=oracle.pyramid(depth, 1), flags=L.CULL_OCCLUSION | L.CULL_REVERSED_Z)
+    _check(gpu, ref)
+    assert ref[2][4] > 10_000
