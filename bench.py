@@ -140,18 +140,18 @@ def serving_leg(ctx, rates, seconds: float, meshlet_fraction: float = 0.25) -> d
             else:
                 v = synth.perspective_view(g)
             kw = dict(vg=vg, vg_capacity=vg_cap, vg_outputs=False) if g.random() < meshlet_fraction else {}
-            reqs.append((CullServer.request(scenes[si], v.camera, 0, int(sizes[si]), want_ids=False, **kw),
+            reqs.append((CullServer.request(scenes[si], v.camera, 0, int(sizes[si]), want_ids=False, outputs=False, **kw),
                          int(sizes[si]), bool(kw)))
         return reqs
 
     out = {"scenes": [int(x) for x in sizes], "meshlet_scene": int(vg.meshlet_count),
-           "meshlet_fraction": meshlet_fraction, "rates": {}}
+           "meshlet_fraction": meshlet_fraction, "rates": {}, "max_requests_per_rate": 30_000}
     batched = CullServer(ctx, max_batch_views=64, max_inflight=3, max_batch_candidates=64 << 20,
                          device_budget_bytes=16 << 30, copy_ids=False, use_graph=True, max_vg_draws=16 * vg_cap)
     plain = CullServer(ctx, max_batch_views=1, max_inflight=1, max_batch_candidates=8 << 20,
                        device_budget_bytes=16 << 30, copy_ids=False, use_graph=False, max_vg_draws=vg_cap)
     for rate in rates:
-        n = max(200, int(rate * seconds))
+        n = min(max(200, int(rate * seconds)), 30_000)
         row = {}
         for name, srv in (("batched", batched), ("plain", plain)):
             reqs = trace(n)
@@ -174,6 +174,7 @@ def serving_leg(ctx, rates, seconds: float, meshlet_fraction: float = 0.25) -> d
             log(f"serving {name} @ {rate}/s: {row[name]['views_per_s']:.0f} views/s, "
                 f"p50 {row[name]['latency_ms_p50']:.3f} ms, p99 {row[name]['latency_ms_p99']:.3f} ms, "
                 f"batch {st['mean_batch_views']:.1f}")
+        row["batched_over_plain_views_per_s"] = row["batched"]["views_per_s"] / row["plain"]["views_per_s"]
         out["rates"][str(rate)] = row
     batched.close()
     plain.close()
@@ -330,7 +331,8 @@ def main() -> int:
     ap.add_argument("--instances", type=int, default=N_INSTANCES)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the occlusion and VG legs")
-    ap.add_argument("--serving-rates", default="2000,10000,50000")
+    ap.add_argument("--serving-rates", default="2000,10000,50000,200000",
+                    help="offered Poisson rates (requests/s); the last one saturates both servers")
     ap.add_argument("--serving-seconds", type=float, default=0.5)
     ap.add_argument("--launch-shape", default=None, metavar="WGxIPT",
                     help="classify launch shape, e.g. 512x4 (default: autotuned on the batch)")

@@ -609,24 +609,27 @@ class CullServer:
     @staticmethod
     def request(scene: Scene | None, camera: np.ndarray, first: int = 0, count: int = 0, want_ids: bool = True,
                 vg: VgScene | None = None, vg_capacity: int = 0, screen=(1920, 1080), threshold_px: float = 0.5,
-                hiz: "HiZBuildPass | None" = None, reversed_z: bool = False, vg_outputs: bool = True):
+                hiz: "HiZBuildPass | None" = None, reversed_z: bool = False, vg_outputs: bool = True,
+                outputs: bool = True):
         """Build a request plus the host buffers it writes into. `vg` adds a
         meshlet cull of that resident scene from the same camera (its
         published commands are copied back unless vg_outputs is False; the
-        counters always are)."""
+        counters always are). outputs=False copies no instance results back
+        (load generation: only the latency is wanted)."""
         cam = np.ascontiguousarray(camera, dtype=L.CAMERA).reshape(1)
         r = ServerRequest()
         r.view.camera = cam.ctypes.data
         ids = args = None
         stats = np.zeros(9, np.uint32)
         if scene is not None:
-            nb = scene.info()["meshes"] * scene.info()["lods"]
-            ids = np.zeros(max(count, 1), np.uint32) if want_ids else None
-            args = np.zeros(max(nb, 1), L.DRAW_INDEXED_INDIRECT)
             r.view.scene = scene.handle.value
             r.view.first, r.view.count = first, count
-            r.ids_out = None if ids is None else ids.ctypes.data
-            r.args_out, r.stats_out = args.ctypes.data, stats.ctypes.data
+            if outputs:
+                nb = scene.info()["meshes"] * scene.info()["lods"]
+                ids = np.zeros(max(count, 1), np.uint32) if want_ids else None
+                args = np.zeros(max(nb, 1), L.DRAW_INDEXED_INDIRECT)
+                r.ids_out = None if ids is None else ids.ctypes.data
+                r.args_out, r.stats_out = args.ctypes.data, stats.ctypes.data
         vgo = None
         if vg is not None:
             cap = vg_capacity or vg.meshlet_count
