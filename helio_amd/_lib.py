@@ -174,6 +174,10 @@ _SIGS: dict[str, tuple] = {
     "hv_scene_upload": (i32, [vp, vp, vp, u32, u32, vp]),
     "hv_scene_upload_soa": (i32, [vp, vp, vp, vp, u32, u32, vp]),
     "hv_scene_set_mesh_lods": (i32, [vp, vp, u32, u32]),
+    "hv_scene_upload_device": (i32, [vp, vp, u32, u32, vp]),
+    "hv_scene_swap_remove": (i32, [vp, u32, vp]),
+    "hv_scene_truncate": (i32, [vp, u32]),
+    "hv_scene_read_soa": (i32, [vp, vp, vp, u32, u32]),
     "hv_scene_info": (i32, [vp, P(u32), P(u32), P(u32), P(u64)]),
     "hv_hiz_create": (i32, [vp, u32, u32, u32, P(vp)]),
     "hv_hiz_destroy": (None, [vp]),

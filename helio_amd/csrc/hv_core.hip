@@ -76,6 +76,27 @@ __global__ __launch_bounds__(256) void k_scene_from_aos(const hv_instance_data* 
     meta[i] = (src[i].mesh_id & 0x7fffffffu) | ((src[i].flags & 1u) << 31);
 }
 
+// k_scene_from_aos for a device-resident source, also reducing the largest
+// mesh id of the range (one atomic per workgroup) so the host can validate it.
+__global__ __launch_bounds__(256) void k_scene_from_aos_max(const hv_instance_data* __restrict__ src, uint32_t count,
+                                                            float4* __restrict__ bounds, uint32_t* __restrict__ meta,
+                                                            uint32_t* __restrict__ max_id) {
+    __shared__ uint32_t s_max[4];
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    uint32_t id = 0;
+    if (i < count) {
+        const float4 b = *reinterpret_cast<const float4*>(src[i].bounds);
+        id = src[i].mesh_id;
+        bounds[i] = b;
+        meta[i] = (id & 0x7fffffffu) | ((src[i].flags & 1u) << 31);
+    }
+#pragma unroll
+    for (uint32_t off = 32; off; off >>= 1) id = max(id, (uint32_t)__shfl_xor(id, off));
+    if ((threadIdx.x & 63u) == 0) s_max[threadIdx.x >> 6] = id;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(max_id, max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3])));
+}
+
 extern "C" {
 
 const char* hv_last_error(void) { return g_err; }
@@ -399,6 +420,71 @@ hv_status hv_scene_upload_soa(hv_scene* s, const float* bounds4, const uint32_t*
         HV_HIP(hipStreamSynchronize(st));
     }
     if (first + count > s->count) s->count = first + count;
+    return HV_OK;
+}
+
+hv_status hv_scene_upload_device(hv_scene* s, const hv_instance_data* instances, uint32_t first, uint32_t count,
+                                 void* stream) {
+    if (!s || (!instances && count)) return set_error(HV_EINVAL, "hv_scene_upload_device: NULL argument");
+    if ((uint64_t)first + count > 0xffffffffull)
+        return set_error(HV_ELIMIT, "hv_scene_upload_device: range overflows u32");
+    if (s->explicit_aabb)
+        return set_error(HV_ESTATE, "hv_scene_upload_device: scene holds explicit AABBs; use hv_scene_upload");
+    if (!count) return HV_OK;
+    hipStream_t st = pick(s->ctx, stream);
+    HV_HIP(hipSetDevice(s->ctx->device));
+    HV_TRY(scene_grow(s, first + count, st));
+    HV_TRY(s->staging.reserve(sizeof(uint32_t), false, st));
+    uint32_t* d_max = s->staging.as<uint32_t>();
+    HV_HIP(hipMemsetAsync(d_max, 0, 4, st));
+    k_scene_from_aos_max<<<(count + 255) / 256, 256, 0, st>>>(instances, count, s->bounds.as<float4>() + first,
+                                                             s->meta.as<uint32_t>() + first, d_max);
+    HV_HIP(hipGetLastError());
+    uint32_t max_id = 0;
+    HV_HIP(hipMemcpyAsync(&max_id, d_max, 4, hipMemcpyDeviceToHost, st));
+    HV_HIP(hipStreamSynchronize(st));
+    if (max_id >= 0x80000000u)
+        return set_error(HV_EINVAL, "hv_scene_upload_device: a mesh_id of [%u, %u) is >= 2^31 (range written "
+                                    "with the id masked; the instance count is unchanged)", first, first + count);
+    if (first + count > s->count) s->count = first + count;
+    s->max_mesh_id = std::max(s->max_mesh_id, max_id);
+    return HV_OK;
+}
+
+hv_status hv_scene_swap_remove(hv_scene* s, uint32_t index, void* stream) {
+    if (!s) return set_error(HV_EINVAL, "hv_scene_swap_remove: NULL scene");
+    if (index >= s->count) return set_error(HV_EINVAL, "hv_scene_swap_remove: index %u >= %u instances", index, s->count);
+    const uint32_t last = s->count - 1u;
+    if (index < last) {
+        hipStream_t st = pick(s->ctx, stream);
+        HV_HIP(hipSetDevice(s->ctx->device));
+        HV_HIP(hipMemcpyAsync(s->bounds.as<float4>() + index, s->bounds.as<float4>() + last, 16,
+                              hipMemcpyDeviceToDevice, st));
+        HV_HIP(hipMemcpyAsync(s->meta.as<uint32_t>() + index, s->meta.as<uint32_t>() + last, 4,
+                              hipMemcpyDeviceToDevice, st));
+        if (s->explicit_aabb)
+            HV_HIP(hipMemcpyAsync(s->aabb.as<hv_instance_aabb>() + index, s->aabb.as<hv_instance_aabb>() + last, 32,
+                                  hipMemcpyDeviceToDevice, st));
+    }
+    s->count = last;
+    return HV_OK;
+}
+
+hv_status hv_scene_truncate(hv_scene* s, uint32_t new_len) {
+    if (!s) return set_error(HV_EINVAL, "hv_scene_truncate: NULL scene");
+    if (new_len < s->count) s->count = new_len;
+    return HV_OK;
+}
+
+hv_status hv_scene_read_soa(hv_scene* s, float* bounds4, uint32_t* meta, uint32_t first, uint32_t count) {
+    if (!s) return set_error(HV_EINVAL, "hv_scene_read_soa: NULL scene");
+    if ((uint64_t)first + count > s->count)
+        return set_error(HV_EINVAL, "hv_scene_read_soa: [%u, %u) exceeds %u instances", first, first + count, s->count);
+    HV_HIP(hipSetDevice(s->ctx->device));
+    HV_HIP(hipDeviceSynchronize());  // uploads may have gone to any caller stream
+    if (bounds4 && count)
+        HV_HIP(hipMemcpy(bounds4, s->bounds.as<float4>() + first, (size_t)count * 16, hipMemcpyDeviceToHost));
+    if (meta && count) HV_HIP(hipMemcpy(meta, s->meta.as<uint32_t>() + first, (size_t)count * 4, hipMemcpyDeviceToHost));
     return HV_OK;
 }
 

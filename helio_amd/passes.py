@@ -115,6 +115,26 @@ class Scene:
         assert len(t) == mesh_count * lod_count
         call("hv_scene_set_mesh_lods", self.handle, ptr(t), mesh_count, lod_count)
 
+    def upload_device(self, instances, count: int, first: int = 0) -> None:
+        """Instances from a device array of INSTANCE records (a DeviceBuffer or a raw device pointer)."""
+        src = instances.ptr if isinstance(instances, DeviceBuffer) else instances
+        call("hv_scene_upload_device", self.handle, src, first, count, None)
+
+    def swap_remove(self, index: int) -> None:
+        call("hv_scene_swap_remove", self.handle, index, None)
+
+    def truncate(self, new_len: int) -> None:
+        call("hv_scene_truncate", self.handle, new_len)
+
+    def read_soa(self, first: int = 0, count: int | None = None):
+        """Blocking readback of the resident (bounds float4, meta u32) arrays."""
+        if count is None:
+            count = self.info()["instances"] - first
+        b = np.empty((count, 4), np.float32)
+        m = np.empty(count, np.uint32)
+        call("hv_scene_read_soa", self.handle, ptr(b), ptr(m), first, count)
+        return b, m
+
     def info(self) -> dict:
         n, m, l, b = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint64()
         call("hv_scene_info", self.handle, C.byref(n), C.byref(m), C.byref(l), C.byref(b))

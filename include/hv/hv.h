@@ -284,6 +284,20 @@ hv_status hv_scene_upload(hv_scene* scene, const hv_instance_data* instances,
 /* Upload already-SoA data: bounds (count x float4), mesh ids and flags. */
 hv_status hv_scene_upload_soa(hv_scene* scene, const float* bounds4, const uint32_t* mesh_ids,
                               const uint32_t* flags, uint32_t first, uint32_t count, void* stream);
+/* Upload instances [first, first + count) from a DEVICE array of
+ * GpuInstanceData (the reference's GpuScene instance buffer, no PCIe copy):
+ * converted to the resident SoA on the device; sphere-derived AABBs only.
+ * Synchronises once to validate the mesh ids (< 2^31). */
+hv_status hv_scene_upload_device(hv_scene* scene, const hv_instance_data* instances, uint32_t first,
+                                 uint32_t count, void* stream);
+/* GrowableBuffer::swap_remove (crates/helio-core/src/scene/managers.rs:147-160):
+ * the last instance moves into `index`, the count drops by one. */
+hv_status hv_scene_swap_remove(hv_scene* scene, uint32_t index, void* stream);
+/* GrowableBuffer::truncate (managers.rs:162-180): keeps the first new_len instances. */
+hv_status hv_scene_truncate(hv_scene* scene, uint32_t new_len);
+/* Blocking readback of the resident SoA (bounds: count x float4; meta:
+ * mesh_id | casts_shadow << 31); either pointer may be NULL. */
+hv_status hv_scene_read_soa(hv_scene* scene, float* bounds4, uint32_t* meta, uint32_t first, uint32_t count);
 /* Per-(mesh, LOD) draw ranges: table[mesh * lod_count + lod]. */
 hv_status hv_scene_set_mesh_lods(hv_scene* scene, const hv_mesh_lod* table, uint32_t mesh_count,
                                  uint32_t lod_count);
