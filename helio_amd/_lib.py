@@ -146,6 +146,21 @@ class ShadowCullBuffers(C.Structure):
     ]
 
 
+class RebuildInputs(C.Structure):
+    _fields_ = [
+        ("instances", vp), ("aabbs", vp), ("draws", vp), ("objects", vp), ("materials", vp),
+        ("object_count", u32), ("material_count", u32), ("group_hidden", C.c_uint64),
+    ]
+
+
+class RebuildOutputs(C.Structure):
+    _fields_ = [
+        ("instances", vp), ("aabbs", vp), ("draw_calls", vp), ("indirect", vp), ("visibility", vp),
+        ("gpu_slots", vp), ("ranges", vp), ("shadow_static", vp), ("shadow_movable", vp), ("counts", vp),
+        ("scene", vp),
+    ]
+
+
 class TuneResult(C.Structure):
     _fields_ = [("wg", u32), ("ipt", u32), ("classify_ms", f32), ("total_ms", f32)]
 
@@ -227,6 +242,9 @@ _SIGS: dict[str, tuple] = {
     "hv_occlusion_cull_params": (i32, [vp, vp]),
     "hv_occlusion_cull_execute": (i32, [vp, P(DrawBuffers), vp, vp]),
     "hv_compact_draws": (i32, [vp, vp, u32, vp, vp, vp]),
+    "hv_scene_rebuild_create": (i32, [vp, P(vp)]),
+    "hv_scene_rebuild_destroy": (None, [vp]),
+    "hv_scene_rebuild_execute": (i32, [vp, P(RebuildInputs), P(RebuildOutputs), u32, vp]),
     "hv_shadow_matrix_create": (i32, [vp, u32, P(vp)]),
     "hv_shadow_matrix_destroy": (None, [vp]),
     "hv_shadow_matrix_execute": (i32, [vp, P(ShadowMatrixBuffers), vp]),

@@ -280,7 +280,7 @@ hv_status hv_instance_cull_data_from_instances(const hv_instance_data* instances
 // ---------------------------------------------------------------------------
 // Resident scene.
 // ---------------------------------------------------------------------------
-static hv_status scene_grow(hv_scene* s, uint32_t need, hipStream_t stream) {
+extern "C++" hv_status scene_grow(hv_scene* s, uint32_t need, hipStream_t stream) {
     if (need <= s->capacity) return HV_OK;
     uint64_t cap = s->capacity ? s->capacity : 1024;
     while (cap < need) cap *= 2;

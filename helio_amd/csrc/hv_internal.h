@@ -38,6 +38,9 @@ static_assert(sizeof(hv_occl_cull_params) == 56, "occlusion CullParams is 56 byt
 static_assert(sizeof(hv_vg_cull_uniforms) == 48, "VG CullUniforms is 48 bytes");
 static_assert(sizeof(hv_hiz_uniforms) == 16, "HiZUniforms is 16 bytes");
 static_assert(sizeof(hv_mesh_lod) == 12, "hv_mesh_lod is 12 bytes");
+static_assert(sizeof(hv_object_info) == 16, "hv_object_info is 16 bytes");
+static_assert(sizeof(hv_material_key) == 16, "hv_material_key is 16 bytes");
+static_assert(sizeof(hv_material_range) == 24, "hv_material_range is 24 bytes");
 
 namespace hv {
 
@@ -110,6 +113,9 @@ struct hv_scene {
     uint32_t lod_count = 0;
     uint32_t max_mesh_id = 0;    // largest mesh_id ever uploaded (host-side check)
 };
+
+// Grows the resident arrays to hold `need` instances (x2 steps, contents kept).
+hv_status scene_grow(hv_scene* s, uint32_t need, hipStream_t stream);
 
 struct hv_hiz {
     hv_ctx* ctx = nullptr;

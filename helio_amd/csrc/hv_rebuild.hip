@@ -1,0 +1,566 @@
+// Instance/draw rebuild on the device: the scene's dense object records are
+// sorted, gathered into the GPU-scene buffers and turned into draw calls,
+// material-class ranges and the static/movable shadow lists.
+//
+// Pipeline (one stream; the host synchronises once, after k_rb_prep):
+//   k_rb_mat_init, two stable radix sorts, k_rb_mat_flags, scan, k_rb_mat_rank
+//                  dense rank of every material's (class, graph_hash) pair, so
+//                  the 96-bit pair becomes a few-bit key; row M is the (0, 0)
+//                  pair a missing material reads.
+//   k_rb_prep      per object: its pair rank, (mesh_id, material_id) and
+//                  movable flag; the largest mesh and material ids.
+//   k_rb_keys + radix sort
+//                  packed key rank | mesh | material with the widths the host
+//                  read back, or two stable passes when it needs more than 64
+//                  bits; values are dense indices, so order[slot] = object.
+//   k_rb_gather    sorted instances and aabbs (a workgroup copies its 256
+//                  records as 16-byte chunks, writes fully coalesced), the
+//                  optional scene SoA, visibility, gpu_slots, sorted ranks and
+//                  the group-head flags.
+//   scan + k_rb_group_heads       first slot of every draw group.
+//   k_rb_draws     GpuDrawCall + DrawIndexedIndirectArgs per group, range flags.
+//   scan + k_rb_ranges + k_rb_range_counts   material_class_ranges.
+//   scan + k_rb_shadow            static / movable shadow lists in dense order.
+//
+// Sorting, scans and compaction are integer work: bit-exact by construction.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "hv_internal.h"
+
+namespace hv {
+namespace {
+
+constexpr uint32_t kRbThreads = 256;
+
+inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kRbThreads - 1) / kRbThreads); }
+
+inline uint32_t bit_width(uint32_t x) { return x ? 32u - (uint32_t)__builtin_clz(x) : 0u; }
+
+// small[0] = largest mesh_id, small[1] = largest material_id, small[2] = distinct (class, graph_hash) pairs.
+__global__ __launch_bounds__(64) void k_rb_clear(uint32_t* __restrict__ small) {
+    if (threadIdx.x < 8) small[threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(64) void k_rb_empty(uint32_t* __restrict__ counts) {
+    if (threadIdx.x < 2) counts[threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_rb_mat_init(const hv_material_key* __restrict__ mats, uint32_t m_count,
+                                                     uint64_t* __restrict__ gh, uint32_t* __restrict__ cls,
+                                                     uint32_t* __restrict__ idx) {
+    const uint32_t m = blockIdx.x * kRbThreads + threadIdx.x;
+    if (m > m_count) return;
+    const bool real = m < m_count;
+    gh[m] = real ? mats[m].graph_hash : 0ull;
+    cls[m] = real ? mats[m].material_class : 0u;
+    idx[m] = m;
+}
+
+__global__ __launch_bounds__(256) void k_rb_gather_u32(const uint32_t* __restrict__ src,
+                                                       const uint32_t* __restrict__ perm, uint32_t n,
+                                                       uint32_t* __restrict__ dst) {
+    const uint32_t s = blockIdx.x * kRbThreads + threadIdx.x;
+    if (s < n) dst[s] = src[perm[s]];
+}
+
+__global__ __launch_bounds__(256) void k_rb_iota(uint32_t* __restrict__ v, uint32_t n) {
+    const uint32_t s = blockIdx.x * kRbThreads + threadIdx.x;
+    if (s < n) v[s] = s;
+}
+
+// perm lists the material rows in (class, graph_hash) order.
+__global__ __launch_bounds__(256) void k_rb_mat_flags(const uint32_t* __restrict__ perm,
+                                                      const uint64_t* __restrict__ gh,
+                                                      const uint32_t* __restrict__ cls, uint32_t rows,
+                                                      uint32_t* __restrict__ flag) {
+    const uint32_t s = blockIdx.x * kRbThreads + threadIdx.x;
+    if (s >= rows) return;
+    uint32_t f = 1u;
+    if (s) {
+        const uint32_t a = perm[s], b = perm[s - 1];
+        f = (cls[a] != cls[b] || gh[a] != gh[b]) ? 1u : 0u;
+    }
+    flag[s] = f;
+}
+
+__global__ __launch_bounds__(256) void k_rb_mat_rank(const uint32_t* __restrict__ perm,
+                                                     const uint32_t* __restrict__ incl, uint32_t rows,
+                                                     uint32_t* __restrict__ rank, uint32_t* __restrict__ small) {
+    const uint32_t s = blockIdx.x * kRbThreads + threadIdx.x;
+    if (s >= rows) return;
+    rank[perm[s]] = incl[s] - 1u;
+    if (s == rows - 1u) small[2] = incl[s];
+}
+
+__global__ __launch_bounds__(256) void k_rb_prep(const hv_instance_data* __restrict__ inst,
+                                                 const hv_object_info* __restrict__ obj, uint32_t n,
+                                                 const uint32_t* __restrict__ mat_rank, uint32_t m_count,
+                                                 uint32_t* __restrict__ rank_of, uint2* __restrict__ mm,
+                                                 uint32_t* __restrict__ mv, uint32_t* __restrict__ small) {
+    __shared__ uint32_t s_max[2][kRbThreads / 64];
+    const uint32_t i = blockIdx.x * kRbThreads + threadIdx.x;
+    uint32_t mesh = 0u, mat = 0u;
+    if (i < n) {
+        const uint2 ids = *reinterpret_cast<const uint2*>(&inst[i].mesh_id);  // mesh_id, material_id
+        mesh = ids.x;
+        mat = ids.y;
+        const uint32_t slot = obj[i].material_slot;
+        rank_of[i] = mat_rank[slot < m_count ? slot : m_count];
+        mm[i] = ids;
+        mv[i] = obj[i].movability == 2u ? 1u : 0u;
+    }
+#pragma unroll
+    for (uint32_t off = 32; off; off >>= 1) {
+        mesh = max(mesh, (uint32_t)__shfl_xor(mesh, off));
+        mat = max(mat, (uint32_t)__shfl_xor(mat, off));
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        s_max[0][threadIdx.x >> 6] = mesh;
+        s_max[1][threadIdx.x >> 6] = mat;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t m = 0;
+        for (uint32_t w = 0; w < kRbThreads / 64; ++w) m = max(m, s_max[threadIdx.x][w]);
+        atomicMax(&small[threadIdx.x], m);
+    }
+}
+
+// key = rank << rank_shift | mesh << mesh_shift | material (terms switched by use_rank / use_mm).
+__global__ __launch_bounds__(256) void k_rb_keys(const uint32_t* __restrict__ rank_of, const uint2* __restrict__ mm,
+                                                 uint32_t n, uint32_t use_rank, uint32_t rank_shift, uint32_t use_mm,
+                                                 uint32_t mesh_shift, uint64_t* __restrict__ key,
+                                                 uint32_t* __restrict__ val) {
+    const uint32_t i = blockIdx.x * kRbThreads + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = 0ull;
+    if (use_mm) {
+        const uint2 v = mm[i];
+        k = ((uint64_t)v.x << mesh_shift) | v.y;
+    }
+    if (use_rank) k |= (uint64_t)rank_of[i] << rank_shift;
+    key[i] = k;
+    val[i] = i;
+}
+
+template <bool kGroups>
+__global__ __launch_bounds__(256) void k_rb_gather(
+    const float4* __restrict__ inst, const float4* __restrict__ aabb, const hv_object_info* __restrict__ obj,
+    const uint2* __restrict__ mm, const uint32_t* __restrict__ rank_of, const uint32_t* __restrict__ order,
+    uint32_t n, uint64_t hidden, float4* __restrict__ out_inst, float4* __restrict__ out_aabb,
+    uint32_t* __restrict__ vis, uint32_t* __restrict__ gpu_slots, uint32_t* __restrict__ srank,
+    uint32_t* __restrict__ gflag, float4* __restrict__ sc_bounds, uint32_t* __restrict__ sc_meta) {
+    __shared__ uint32_t s_o[kRbThreads];
+    const uint32_t t = threadIdx.x, base = blockIdx.x * kRbThreads, s = base + t;
+    const uint32_t live = min(kRbThreads, n - base);
+    uint32_t o = 0u;
+    if (s < n) {
+        o = order[s];
+        s_o[t] = o;
+    }
+    __syncthreads();
+    // 144-byte instances: 9 chunks of 16 bytes; chunk c is part c % 9 of record c / 9.
+    for (uint32_t c = t; c < live * 9u; c += kRbThreads) {
+        const uint32_t r = c / 9u, part = c - r * 9u;
+        const float4 v = inst[(size_t)s_o[r] * 9u + part];
+        out_inst[(size_t)(base + r) * 9u + part] = v;
+        if (sc_bounds && part == 7u) sc_bounds[base + r] = v;  // bounds @112
+        if (sc_meta && part == 8u)                             // mesh_id @128, flags @136
+            sc_meta[base + r] = (__float_as_uint(v.x) & 0x7fffffffu) | ((__float_as_uint(v.z) & 1u) << 31);
+    }
+    for (uint32_t c = t; c < live * 2u; c += kRbThreads) {
+        const uint32_t r = c >> 1, part = c & 1u;
+        out_aabb[(size_t)(base + r) * 2u + part] = aabb[(size_t)s_o[r] * 2u + part];
+    }
+    if (s < n) {
+        gpu_slots[o] = s;
+        const uint64_t g = obj[o].groups;
+        vis[s] = (g == 0ull || (g & hidden) == 0ull) ? 1u : 0u;  // object_is_visible
+        srank[s] = rank_of[o];
+        if (kGroups) {
+            uint32_t f = 1u;
+            if (s) {
+                const uint32_t p = t ? s_o[t - 1] : order[s - 1];
+                const uint2 a = mm[o], b = mm[p];
+                f = (a.x != b.x || a.y != b.y) ? 1u : 0u;
+            }
+            gflag[s] = f;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rb_group_heads(const uint32_t* __restrict__ gflag,
+                                                        const uint32_t* __restrict__ gid, uint32_t n,
+                                                        uint32_t* __restrict__ ghead) {
+    const uint32_t s = blockIdx.x * kRbThreads + threadIdx.x;
+    if (s < n && gflag[s]) ghead[gid[s] - 1u] = s;
+}
+
+// Groups are runs of slots (kIdentity: every slot is its own group). Slot
+// counts are read on the device: G = gid[n - 1].
+template <bool kIdentity>
+__global__ __launch_bounds__(256) void k_rb_draws(const uint32_t* __restrict__ ghead, const uint32_t* __restrict__ gid,
+                                                  const uint32_t* __restrict__ order,
+                                                  const hv_draw_call* __restrict__ draws,
+                                                  const uint32_t* __restrict__ srank, uint32_t n,
+                                                  hv_draw_call* __restrict__ dc, hv_draw_indexed_indirect* __restrict__ ind,
+                                                  uint32_t* __restrict__ rflag) {
+    const uint32_t g = blockIdx.x * kRbThreads + threadIdx.x;
+    if (g >= n) return;
+    const uint32_t G = kIdentity ? n : gid[n - 1u];
+    if (g >= G) {
+        rflag[g] = 0u;
+        return;
+    }
+    const uint32_t h = kIdentity ? g : ghead[g];
+    const uint32_t e = kIdentity ? g + 1u : (g + 1u < G ? ghead[g + 1u] : n);
+    const hv_draw_call d = draws[order[h]];
+    dc[g].index_count = d.index_count;
+    dc[g].first_index = d.first_index;
+    dc[g].vertex_offset = d.vertex_offset;
+    dc[g].first_instance = h;
+    dc[g].instance_count = e - h;
+    ind[g].index_count = d.index_count;
+    ind[g].instance_count = e - h;
+    ind[g].first_index = d.first_index;
+    ind[g].base_vertex = d.vertex_offset;
+    ind[g].first_instance = h;
+    uint32_t f = 1u;
+    if (g) f = srank[h] != srank[kIdentity ? g - 1u : ghead[g - 1u]] ? 1u : 0u;
+    rflag[g] = f;
+}
+
+template <bool kIdentity>
+__global__ __launch_bounds__(256) void k_rb_ranges(const uint32_t* __restrict__ rflag, const uint32_t* __restrict__ rid,
+                                                   const uint32_t* __restrict__ ghead, const uint32_t* __restrict__ order,
+                                                   const hv_object_info* __restrict__ obj,
+                                                   const hv_material_key* __restrict__ mats, uint32_t m_count,
+                                                   uint32_t n, uint32_t* __restrict__ rhead,
+                                                   hv_material_range* __restrict__ ranges) {
+    const uint32_t g = blockIdx.x * kRbThreads + threadIdx.x;
+    if (g >= n || !rflag[g]) return;
+    const uint32_t r = rid[g] - 1u;
+    const uint32_t o = order[kIdentity ? g : ghead[g]];
+    const uint32_t slot = obj[o].material_slot;
+    uint64_t gh = 0ull;
+    uint32_t cls = 0u;
+    if (slot < m_count) {
+        gh = mats[slot].graph_hash;
+        cls = mats[slot].material_class;
+    }
+    rhead[r] = g;
+    ranges[r].graph_hash = gh;
+    ranges[r].material_class = cls;
+    ranges[r].start = g;
+    ranges[r]._pad = 0u;
+}
+
+template <bool kIdentity>
+__global__ __launch_bounds__(256) void k_rb_range_counts(const uint32_t* __restrict__ rhead,
+                                                         const uint32_t* __restrict__ rid,
+                                                         const uint32_t* __restrict__ gid, uint32_t n,
+                                                         hv_material_range* __restrict__ ranges,
+                                                         uint32_t* __restrict__ counts) {
+    const uint32_t r = blockIdx.x * kRbThreads + threadIdx.x;
+    const uint32_t R = rid[n - 1u];
+    const uint32_t G = kIdentity ? n : gid[n - 1u];
+    if (r == 0) {
+        counts[0] = G;
+        counts[1] = R;
+    }
+    if (r >= R) return;
+    ranges[r].count = (r + 1u < R ? rhead[r + 1u] : G) - rhead[r];
+}
+
+__global__ __launch_bounds__(256) void k_rb_shadow(const hv_draw_call* __restrict__ draws,
+                                                   const uint32_t* __restrict__ gpu_slots,
+                                                   const uint32_t* __restrict__ mv, const uint32_t* __restrict__ mv_incl,
+                                                   uint32_t n, hv_draw_indexed_indirect* __restrict__ st,
+                                                   hv_draw_indexed_indirect* __restrict__ mov,
+                                                   uint32_t* __restrict__ counts) {
+    const uint32_t i = blockIdx.x * kRbThreads + threadIdx.x;
+    if (i >= n) return;
+    const hv_draw_call d = draws[i];
+    const uint32_t before = mv_incl[i] - mv[i];
+    hv_draw_indexed_indirect* e = mv[i] ? &mov[before] : &st[i - before];
+    e->index_count = d.index_count;
+    e->instance_count = 1u;
+    e->first_index = d.first_index;
+    e->base_vertex = d.vertex_offset;
+    e->first_instance = gpu_slots[i];
+    if (i == n - 1u) {
+        counts[2] = n - mv_incl[i];
+        counts[3] = mv_incl[i];
+    }
+}
+
+}  // namespace
+}  // namespace hv
+
+using namespace hv;
+
+struct hv_scene_rebuild {
+    hv_ctx* ctx = nullptr;
+    // material ranking (rows = material_count + 1)
+    DevBuf m_gh, m_gh_s, m_cls, m_cls_k, m_cls_s, m_idx, m_p1, m_p2, m_flag, m_incl, m_rank;
+    // per object / per slot
+    DevBuf rank_of, mm, mv, mv_incl, key_a, key_b, key32_a, key32_b, val, order1, order;
+    DevBuf srank, gflag, gid, ghead, rflag, rid, rhead;
+    DevBuf small, temp;
+    uint32_t* host_small = nullptr;  // pinned
+};
+
+namespace {
+
+template <class K>
+hv_status sort_pairs(hv_scene_rebuild* rb, const K* kin, K* kout, const uint32_t* vin, uint32_t* vout, uint32_t n,
+                     uint32_t bits, hipStream_t st) {
+    size_t bytes = 0;
+    HV_HIP(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0u, bits, st));
+    if (bytes > rb->temp.bytes) return set_error(HV_ELIMIT, "radix sort needs %zu temporary bytes", bytes);
+    bytes = rb->temp.bytes;
+    HV_HIP(rocprim::radix_sort_pairs(rb->temp.ptr, bytes, kin, kout, vin, vout, n, 0u, bits, st));
+    return HV_OK;
+}
+
+hv_status scan(hv_scene_rebuild* rb, const uint32_t* in, uint32_t* out, uint32_t n, hipStream_t st) {
+    size_t bytes = 0;
+    HV_HIP(rocprim::inclusive_scan(nullptr, bytes, in, out, n, rocprim::plus<uint32_t>(), st));
+    if (bytes > rb->temp.bytes) return set_error(HV_ELIMIT, "scan needs %zu temporary bytes", bytes);
+    bytes = rb->temp.bytes;
+    HV_HIP(rocprim::inclusive_scan(rb->temp.ptr, bytes, in, out, n, rocprim::plus<uint32_t>(), st));
+    return HV_OK;
+}
+
+// Largest temporary storage any rocPRIM call below needs for n objects and `rows` material rows.
+hv_status temp_bytes(uint32_t n, uint32_t rows, hipStream_t st, size_t* out) {
+    size_t best = 0, b = 0;
+    const uint32_t sizes[2] = {n, rows};
+    for (uint32_t sz : sizes) {
+        b = 0;
+        HV_HIP(rocprim::radix_sort_pairs(nullptr, b, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                         (const uint32_t*)nullptr, (uint32_t*)nullptr, sz, 0u, 64u, st));
+        best = std::max(best, b);
+        b = 0;
+        HV_HIP(rocprim::radix_sort_pairs(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                         (const uint32_t*)nullptr, (uint32_t*)nullptr, sz, 0u, 32u, st));
+        best = std::max(best, b);
+        b = 0;
+        HV_HIP(rocprim::inclusive_scan(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr, sz,
+                                       rocprim::plus<uint32_t>(), st));
+        best = std::max(best, b);
+    }
+    *out = best;
+    return HV_OK;
+}
+
+bool misaligned(const void* p, uintptr_t a) { return p && ((uintptr_t)p & (a - 1)) != 0; }
+
+}  // namespace
+
+extern "C" {
+
+hv_status hv_scene_rebuild_create(hv_ctx* ctx, hv_scene_rebuild** out) {
+    if (!ctx || !out) return set_error(HV_EINVAL, "hv_scene_rebuild_create: NULL argument");
+    HV_HIP(hipSetDevice(ctx->device));
+    hv_scene_rebuild* rb = new hv_scene_rebuild();
+    rb->ctx = ctx;
+    hipError_t e = hipHostMalloc((void**)&rb->host_small, 8 * sizeof(uint32_t), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        delete rb;
+        return hip_error(e, "hipHostMalloc", __FILE__, __LINE__);
+    }
+    *out = rb;
+    return HV_OK;
+}
+
+void hv_scene_rebuild_destroy(hv_scene_rebuild* rb) {
+    if (!rb) return;
+    (void)hipSetDevice(rb->ctx->device);
+    (void)hipDeviceSynchronize();
+    DevBuf* bufs[] = {&rb->m_gh,    &rb->m_gh_s,   &rb->m_cls,   &rb->m_cls_k, &rb->m_cls_s, &rb->m_idx,
+                      &rb->m_p1,    &rb->m_p2,     &rb->m_flag,  &rb->m_incl,  &rb->m_rank,  &rb->rank_of,
+                      &rb->mm,      &rb->mv,       &rb->mv_incl, &rb->key_a,   &rb->key_b,   &rb->key32_a,
+                      &rb->key32_b, &rb->val,      &rb->order1,  &rb->order,   &rb->srank,   &rb->gflag,
+                      &rb->gid,     &rb->ghead,    &rb->rflag,   &rb->rid,     &rb->rhead,   &rb->small,
+                      &rb->temp};
+    for (DevBuf* b : bufs) b->release();
+    if (rb->host_small) (void)hipHostFree(rb->host_small);
+    delete rb;
+}
+
+hv_status hv_scene_rebuild_execute(hv_scene_rebuild* rb, const hv_rebuild_inputs* in, const hv_rebuild_outputs* out,
+                                   uint32_t mode, void* stream) {
+    if (!rb || !in || !out || !out->counts) return set_error(HV_EINVAL, "hv_scene_rebuild_execute: NULL argument");
+    if (mode != HV_REBUILD_PERSISTENT && mode != HV_REBUILD_OPTIMIZED)
+        return set_error(HV_EINVAL, "hv_scene_rebuild_execute: unknown mode %u", mode);
+    const uint32_t n = in->object_count, m_count = in->material_count;
+    if (n > HV_REBUILD_MAX_OBJECTS)
+        return set_error(HV_ELIMIT, "hv_scene_rebuild_execute: %u objects exceed %u", n, HV_REBUILD_MAX_OBJECTS);
+    if (m_count > HV_REBUILD_MAX_MATERIALS)
+        return set_error(HV_ELIMIT, "hv_scene_rebuild_execute: %u materials exceed %u", m_count,
+                         HV_REBUILD_MAX_MATERIALS);
+    if (m_count && !in->materials) return set_error(HV_EINVAL, "hv_scene_rebuild_execute: NULL materials");
+    if (n && (!in->instances || !in->aabbs || !in->draws || !in->objects || !out->instances || !out->aabbs ||
+              !out->draw_calls || !out->indirect || !out->visibility || !out->gpu_slots || !out->ranges ||
+              !out->shadow_static || !out->shadow_movable))
+        return set_error(HV_EINVAL, "hv_scene_rebuild_execute: NULL buffer");
+    if (misaligned(in->instances, 16) || misaligned(in->aabbs, 16) || misaligned(out->instances, 16) ||
+        misaligned(out->aabbs, 16) || misaligned(in->objects, 8) || misaligned(in->materials, 8) ||
+        misaligned(out->ranges, 8) || misaligned(in->draws, 4) || misaligned(out->draw_calls, 4) ||
+        misaligned(out->indirect, 4) || misaligned(out->shadow_static, 4) || misaligned(out->shadow_movable, 4) ||
+        misaligned(out->visibility, 4) || misaligned(out->gpu_slots, 4) || misaligned(out->counts, 4))
+        return set_error(HV_EINVAL, "hv_scene_rebuild_execute: misaligned buffer (instances/aabbs 16 B, "
+                                    "objects/materials/ranges 8 B, the rest 4 B)");
+    HV_HIP(hipSetDevice(rb->ctx->device));
+    const hipStream_t st = stream ? (hipStream_t)stream : rb->ctx->stream;
+    if (!n) {
+        hipLaunchKernelGGL(k_rb_empty, dim3(1), dim3(64), 0, st, out->counts);
+        HV_HIP(hipGetLastError());
+        return HV_OK;
+    }
+    const uint32_t rows = m_count + 1u;
+    const bool grouped = mode == HV_REBUILD_OPTIMIZED;
+
+    // Every allocation before the first launch (a growing DevBuf frees its old block).
+    size_t tb = 0;
+    HV_TRY(temp_bytes(n, rows, st, &tb));
+    HV_TRY(rb->temp.reserve(tb ? tb : 4, false, st));
+    HV_TRY(rb->small.reserve(8 * 4, false, st));
+    DevBuf* rows_u32[] = {&rb->m_cls, &rb->m_cls_k, &rb->m_cls_s, &rb->m_idx, &rb->m_p1,
+                          &rb->m_p2,  &rb->m_flag,  &rb->m_incl,  &rb->m_rank};
+    for (DevBuf* b : rows_u32) HV_TRY(b->reserve((uint64_t)rows * 4, false, st));
+    HV_TRY(rb->m_gh.reserve((uint64_t)rows * 8, false, st));
+    HV_TRY(rb->m_gh_s.reserve((uint64_t)rows * 8, false, st));
+    DevBuf* obj_u32[] = {&rb->rank_of, &rb->mv,    &rb->mv_incl, &rb->key32_a, &rb->key32_b, &rb->val,
+                         &rb->order1,  &rb->order, &rb->srank,   &rb->gflag,   &rb->gid,     &rb->ghead,
+                         &rb->rflag,   &rb->rid,   &rb->rhead};
+    for (DevBuf* b : obj_u32) HV_TRY(b->reserve((uint64_t)n * 4, false, st));
+    HV_TRY(rb->mm.reserve((uint64_t)n * 8, false, st));
+    HV_TRY(rb->key_a.reserve((uint64_t)n * 8, false, st));
+    HV_TRY(rb->key_b.reserve((uint64_t)n * 8, false, st));
+    if (out->scene) {
+        if (out->scene->ctx->device != rb->ctx->device)
+            return set_error(HV_EINVAL, "hv_scene_rebuild_execute: scene lives on another device");
+        HV_TRY(scene_grow(out->scene, n, st));
+    }
+
+    uint32_t* small = rb->small.as<uint32_t>();
+    // 1. Dense ranks of the (class, graph_hash) pairs.
+    hipLaunchKernelGGL(k_rb_clear, dim3(1), dim3(64), 0, st, small);
+    hipLaunchKernelGGL(k_rb_mat_init, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, in->materials, m_count,
+                       rb->m_gh.as<uint64_t>(), rb->m_cls.as<uint32_t>(), rb->m_idx.as<uint32_t>());
+    HV_HIP(hipGetLastError());
+    HV_TRY(sort_pairs<uint64_t>(rb, rb->m_gh.as<uint64_t>(), rb->m_gh_s.as<uint64_t>(), rb->m_idx.as<uint32_t>(),
+                                rb->m_p1.as<uint32_t>(), rows, 64u, st));
+    hipLaunchKernelGGL(k_rb_gather_u32, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, rb->m_cls.as<uint32_t>(),
+                       rb->m_p1.as<uint32_t>(), rows, rb->m_cls_k.as<uint32_t>());
+    HV_TRY(sort_pairs<uint32_t>(rb, rb->m_cls_k.as<uint32_t>(), rb->m_cls_s.as<uint32_t>(), rb->m_p1.as<uint32_t>(),
+                                rb->m_p2.as<uint32_t>(), rows, 32u, st));
+    hipLaunchKernelGGL(k_rb_mat_flags, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, rb->m_p2.as<uint32_t>(),
+                       rb->m_gh.as<uint64_t>(), rb->m_cls.as<uint32_t>(), rows, rb->m_flag.as<uint32_t>());
+    HV_TRY(scan(rb, rb->m_flag.as<uint32_t>(), rb->m_incl.as<uint32_t>(), rows, st));
+    hipLaunchKernelGGL(k_rb_mat_rank, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, rb->m_p2.as<uint32_t>(),
+                       rb->m_incl.as<uint32_t>(), rows, rb->m_rank.as<uint32_t>(), small);
+    // 2. Per-object rank, ids, movable flag and the id maxima.
+    hipLaunchKernelGGL(k_rb_prep, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, in->instances, in->objects, n,
+                       rb->m_rank.as<uint32_t>(), m_count, rb->rank_of.as<uint32_t>(), rb->mm.as<uint2>(),
+                       rb->mv.as<uint32_t>(), small);
+    HV_HIP(hipGetLastError());
+    HV_HIP(hipMemcpyAsync(rb->host_small, small, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HV_HIP(hipStreamSynchronize(st));
+    const uint32_t max_mesh = rb->host_small[0], max_mat = rb->host_small[1], pairs = rb->host_small[2];
+    if (out->scene && max_mesh >= 0x80000000u)
+        return set_error(HV_EINVAL, "hv_scene_rebuild_execute: mesh_id %u >= 2^31 cannot enter a resident scene",
+                         max_mesh);
+    const uint32_t rbits = bit_width(pairs ? pairs - 1u : 0u);
+    const uint32_t mbits = grouped ? bit_width(max_mesh) : 0u, tbits = grouped ? bit_width(max_mat) : 0u;
+
+    // 3. Stable sort of the dense indices.
+    uint32_t* order = rb->order.as<uint32_t>();
+    if (rbits + mbits + tbits == 0u) {
+        hipLaunchKernelGGL(k_rb_iota, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, order, n);
+    } else if (rbits + mbits + tbits <= 64u) {
+        hipLaunchKernelGGL(k_rb_keys, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, rb->rank_of.as<uint32_t>(),
+                           rb->mm.as<uint2>(), n, rbits ? 1u : 0u, mbits + tbits, (mbits + tbits) ? 1u : 0u, tbits,
+                           rb->key_a.as<uint64_t>(), rb->val.as<uint32_t>());
+        HV_TRY(sort_pairs<uint64_t>(rb, rb->key_a.as<uint64_t>(), rb->key_b.as<uint64_t>(), rb->val.as<uint32_t>(),
+                                    order, n, rbits + mbits + tbits, st));
+    } else {  // (mesh, material) first, then the rank; both passes stable
+        hipLaunchKernelGGL(k_rb_keys, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, rb->rank_of.as<uint32_t>(),
+                           rb->mm.as<uint2>(), n, 0u, 0u, 1u, tbits, rb->key_a.as<uint64_t>(),
+                           rb->val.as<uint32_t>());
+        HV_TRY(sort_pairs<uint64_t>(rb, rb->key_a.as<uint64_t>(), rb->key_b.as<uint64_t>(), rb->val.as<uint32_t>(),
+                                    rb->order1.as<uint32_t>(), n, mbits + tbits, st));
+        hipLaunchKernelGGL(k_rb_gather_u32, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, rb->rank_of.as<uint32_t>(),
+                           rb->order1.as<uint32_t>(), n, rb->key32_a.as<uint32_t>());
+        HV_TRY(sort_pairs<uint32_t>(rb, rb->key32_a.as<uint32_t>(), rb->key32_b.as<uint32_t>(),
+                                    rb->order1.as<uint32_t>(), order, n, rbits, st));
+    }
+    HV_HIP(hipGetLastError());
+
+    // 4. Gather into sorted slots.
+    float4* sc_bounds = out->scene ? out->scene->bounds.as<float4>() : nullptr;
+    uint32_t* sc_meta = out->scene ? out->scene->meta.as<uint32_t>() : nullptr;
+    const auto* inst4 = reinterpret_cast<const float4*>(in->instances);
+    const auto* aabb4 = reinterpret_cast<const float4*>(in->aabbs);
+    auto* oinst4 = reinterpret_cast<float4*>(out->instances);
+    auto* oaabb4 = reinterpret_cast<float4*>(out->aabbs);
+    if (grouped)
+        hipLaunchKernelGGL(k_rb_gather<true>, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, inst4, aabb4, in->objects,
+                           rb->mm.as<uint2>(), rb->rank_of.as<uint32_t>(), order, n, in->group_hidden, oinst4, oaabb4,
+                           out->visibility, out->gpu_slots, rb->srank.as<uint32_t>(), rb->gflag.as<uint32_t>(),
+                           sc_bounds, sc_meta);
+    else
+        hipLaunchKernelGGL(k_rb_gather<false>, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, inst4, aabb4,
+                           in->objects, rb->mm.as<uint2>(), rb->rank_of.as<uint32_t>(), order, n, in->group_hidden,
+                           oinst4, oaabb4, out->visibility, out->gpu_slots, rb->srank.as<uint32_t>(),
+                           rb->gflag.as<uint32_t>(), sc_bounds, sc_meta);
+    HV_HIP(hipGetLastError());
+
+    // 5. Draw groups, material ranges, shadow lists.
+    if (grouped) {
+        HV_TRY(scan(rb, rb->gflag.as<uint32_t>(), rb->gid.as<uint32_t>(), n, st));
+        hipLaunchKernelGGL(k_rb_group_heads, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, rb->gflag.as<uint32_t>(),
+                           rb->gid.as<uint32_t>(), n, rb->ghead.as<uint32_t>());
+        hipLaunchKernelGGL(k_rb_draws<false>, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, rb->ghead.as<uint32_t>(),
+                           rb->gid.as<uint32_t>(), order, in->draws, rb->srank.as<uint32_t>(), n, out->draw_calls,
+                           out->indirect, rb->rflag.as<uint32_t>());
+    } else {
+        hipLaunchKernelGGL(k_rb_draws<true>, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, rb->ghead.as<uint32_t>(),
+                           rb->gid.as<uint32_t>(), order, in->draws, rb->srank.as<uint32_t>(), n, out->draw_calls,
+                           out->indirect, rb->rflag.as<uint32_t>());
+    }
+    HV_HIP(hipGetLastError());
+    HV_TRY(scan(rb, rb->rflag.as<uint32_t>(), rb->rid.as<uint32_t>(), n, st));
+    if (grouped) {
+        hipLaunchKernelGGL(k_rb_ranges<false>, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, rb->rflag.as<uint32_t>(),
+                           rb->rid.as<uint32_t>(), rb->ghead.as<uint32_t>(), order, in->objects, in->materials,
+                           m_count, n, rb->rhead.as<uint32_t>(), out->ranges);
+        hipLaunchKernelGGL(k_rb_range_counts<false>, dim3(blocks_for(n)), dim3(kRbThreads), 0, st,
+                           rb->rhead.as<uint32_t>(), rb->rid.as<uint32_t>(), rb->gid.as<uint32_t>(), n, out->ranges,
+                           out->counts);
+    } else {
+        hipLaunchKernelGGL(k_rb_ranges<true>, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, rb->rflag.as<uint32_t>(),
+                           rb->rid.as<uint32_t>(), rb->ghead.as<uint32_t>(), order, in->objects, in->materials,
+                           m_count, n, rb->rhead.as<uint32_t>(), out->ranges);
+        hipLaunchKernelGGL(k_rb_range_counts<true>, dim3(blocks_for(n)), dim3(kRbThreads), 0, st,
+                           rb->rhead.as<uint32_t>(), rb->rid.as<uint32_t>(), rb->gid.as<uint32_t>(), n, out->ranges,
+                           out->counts);
+    }
+    HV_HIP(hipGetLastError());
+    HV_TRY(scan(rb, rb->mv.as<uint32_t>(), rb->mv_incl.as<uint32_t>(), n, st));
+    hipLaunchKernelGGL(k_rb_shadow, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, in->draws, out->gpu_slots,
+                       rb->mv.as<uint32_t>(), rb->mv_incl.as<uint32_t>(), n, out->shadow_static, out->shadow_movable,
+                       out->counts);
+    HV_HIP(hipGetLastError());
+    if (out->scene) {
+        out->scene->count = n;
+        out->scene->max_mesh_id = max_mesh;
+        out->scene->explicit_aabb = false;
+    }
+    return HV_OK;
+}
+
+}  // extern "C"

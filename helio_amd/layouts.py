@@ -158,3 +158,17 @@ HIZ_MIN = 1
 CULL_OCCLUSION = 1
 CULL_REVERSED_Z = 2
 VG_CLEAR_INDIRECT = 0x100
+
+# Instance/draw rebuild (crates/helio/src/scene/objects/rebuild.rs).
+# hv_object_info: the ObjectRecord fields the rebuild reads besides instance,
+# aabb and draw (crates/helio/src/scene/types.rs:157-187). 16 bytes.
+OBJECT_INFO = np.dtype([("groups", "<u8"), ("material_slot", "<u4"), ("movability", "<u4")])
+# (graph_hash, material_class) of one material. 16 bytes.
+MATERIAL_KEY = np.dtype([("graph_hash", "<u8"), ("material_class", "<u4"), ("_pad", "<u4")])
+# GpuScene::material_class_ranges entry (crates/helio-core/src/scene/gpu_scene.rs:236-237). 24 bytes.
+MATERIAL_RANGE = np.dtype([("graph_hash", "<u8"), ("material_class", "<u4"), ("start", "<u4"), ("count", "<u4"),
+                           ("_pad", "<u4")])
+MOVABILITY_STATIC, MOVABILITY_STATIONARY, MOVABILITY_MOVABLE = 0, 1, 2
+REBUILD_PERSISTENT, REBUILD_OPTIMIZED = 0, 1
+REBUILD_MAX_OBJECTS = 1 << 28
+

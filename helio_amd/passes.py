@@ -12,7 +12,8 @@ import ctypes as C
 import numpy as np
 
 from . import layouts as L
-from ._lib import (CullConfig, CullOutputs, DrawBuffers, ServerConfig, ServerRequest, ServerStats, ShadowCullBuffers,
+from ._lib import (CullConfig, CullOutputs, DrawBuffers, RebuildInputs, RebuildOutputs, ServerConfig, ServerRequest,
+                   ServerStats, ShadowCullBuffers,
                    ShadowMatrixBuffers, VgFrameData, VgOutputs, VgRequest, ViewRequest, call, load, ptr)
 
 
@@ -800,3 +801,80 @@ class ShadowCullPass:
             else None
         call("hv_shadow_cull_read", self.handle, ptr(counts), None if cmds is None else ptr(cmds))
         return counts, cmds
+
+
+class SceneRebuild:
+    """Instance/draw rebuild on the device: Scene::rebuild_instance_buffers_persistent
+    (mode REBUILD_PERSISTENT) or _optimized (REBUILD_OPTIMIZED) followed by
+    rebuild_shadow_partition_buffers (crates/helio/src/scene/objects/rebuild.rs:91-425).
+
+    bind() uploads the dense object arrays and sizes the outputs; execute()
+    enqueues one rebuild (one host synchronisation inside); read() returns the
+    outputs trimmed to their counts."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = C.c_void_p()
+        call("hv_scene_rebuild_create", ctx.handle, C.byref(h))
+        self.handle = h
+        self.n = 0
+
+    def close(self):
+        if self.handle:
+            load().hv_scene_rebuild_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bind(self, instances: np.ndarray, aabbs: np.ndarray, draws: np.ndarray, objects: np.ndarray,
+             materials: np.ndarray, group_hidden: int = 0) -> None:
+        ctx, n = self.ctx, len(instances)
+        assert len(aabbs) == n and len(draws) == n and len(objects) == n
+        self.n, self.m, self.group_hidden = n, len(materials), int(group_hidden)
+
+        def dev(a, dtype):
+            a = np.ascontiguousarray(a, dtype=dtype)
+            return DeviceBuffer.from_array(ctx, a) if a.nbytes else DeviceBuffer(ctx, 16)
+
+        self.d_in = [dev(instances, L.INSTANCE), dev(aabbs, L.AABB), dev(draws, L.DRAW_CALL),
+                     dev(objects, L.OBJECT_INFO), dev(materials, L.MATERIAL_KEY)]
+        k = max(n, 1)
+        self.d_out = {
+            "instances": DeviceBuffer(ctx, 144 * k), "aabbs": DeviceBuffer(ctx, 32 * k),
+            "draw_calls": DeviceBuffer(ctx, 20 * k), "indirect": DeviceBuffer(ctx, 20 * k),
+            "visibility": DeviceBuffer(ctx, 4 * k), "gpu_slots": DeviceBuffer(ctx, 4 * k),
+            "ranges": DeviceBuffer(ctx, 24 * k), "shadow_static": DeviceBuffer(ctx, 20 * k),
+            "shadow_movable": DeviceBuffer(ctx, 20 * k), "counts": DeviceBuffer(ctx, 16),
+        }
+
+    def execute(self, mode: int = L.REBUILD_OPTIMIZED, scene: "Scene | None" = None, stream=None) -> None:
+        i = RebuildInputs()
+        i.instances, i.aabbs, i.draws, i.objects, i.materials = (b.ptr for b in self.d_in)
+        i.object_count, i.material_count, i.group_hidden = self.n, self.m, self.group_hidden
+        o = RebuildOutputs()
+        for name, b in self.d_out.items():
+            setattr(o, name, b.ptr)
+        o.scene = scene.handle if scene is not None else None
+        call("hv_scene_rebuild_execute", self.handle, C.byref(i), C.byref(o), mode, stream)
+
+    def read(self) -> dict:
+        """Blocking readback: counts[4] and every output trimmed to its count."""
+        self.ctx.synchronize()
+        d = self.d_out
+        c = d["counts"].read(np.uint32, 4)
+        n, g, r, ns, nm = self.n, int(c[0]), int(c[1]), int(c[2]), int(c[3])
+        return {
+            "counts": c,
+            "instances": d["instances"].read(L.INSTANCE, n), "aabbs": d["aabbs"].read(L.AABB, n),
+            "draw_calls": d["draw_calls"].read(L.DRAW_CALL, min(g, n)),
+            "indirect": d["indirect"].read(L.DRAW_INDEXED_INDIRECT, min(g, n)),
+            "visibility": d["visibility"].read(np.uint32, n), "gpu_slots": d["gpu_slots"].read(np.uint32, n),
+            "ranges": d["ranges"].read(L.MATERIAL_RANGE, min(r, n)),
+            "shadow_static": d["shadow_static"].read(L.DRAW_INDEXED_INDIRECT, min(ns, n)),
+            "shadow_movable": d["shadow_movable"].read(L.DRAW_INDEXED_INDIRECT, min(nm, n)),
+        }
+

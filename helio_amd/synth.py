@@ -511,3 +511,79 @@ def shadow_draws(g: np.random.Generator, n_instances: int, n_draws: int, centers
     draws["base_vertex"] = g.integers(-1000, 1 << 20, n_draws)
     draws["first_instance"] = g.integers(0, n_instances, n_draws)
     return inst, draws
+
+
+# ---------------------------------------------------------------------------
+# Scene object sets for the instance/draw rebuild.
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class ObjectSet:
+    instances: np.ndarray   # L.INSTANCE, dense object order
+    aabbs: np.ndarray       # L.AABB
+    draws: np.ndarray       # L.DRAW_CALL (the object's draw template)
+    objects: np.ndarray     # L.OBJECT_INFO
+    materials: np.ndarray   # L.MATERIAL_KEY
+
+
+def scene_objects(n: int, mesh_count: int = 256, material_count: int = 64, class_count: int = 4,
+                  hash_pool: int = 8, p_missing: float = 0.02, p_foreign_material_id: float = 0.05,
+                  seed: int = 0, wide_ids: bool = False) -> ObjectSet:
+    """n objects over mesh_count meshes and material_count materials.
+
+    Materials draw (class, graph_hash) from class_count classes and a pool of
+    hash_pool hashes, so several materials share a pair. An object's
+    instance.material_id is its material slot except for a p_foreign_material_id
+    share (ids that disagree with the slot, so equal (mesh, material_id) runs
+    can span classes); p_missing of the objects name a material slot past the
+    table. Every other instance word is random, so a misplaced copy shows.
+    wide_ids draws mesh and material ids from the full 31/32-bit range (keys
+    wider than 64 bits)."""
+    g = np.random.Generator(np.random.PCG64([SEED_BASE + 7, seed]))
+    mats = np.zeros(material_count, L.MATERIAL_KEY)
+    hashes = g.integers(0, 2**63, hash_pool, dtype=np.uint64) * np.uint64(2) + np.uint64(1)
+    mats["graph_hash"] = hashes[g.integers(0, hash_pool, material_count)]
+    mats["material_class"] = g.integers(0, class_count, material_count, dtype=np.uint32)
+    inst = np.frombuffer(g.integers(0, 2**32, n * 36, dtype=np.uint32).astype(np.uint32).tobytes(),
+                         L.INSTANCE).copy()
+    inst["bounds"][:, :3] = g.uniform(0.0, 1000.0, (n, 3)).astype(np.float32)
+    inst["bounds"][:, 3] = g.uniform(0.5, 5.0, n).astype(np.float32)
+    slot = g.integers(0, max(material_count, 1), n, dtype=np.uint32)
+    missing = g.random(n) < p_missing
+    slot[missing] = material_count + g.integers(0, 1000, int(missing.sum()), dtype=np.uint32)
+    if wide_ids:
+        mesh = g.integers(0, 2**31, n, dtype=np.uint32)
+        mat = g.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+        mesh[: n // 2] = mesh[0]  # keep some equal (mesh, material) runs
+        mat[: n // 2] = mat[0]
+    else:
+        mesh = g.integers(0, mesh_count, n, dtype=np.uint32)
+        mat = slot.copy()
+        foreign = g.random(n) < p_foreign_material_id
+        mat[foreign] = g.integers(0, max(material_count, 1), int(foreign.sum()), dtype=np.uint32)
+    inst["mesh_id"] = mesh
+    inst["material_id"] = mat
+    ab = np.zeros(n, L.AABB)
+    ab["min"] = inst["bounds"][:, :3] - inst["bounds"][:, 3:4]
+    ab["max"] = inst["bounds"][:, :3] + inst["bounds"][:, 3:4]
+    ab["_pad0"] = g.standard_normal(n).astype(np.float32)
+    ab["_pad1"] = g.standard_normal(n).astype(np.float32)
+    dr = np.zeros(n, L.DRAW_CALL)
+    tmpl_ic = g.integers(3, 30000, max(mesh_count, 1), dtype=np.uint32)
+    tmpl_fi = g.integers(0, 2**24, max(mesh_count, 1), dtype=np.uint32)
+    tmpl_vo = g.integers(-(2**20), 2**20, max(mesh_count, 1)).astype(np.int32)
+    midx = mesh % np.uint32(max(mesh_count, 1))
+    dr["index_count"] = tmpl_ic[midx]
+    dr["first_index"] = tmpl_fi[midx]
+    dr["vertex_offset"] = tmpl_vo[midx]
+    dr["first_instance"] = np.arange(n, dtype=np.uint32)
+    dr["instance_count"] = 1
+    ob = np.zeros(n, L.OBJECT_INFO)
+    grp = (np.uint64(1) << g.integers(0, 8, n, dtype=np.uint64)) | (np.uint64(1) << g.integers(0, 8, n, dtype=np.uint64))
+    grp[g.random(n) < 0.5] = 0
+    ob["groups"] = grp
+    ob["material_slot"] = slot
+    ob["movability"] = g.integers(0, 3, n, dtype=np.uint32)
+    return ObjectSet(inst, ab, dr, ob, mats)
+

@@ -305,6 +305,89 @@ hv_status hv_scene_info(const hv_scene* scene, uint32_t* instance_count, uint32_
                         uint32_t* lod_count, uint64_t* device_bytes);
 
 /* ------------------------------------------------------------------------- */
+/* Instance/draw rebuild (Scene::rebuild_instance_buffers_persistent and      */
+/* _optimized, crates/helio/src/scene/objects/rebuild.rs:91-191,234-361, and  */
+/* rebuild_shadow_partition_buffers, rebuild.rs:374-425), on the device.      */
+/* ------------------------------------------------------------------------- */
+
+/* The fields of one ObjectRecord (crates/helio/src/scene/types.rs:157-187)
+ * that the rebuild reads besides instance, aabb and draw. 16 bytes. */
+typedef struct hv_object_info {
+    uint64_t groups;          /* GroupMask (crates/helio/src/groups.rs:99) */
+    uint32_t material_slot;   /* row of the material table; >= material_count reads (class 0, graph_hash 0),
+                                 as the reference's failed materials.get() */
+    uint32_t movability;      /* Movability (crates/libhelio/src/movability.rs): 0 static, 1 stationary,
+                                 2 movable; only 2 goes to the movable shadow list */
+} hv_object_info;
+
+/* (material_class, graph_hash) of one material, the pair rebuild.rs:108-117 sorts by. 16 bytes. */
+typedef struct hv_material_key {
+    uint64_t graph_hash;
+    uint32_t material_class;
+    uint32_t _pad;
+} hv_material_key;
+
+/* One entry of GpuScene::material_class_ranges (class, graph_hash, start, count),
+ * crates/helio-core/src/scene/gpu_scene.rs:236-237. 24 bytes. */
+typedef struct hv_material_range {
+    uint64_t graph_hash;
+    uint32_t material_class;
+    uint32_t start;
+    uint32_t count;
+    uint32_t _pad;
+} hv_material_range;
+
+/* Device inputs, in the dense object order. */
+typedef struct hv_rebuild_inputs {
+    const hv_instance_data* instances;   /* 16-byte aligned */
+    const hv_instance_aabb* aabbs;       /* 16-byte aligned */
+    const hv_draw_call* draws;           /* the draw templates (first_instance, instance_count unused) */
+    const hv_object_info* objects;       /* 8-byte aligned */
+    const hv_material_key* materials;    /* 8-byte aligned; may be NULL when material_count is 0 */
+    uint32_t object_count;
+    uint32_t material_count;
+    uint64_t group_hidden;               /* Scene::group_hidden */
+} hv_rebuild_inputs;
+
+/* Device outputs; each array holds object_count entries (the worst case). */
+typedef struct hv_rebuild_outputs {
+    hv_instance_data* instances;         /* sorted instances (16-byte aligned) */
+    hv_instance_aabb* aabbs;             /* sorted aabbs (16-byte aligned) */
+    hv_draw_call* draw_calls;            /* one per draw group */
+    hv_draw_indexed_indirect* indirect;  /* one per draw group */
+    uint32_t* visibility;                /* per sorted slot: object_is_visible(groups, group_hidden) */
+    uint32_t* gpu_slots;                 /* per dense object: its sorted slot (ObjectRecord::gpu_slot) */
+    hv_material_range* ranges;           /* material_class_ranges (8-byte aligned) */
+    hv_draw_indexed_indirect* shadow_static;   /* static + stationary objects, dense order */
+    hv_draw_indexed_indirect* shadow_movable;  /* movable objects, dense order */
+    uint32_t* counts;                    /* [0] draw groups, [1] ranges, [2] shadow_static_draw_count,
+                                            [3] shadow_movable_draw_count */
+    hv_scene* scene;                     /* optional: resident SoA replaced by the sorted instances
+                                            (count = object_count, sphere-derived AABBs) */
+} hv_rebuild_outputs;
+
+#define HV_REBUILD_PERSISTENT 0u  /* sort by (class, graph_hash); one draw per object */
+#define HV_REBUILD_OPTIMIZED 1u   /* sort by (class, graph_hash, mesh_id, material_id); one draw per run of
+                                     equal (mesh_id, material_id) with instance_count = run length */
+#define HV_REBUILD_MAX_OBJECTS (1u << 28)
+#define HV_REBUILD_MAX_MATERIALS (1u << 24)
+
+typedef struct hv_scene_rebuild hv_scene_rebuild;
+
+hv_status hv_scene_rebuild_create(hv_ctx* ctx, hv_scene_rebuild** out);
+void hv_scene_rebuild_destroy(hv_scene_rebuild* rb);
+/* Both sorts are stable (ties keep the dense order, as Rust's sort_by_key).
+ * Ranges run over objects (persistent) or over draw groups (optimized) and
+ * split where (class, graph_hash) changes between neighbours; a group takes
+ * the class, graph_hash and draw template of its first object. With zero
+ * objects counts[0..1] become 0 and counts[2..3] are left as they are (the
+ * reference returns before rebuilding the shadow partition). Synchronises
+ * the stream once (key widths come from the material ranks and the largest
+ * mesh and material ids); the rest is enqueued. */
+hv_status hv_scene_rebuild_execute(hv_scene_rebuild* rb, const hv_rebuild_inputs* in, const hv_rebuild_outputs* out,
+                                   uint32_t mode, void* stream);
+
+/* ------------------------------------------------------------------------- */
 /* Hi-Z pyramid (HiZBuildPass, crates/helio-pass-hiz/src/lib.rs:45-548;       */
 /* hiz_depth_copy.wgsl:13-19, hiz_build.wgsl:12-23).                         */
 /* Level 0 is the depth buffer itself (no copy); levels 1.. are packed in one */

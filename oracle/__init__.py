@@ -81,6 +81,8 @@ _SIGS = {
     "or_fnv_mat": (u32, [vp]),
     "or_shadow_matrices": (None, [vp, u32, vp, u32, vp, u32, vp, vp, u32]),
     "or_shadow_cull": (None, [vp, vp, u32, vp, u32, vp, u32, u32, vp, vp]),
+    "or_scene_rebuild": (None, [u32, u32, vp, vp, vp, vp, vp, u32, C.c_uint64, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                vp]),
 }
 
 _lib = None
@@ -206,3 +208,43 @@ def shadow_cull(mats: np.ndarray, instances: np.ndarray, instance_capacity: int,
     lib().or_shadow_cull(_p(m), _p(inst), instance_capacity, _p(src), src.size // 5, _p(fd), max_per_face,
                          plane_mode, _p(counts), _p(dst))
     return counts, dst
+
+
+_REBUILD_DT = {
+    "instance": np.dtype([("raw", np.uint8, 144)]),
+    "range": np.dtype([("graph_hash", "<u8"), ("material_class", "<u4"), ("start", "<u4"), ("count", "<u4"),
+                       ("_pad", "<u4")]),
+}
+
+
+def scene_rebuild(mode: int, instances: np.ndarray, aabbs: np.ndarray, draws: np.ndarray, objects: np.ndarray,
+                  materials: np.ndarray, group_hidden: int = 0, counts_init=(0, 0, 0, 0)) -> dict:
+    """rebuild_instance_buffers_{persistent,optimized} + rebuild_shadow_partition_buffers on the CPU.
+
+    Arrays are the structured records of helio_amd.layouts (INSTANCE, AABB,
+    DRAW_CALL, OBJECT_INFO, MATERIAL_KEY); outputs are trimmed to the counts."""
+    inst = np.ascontiguousarray(instances)
+    n = len(inst)
+    ab = np.ascontiguousarray(aabbs)
+    dr = np.ascontiguousarray(draws).view(np.uint32).reshape(-1)
+    ob = np.ascontiguousarray(objects)
+    mt = np.ascontiguousarray(materials)
+    k = max(n, 1)
+    o_inst = np.zeros(k, inst.dtype) if n else np.zeros(1, inst.dtype)
+    o_ab = np.zeros(k, ab.dtype) if n else np.zeros(1, ab.dtype)
+    dc = np.zeros((k, 5), np.uint32)
+    ind = np.zeros((k, 5), np.uint32)
+    vis = np.zeros(k, np.uint32)
+    slots = np.zeros(k, np.uint32)
+    rg = np.zeros(k, _REBUILD_DT["range"])
+    st = np.zeros((k, 5), np.uint32)
+    mv = np.zeros((k, 5), np.uint32)
+    counts = np.array(counts_init, np.uint32)
+    lib().or_scene_rebuild(mode, n, _p(inst), _p(ab), _p(dr) if n else None, _p(ob), _p(mt) if len(mt) else None,
+                           len(mt), C.c_uint64(group_hidden), _p(o_inst), _p(o_ab), _p(dc), _p(ind), _p(vis),
+                           _p(slots), _p(rg), _p(st), _p(mv), _p(counts))
+    g, r, ns, nm = (int(x) for x in counts)
+    return {"counts": counts, "instances": o_inst[:n], "aabbs": o_ab[:n], "draw_calls": dc[:min(g, n)],
+            "indirect": ind[:min(g, n)], "visibility": vis[:n], "gpu_slots": slots[:n], "ranges": rg[:min(r, n)],
+            "shadow_static": st[:min(ns, n)], "shadow_movable": mv[:min(nm, n)]}
+

@@ -4,9 +4,11 @@
 // cpu_baseline leg load this library.
 #include "hv_oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <thread>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -941,3 +943,126 @@ void or_shadow_cull(const float* mats, const void* instances, uint32_t instance_
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Instance/draw rebuild (rebuild.rs:91-425), restated step by step: a stable
+// sort of the dense indices by the tuple key, a walk that consumes one object
+// (persistent) or a run of equal (mesh_id, material_id) (optimized) per draw,
+// ranges over consecutive equal (class, graph_hash) draw keys, then the
+// shadow split in dense order with first_instance = the new slot.
+// ---------------------------------------------------------------------------
+namespace {
+struct OrObjInfo {
+    uint64_t groups;
+    uint32_t material_slot, movability;
+};
+struct OrMatKey {
+    uint64_t graph_hash;
+    uint32_t material_class, pad;
+};
+struct OrRange {
+    uint64_t graph_hash;
+    uint32_t material_class, start, count, pad;
+};
+}  // namespace
+
+extern "C" void or_scene_rebuild(uint32_t mode, uint32_t n, const void* instances, const void* aabbs,
+                                 const uint32_t* draws, const void* objects, const void* materials, uint32_t m_count,
+                                 uint64_t group_hidden, void* out_instances, void* out_aabbs, uint32_t* out_draw_calls,
+                                 uint32_t* out_indirect, uint32_t* visibility, uint32_t* gpu_slots, void* ranges,
+                                 uint32_t* shadow_static, uint32_t* shadow_movable, uint32_t* counts) {
+    if (n == 0) {  // rebuild.rs:93-101 and :236-244 return before the shadow partition
+        counts[0] = counts[1] = 0;
+        return;
+    }
+    const auto* inst = static_cast<const uint8_t*>(instances);
+    const auto* obj = static_cast<const OrObjInfo*>(objects);
+    const auto* mats = static_cast<const OrMatKey*>(materials);
+    auto u32_at = [&](uint32_t i, size_t off) {
+        uint32_t v;
+        std::memcpy(&v, inst + 144 * (size_t)i + off, 4);
+        return v;
+    };
+    // materials.get(r.material).map(|m| (class, graph_hash)).unwrap_or((0, 0))
+    auto class_hash = [&](uint32_t i) {
+        const uint32_t slot = obj[i].material_slot;
+        return slot < m_count ? std::make_pair(mats[slot].material_class, mats[slot].graph_hash)
+                              : std::make_pair(0u, (uint64_t)0);
+    };
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    if (mode == 0) {
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t a, uint32_t b) { return class_hash(a) < class_hash(b); });
+    } else {
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+            const auto ka = class_hash(a), kb = class_hash(b);
+            if (ka != kb) return ka < kb;
+            const uint32_t ma = u32_at(a, 128), mb = u32_at(b, 128);
+            if (ma != mb) return ma < mb;
+            return u32_at(a, 132) < u32_at(b, 132);
+        });
+    }
+    std::vector<std::pair<uint32_t, uint64_t>> group_keys;
+    uint32_t groups = 0;
+    uint32_t i = 0;
+    while (i < n) {
+        const uint32_t o0 = order[i];
+        const uint32_t mesh0 = u32_at(o0, 128), mat0 = u32_at(o0, 132);
+        const uint32_t start = i;
+        while (i < n) {
+            const uint32_t o = order[i];
+            if (mode == 1 && (u32_at(o, 128) != mesh0 || u32_at(o, 132) != mat0)) break;
+            if (mode == 0 && i > start) break;
+            gpu_slots[o] = i;
+            std::memcpy(static_cast<uint8_t*>(out_instances) + 144 * (size_t)i, inst + 144 * (size_t)o, 144);
+            std::memcpy(static_cast<uint8_t*>(out_aabbs) + 32 * (size_t)i,
+                        static_cast<const uint8_t*>(aabbs) + 32 * (size_t)o, 32);
+            const uint64_t g = obj[o].groups;
+            visibility[i] = (g == 0 || (g & group_hidden) == 0) ? 1u : 0u;  // helpers.rs:87-89
+            ++i;
+        }
+        const uint32_t* d = draws + 5 * (size_t)o0;  // index_count, first_index, vertex_offset, ...
+        uint32_t* dc = out_draw_calls + 5 * (size_t)groups;
+        dc[0] = d[0];
+        dc[1] = d[1];
+        dc[2] = d[2];
+        dc[3] = start;
+        dc[4] = i - start;
+        uint32_t* ind = out_indirect + 5 * (size_t)groups;
+        ind[0] = d[0];
+        ind[1] = i - start;
+        ind[2] = d[1];
+        ind[3] = d[2];
+        ind[4] = start;
+        group_keys.push_back(class_hash(o0));
+        ++groups;
+    }
+    auto* rg = static_cast<OrRange*>(ranges);
+    uint32_t nr = 0;
+    for (uint32_t g = 0; g < groups;) {
+        const auto key = group_keys[g];
+        const uint32_t start = g;
+        while (g < groups && group_keys[g] == key) ++g;
+        rg[nr].graph_hash = key.second;
+        rg[nr].material_class = key.first;
+        rg[nr].start = start;
+        rg[nr].count = g - start;
+        rg[nr].pad = 0;
+        ++nr;
+    }
+    counts[0] = groups;
+    counts[1] = nr;
+    uint32_t ns = 0, nm = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t* d = draws + 5 * (size_t)k;
+        uint32_t* e = obj[k].movability == 2 ? shadow_movable + 5 * (size_t)nm++ : shadow_static + 5 * (size_t)ns++;
+        e[0] = d[0];
+        e[1] = 1;
+        e[2] = d[1];
+        e[3] = d[2];
+        e[4] = gpu_slots[k];
+    }
+    counts[2] = ns;
+    counts[3] = nm;
+}

@@ -126,6 +126,21 @@ void or_shadow_cull(const float* mats, const void* instances, uint32_t instance_
                     const uint32_t* src_indirect, uint32_t draw_count, const uint32_t* face_dirty,
                     uint32_t max_per_face, uint32_t plane_mode, uint32_t* counts, uint32_t* dst);
 
+/* --- instance/draw rebuild ----------------------------------------------------
+ * Scene::rebuild_instance_buffers_persistent (rebuild.rs:91-191, mode 0) or
+ * _optimized (rebuild.rs:234-361, mode 1), then rebuild_shadow_partition_buffers
+ * (rebuild.rs:374-425). instances: n 144-byte GpuInstanceData; aabbs: n 32-byte
+ * records; draws: n GpuDrawCall (5 u32); objects: n hv_object_info (u64 groups,
+ * u32 material_slot, u32 movability); materials: m_count (u64 graph_hash, u32
+ * class, u32 pad). Outputs hold n entries each; ranges are 24-byte (u64
+ * graph_hash, u32 class, start, count, pad). counts = [draws, ranges,
+ * shadow static, shadow movable]; with n == 0 only counts[0..1] are written. */
+void or_scene_rebuild(uint32_t mode, uint32_t n, const void* instances, const void* aabbs, const uint32_t* draws,
+                      const void* objects, const void* materials, uint32_t m_count, uint64_t group_hidden,
+                      void* out_instances, void* out_aabbs, uint32_t* out_draw_calls, uint32_t* out_indirect,
+                      uint32_t* visibility, uint32_t* gpu_slots, void* ranges, uint32_t* shadow_static,
+                      uint32_t* shadow_movable, uint32_t* counts);
+
 #ifdef __cplusplus
 }
 #endif
