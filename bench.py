@@ -226,6 +226,64 @@ def shadow_leg(ctx, steps: int) -> dict:
             "visible_draw_faces": int(counts.sum())}
 
 
+REBUILD_OBJECTS = 1 << 20
+# Least HBM bytes one object moves through a rebuild: instance 144 + aabb 32 +
+# draw template 20 + object info 16 read; instance 144 + aabb 32 + draw call
+# 20 + indirect 20 + visibility 4 + gpu_slot 4 + shadow entry 20 written.
+REBUILD_MIN_BYTES = 212 + 244
+
+
+def rebuild_objects():
+    from helio_amd import synth
+
+    return synth.scene_objects(REBUILD_OBJECTS, mesh_count=4096, material_count=1024, seed=1)
+
+
+def rebuild_leg(ctx, steps: int, objs) -> dict:
+    """Instance/draw rebuild of 1,048,576 scene objects (4,096 meshes, 1,024
+    materials over 4 classes), optimized and persistent: sort, gather, draw
+    groups, material ranges and shadow lists, with the one host
+    synchronisation inside (HIP events bracket the whole call)."""
+    from helio_amd.passes import SceneRebuild
+    from helio_amd._lib import call
+    import ctypes as C
+
+    rb = SceneRebuild(ctx)
+    rb.bind(objs.instances, objs.aabbs, objs.draws, objs.objects, objs.materials, group_hidden=6)
+    out = {"objects": REBUILD_OBJECTS, "materials": int(len(objs.materials)), "min_bytes_per_object": REBUILD_MIN_BYTES}
+    for mode, name in ((1, "optimized"), (0, "persistent")):
+        ts = []
+        for i in range(steps + 3):
+            call("hv_timer_record", ctx.handle, 0, None)
+            rb.execute(mode)
+            call("hv_timer_record", ctx.handle, 1, None)
+            ms = C.c_float()
+            call("hv_timer_elapsed_ms", ctx.handle, 0, 1, C.byref(ms))
+            if i >= 3:
+                ts.append(ms.value)
+        c = rb.d_out["counts"].read(np.uint32, 4)
+        ms = float(np.median(ts))
+        out[name] = {"ms": ms, "objects_per_s": REBUILD_OBJECTS / (ms * 1e-3),
+                     "min_bytes_gb_s": REBUILD_OBJECTS * REBUILD_MIN_BYTES / (ms * 1e-3) / 1e9,
+                     "draw_groups": int(c[0]), "ranges": int(c[1]), "shadow_static": int(c[2]),
+                     "shadow_movable": int(c[3])}
+    rb.close()
+    return out
+
+
+def rebuild_cpu_ms(objs) -> dict:
+    """The oracle's rebuild (std::stable_sort + the reference's walk, one thread) on the same objects."""
+    import oracle  # test infrastructure: the checker, timed here as the baseline
+
+    out = {}
+    for mode, name in ((1, "optimized"), (0, "persistent")):
+        t = time.perf_counter()
+        oracle.scene_rebuild(mode, objs.instances, objs.aabbs, objs.draws, objs.objects, objs.materials, 6)
+        out[name + "_ms"] = (time.perf_counter() - t) * 1e3
+    out["cores"] = 1
+    return out
+
+
 def occlusion_leg(ctx, steps: int) -> dict:
     """Config 1: 1920x1080 reversed-Z depth from 2,000 seeded boxes, 11-level
     min pyramid, 1,048,576 spheres frustum + sub-pixel + Hi-Z tested."""
@@ -330,7 +388,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--instances", type=int, default=N_INSTANCES)
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the occlusion and VG legs")
+    ap.add_argument("--no-extra", action="store_true", help="skip the occlusion, VG, shadow and rebuild legs")
+    ap.add_argument("--legs", default="occlusion,vg,shadow,rebuild",
+                    help="comma list of the side legs to run (occlusion, vg, shadow, rebuild)")
     ap.add_argument("--serving-rates", default="2000,10000,50000,200000",
                     help="offered Poisson rates (requests/s); the last one saturates both servers")
     ap.add_argument("--serving-seconds", type=float, default=0.5)
@@ -470,17 +530,26 @@ def main() -> int:
         ctx.synchronize()
         lat0.append((time.perf_counter() - t) * 1e3)
 
-    occl = vgres = shadow = None
-    if not args.no_extra:
+    occl = vgres = shadow = rebuild = objs = None
+    legs = set() if args.no_extra else {x.strip() for x in args.legs.split(",") if x.strip()}
+    side_steps = max(10, args.steps // 2)
+    if "occlusion" in legs:
         log("occlusion leg")
-        occl = occlusion_leg(ctx, max(10, args.steps // 2))
+        occl = occlusion_leg(ctx, side_steps)
         log(f"occlusion: {occl}")
+    if "vg" in legs:
         log("vg leg")
-        vgres = vg_leg(ctx, max(10, args.steps // 2))
+        vgres = vg_leg(ctx, side_steps)
         log(f"vg: {vgres}")
+    if "shadow" in legs:
         log("shadow leg")
-        shadow = shadow_leg(ctx, max(10, args.steps // 2))
+        shadow = shadow_leg(ctx, side_steps)
         log(f"shadow: {shadow}")
+    if "rebuild" in legs:
+        log("rebuild leg")
+        objs = rebuild_objects()
+        rebuild = rebuild_leg(ctx, side_steps, objs)
+        log(f"rebuild: {rebuild}")
 
     serving = None
     if not args.no_serving:
@@ -491,6 +560,8 @@ def main() -> int:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline: {threads} threads, budget {args.cpu_budget} s")
         cpu = cpu_baseline(sc, views, args.cpu_budget, threads)
+        if objs is not None:
+            cpu["scene_rebuild_1m"] = rebuild_cpu_ms(objs)
 
     value = tests_per_step * args.steps / dt_batched
     out = {
@@ -546,6 +617,7 @@ def main() -> int:
         "occlusion_1m": occl,
         "virtual_geometry": vgres,
         "shadow_producers": shadow,
+        "scene_rebuild": rebuild,
         "single_view_65536": {
             "latency_ms_p50": pct(lat0, 50),
             "latency_ms_p99": pct(lat0, 99),

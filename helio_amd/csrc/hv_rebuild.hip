@@ -3,20 +3,23 @@
 // material-class ranges and the static/movable shadow lists.
 //
 // Pipeline (one stream; the host synchronises once, after k_rb_prep):
-//   k_rb_mat_init, two stable radix sorts, k_rb_mat_flags, scan, k_rb_mat_rank
-//                  dense rank of every material's (class, graph_hash) pair, so
-//                  the 96-bit pair becomes a few-bit key; row M is the (0, 0)
-//                  pair a missing material reads.
-//   k_rb_prep      per object: its pair rank, (mesh_id, material_id) and
-//                  movable flag; the largest mesh and material ids.
-//   k_rb_keys + radix sort
-//                  packed key rank | mesh | material with the widths the host
-//                  read back, or two stable passes when it needs more than 64
-//                  bits; values are dense indices, so order[slot] = object.
-//   k_rb_gather    sorted instances and aabbs (a workgroup copies its 256
-//                  records as 16-byte chunks, writes fully coalesced), the
-//                  optional scene SoA, visibility, gpu_slots, sorted ranks and
-//                  the group-head flags.
+//   k_rb_mat_rank_block
+//                  dense rank of every material's (class, graph_hash) pair
+//                  (one workgroup, bitonic sort in LDS; above 2,048 rows two
+//                  stable radix sorts + a scan), so the 96-bit pair becomes a
+//                  few-bit key; row M is the (0, 0) pair of a missing material.
+//   k_rb_prep      per object: pair rank, (mesh_id, material_id), movable flag,
+//                  sort value = index | visible << 31; the largest mesh and
+//                  material ids.
+//   k_rb_keys + onesweep radix sort
+//                  packed key rank | mesh | material, 32 or 64 bits wide from
+//                  the widths the host read back, or two stable passes when it
+//                  needs more than 64 bits: order[slot] = sort value.
+//   k_rb_gather    sorted instances and aabbs (a workgroup loads its 256
+//                  records as 16-byte chunks, all loads before the stores,
+//                  writes contiguous), the optional scene SoA, visibility,
+//                  gpu_slots, the sorted rank (read back out of the sorted
+//                  key) and group-head flags from the copied records in LDS.
 //   scan + k_rb_group_heads       first slot of every draw group.
 //   k_rb_draws     GpuDrawCall + DrawIndexedIndirectArgs per group, range flags.
 //   scan + k_rb_ranges + k_rb_range_counts   material_class_ranges.
@@ -32,6 +35,15 @@ namespace hv {
 namespace {
 
 constexpr uint32_t kRbThreads = 256;
+constexpr uint32_t kVisBit = 0x80000000u;  // sort values: dense index | visible << 31
+constexpr uint32_t kIdxMask = 0x7fffffffu;
+constexpr uint32_t kMatBlockRows = 2048;   // material rows ranked inside one workgroup
+constexpr uint32_t kPrepBlocks = 1024;     // k_rb_prep grid cap: one max-atomic pair per workgroup
+
+// Onesweep radix sort for every size above one workgroup's tile (rocPRIM's
+// default switches to a block sort + merge passes up to 2^20 keys).
+using SortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                              rocprim::default_config, 0>;
 
 inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kRbThreads - 1) / kRbThreads); }
 
@@ -46,6 +58,72 @@ __global__ __launch_bounds__(64) void k_rb_empty(uint32_t* __restrict__ counts) 
     if (threadIdx.x < 2) counts[threadIdx.x] = 0u;
 }
 
+// Dense ranks of up to kMatBlockRows (class, graph_hash) rows in one
+// workgroup: bitonic sort of (class, graph_hash, row) in LDS (padding rows
+// sort last), neighbour-difference flags, a block scan.
+__global__ __launch_bounds__(1024) void k_rb_mat_rank_block(const hv_material_key* __restrict__ mats,
+                                                            uint32_t m_count, uint32_t* __restrict__ rank,
+                                                            uint32_t* __restrict__ small) {
+    __shared__ uint64_t s_gh[kMatBlockRows];
+    __shared__ uint32_t s_cls[kMatBlockRows];
+    __shared__ uint32_t s_idx[kMatBlockRows];
+    __shared__ uint32_t s_wave[16];
+    const uint32_t t = threadIdx.x, lane = t & 63u, rows = m_count + 1u;
+    uint32_t N = 64;
+    while (N < rows) N <<= 1;
+    for (uint32_t i = t; i < N; i += 1024u) {
+        if (i < rows) {
+            const bool real = i < m_count;
+            s_gh[i] = real ? mats[i].graph_hash : 0ull;
+            s_cls[i] = real ? mats[i].material_class : 0u;
+            s_idx[i] = i;
+        } else {
+            s_gh[i] = ~0ull;
+            s_cls[i] = ~0u;
+            s_idx[i] = ~0u;
+        }
+    }
+    __syncthreads();
+    // One compare-exchange pair (i, i + j) per thread and step: N / 2 <= 1024 pairs.
+    for (uint32_t k = 2; k <= N; k <<= 1) {
+        for (uint32_t j = k >> 1; j; j >>= 1) {
+            if (t < (N >> 1)) {
+                const uint32_t i = ((t & ~(j - 1u)) << 1) | (t & (j - 1u)), l = i + j;
+                const uint32_t ca = s_cls[i], cb = s_cls[l];
+                const uint64_t ga = s_gh[i], gb = s_gh[l];
+                const uint32_t ia = s_idx[i], ib = s_idx[l];
+                const bool a_gt_b = ca != cb ? ca > cb : (ga != gb ? ga > gb : ia > ib);
+                if (a_gt_b == ((i & k) == 0u)) {
+                    s_cls[i] = cb;
+                    s_cls[l] = ca;
+                    s_gh[i] = gb;
+                    s_gh[l] = ga;
+                    s_idx[i] = ib;
+                    s_idx[l] = ia;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const uint32_t s0 = 2u * t, s1 = s0 + 1u;
+    const uint32_t f0 = s0 < rows ? (s0 == 0u || s_cls[s0] != s_cls[s0 - 1u] || s_gh[s0] != s_gh[s0 - 1u]) : 0u;
+    const uint32_t f1 = s1 < rows ? (s_cls[s1] != s_cls[s0] || s_gh[s1] != s_gh[s0]) : 0u;
+    uint32_t v = f0 + f1;
+#pragma unroll
+    for (uint32_t off = 1; off < 64u; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)v, off);
+        if (lane >= off) v += y;
+    }
+    if (lane == 63u) s_wave[t >> 6] = v;
+    __syncthreads();
+    uint32_t base = 0;
+    for (uint32_t w = 0; w < (t >> 6); ++w) base += s_wave[w];
+    const uint32_t incl1 = base + v, incl0 = incl1 - f1;
+    if (s0 < rows) rank[s_idx[s0]] = incl0 - 1u;
+    if (s1 < rows) rank[s_idx[s1]] = incl1 - 1u;
+    if (t == 1023u) small[2] = incl1;
+}
+
 __global__ __launch_bounds__(256) void k_rb_mat_init(const hv_material_key* __restrict__ mats, uint32_t m_count,
                                                      uint64_t* __restrict__ gh, uint32_t* __restrict__ cls,
                                                      uint32_t* __restrict__ idx) {
@@ -57,16 +135,12 @@ __global__ __launch_bounds__(256) void k_rb_mat_init(const hv_material_key* __re
     idx[m] = m;
 }
 
+// dst[s] = src[perm[s] & kIdxMask]
 __global__ __launch_bounds__(256) void k_rb_gather_u32(const uint32_t* __restrict__ src,
                                                        const uint32_t* __restrict__ perm, uint32_t n,
                                                        uint32_t* __restrict__ dst) {
     const uint32_t s = blockIdx.x * kRbThreads + threadIdx.x;
-    if (s < n) dst[s] = src[perm[s]];
-}
-
-__global__ __launch_bounds__(256) void k_rb_iota(uint32_t* __restrict__ v, uint32_t n) {
-    const uint32_t s = blockIdx.x * kRbThreads + threadIdx.x;
-    if (s < n) v[s] = s;
+    if (s < n) dst[s] = src[perm[s] & kIdxMask];
 }
 
 // perm lists the material rows in (class, graph_hash) order.
@@ -93,22 +167,26 @@ __global__ __launch_bounds__(256) void k_rb_mat_rank(const uint32_t* __restrict_
     if (s == rows - 1u) small[2] = incl[s];
 }
 
+// Per object (grid-stride): pair rank, (mesh_id, material_id), movable flag,
+// sort value (index | visible << 31); the largest mesh and material ids.
 __global__ __launch_bounds__(256) void k_rb_prep(const hv_instance_data* __restrict__ inst,
                                                  const hv_object_info* __restrict__ obj, uint32_t n,
                                                  const uint32_t* __restrict__ mat_rank, uint32_t m_count,
-                                                 uint32_t* __restrict__ rank_of, uint2* __restrict__ mm,
-                                                 uint32_t* __restrict__ mv, uint32_t* __restrict__ small) {
+                                                 uint64_t hidden, uint32_t* __restrict__ rank_of,
+                                                 uint2* __restrict__ mm, uint32_t* __restrict__ mv,
+                                                 uint32_t* __restrict__ val, uint32_t* __restrict__ small) {
     __shared__ uint32_t s_max[2][kRbThreads / 64];
-    const uint32_t i = blockIdx.x * kRbThreads + threadIdx.x;
     uint32_t mesh = 0u, mat = 0u;
-    if (i < n) {
+    for (uint32_t i = blockIdx.x * kRbThreads + threadIdx.x; i < n; i += gridDim.x * kRbThreads) {
         const uint2 ids = *reinterpret_cast<const uint2*>(&inst[i].mesh_id);  // mesh_id, material_id
-        mesh = ids.x;
-        mat = ids.y;
-        const uint32_t slot = obj[i].material_slot;
-        rank_of[i] = mat_rank[slot < m_count ? slot : m_count];
+        const hv_object_info o = obj[i];
+        rank_of[i] = mat_rank[o.material_slot < m_count ? o.material_slot : m_count];
         mm[i] = ids;
-        mv[i] = obj[i].movability == 2u ? 1u : 0u;
+        mv[i] = o.movability == 2u ? 1u : 0u;
+        const bool visible = o.groups == 0ull || (o.groups & hidden) == 0ull;  // object_is_visible
+        val[i] = i | (visible ? kVisBit : 0u);
+        mesh = max(mesh, ids.x);
+        mat = max(mat, ids.y);
     }
 #pragma unroll
     for (uint32_t off = 32; off; off >>= 1) {
@@ -128,10 +206,10 @@ __global__ __launch_bounds__(256) void k_rb_prep(const hv_instance_data* __restr
 }
 
 // key = rank << rank_shift | mesh << mesh_shift | material (terms switched by use_rank / use_mm).
+template <class K>
 __global__ __launch_bounds__(256) void k_rb_keys(const uint32_t* __restrict__ rank_of, const uint2* __restrict__ mm,
                                                  uint32_t n, uint32_t use_rank, uint32_t rank_shift, uint32_t use_mm,
-                                                 uint32_t mesh_shift, uint64_t* __restrict__ key,
-                                                 uint32_t* __restrict__ val) {
+                                                 uint32_t mesh_shift, K* __restrict__ key) {
     const uint32_t i = blockIdx.x * kRbThreads + threadIdx.x;
     if (i >= n) return;
     uint64_t k = 0ull;
@@ -140,49 +218,72 @@ __global__ __launch_bounds__(256) void k_rb_keys(const uint32_t* __restrict__ ra
         k = ((uint64_t)v.x << mesh_shift) | v.y;
     }
     if (use_rank) k |= (uint64_t)rank_of[i] << rank_shift;
-    key[i] = k;
-    val[i] = i;
+    key[i] = (K)k;
 }
 
 template <bool kGroups>
 __global__ __launch_bounds__(256) void k_rb_gather(
-    const float4* __restrict__ inst, const float4* __restrict__ aabb, const hv_object_info* __restrict__ obj,
-    const uint2* __restrict__ mm, const uint32_t* __restrict__ rank_of, const uint32_t* __restrict__ order,
-    uint32_t n, uint64_t hidden, float4* __restrict__ out_inst, float4* __restrict__ out_aabb,
-    uint32_t* __restrict__ vis, uint32_t* __restrict__ gpu_slots, uint32_t* __restrict__ srank,
-    uint32_t* __restrict__ gflag, float4* __restrict__ sc_bounds, uint32_t* __restrict__ sc_meta) {
+    const float4* __restrict__ inst, const float4* __restrict__ aabb, const uint2* __restrict__ mm,
+    const uint32_t* __restrict__ order, uint32_t n, const uint64_t* __restrict__ skey64,
+    const uint32_t* __restrict__ skey32, uint32_t rank_shift, float4* __restrict__ out_inst,
+    float4* __restrict__ out_aabb, uint32_t* __restrict__ vis, uint32_t* __restrict__ gpu_slots,
+    uint32_t* __restrict__ srank, uint32_t* __restrict__ gflag, float4* __restrict__ sc_bounds,
+    uint32_t* __restrict__ sc_meta) {
     __shared__ uint32_t s_o[kRbThreads];
+    __shared__ uint2 s_mm[kRbThreads];
     const uint32_t t = threadIdx.x, base = blockIdx.x * kRbThreads, s = base + t;
     const uint32_t live = min(kRbThreads, n - base);
-    uint32_t o = 0u;
+    uint32_t v = 0u;
     if (s < n) {
-        o = order[s];
-        s_o[t] = o;
+        v = order[s];
+        s_o[t] = v & kIdxMask;
     }
     __syncthreads();
-    // 144-byte instances: 9 chunks of 16 bytes; chunk c is part c % 9 of record c / 9.
-    for (uint32_t c = t; c < live * 9u; c += kRbThreads) {
-        const uint32_t r = c / 9u, part = c - r * 9u;
-        const float4 v = inst[(size_t)s_o[r] * 9u + part];
-        out_inst[(size_t)(base + r) * 9u + part] = v;
-        if (sc_bounds && part == 7u) sc_bounds[base + r] = v;  // bounds @112
-        if (sc_meta && part == 8u)                             // mesh_id @128, flags @136
-            sc_meta[base + r] = (__float_as_uint(v.x) & 0x7fffffffu) | ((__float_as_uint(v.z) & 1u) << 31);
+    // 144-byte instances: 9 chunks of 16 bytes, chunk c is part c % 9 of record
+    // c / 9; 32-byte aabbs: 2 chunks. Writes are contiguous across the workgroup.
+    auto put = [&](uint32_t r, uint32_t part, const float4& x) {
+        out_inst[(size_t)(base + r) * 9u + part] = x;
+        if (part == 8u) {  // mesh_id @128, material_id @132, flags @136
+            s_mm[r] = make_uint2(__float_as_uint(x.x), __float_as_uint(x.y));
+            if (sc_meta) sc_meta[base + r] = (__float_as_uint(x.x) & 0x7fffffffu) | ((__float_as_uint(x.z) & 1u) << 31);
+        }
+        if (sc_bounds && part == 7u) sc_bounds[base + r] = x;  // bounds @112
+    };
+    if (live == kRbThreads) {
+        float4 x[9];
+#pragma unroll
+        for (uint32_t k = 0; k < 9u; ++k) {
+            const uint32_t c = t + k * kRbThreads, r = c / 9u;
+            x[k] = inst[(size_t)s_o[r] * 9u + (c - r * 9u)];
+        }
+        const uint32_t c1 = t + kRbThreads;
+        const float4 y0 = aabb[(size_t)s_o[t >> 1] * 2u + (t & 1u)];
+        const float4 y1 = aabb[(size_t)s_o[c1 >> 1] * 2u + (c1 & 1u)];
+#pragma unroll
+        for (uint32_t k = 0; k < 9u; ++k) {
+            const uint32_t c = t + k * kRbThreads, r = c / 9u;
+            put(r, c - r * 9u, x[k]);
+        }
+        out_aabb[(size_t)base * 2u + t] = y0;
+        out_aabb[(size_t)base * 2u + c1] = y1;
+    } else {
+        for (uint32_t c = t; c < live * 9u; c += kRbThreads) {
+            const uint32_t r = c / 9u, part = c - r * 9u;
+            put(r, part, inst[(size_t)s_o[r] * 9u + part]);
+        }
+        for (uint32_t c = t; c < live * 2u; c += kRbThreads)
+            out_aabb[(size_t)base * 2u + c] = aabb[(size_t)s_o[c >> 1] * 2u + (c & 1u)];
     }
-    for (uint32_t c = t; c < live * 2u; c += kRbThreads) {
-        const uint32_t r = c >> 1, part = c & 1u;
-        out_aabb[(size_t)(base + r) * 2u + part] = aabb[(size_t)s_o[r] * 2u + part];
-    }
+    __syncthreads();
     if (s < n) {
-        gpu_slots[o] = s;
-        const uint64_t g = obj[o].groups;
-        vis[s] = (g == 0ull || (g & hidden) == 0ull) ? 1u : 0u;  // object_is_visible
-        srank[s] = rank_of[o];
+        gpu_slots[s_o[t]] = s;
+        vis[s] = v >> 31;
+        srank[s] = skey64 ? (uint32_t)(skey64[s] >> rank_shift) : (skey32 ? skey32[s] >> rank_shift : 0u);
         if (kGroups) {
             uint32_t f = 1u;
             if (s) {
-                const uint32_t p = t ? s_o[t - 1] : order[s - 1];
-                const uint2 a = mm[o], b = mm[p];
+                const uint2 a = s_mm[t];
+                const uint2 b = t ? s_mm[t - 1u] : mm[order[s - 1u] & kIdxMask];
                 f = (a.x != b.x || a.y != b.y) ? 1u : 0u;
             }
             gflag[s] = f;
@@ -215,7 +316,7 @@ __global__ __launch_bounds__(256) void k_rb_draws(const uint32_t* __restrict__ g
     }
     const uint32_t h = kIdentity ? g : ghead[g];
     const uint32_t e = kIdentity ? g + 1u : (g + 1u < G ? ghead[g + 1u] : n);
-    const hv_draw_call d = draws[order[h]];
+    const hv_draw_call d = draws[order[h] & kIdxMask];
     dc[g].index_count = d.index_count;
     dc[g].first_index = d.first_index;
     dc[g].vertex_offset = d.vertex_offset;
@@ -241,7 +342,7 @@ __global__ __launch_bounds__(256) void k_rb_ranges(const uint32_t* __restrict__ 
     const uint32_t g = blockIdx.x * kRbThreads + threadIdx.x;
     if (g >= n || !rflag[g]) return;
     const uint32_t r = rid[g] - 1u;
-    const uint32_t o = order[kIdentity ? g : ghead[g]];
+    const uint32_t o = order[kIdentity ? g : ghead[g]] & kIdxMask;
     const uint32_t slot = obj[o].material_slot;
     uint64_t gh = 0ull;
     uint32_t cls = 0u;
@@ -317,10 +418,10 @@ template <class K>
 hv_status sort_pairs(hv_scene_rebuild* rb, const K* kin, K* kout, const uint32_t* vin, uint32_t* vout, uint32_t n,
                      uint32_t bits, hipStream_t st) {
     size_t bytes = 0;
-    HV_HIP(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, n, 0u, bits, st));
+    HV_HIP(rocprim::radix_sort_pairs<SortConfig>(nullptr, bytes, kin, kout, vin, vout, n, 0u, bits, st));
     if (bytes > rb->temp.bytes) return set_error(HV_ELIMIT, "radix sort needs %zu temporary bytes", bytes);
     bytes = rb->temp.bytes;
-    HV_HIP(rocprim::radix_sort_pairs(rb->temp.ptr, bytes, kin, kout, vin, vout, n, 0u, bits, st));
+    HV_HIP(rocprim::radix_sort_pairs<SortConfig>(rb->temp.ptr, bytes, kin, kout, vin, vout, n, 0u, bits, st));
     return HV_OK;
 }
 
@@ -339,12 +440,12 @@ hv_status temp_bytes(uint32_t n, uint32_t rows, hipStream_t st, size_t* out) {
     const uint32_t sizes[2] = {n, rows};
     for (uint32_t sz : sizes) {
         b = 0;
-        HV_HIP(rocprim::radix_sort_pairs(nullptr, b, (const uint64_t*)nullptr, (uint64_t*)nullptr,
-                                         (const uint32_t*)nullptr, (uint32_t*)nullptr, sz, 0u, 64u, st));
+        HV_HIP(rocprim::radix_sort_pairs<SortConfig>(nullptr, b, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                                     (const uint32_t*)nullptr, (uint32_t*)nullptr, sz, 0u, 64u, st));
         best = std::max(best, b);
         b = 0;
-        HV_HIP(rocprim::radix_sort_pairs(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                         (const uint32_t*)nullptr, (uint32_t*)nullptr, sz, 0u, 32u, st));
+        HV_HIP(rocprim::radix_sort_pairs<SortConfig>(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                     (const uint32_t*)nullptr, (uint32_t*)nullptr, sz, 0u, 32u, st));
         best = std::max(best, b);
         b = 0;
         HV_HIP(rocprim::inclusive_scan(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr, sz,
@@ -449,24 +550,31 @@ hv_status hv_scene_rebuild_execute(hv_scene_rebuild* rb, const hv_rebuild_inputs
     uint32_t* small = rb->small.as<uint32_t>();
     // 1. Dense ranks of the (class, graph_hash) pairs.
     hipLaunchKernelGGL(k_rb_clear, dim3(1), dim3(64), 0, st, small);
-    hipLaunchKernelGGL(k_rb_mat_init, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, in->materials, m_count,
-                       rb->m_gh.as<uint64_t>(), rb->m_cls.as<uint32_t>(), rb->m_idx.as<uint32_t>());
+    if (rows <= kMatBlockRows) {
+        hipLaunchKernelGGL(k_rb_mat_rank_block, dim3(1), dim3(1024), 0, st, in->materials, m_count,
+                           rb->m_rank.as<uint32_t>(), small);
+    } else {
+        hipLaunchKernelGGL(k_rb_mat_init, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, in->materials, m_count,
+                           rb->m_gh.as<uint64_t>(), rb->m_cls.as<uint32_t>(), rb->m_idx.as<uint32_t>());
+        HV_HIP(hipGetLastError());
+        HV_TRY(sort_pairs<uint64_t>(rb, rb->m_gh.as<uint64_t>(), rb->m_gh_s.as<uint64_t>(), rb->m_idx.as<uint32_t>(),
+                                    rb->m_p1.as<uint32_t>(), rows, 64u, st));
+        hipLaunchKernelGGL(k_rb_gather_u32, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st,
+                           rb->m_cls.as<uint32_t>(), rb->m_p1.as<uint32_t>(), rows, rb->m_cls_k.as<uint32_t>());
+        HV_TRY(sort_pairs<uint32_t>(rb, rb->m_cls_k.as<uint32_t>(), rb->m_cls_s.as<uint32_t>(),
+                                    rb->m_p1.as<uint32_t>(), rb->m_p2.as<uint32_t>(), rows, 32u, st));
+        hipLaunchKernelGGL(k_rb_mat_flags, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, rb->m_p2.as<uint32_t>(),
+                           rb->m_gh.as<uint64_t>(), rb->m_cls.as<uint32_t>(), rows, rb->m_flag.as<uint32_t>());
+        HV_TRY(scan(rb, rb->m_flag.as<uint32_t>(), rb->m_incl.as<uint32_t>(), rows, st));
+        hipLaunchKernelGGL(k_rb_mat_rank, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, rb->m_p2.as<uint32_t>(),
+                           rb->m_incl.as<uint32_t>(), rows, rb->m_rank.as<uint32_t>(), small);
+    }
     HV_HIP(hipGetLastError());
-    HV_TRY(sort_pairs<uint64_t>(rb, rb->m_gh.as<uint64_t>(), rb->m_gh_s.as<uint64_t>(), rb->m_idx.as<uint32_t>(),
-                                rb->m_p1.as<uint32_t>(), rows, 64u, st));
-    hipLaunchKernelGGL(k_rb_gather_u32, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, rb->m_cls.as<uint32_t>(),
-                       rb->m_p1.as<uint32_t>(), rows, rb->m_cls_k.as<uint32_t>());
-    HV_TRY(sort_pairs<uint32_t>(rb, rb->m_cls_k.as<uint32_t>(), rb->m_cls_s.as<uint32_t>(), rb->m_p1.as<uint32_t>(),
-                                rb->m_p2.as<uint32_t>(), rows, 32u, st));
-    hipLaunchKernelGGL(k_rb_mat_flags, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, rb->m_p2.as<uint32_t>(),
-                       rb->m_gh.as<uint64_t>(), rb->m_cls.as<uint32_t>(), rows, rb->m_flag.as<uint32_t>());
-    HV_TRY(scan(rb, rb->m_flag.as<uint32_t>(), rb->m_incl.as<uint32_t>(), rows, st));
-    hipLaunchKernelGGL(k_rb_mat_rank, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, rb->m_p2.as<uint32_t>(),
-                       rb->m_incl.as<uint32_t>(), rows, rb->m_rank.as<uint32_t>(), small);
-    // 2. Per-object rank, ids, movable flag and the id maxima.
-    hipLaunchKernelGGL(k_rb_prep, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, in->instances, in->objects, n,
-                       rb->m_rank.as<uint32_t>(), m_count, rb->rank_of.as<uint32_t>(), rb->mm.as<uint2>(),
-                       rb->mv.as<uint32_t>(), small);
+    // 2. Per-object rank, ids, movable flag, sort value and the id maxima.
+    hipLaunchKernelGGL(k_rb_prep, dim3(std::min(blocks_for(n), kPrepBlocks)), dim3(kRbThreads), 0, st, in->instances,
+                       in->objects, n, rb->m_rank.as<uint32_t>(), m_count, in->group_hidden,
+                       rb->rank_of.as<uint32_t>(), rb->mm.as<uint2>(), rb->mv.as<uint32_t>(), rb->val.as<uint32_t>(),
+                       small);
     HV_HIP(hipGetLastError());
     HV_HIP(hipMemcpyAsync(rb->host_small, small, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HV_HIP(hipStreamSynchronize(st));
@@ -476,27 +584,39 @@ hv_status hv_scene_rebuild_execute(hv_scene_rebuild* rb, const hv_rebuild_inputs
                          max_mesh);
     const uint32_t rbits = bit_width(pairs ? pairs - 1u : 0u);
     const uint32_t mbits = grouped ? bit_width(max_mesh) : 0u, tbits = grouped ? bit_width(max_mat) : 0u;
+    const uint32_t total = rbits + mbits + tbits;
 
-    // 3. Stable sort of the dense indices.
+    // 3. Stable sort of the sort values (dense index | visible << 31). The
+    // sorted keys keep each slot's rank: skey >> rank_shift.
     uint32_t* order = rb->order.as<uint32_t>();
-    if (rbits + mbits + tbits == 0u) {
-        hipLaunchKernelGGL(k_rb_iota, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, order, n);
-    } else if (rbits + mbits + tbits <= 64u) {
-        hipLaunchKernelGGL(k_rb_keys, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, rb->rank_of.as<uint32_t>(),
-                           rb->mm.as<uint2>(), n, rbits ? 1u : 0u, mbits + tbits, (mbits + tbits) ? 1u : 0u, tbits,
-                           rb->key_a.as<uint64_t>(), rb->val.as<uint32_t>());
+    const uint64_t* skey64 = nullptr;
+    const uint32_t* skey32 = nullptr;
+    uint32_t rank_shift = 0u;
+    const dim3 grid(blocks_for(n)), block(kRbThreads);
+    if (total == 0u) {
+        HV_HIP(hipMemcpyAsync(order, rb->val.ptr, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    } else if (total <= 32u) {
+        hipLaunchKernelGGL(k_rb_keys<uint32_t>, grid, block, 0, st, rb->rank_of.as<uint32_t>(), rb->mm.as<uint2>(), n,
+                           rbits ? 1u : 0u, mbits + tbits, (mbits + tbits) ? 1u : 0u, tbits, rb->key32_a.as<uint32_t>());
+        HV_TRY(sort_pairs<uint32_t>(rb, rb->key32_a.as<uint32_t>(), rb->key32_b.as<uint32_t>(), rb->val.as<uint32_t>(),
+                                    order, n, total, st));
+        if (rbits) skey32 = rb->key32_b.as<uint32_t>(), rank_shift = mbits + tbits;
+    } else if (total <= 64u) {
+        hipLaunchKernelGGL(k_rb_keys<uint64_t>, grid, block, 0, st, rb->rank_of.as<uint32_t>(), rb->mm.as<uint2>(), n,
+                           rbits ? 1u : 0u, mbits + tbits, 1u, tbits, rb->key_a.as<uint64_t>());
         HV_TRY(sort_pairs<uint64_t>(rb, rb->key_a.as<uint64_t>(), rb->key_b.as<uint64_t>(), rb->val.as<uint32_t>(),
-                                    order, n, rbits + mbits + tbits, st));
+                                    order, n, total, st));
+        if (rbits) skey64 = rb->key_b.as<uint64_t>(), rank_shift = mbits + tbits;
     } else {  // (mesh, material) first, then the rank; both passes stable
-        hipLaunchKernelGGL(k_rb_keys, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, rb->rank_of.as<uint32_t>(),
-                           rb->mm.as<uint2>(), n, 0u, 0u, 1u, tbits, rb->key_a.as<uint64_t>(),
-                           rb->val.as<uint32_t>());
+        hipLaunchKernelGGL(k_rb_keys<uint64_t>, grid, block, 0, st, rb->rank_of.as<uint32_t>(), rb->mm.as<uint2>(), n,
+                           0u, 0u, 1u, tbits, rb->key_a.as<uint64_t>());
         HV_TRY(sort_pairs<uint64_t>(rb, rb->key_a.as<uint64_t>(), rb->key_b.as<uint64_t>(), rb->val.as<uint32_t>(),
                                     rb->order1.as<uint32_t>(), n, mbits + tbits, st));
-        hipLaunchKernelGGL(k_rb_gather_u32, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, rb->rank_of.as<uint32_t>(),
-                           rb->order1.as<uint32_t>(), n, rb->key32_a.as<uint32_t>());
+        hipLaunchKernelGGL(k_rb_gather_u32, grid, block, 0, st, rb->rank_of.as<uint32_t>(), rb->order1.as<uint32_t>(),
+                           n, rb->key32_a.as<uint32_t>());
         HV_TRY(sort_pairs<uint32_t>(rb, rb->key32_a.as<uint32_t>(), rb->key32_b.as<uint32_t>(),
                                     rb->order1.as<uint32_t>(), order, n, rbits, st));
+        skey32 = rb->key32_b.as<uint32_t>();
     }
     HV_HIP(hipGetLastError());
 
@@ -508,15 +628,13 @@ hv_status hv_scene_rebuild_execute(hv_scene_rebuild* rb, const hv_rebuild_inputs
     auto* oinst4 = reinterpret_cast<float4*>(out->instances);
     auto* oaabb4 = reinterpret_cast<float4*>(out->aabbs);
     if (grouped)
-        hipLaunchKernelGGL(k_rb_gather<true>, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, inst4, aabb4, in->objects,
-                           rb->mm.as<uint2>(), rb->rank_of.as<uint32_t>(), order, n, in->group_hidden, oinst4, oaabb4,
-                           out->visibility, out->gpu_slots, rb->srank.as<uint32_t>(), rb->gflag.as<uint32_t>(),
-                           sc_bounds, sc_meta);
+        hipLaunchKernelGGL(k_rb_gather<true>, grid, block, 0, st, inst4, aabb4, rb->mm.as<uint2>(), order, n, skey64,
+                           skey32, rank_shift, oinst4, oaabb4, out->visibility, out->gpu_slots,
+                           rb->srank.as<uint32_t>(), rb->gflag.as<uint32_t>(), sc_bounds, sc_meta);
     else
-        hipLaunchKernelGGL(k_rb_gather<false>, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, inst4, aabb4,
-                           in->objects, rb->mm.as<uint2>(), rb->rank_of.as<uint32_t>(), order, n, in->group_hidden,
-                           oinst4, oaabb4, out->visibility, out->gpu_slots, rb->srank.as<uint32_t>(),
-                           rb->gflag.as<uint32_t>(), sc_bounds, sc_meta);
+        hipLaunchKernelGGL(k_rb_gather<false>, grid, block, 0, st, inst4, aabb4, rb->mm.as<uint2>(), order, n, skey64,
+                           skey32, rank_shift, oinst4, oaabb4, out->visibility, out->gpu_slots,
+                           rb->srank.as<uint32_t>(), rb->gflag.as<uint32_t>(), sc_bounds, sc_meta);
     HV_HIP(hipGetLastError());
 
     // 5. Draw groups, material ranges, shadow lists.

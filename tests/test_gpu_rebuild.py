@@ -46,6 +46,8 @@ CASES = {
     "n257": (257, dict(material_count=1)),
     "n5000_no_materials": (5000, dict(material_count=0)),
     "n20000_shared_pairs": (20_000, dict(material_count=5000, hash_pool=3, class_count=2)),
+    "n9000_2047_materials": (9000, dict(material_count=2047, hash_pool=50, class_count=7)),
+    "n9000_2048_materials": (9000, dict(material_count=2048, hash_pool=50, class_count=7)),
     "n4099_one_key": (4099, dict(mesh_count=1, material_count=1, class_count=1, hash_pool=1, p_missing=0.0,
                                  p_foreign_material_id=0.0)),
     "n30000_wide_ids": (30_000, dict(wide_ids=True)),
