@@ -11,10 +11,11 @@
 //   k_rb_prep      per object: pair rank, (mesh_id, material_id), movable flag,
 //                  sort value = index | visible << 31; the largest mesh and
 //                  material ids.
-//   k_rb_keys + onesweep radix sort
+//   k_rb_keys + radix sort (hv_radix.h)
 //                  packed key rank | mesh | material, 32 or 64 bits wide from
-//                  the widths the host read back, or two stable passes when it
-//                  needs more than 64 bits: order[slot] = sort value.
+//                  the widths the host read back (8-bit digits, as many passes
+//                  as the key has bits), or two stable sorts when it needs more
+//                  than 64 bits: order[slot] = sort value.
 //   k_rb_gather    sorted instances and aabbs (a workgroup loads its 256
 //                  records as 16-byte chunks, all loads before the stores,
 //                  writes contiguous), the optional scene SoA, visibility,
@@ -26,10 +27,8 @@
 //   scan + k_rb_shadow            static / movable shadow lists in dense order.
 //
 // Sorting, scans and compaction are integer work: bit-exact by construction.
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
-
 #include "hv_internal.h"
+#include "hv_radix.h"
 
 namespace hv {
 namespace {
@@ -39,11 +38,6 @@ constexpr uint32_t kVisBit = 0x80000000u;  // sort values: dense index | visible
 constexpr uint32_t kIdxMask = 0x7fffffffu;
 constexpr uint32_t kMatBlockRows = 2048;   // material rows ranked inside one workgroup
 constexpr uint32_t kPrepBlocks = 1024;     // k_rb_prep grid cap: one max-atomic pair per workgroup
-
-// Onesweep radix sort for every size above one workgroup's tile (rocPRIM's
-// default switches to a block sort + merge passes up to 2^20 keys).
-using SortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
-                                              rocprim::default_config, 0>;
 
 inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kRbThreads - 1) / kRbThreads); }
 
@@ -404,7 +398,7 @@ using namespace hv;
 struct hv_scene_rebuild {
     hv_ctx* ctx = nullptr;
     // material ranking (rows = material_count + 1)
-    DevBuf m_gh, m_gh_s, m_cls, m_cls_k, m_cls_s, m_idx, m_p1, m_p2, m_flag, m_incl, m_rank;
+    DevBuf m_gh, m_gh_s, m_gh_t, m_cls, m_cls_k, m_idx, m_p1, m_p2, m_flag, m_incl, m_rank;
     // per object / per slot
     DevBuf rank_of, mm, mv, mv_incl, key_a, key_b, key32_a, key32_b, val, order1, order;
     DevBuf srank, gflag, gid, ghead, rflag, rid, rhead;
@@ -413,48 +407,6 @@ struct hv_scene_rebuild {
 };
 
 namespace {
-
-template <class K>
-hv_status sort_pairs(hv_scene_rebuild* rb, const K* kin, K* kout, const uint32_t* vin, uint32_t* vout, uint32_t n,
-                     uint32_t bits, hipStream_t st) {
-    size_t bytes = 0;
-    HV_HIP(rocprim::radix_sort_pairs<SortConfig>(nullptr, bytes, kin, kout, vin, vout, n, 0u, bits, st));
-    if (bytes > rb->temp.bytes) return set_error(HV_ELIMIT, "radix sort needs %zu temporary bytes", bytes);
-    bytes = rb->temp.bytes;
-    HV_HIP(rocprim::radix_sort_pairs<SortConfig>(rb->temp.ptr, bytes, kin, kout, vin, vout, n, 0u, bits, st));
-    return HV_OK;
-}
-
-hv_status scan(hv_scene_rebuild* rb, const uint32_t* in, uint32_t* out, uint32_t n, hipStream_t st) {
-    size_t bytes = 0;
-    HV_HIP(rocprim::inclusive_scan(nullptr, bytes, in, out, n, rocprim::plus<uint32_t>(), st));
-    if (bytes > rb->temp.bytes) return set_error(HV_ELIMIT, "scan needs %zu temporary bytes", bytes);
-    bytes = rb->temp.bytes;
-    HV_HIP(rocprim::inclusive_scan(rb->temp.ptr, bytes, in, out, n, rocprim::plus<uint32_t>(), st));
-    return HV_OK;
-}
-
-// Largest temporary storage any rocPRIM call below needs for n objects and `rows` material rows.
-hv_status temp_bytes(uint32_t n, uint32_t rows, hipStream_t st, size_t* out) {
-    size_t best = 0, b = 0;
-    const uint32_t sizes[2] = {n, rows};
-    for (uint32_t sz : sizes) {
-        b = 0;
-        HV_HIP(rocprim::radix_sort_pairs<SortConfig>(nullptr, b, (const uint64_t*)nullptr, (uint64_t*)nullptr,
-                                                     (const uint32_t*)nullptr, (uint32_t*)nullptr, sz, 0u, 64u, st));
-        best = std::max(best, b);
-        b = 0;
-        HV_HIP(rocprim::radix_sort_pairs<SortConfig>(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                     (const uint32_t*)nullptr, (uint32_t*)nullptr, sz, 0u, 32u, st));
-        best = std::max(best, b);
-        b = 0;
-        HV_HIP(rocprim::inclusive_scan(nullptr, b, (const uint32_t*)nullptr, (uint32_t*)nullptr, sz,
-                                       rocprim::plus<uint32_t>(), st));
-        best = std::max(best, b);
-    }
-    *out = best;
-    return HV_OK;
-}
 
 bool misaligned(const void* p, uintptr_t a) { return p && ((uintptr_t)p & (a - 1)) != 0; }
 
@@ -480,7 +432,7 @@ void hv_scene_rebuild_destroy(hv_scene_rebuild* rb) {
     if (!rb) return;
     (void)hipSetDevice(rb->ctx->device);
     (void)hipDeviceSynchronize();
-    DevBuf* bufs[] = {&rb->m_gh,    &rb->m_gh_s,   &rb->m_cls,   &rb->m_cls_k, &rb->m_cls_s, &rb->m_idx,
+    DevBuf* bufs[] = {&rb->m_gh,    &rb->m_gh_s,   &rb->m_cls,   &rb->m_cls_k, &rb->m_gh_t,  &rb->m_idx,
                       &rb->m_p1,    &rb->m_p2,     &rb->m_flag,  &rb->m_incl,  &rb->m_rank,  &rb->rank_of,
                       &rb->mm,      &rb->mv,       &rb->mv_incl, &rb->key_a,   &rb->key_b,   &rb->key32_a,
                       &rb->key32_b, &rb->val,      &rb->order1,  &rb->order,   &rb->srank,   &rb->gflag,
@@ -525,15 +477,14 @@ hv_status hv_scene_rebuild_execute(hv_scene_rebuild* rb, const hv_rebuild_inputs
     const bool grouped = mode == HV_REBUILD_OPTIMIZED;
 
     // Every allocation before the first launch (a growing DevBuf frees its old block).
-    size_t tb = 0;
-    HV_TRY(temp_bytes(n, rows, st, &tb));
-    HV_TRY(rb->temp.reserve(tb ? tb : 4, false, st));
+    HV_TRY(rb->temp.reserve(radix::scratch_words(std::max(n, rows)) * 4, false, st));
     HV_TRY(rb->small.reserve(8 * 4, false, st));
-    DevBuf* rows_u32[] = {&rb->m_cls, &rb->m_cls_k, &rb->m_cls_s, &rb->m_idx, &rb->m_p1,
+    DevBuf* rows_u32[] = {&rb->m_cls, &rb->m_cls_k, &rb->m_idx, &rb->m_p1,
                           &rb->m_p2,  &rb->m_flag,  &rb->m_incl,  &rb->m_rank};
     for (DevBuf* b : rows_u32) HV_TRY(b->reserve((uint64_t)rows * 4, false, st));
     HV_TRY(rb->m_gh.reserve((uint64_t)rows * 8, false, st));
     HV_TRY(rb->m_gh_s.reserve((uint64_t)rows * 8, false, st));
+    HV_TRY(rb->m_gh_t.reserve((uint64_t)rows * 8, false, st));
     DevBuf* obj_u32[] = {&rb->rank_of, &rb->mv,    &rb->mv_incl, &rb->key32_a, &rb->key32_b, &rb->val,
                          &rb->order1,  &rb->order, &rb->srank,   &rb->gflag,   &rb->gid,     &rb->ghead,
                          &rb->rflag,   &rb->rid,   &rb->rhead};
@@ -557,16 +508,23 @@ hv_status hv_scene_rebuild_execute(hv_scene_rebuild* rb, const hv_rebuild_inputs
         hipLaunchKernelGGL(k_rb_mat_init, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, in->materials, m_count,
                            rb->m_gh.as<uint64_t>(), rb->m_cls.as<uint32_t>(), rb->m_idx.as<uint32_t>());
         HV_HIP(hipGetLastError());
-        HV_TRY(sort_pairs<uint64_t>(rb, rb->m_gh.as<uint64_t>(), rb->m_gh_s.as<uint64_t>(), rb->m_idx.as<uint32_t>(),
-                                    rb->m_p1.as<uint32_t>(), rows, 64u, st));
-        hipLaunchKernelGGL(k_rb_gather_u32, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st,
-                           rb->m_cls.as<uint32_t>(), rb->m_p1.as<uint32_t>(), rows, rb->m_cls_k.as<uint32_t>());
-        HV_TRY(sort_pairs<uint32_t>(rb, rb->m_cls_k.as<uint32_t>(), rb->m_cls_s.as<uint32_t>(),
-                                    rb->m_p1.as<uint32_t>(), rb->m_p2.as<uint32_t>(), rows, 32u, st));
-        hipLaunchKernelGGL(k_rb_mat_flags, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, rb->m_p2.as<uint32_t>(),
+        // (class, graph_hash) order: stable by graph_hash, then stable by class.
+        // The sorts consume their inputs, so they run on copies (m_gh_s, m_p1).
+        uint32_t* scratch = rb->temp.as<uint32_t>();
+        HV_HIP(hipMemcpyAsync(rb->m_gh_s.ptr, rb->m_gh.ptr, (size_t)rows * 8, hipMemcpyDeviceToDevice, st));
+        uint64_t *ghk = nullptr, *ghk2 = rb->m_gh_s.as<uint64_t>();
+        uint32_t *p1 = nullptr, *cls_k = nullptr, *p2 = nullptr;
+        HV_TRY(radix::sort_pairs<uint64_t>(ghk2, rb->m_gh_t.as<uint64_t>(), rb->m_idx.as<uint32_t>(),
+                                           rb->m_p1.as<uint32_t>(), rows, 64u, scratch, st, &ghk, &p1));
+        hipLaunchKernelGGL(k_rb_gather_u32, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, rb->m_cls.as<uint32_t>(),
+                           p1, rows, rb->m_cls_k.as<uint32_t>());
+        uint32_t* p_free = p1 == rb->m_p1.as<uint32_t>() ? rb->m_idx.as<uint32_t>() : rb->m_p1.as<uint32_t>();
+        HV_TRY(radix::sort_pairs<uint32_t>(rb->m_cls_k.as<uint32_t>(), rb->m_p2.as<uint32_t>(), p1, p_free, rows, 32u,
+                                           scratch, st, &cls_k, &p2));
+        hipLaunchKernelGGL(k_rb_mat_flags, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, p2,
                            rb->m_gh.as<uint64_t>(), rb->m_cls.as<uint32_t>(), rows, rb->m_flag.as<uint32_t>());
-        HV_TRY(scan(rb, rb->m_flag.as<uint32_t>(), rb->m_incl.as<uint32_t>(), rows, st));
-        hipLaunchKernelGGL(k_rb_mat_rank, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, rb->m_p2.as<uint32_t>(),
+        HV_TRY(radix::scan_u32(rb->m_flag.as<uint32_t>(), rb->m_incl.as<uint32_t>(), rows, scratch, st));
+        hipLaunchKernelGGL(k_rb_mat_rank, dim3(blocks_for(rows)), dim3(kRbThreads), 0, st, p2,
                            rb->m_incl.as<uint32_t>(), rows, rb->m_rank.as<uint32_t>(), small);
     }
     HV_HIP(hipGetLastError());
@@ -588,35 +546,42 @@ hv_status hv_scene_rebuild_execute(hv_scene_rebuild* rb, const hv_rebuild_inputs
 
     // 3. Stable sort of the sort values (dense index | visible << 31). The
     // sorted keys keep each slot's rank: skey >> rank_shift.
+    uint32_t* scratch = rb->temp.as<uint32_t>();
     uint32_t* order = rb->order.as<uint32_t>();
     const uint64_t* skey64 = nullptr;
     const uint32_t* skey32 = nullptr;
     uint32_t rank_shift = 0u;
     const dim3 grid(blocks_for(n)), block(kRbThreads);
     if (total == 0u) {
-        HV_HIP(hipMemcpyAsync(order, rb->val.ptr, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+        order = rb->val.as<uint32_t>();
     } else if (total <= 32u) {
         hipLaunchKernelGGL(k_rb_keys<uint32_t>, grid, block, 0, st, rb->rank_of.as<uint32_t>(), rb->mm.as<uint2>(), n,
                            rbits ? 1u : 0u, mbits + tbits, (mbits + tbits) ? 1u : 0u, tbits, rb->key32_a.as<uint32_t>());
-        HV_TRY(sort_pairs<uint32_t>(rb, rb->key32_a.as<uint32_t>(), rb->key32_b.as<uint32_t>(), rb->val.as<uint32_t>(),
-                                    order, n, total, st));
-        if (rbits) skey32 = rb->key32_b.as<uint32_t>(), rank_shift = mbits + tbits;
+        uint32_t* k = nullptr;
+        HV_TRY(radix::sort_pairs<uint32_t>(rb->key32_a.as<uint32_t>(), rb->key32_b.as<uint32_t>(),
+                                           rb->val.as<uint32_t>(), rb->order.as<uint32_t>(), n, total, scratch, st, &k,
+                                           &order));
+        if (rbits) skey32 = k, rank_shift = mbits + tbits;
     } else if (total <= 64u) {
         hipLaunchKernelGGL(k_rb_keys<uint64_t>, grid, block, 0, st, rb->rank_of.as<uint32_t>(), rb->mm.as<uint2>(), n,
                            rbits ? 1u : 0u, mbits + tbits, 1u, tbits, rb->key_a.as<uint64_t>());
-        HV_TRY(sort_pairs<uint64_t>(rb, rb->key_a.as<uint64_t>(), rb->key_b.as<uint64_t>(), rb->val.as<uint32_t>(),
-                                    order, n, total, st));
-        if (rbits) skey64 = rb->key_b.as<uint64_t>(), rank_shift = mbits + tbits;
+        uint64_t* k = nullptr;
+        HV_TRY(radix::sort_pairs<uint64_t>(rb->key_a.as<uint64_t>(), rb->key_b.as<uint64_t>(), rb->val.as<uint32_t>(),
+                                           rb->order.as<uint32_t>(), n, total, scratch, st, &k, &order));
+        if (rbits) skey64 = k, rank_shift = mbits + tbits;
     } else {  // (mesh, material) first, then the rank; both passes stable
         hipLaunchKernelGGL(k_rb_keys<uint64_t>, grid, block, 0, st, rb->rank_of.as<uint32_t>(), rb->mm.as<uint2>(), n,
                            0u, 0u, 1u, tbits, rb->key_a.as<uint64_t>());
-        HV_TRY(sort_pairs<uint64_t>(rb, rb->key_a.as<uint64_t>(), rb->key_b.as<uint64_t>(), rb->val.as<uint32_t>(),
-                                    rb->order1.as<uint32_t>(), n, mbits + tbits, st));
-        hipLaunchKernelGGL(k_rb_gather_u32, grid, block, 0, st, rb->rank_of.as<uint32_t>(), rb->order1.as<uint32_t>(),
-                           n, rb->key32_a.as<uint32_t>());
-        HV_TRY(sort_pairs<uint32_t>(rb, rb->key32_a.as<uint32_t>(), rb->key32_b.as<uint32_t>(),
-                                    rb->order1.as<uint32_t>(), order, n, rbits, st));
-        skey32 = rb->key32_b.as<uint32_t>();
+        uint64_t* k1 = nullptr;
+        uint32_t* v1 = nullptr;
+        HV_TRY(radix::sort_pairs<uint64_t>(rb->key_a.as<uint64_t>(), rb->key_b.as<uint64_t>(), rb->val.as<uint32_t>(),
+                                           rb->order1.as<uint32_t>(), n, mbits + tbits, scratch, st, &k1, &v1));
+        hipLaunchKernelGGL(k_rb_gather_u32, grid, block, 0, st, rb->rank_of.as<uint32_t>(), v1, n,
+                           rb->key32_a.as<uint32_t>());
+        uint32_t* k = nullptr;
+        HV_TRY(radix::sort_pairs<uint32_t>(rb->key32_a.as<uint32_t>(), rb->key32_b.as<uint32_t>(), v1,
+                                           rb->order.as<uint32_t>(), n, rbits, scratch, st, &k, &order));
+        skey32 = k;
     }
     HV_HIP(hipGetLastError());
 
@@ -639,7 +604,7 @@ hv_status hv_scene_rebuild_execute(hv_scene_rebuild* rb, const hv_rebuild_inputs
 
     // 5. Draw groups, material ranges, shadow lists.
     if (grouped) {
-        HV_TRY(scan(rb, rb->gflag.as<uint32_t>(), rb->gid.as<uint32_t>(), n, st));
+        HV_TRY(radix::scan_u32(rb->gflag.as<uint32_t>(), rb->gid.as<uint32_t>(), n, scratch, st));
         hipLaunchKernelGGL(k_rb_group_heads, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, rb->gflag.as<uint32_t>(),
                            rb->gid.as<uint32_t>(), n, rb->ghead.as<uint32_t>());
         hipLaunchKernelGGL(k_rb_draws<false>, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, rb->ghead.as<uint32_t>(),
@@ -651,7 +616,7 @@ hv_status hv_scene_rebuild_execute(hv_scene_rebuild* rb, const hv_rebuild_inputs
                            out->indirect, rb->rflag.as<uint32_t>());
     }
     HV_HIP(hipGetLastError());
-    HV_TRY(scan(rb, rb->rflag.as<uint32_t>(), rb->rid.as<uint32_t>(), n, st));
+    HV_TRY(radix::scan_u32(rb->rflag.as<uint32_t>(), rb->rid.as<uint32_t>(), n, scratch, st));
     if (grouped) {
         hipLaunchKernelGGL(k_rb_ranges<false>, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, rb->rflag.as<uint32_t>(),
                            rb->rid.as<uint32_t>(), rb->ghead.as<uint32_t>(), order, in->objects, in->materials,
@@ -668,7 +633,7 @@ hv_status hv_scene_rebuild_execute(hv_scene_rebuild* rb, const hv_rebuild_inputs
                            out->counts);
     }
     HV_HIP(hipGetLastError());
-    HV_TRY(scan(rb, rb->mv.as<uint32_t>(), rb->mv_incl.as<uint32_t>(), n, st));
+    HV_TRY(radix::scan_u32(rb->mv.as<uint32_t>(), rb->mv_incl.as<uint32_t>(), n, scratch, st));
     hipLaunchKernelGGL(k_rb_shadow, dim3(blocks_for(n)), dim3(kRbThreads), 0, st, in->draws, out->gpu_slots,
                        rb->mv.as<uint32_t>(), rb->mv_incl.as<uint32_t>(), n, out->shadow_static, out->shadow_movable,
                        out->counts);
