@@ -10,7 +10,10 @@ sub-pixel test, distance LOD (25/75/200 m), compaction into bucket-major
 visible ids and DrawIndexedIndirect args.
 
   value          culled instances/s (view x instance tests per second) of the
-                 batched path: the 64 views in one hipGraph replay.
+                 batched path: the 64 views in one hipGraph replay per step,
+                 two copies of the batch in flight on two streams (one step's
+                 column scan and scatter overlap the next step's classify);
+                 one_in_flight gives the same steps on one stream.
   plain          the same 64 views as a per-view launch loop (prepared once,
                  3 eager launches per view), views/s and p50/p99.
   roofline       the classify kernel: algorithmic bytes (20 B per test) /
@@ -40,6 +43,7 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 N_INSTANCES = 8_388_608
 BYTES_PER_TEST = 20  # float4 bounds + u32 meta
+IN_FLIGHT = 2  # batches in flight in the steady-state batched leg
 CALIBRATION_COPY_BYTES = 256 << 20  # device-to-device copy used to scale the HBM counters
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_k_classify_multiview.json"
 
@@ -407,7 +411,7 @@ def main() -> int:
         return 2
 
     from helio_amd import synth
-    from helio_amd.passes import BatchCull, Context, InstanceCullPass, Scene
+    from helio_amd.passes import BatchCull, Context, InstanceCullPass, Scene, Stream
 
     t_setup = time.perf_counter()
     ctx = Context(0)
@@ -439,6 +443,7 @@ def main() -> int:
         # the fastest; every later pass of this run uses it.
         tuned = autotune.tune_batch(batch, reps=5)["best"]
         log(f"launch shape wg={tuned['wg']} ipt={tuned['ipt']} ({tuned['total_ms'] * 1e3:.1f} us kernels)")
+    # Sequential: one batch, one graph replay per step on one stream.
     for _ in range(args.warmup):
         batch.execute(graph=True)
     batch.wait()
@@ -447,9 +452,40 @@ def main() -> int:
         batch.execute(graph=True)
     batch.wait()
     ctx.synchronize()
+    dt_seq = time.perf_counter() - t0
+    ms_seq = dt_seq / args.steps * 1e3
+    log(f"batched, one in flight: {ms_seq:.3f} ms/step")
+    # Steady state: IN_FLIGHT prepared copies of the batch replay their graphs
+    # on their own streams, steps alternating, so one step's latency-bound
+    # column scan and scatter overlap the next step's classify. Every step
+    # still culls all 64 views.
+    batches = [batch] + [BatchCull(ctx, len(views)) for _ in range(IN_FLIGHT - 1)]
+    for b in batches[1:]:
+        b.set_shape(tuned["wg"], tuned["ipt"])
+        b.prepare([(scene, v.camera, f, c) for v, f, c in views])
+    streams = [Stream(ctx) for _ in batches]
+    for i in range(max(args.warmup, IN_FLIGHT)):
+        batches[i % IN_FLIGHT].execute(stream=streams[i % IN_FLIGHT].handle, graph=True)
+    for b in batches:
+        b.wait()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        batches[i % IN_FLIGHT].execute(stream=streams[i % IN_FLIGHT].handle, graph=True)
+    for b in batches:
+        b.wait()
+    ctx.synchronize()
     dt_batched = time.perf_counter() - t0
     ms_batched = dt_batched / args.steps * 1e3
-    log(f"batched: {ms_batched:.3f} ms/step")
+    log(f"batched, {IN_FLIGHT} in flight: {ms_batched:.3f} ms/step")
+    for vi in (0, len(views) // 2, len(views) - 1):  # the copies computed the same culls
+        a = batches[0].read(vi)
+        for b in batches[1:]:
+            c = b.read(vi)
+            assert np.array_equal(a[2], c[2]) and np.array_equal(a[1].view(np.uint32), c[1].view(np.uint32)), vi
+    for b in batches[1:]:
+        b.close()
+    for st in streams:
+        st.close()
 
     # batched launch latency (graph launch -> results ready)
     lat_b = []
@@ -585,7 +621,9 @@ def main() -> int:
             "tests_per_step": tests_per_step,
             "lod_distances_m": [25, 75, 200],
             "buckets_per_view": sc.mesh_count * sc.lod_count,
-            "launch": "hipGraph replay (classify, column scan, scatter)",
+            "launch": f"hipGraph replay (classify, column scan, scatter); {IN_FLIGHT} copies of the batch in "
+                      f"flight on {IN_FLIGHT} streams, steps alternating",
+            "in_flight": IN_FLIGHT,
             "launch_shape": {"wg": tuned["wg"], "ipt": tuned["ipt"]},
             "visible_per_step": visible,
         },
@@ -602,10 +640,16 @@ def main() -> int:
             "unique_bytes_per_launch": unique * BYTES_PER_TEST,
         },
         "cpu_baseline": cpu,
+        "one_in_flight": {
+            "ms_per_step": ms_seq,
+            "value": tests_per_step * args.steps / dt_seq,
+        },
         "batched_vs_plain": {
             "batched_views_per_s": len(views) / (ms_batched * 1e-3),
+            "batched_one_in_flight_views_per_s": len(views) / (ms_seq * 1e-3),
             "plain_views_per_s": len(views) / (ms_plain * 1e-3),
             "speedup": ms_plain / ms_batched,
+            "speedup_one_in_flight": ms_plain / ms_seq,
             "batched_latency_ms_p50": pct(lat_b, 50),
             "batched_latency_ms_p99": pct(lat_b, 99),
             "plain_loop_latency_ms_p50": pct(lat_p, 50),

@@ -190,6 +190,9 @@ _SIGS: dict[str, tuple] = {
     "hv_scene_upload_soa": (i32, [vp, vp, vp, vp, u32, u32, vp]),
     "hv_scene_set_mesh_lods": (i32, [vp, vp, u32, u32]),
     "hv_scene_upload_device": (i32, [vp, vp, u32, u32, vp]),
+    "hv_stream_create": (i32, [vp, P(vp)]),
+    "hv_stream_destroy": (i32, [vp, vp]),
+    "hv_stream_synchronize": (i32, [vp]),
     "hv_scene_swap_remove": (i32, [vp, u32, vp]),
     "hv_scene_truncate": (i32, [vp, u32]),
     "hv_scene_read_soa": (i32, [vp, vp, vp, u32, u32]),

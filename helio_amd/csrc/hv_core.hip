@@ -141,6 +141,29 @@ hv_status hv_ctx_stream(hv_ctx* ctx, void** stream) {
     return HV_OK;
 }
 
+hv_status hv_stream_create(hv_ctx* ctx, void** stream) {
+    if (!ctx || !stream) return set_error(HV_EINVAL, "hv_stream_create: NULL argument");
+    HV_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = nullptr;
+    HV_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return HV_OK;
+}
+
+hv_status hv_stream_destroy(hv_ctx* ctx, void* stream) {
+    if (!ctx || !stream) return set_error(HV_EINVAL, "hv_stream_destroy: NULL argument");
+    HV_HIP(hipSetDevice(ctx->device));
+    HV_HIP(hipStreamSynchronize((hipStream_t)stream));
+    HV_HIP(hipStreamDestroy((hipStream_t)stream));
+    return HV_OK;
+}
+
+hv_status hv_stream_synchronize(void* stream) {
+    if (!stream) return set_error(HV_EINVAL, "hv_stream_synchronize: NULL stream");
+    HV_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return HV_OK;
+}
+
 hv_status hv_ctx_synchronize(hv_ctx* ctx) {
     if (!ctx) return set_error(HV_EINVAL, "hv_ctx_synchronize: NULL ctx");
     HV_HIP(hipStreamSynchronize(ctx->stream));

@@ -79,6 +79,30 @@ def instance_cull_data(instances: np.ndarray) -> np.ndarray:
     return out
 
 
+class Stream:
+    """A non-blocking HIP stream at the default priority (hv_stream_create)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = C.c_void_p()
+        call("hv_stream_create", ctx.handle, C.byref(h))
+        self.handle = h.value
+
+    def synchronize(self) -> None:
+        call("hv_stream_synchronize", self.handle)
+
+    def close(self) -> None:
+        if self.handle:
+            call("hv_stream_destroy", self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Scene:
     """Resident instance scene (GpuScene + GrowableBuffer counterpart)."""
 

@@ -224,6 +224,12 @@ hv_status hv_ctx_create(int32_t device, hv_ctx** out);
 void hv_ctx_destroy(hv_ctx* ctx);
 hv_status hv_ctx_stream(hv_ctx* ctx, void** stream);
 hv_status hv_ctx_synchronize(hv_ctx* ctx);
+/* Extra non-blocking streams at the default priority, for callers that keep
+ * several batches in flight (the encoder/queue split of the reference has no
+ * counterpart: wgpu submits in order). */
+hv_status hv_stream_create(hv_ctx* ctx, void** stream);
+hv_status hv_stream_destroy(hv_ctx* ctx, void* stream);
+hv_status hv_stream_synchronize(void* stream);
 /* Number of compute units and the L2/LLC sizes seen by the library. */
 hv_status hv_ctx_device_info(hv_ctx* ctx, uint32_t* cu_count, uint64_t* hbm_bytes);
 

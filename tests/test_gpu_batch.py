@@ -8,7 +8,7 @@ import pytest
 import oracle
 from helio_amd import layouts as L
 from helio_amd import synth
-from helio_amd.passes import BatchCull, InstanceCullPass, Scene, extract_frustum_planes
+from helio_amd.passes import BatchCull, InstanceCullPass, Scene, Stream, extract_frustum_planes
 from tests.conftest import args_as_u32, canonical_ids
 
 pytestmark = pytest.mark.gpu
@@ -48,6 +48,32 @@ def test_batch_graph_replay_matches_oracle(ctx):
     b.execute(graph=True)
     for i, (v, f, c) in enumerate(views):
         _check(b.read(i), _oracle(sc, v.camera, f, c))
+
+
+def test_two_batches_in_flight_on_two_streams(ctx):
+    """Two copies of a batch replaying their graphs alternately on two library
+    streams (the benchmark's steady state): after 5 steps each, every view of
+    both copies equals the oracle."""
+    sc = synth.instance_scene(262_144, config=3, index=2)
+    scene = _scene(ctx, sc)
+    views = synth.multiview_views(sc.n, max_count=100_000)
+    reqs = [(scene, v.camera, f, c) for v, f, c in views]
+    batches = [BatchCull(ctx, len(views)) for _ in range(2)]
+    streams = [Stream(ctx) for _ in range(2)]
+    for b in batches:
+        b.prepare(reqs)
+    for i in range(10):
+        batches[i % 2].execute(stream=streams[i % 2].handle, graph=True)
+    for b in batches:
+        b.wait()
+    for i, (v, f, c) in enumerate(views):
+        ref = _oracle(sc, v.camera, f, c)
+        for b in batches:
+            _check(b.read(i), ref)
+    for b in batches:
+        b.close()
+    for st in streams:
+        st.close()
 
 
 def test_batch_reprepare_after_replay(ctx):
