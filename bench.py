@@ -407,7 +407,8 @@ def main() -> int:
         args.no_serving = args.no_extra = args.no_cpu_baseline = True
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the visibility path is "
-              "single-device; see the sharding notes in DESIGN.md)", file=sys.stderr)
+              "single-device: more GPUs would each hold a replica of the scene and take their own views)",
+              file=sys.stderr)
         return 2
 
     from helio_amd import synth
