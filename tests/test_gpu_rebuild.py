@@ -144,3 +144,18 @@ def test_rebuilt_draw_list_feeds_indirect_dispatch(ctx):
     assert 0 < int(rst[1]) < g  # some groups culled, some kept
     p.close()
     rb.close()
+
+
+@pytest.mark.parametrize("mode", [L.REBUILD_PERSISTENT, L.REBUILD_OPTIMIZED], ids=["persistent", "optimized"])
+def test_rebuild_benchmark_size_matches_oracle(ctx, mode):
+    """The benchmarked rebuild workload itself: 1,048,576 objects over 4,096
+    meshes and 1,024 materials, every output against the oracle."""
+    s = synth.scene_objects(1 << 20, mesh_count=4096, material_count=1024, seed=1)
+    print(f"rebuild benchmark size mode={mode}", flush=True)
+    rb = SceneRebuild(ctx)
+    rb.bind(s.instances, s.aabbs, s.draws, s.objects, s.materials, group_hidden=6)
+    rb.execute(mode)
+    got = rb.read()
+    want = oracle.scene_rebuild(mode, s.instances, s.aabbs, s.draws, s.objects, s.materials, group_hidden=6)
+    _compare(got, want)
+    rb.close()
