@@ -159,3 +159,41 @@ def test_rebuild_benchmark_size_matches_oracle(ctx, mode):
     want = oracle.scene_rebuild(mode, s.instances, s.aabbs, s.draws, s.objects, s.materials, group_hidden=6)
     _compare(got, want)
     rb.close()
+
+
+def test_rebuild_argument_errors(ctx):
+    """Unknown modes, sizes past the limits, missing buffers and misaligned
+    pointers are refused before anything is launched."""
+    from helio_amd._lib import HvError, RebuildInputs, RebuildOutputs, call
+    import ctypes as C
+
+    s = synth.scene_objects(100, seed=3)
+    rb = SceneRebuild(ctx)
+    rb.bind(s.instances, s.aabbs, s.draws, s.objects, s.materials)
+    with pytest.raises(HvError):
+        rb.execute(7)
+    i = RebuildInputs()
+    i.instances, i.aabbs, i.draws, i.objects, i.materials = (b.ptr for b in rb.d_in)
+    i.object_count, i.material_count = len(s.instances), len(s.materials)
+    o = RebuildOutputs()
+    for name, b in rb.d_out.items():
+        setattr(o, name, b.ptr)
+    call("hv_scene_rebuild_execute", rb.handle, C.byref(i), C.byref(o), L.REBUILD_OPTIMIZED, None)  # valid
+    bad = RebuildInputs.from_buffer_copy(i)
+    bad.object_count = L.REBUILD_MAX_OBJECTS + 1
+    with pytest.raises(HvError):
+        call("hv_scene_rebuild_execute", rb.handle, C.byref(bad), C.byref(o), L.REBUILD_OPTIMIZED, None)
+    bad = RebuildInputs.from_buffer_copy(i)
+    bad.instances = i.instances + 4  # instances must be 16-byte aligned
+    with pytest.raises(HvError):
+        call("hv_scene_rebuild_execute", rb.handle, C.byref(bad), C.byref(o), L.REBUILD_OPTIMIZED, None)
+    bad_o = RebuildOutputs.from_buffer_copy(o)
+    bad_o.shadow_movable = None
+    with pytest.raises(HvError):
+        call("hv_scene_rebuild_execute", rb.handle, C.byref(i), C.byref(bad_o), L.REBUILD_OPTIMIZED, None)
+    bad = RebuildInputs.from_buffer_copy(i)
+    bad.materials = None  # material_count > 0 needs the table
+    with pytest.raises(HvError):
+        call("hv_scene_rebuild_execute", rb.handle, C.byref(bad), C.byref(o), L.REBUILD_OPTIMIZED, None)
+    ctx.synchronize()
+    rb.close()
