@@ -108,8 +108,20 @@ def cpu_baseline(sc, views, budget_s: float, threads: int) -> dict:
         "cores": threads,
         "kind": "port",
         "sample": f"{reps} pass(es) over the 64-view batch ({tests // reps} tests each) in {dt:.1f} s, "
-                  f"C++ oracle -O3 -ffp-contract=off, std::thread x {threads}, cpu={platform.processor() or 'x86_64'}",
+                  f"C++ oracle -O3 -ffp-contract=off, std::thread x {threads}",
+        "nproc": os.cpu_count(),
+        "cpu_model": cpu_model(),
     }
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 def serving_leg(ctx, rates, seconds: float, meshlet_fraction: float = 0.25) -> dict:
@@ -395,7 +407,7 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip the occlusion, VG, shadow and rebuild legs")
     ap.add_argument("--legs", default="occlusion,vg,shadow,rebuild",
                     help="comma list of the side legs to run (occlusion, vg, shadow, rebuild)")
-    ap.add_argument("--serving-rates", default="2000,10000,50000,200000",
+    ap.add_argument("--serving-rates", default="2000,5000,10000,20000,50000,200000",
                     help="offered Poisson rates (requests/s); the last one saturates both servers")
     ap.add_argument("--serving-seconds", type=float, default=0.5)
     ap.add_argument("--launch-shape", default=None, metavar="WGxIPT",
