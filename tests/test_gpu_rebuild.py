@@ -197,3 +197,37 @@ def test_rebuild_argument_errors(ctx):
         call("hv_scene_rebuild_execute", rb.handle, C.byref(bad), C.byref(o), L.REBUILD_OPTIMIZED, None)
     ctx.synchronize()
     rb.close()
+
+
+def test_rebuilt_shadow_list_feeds_shadow_cull(ctx):
+    """The rebuild's movable shadow list and sorted instances, on the device,
+    culled per face by ShadowCullPass: per-face counts and command multisets
+    equal the oracle's rebuild followed by the oracle's shadow cull."""
+    from helio_amd.passes import ShadowCullPass
+    from tests.test_gpu_shadow import _compare_faces, _lights, _oracle_full
+
+    lights, _ = _lights(3)
+    g = synth.rng(11, 3)
+    cam = synth.perspective_view(g).camera
+    mats, _, _ = oracle.shadow_matrices(lights, cam, 2048, np.zeros((L.SHADOW_MAX_FACES, 16), np.float32),
+                                        np.zeros(42, np.uint32), np.zeros(42, np.uint32))
+    s = synth.scene_objects(40_000, mesh_count=64, material_count=32, seed=9)
+    centers = lights["position_range"][:38, :3]
+    pick = np.random.default_rng(9).integers(0, len(centers), len(s.instances))
+    s.instances["bounds"][:, :3] = centers[pick] + np.random.default_rng(10).normal(0, 15, (len(pick), 3))
+    rb, got = _run(ctx, s, L.REBUILD_OPTIMIZED)
+    want = oracle.scene_rebuild(L.REBUILD_OPTIMIZED, s.instances, s.aabbs, s.draws, s.objects, s.materials,
+                                group_hidden=HIDDEN)
+    _compare(got, want)
+    n_mov = int(want["counts"][3])
+    dirty = np.ones(L.SHADOW_MAX_FACES, np.uint32)
+    d_m, d_f = DeviceBuffer.from_array(ctx, mats), DeviceBuffer.from_array(ctx, dirty)
+    p = ShadowCullPass(ctx, 8192, L.SHADOW_PLANES_ROWS)
+    p.execute(d_m, rb.d_out["instances"], len(s.instances), rb.d_out["shadow_movable"], n_mov, d_f, face_count=256)
+    counts, cmds = p.read()
+    rc, rd = _oracle_full(mats, np.ascontiguousarray(want["instances"]), np.ascontiguousarray(want["shadow_movable"]),
+                          dirty, L.SHADOW_PLANES_ROWS)
+    _compare_faces(counts, cmds, rc, rd, 8192)
+    assert counts.sum() > 1000
+    p.close()
+    rb.close()
