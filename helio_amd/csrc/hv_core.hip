@@ -387,7 +387,7 @@ hv_status hv_scene_upload(hv_scene* s, const hv_instance_data* instances, const 
         HV_TRY(s->staging.reserve((uint64_t)count * sizeof(hv_instance_data), false, st));
         HV_HIP(hipMemcpyAsync(s->staging.ptr, instances, (size_t)count * sizeof(hv_instance_data),
                               hipMemcpyHostToDevice, st));
-        k_scene_from_aos<<<(count + 255) / 256, 256, 0, st>>>(s->staging.as<hv_instance_data>(), count,
+        k_scene_from_aos<<<ceil_div(count, 256), 256, 0, st>>>(s->staging.as<hv_instance_data>(), count,
                                                              s->bounds.as<float4>() + first,
                                                              s->meta.as<uint32_t>() + first);
         HV_HIP(hipGetLastError());
@@ -460,7 +460,7 @@ hv_status hv_scene_upload_device(hv_scene* s, const hv_instance_data* instances,
     HV_TRY(s->staging.reserve(sizeof(uint32_t), false, st));
     uint32_t* d_max = s->staging.as<uint32_t>();
     HV_HIP(hipMemsetAsync(d_max, 0, 4, st));
-    k_scene_from_aos_max<<<(count + 255) / 256, 256, 0, st>>>(instances, count, s->bounds.as<float4>() + first,
+    k_scene_from_aos_max<<<ceil_div(count, 256), 256, 0, st>>>(instances, count, s->bounds.as<float4>() + first,
                                                              s->meta.as<uint32_t>() + first, d_max);
     HV_HIP(hipGetLastError());
     uint32_t max_id = 0;

@@ -50,6 +50,9 @@ namespace hv {
 hv_status set_error(hv_status code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 hv_status hip_error(hipError_t e, const char* what, const char* file, int line);
 
+// Workgroups for n items, computed in 64 bits (n near 2^32 must not wrap).
+inline uint32_t ceil_div(uint64_t n, uint32_t d) { return (uint32_t)((n + d - 1u) / d); }
+
 #define HV_HIP(call)                                                   \
     do {                                                               \
         hipError_t hv_e_ = (call);                                     \

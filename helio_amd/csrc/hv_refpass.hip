@@ -203,7 +203,7 @@ hv_status hv_indirect_dispatch_execute(hv_indirect_dispatch_pass* p, const hv_dr
     const uint32_t n = std::min(p->u.draw_count, b->draw_count);
     if (n == 0) return HV_OK;  // draw_count == 0: no dispatch (:194-197)
     HV_HIP(hipSetDevice(p->ctx->device));
-    hipLaunchKernelGGL(k_indirect_dispatch, dim3((n + 63) / 64), dim3(64), 0,
+    hipLaunchKernelGGL(k_indirect_dispatch, dim3(ceil_div(n, 64)), dim3(64), 0,
                        stream ? (hipStream_t)stream : p->ctx->stream, p->u, p->cam, *b);
     HV_HIP(hipGetLastError());
     return HV_OK;
@@ -258,7 +258,7 @@ hv_status hv_occlusion_cull_execute(hv_occlusion_cull_pass* p, const hv_draw_buf
     hz.h = hiz->height;
     hz.levels = hiz->levels;
     for (uint32_t k = 1; k < hiz->levels && k < 12; ++k) hz.off[k] = (uint32_t)hiz->level_off[k];
-    hipLaunchKernelGGL(k_occlusion_draws, dim3((n + 63) / 64), dim3(64), 0,
+    hipLaunchKernelGGL(k_occlusion_draws, dim3(ceil_div(n, 64)), dim3(64), 0,
                        stream ? (hipStream_t)stream : p->ctx->stream, p->prm, p->cam, *b, hz,
                        hiz->mode == HV_HIZ_MIN ? 1u : 0u);
     HV_HIP(hipGetLastError());

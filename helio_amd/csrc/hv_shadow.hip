@@ -490,7 +490,7 @@ hv_status hv_shadow_matrix_execute(hv_shadow_matrix_pass* p, const hv_shadow_mat
     MatArgs a{b->lights, b->shadow_matrices, b->camera, b->shadow_dirty, b->shadow_hashes,
               b->light_count, b->matrix_capacity, b->slot_capacity, p->atlas};
     const hipStream_t s = stream ? (hipStream_t)stream : p->ctx->stream;
-    hipLaunchKernelGGL(k_shadow_matrices, dim3((b->light_count + 63) / 64), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_shadow_matrices, dim3(ceil_div(b->light_count, 64)), dim3(64), 0, s, a);
     HV_HIP(hipGetLastError());
     return HV_OK;
 }
@@ -549,7 +549,7 @@ hv_status hv_shadow_cull_execute(hv_shadow_cull_pass* p, const hv_shadow_cull_bu
     const hipStream_t s = stream ? (hipStream_t)stream : p->ctx->stream;
     hipLaunchKernelGGL(k_shadow_planes, dim3(1), dim3(HV_SHADOW_MAX_FACES), 0, s, a);
     HV_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_shadow_cull, dim3((b->draw_count + 255) / 256, kFaceGroups), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_shadow_cull, dim3(ceil_div(b->draw_count, 256), kFaceGroups), dim3(256), 0, s, a);
     HV_HIP(hipGetLastError());
     HV_HIP(hipEventRecord(p->done, s));
     p->done_recorded = true;

@@ -530,7 +530,7 @@ hv_status vg_launch_multi(const VgArgs* d_reqs, const uint32_t* d_begin, uint32_
         return set_error(HV_ELIMIT, "vg_launch_multi: %llu workgroups exceed the grid limit",
                          (unsigned long long)total_blocks);
     const uint32_t words = n * HV_VG_COUNTER_COUNT;
-    hipLaunchKernelGGL(k_vg_clear_many, dim3((words + 255) / 256), dim3(256), 0, s, counters, words);
+    hipLaunchKernelGGL(k_vg_clear_many, dim3(ceil_div(words, 256)), dim3(256), 0, s, counters, words);
     HV_HIP(hipGetLastError());
     if (total_blocks) {
         hipLaunchKernelGGL(k_vg_multi, dim3((uint32_t)total_blocks), dim3(kVgThreads), 0, s, d_reqs, d_begin, n);
