@@ -410,6 +410,12 @@ namespace {
 
 bool misaligned(const void* p, uintptr_t a) { return p && ((uintptr_t)p & (a - 1)) != 0; }
 
+// [a, a + na) and [b, b + nb) share a byte.
+bool overlap(const void* a, uint64_t na, const void* b, uint64_t nb) {
+    const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+    return a && b && na && nb && x < y + nb && y < x + na;
+}
+
 }  // namespace
 
 extern "C" {
@@ -466,6 +472,21 @@ hv_status hv_scene_rebuild_execute(hv_scene_rebuild* rb, const hv_rebuild_inputs
         misaligned(out->visibility, 4) || misaligned(out->gpu_slots, 4) || misaligned(out->counts, 4))
         return set_error(HV_EINVAL, "hv_scene_rebuild_execute: misaligned buffer (instances/aabbs 16 B, "
                                     "objects/materials/ranges 8 B, the rest 4 B)");
+    {
+        // The outputs are written while the inputs are read at sorted (random) positions.
+        const void* ins[] = {in->instances, in->aabbs, in->draws, in->objects, in->materials};
+        const uint64_t in_bytes[] = {(uint64_t)n * 144, (uint64_t)n * 32, (uint64_t)n * 20, (uint64_t)n * 16,
+                                     (uint64_t)m_count * 16};
+        const void* outs[] = {out->instances, out->aabbs, out->draw_calls, out->indirect, out->visibility,
+                              out->gpu_slots, out->ranges, out->shadow_static, out->shadow_movable, out->counts};
+        const uint64_t out_bytes[] = {(uint64_t)n * 144, (uint64_t)n * 32, (uint64_t)n * 20, (uint64_t)n * 20,
+                                      (uint64_t)n * 4,   (uint64_t)n * 4,  (uint64_t)n * 24, (uint64_t)n * 20,
+                                      (uint64_t)n * 20,  16};
+        for (size_t i = 0; i < sizeof(ins) / sizeof(ins[0]); ++i)
+            for (size_t o = 0; o < sizeof(outs) / sizeof(outs[0]); ++o)
+                if (overlap(ins[i], in_bytes[i], outs[o], out_bytes[o]))
+                    return set_error(HV_EINVAL, "hv_scene_rebuild_execute: output %zu overlaps input %zu", o, i);
+    }
     HV_HIP(hipSetDevice(rb->ctx->device));
     const hipStream_t st = stream ? (hipStream_t)stream : rb->ctx->stream;
     if (!n) {

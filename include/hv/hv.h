@@ -389,7 +389,8 @@ void hv_scene_rebuild_destroy(hv_scene_rebuild* rb);
  * objects counts[0..1] become 0 and counts[2..3] are left as they are (the
  * reference returns before rebuilding the shadow partition). Synchronises
  * the stream once (key widths come from the material ranks and the largest
- * mesh and material ids); the rest is enqueued. */
+ * mesh and material ids); the rest is enqueued. No output may overlap an
+ * input (HV_EINVAL). */
 hv_status hv_scene_rebuild_execute(hv_scene_rebuild* rb, const hv_rebuild_inputs* in, const hv_rebuild_outputs* out,
                                    uint32_t mode, void* stream);
 

@@ -191,6 +191,14 @@ def test_rebuild_argument_errors(ctx):
     bad_o.shadow_movable = None
     with pytest.raises(HvError):
         call("hv_scene_rebuild_execute", rb.handle, C.byref(i), C.byref(bad_o), L.REBUILD_OPTIMIZED, None)
+    bad_o = RebuildOutputs.from_buffer_copy(o)
+    bad_o.instances = i.instances  # in place: the gather would read what it overwrites
+    with pytest.raises(HvError):
+        call("hv_scene_rebuild_execute", rb.handle, C.byref(i), C.byref(bad_o), L.REBUILD_OPTIMIZED, None)
+    bad_o = RebuildOutputs.from_buffer_copy(o)
+    bad_o.shadow_static = i.draws + 16 * 20  # inside the draw templates
+    with pytest.raises(HvError):
+        call("hv_scene_rebuild_execute", rb.handle, C.byref(i), C.byref(bad_o), L.REBUILD_OPTIMIZED, None)
     bad = RebuildInputs.from_buffer_copy(i)
     bad.materials = None  # material_count > 0 needs the table
     with pytest.raises(HvError):
