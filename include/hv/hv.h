@@ -645,7 +645,9 @@ hv_status hv_occlusion_cull_execute(hv_occlusion_cull_pass* pass, const hv_draw_
 /* and the shadow faces) this copies the non-empty draws, in order, to `out` */
 /* and writes their number to count[0] (count at byte 0, as the VG draw      */
 /* counter buffer keeps it, rendering.rs:648-661). Device pointers; out has  */
-/* room for n records; n <= 2^20. Enqueues on `stream` only.                 */
+/* room for n records; n <= 2^20; out == args compacts in place (each draw   */
+/* moves only towards the front, after its chunk was read). Enqueues on      */
+/* `stream` only.                                                            */
 /* ------------------------------------------------------------------------- */
 hv_status hv_compact_draws(hv_ctx* ctx, const hv_draw_indexed_indirect* args, uint32_t n,
                            hv_draw_indexed_indirect* out, uint32_t* count, void* stream);

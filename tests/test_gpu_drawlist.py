@@ -44,6 +44,20 @@ def test_compact_draws(ctx, n, p_empty):
     assert np.array_equal(out.read(np.uint32, 5 * k).reshape(-1, 5), want)
 
 
+@pytest.mark.parametrize("n", [1, 1024, 1025, 5000])
+def test_compact_draws_in_place(ctx, n):
+    g = np.random.default_rng(40 + n)
+    a5 = g.integers(1, 2**31, (n, 5), dtype=np.int64).astype(np.uint32)
+    a5[g.random(n) < 0.5, 1] = 0
+    buf = DeviceBuffer.from_array(ctx, a5)
+    cnt = DeviceBuffer(ctx, 4)
+    compact_draws(ctx, buf.ptr, n, buf, cnt)
+    ctx.synchronize()
+    want, k = _expected(a5)
+    assert int(cnt.read(np.uint32, 1)[0]) == k
+    assert np.array_equal(buf.read(np.uint32, 5 * k).reshape(-1, 5), want)
+
+
 def test_compact_bucket_args_of_the_instance_pass(ctx):
     sc = synth.instance_scene(65536, config=0)
     cam = synth.perspective_view(synth.rng(0, 1)).camera
