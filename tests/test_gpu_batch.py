@@ -76,6 +76,29 @@ def test_two_batches_in_flight_on_two_streams(ctx):
         st.close()
 
 
+def test_batch_over_explicit_aabbs(ctx):
+    """A batch over a scene that holds explicit AABBs (the <AABB> classify
+    kernel in the batched launch), every view against the oracle."""
+    sc = synth.instance_scene(150_000, config=3, index=4)
+    g = synth.rng(3, 44)
+    aabbs = np.zeros(sc.n, L.AABB)
+    ext = (sc.bounds[:, 3:4] * g.uniform(0.3, 2.0, (sc.n, 3))).astype(np.float32)
+    aabbs["min"] = sc.bounds[:, :3] - ext
+    aabbs["max"] = sc.bounds[:, :3] + ext
+    scene = Scene(ctx, sc.n)
+    scene.upload(sc.instances(), aabbs)
+    scene.set_mesh_lods(sc.mesh_lods, sc.mesh_count, sc.lod_count)
+    views = synth.multiview_views(sc.n, max_count=60_000)[:24]
+    b = BatchCull(ctx, len(views))
+    b.prepare([(scene, v.camera, f, c) for v, f, c in views])
+    b.execute(graph=True)
+    aabb6 = np.concatenate([aabbs["min"], aabbs["max"]], axis=1).astype(np.float32)
+    for i, (v, f, c) in enumerate(views):
+        ov = oracle.view_from_camera(v.camera, mesh_lods=sc.mesh_lods, mesh_count=sc.mesh_count)
+        _check(b.read(i), oracle.cull_view(ov, sc.bounds, sc.meta(), f, c, None, aabb6, threads=8))
+    b.close()
+
+
 def test_batch_reprepare_after_replay(ctx):
     """prepare() right after a replay waits for it: a second table of other
     ranges gives the oracle's results."""

@@ -145,6 +145,27 @@ def test_occlusion_parity(ctx, rev):
     assert ref[2][4] > 0, "no instance was occluded: the test would not pin the Hi-Z path"
 
 
+def test_explicit_aabbs_with_occlusion(ctx):
+    """Explicit AABBs and the Hi-Z test together (the <OCCL, AABB> kernel):
+    the frustum decision from the boxes, the occlusion test from the spheres."""
+    view, depth, _ = synth.occluder_view_and_depth(reversed_z=True, boxes=600)
+    sc = synth.instance_scene(200_000, config=1, index=3)
+    g = synth.rng(1, 33)
+    aabbs = np.zeros(sc.n, L.AABB)
+    ext = (sc.bounds[:, 3:4] * g.uniform(0.5, 1.5, (sc.n, 3))).astype(np.float32)
+    aabbs["min"] = sc.bounds[:, :3] - ext
+    aabbs["max"] = sc.bounds[:, :3] + ext
+    hiz = HiZBuildPass(ctx, 1920, 1080, L.HIZ_MIN)
+    hiz.upload_depth(depth)
+    hiz.execute()
+    gpu = _run(ctx, sc, view.camera, soa=False, aabbs=aabbs, hiz=hiz, occl=True, rev=True)
+    aabb6 = np.concatenate([aabbs["min"], aabbs["max"]], axis=1).astype(np.float32)
+    ref = _oracle(sc, view.camera, pyr=oracle.pyramid(depth, 1), flags=L.CULL_OCCLUSION | L.CULL_REVERSED_Z,
+                  aabb6=aabb6)
+    _check(gpu, ref)
+    assert ref[2][4] > 0 and ref[2][3] > 0
+
+
 def test_occlusion_parity_config1_full(ctx):
     """Config 1 at its full size: 1,048,576 spheres against the 11-level min
     pyramid of 2,000 rasterised occluder boxes (reversed Z)."""
