@@ -207,13 +207,16 @@ constexpr float kTinyLo = 0.001f * (1.0f - 0x1p-20f);
 constexpr float kTinyHi = 0.001f * (1.0f + 0x1p-20f);
 
 // One candidate. Returns its bucket (mesh_id * lod_count + lod) when it is
-// drawn, kInvalid otherwise. Lanes with !valid report all-false.
+// drawn, kInvalid otherwise. Lanes with !valid report all-false. `boxes`
+// (uniform) says the view's scene holds explicit AABBs: a batch that mixes
+// such scenes with sphere-only ones runs the AABB kernel for every view, and
+// the sphere-only views take the sphere path (their `ab` is null).
 template <bool OCCL, bool AABB>
 __device__ __forceinline__ uint32_t classify(const CullView* __restrict__ V, const CullView* __restrict__ Vk,
                                              const float* pl, const float* l1s, const RowLds& rc, bool valid, float4 b,
-                                             uint32_t m, const hv_instance_aabb* ab, Outcome& o) {
+                                             uint32_t m, bool boxes, const hv_instance_aabb* ab, Outcome& o) {
     const float cx = b.x, cy = b.y, cz = b.z, r = b.w;
-    if (AABB) {
+    if (AABB && boxes) {
         o.visible = valid && frustum_visible<true>(pl, nullptr, cx, cy, cz, r, ab);
     } else {
         const float cmax = fmaxf(fmaxf(fabsf(cx), fabsf(cy)), fabsf(cz));
@@ -356,13 +359,14 @@ __device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, cons
 #pragma unroll
     for (int i = 0; i < 6; ++i) l1[i] = ldc(&Vk->plane_l1[i]);
     uint32_t pos = 0;
+    const bool boxes = AABB && c.aabb != nullptr;  // uniform: the view's scene holds explicit AABBs
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
         const uint32_t j = k * WG + tid;
         const bool valid = j < n;
         Outcome o;
-        const uint32_t code = classify<OCCL, AABB>(c.V, Vk, pl, l1, *c.rc, valid, bb[k], mm[k],
-                                                   AABB ? c.aabb + first + (valid ? j : 0u) : nullptr, o);
+        const uint32_t code = classify<OCCL, AABB>(c.V, Vk, pl, l1, *c.rc, valid, bb[k], mm[k], boxes,
+                                                   boxes ? c.aabb + first + (valid ? j : 0u) : nullptr, o);
         const uint64_t keep = __ballot(code != kInvalid);
         if (code != kInvalid) {
             out[pos + mbcnt(keep)] = (code << kEntryShift) | j;

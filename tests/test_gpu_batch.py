@@ -99,6 +99,45 @@ def test_batch_over_explicit_aabbs(ctx):
     b.close()
 
 
+def test_batch_mixed_scenes(ctx):
+    """One batch over three scenes with different sizes and mesh tables (256,
+    64 and 1,024 meshes x 4 LODs) and one with explicit AABBs, perspective and
+    ortho views on random ranges: every view against the oracle."""
+    g = synth.rng(3, 55)
+    cases = []
+    for i, (n, meshes, explicit) in enumerate([(120_000, 256, False), (60_000, 64, True), (30_000, 1024, False)]):
+        sc = synth.instance_scene(n, config=3, index=20 + i, mesh_count=meshes)
+        scene = Scene(ctx, sc.n)
+        aabb6 = None
+        if explicit:
+            aabbs = np.zeros(sc.n, L.AABB)
+            ext = (sc.bounds[:, 3:4] * g.uniform(0.3, 2.0, (sc.n, 3))).astype(np.float32)
+            aabbs["min"] = sc.bounds[:, :3] - ext
+            aabbs["max"] = sc.bounds[:, :3] + ext
+            scene.upload(sc.instances(), aabbs)
+            aabb6 = np.concatenate([aabbs["min"], aabbs["max"]], axis=1).astype(np.float32)
+        else:
+            scene.upload_soa(sc.bounds, sc.mesh_ids, sc.flags)
+        scene.set_mesh_lods(sc.mesh_lods, sc.mesh_count, sc.lod_count)
+        cases.append((sc, scene, aabb6))
+    views = []
+    for k in range(30):
+        sc, scene, aabb6 = cases[k % 3]
+        v = (synth.ortho_view(g, g.uniform(0, 1000, 3), float(g.uniform(50, 400)), 2000.0, g.normal(size=3))
+             if k % 4 == 0 else synth.perspective_view(g))
+        first = int(g.integers(0, sc.n - 1))
+        count = int(min(sc.n - first, g.integers(1, sc.n)))
+        views.append((k % 3, v, first, count))
+    b = BatchCull(ctx, len(views))
+    b.prepare([(cases[c][1], v.camera, f, n) for c, v, f, n in views])
+    b.execute(graph=True)
+    for i, (c, v, f, n) in enumerate(views):
+        sc, _, aabb6 = cases[c]
+        ov = oracle.view_from_camera(v.camera, mesh_lods=sc.mesh_lods, mesh_count=sc.mesh_count)
+        _check(b.read(i), oracle.cull_view(ov, sc.bounds, sc.meta(), f, n, None, aabb6, threads=8))
+    b.close()
+
+
 def test_batch_reprepare_after_replay(ctx):
     """prepare() right after a replay waits for it: a second table of other
     ranges gives the oracle's results."""

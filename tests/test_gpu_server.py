@@ -72,6 +72,31 @@ def test_alone_equals_in_mix_equals_plain(ctx):
     srv.close()
 
 
+def test_scenes_with_and_without_explicit_aabbs_in_one_batch(ctx):
+    """A batch holding views of a scene with explicit AABBs and of sphere-only
+    scenes: each result equals the plain pass on its own scene."""
+    from helio_amd import layouts as L
+
+    scenes = _scenes(ctx)[:2]
+    sc = synth.instance_scene(50_000, config=4, index=9, cube=400.0)
+    g = synth.rng(4, 99)
+    aabbs = np.zeros(sc.n, L.AABB)
+    ext = (sc.bounds[:, 3:4] * g.uniform(0.3, 2.0, (sc.n, 3))).astype(np.float32)
+    aabbs["min"] = sc.bounds[:, :3] - ext
+    aabbs["max"] = sc.bounds[:, :3] + ext
+    boxed = Scene(ctx, sc.n)
+    boxed.upload(sc.instances(), aabbs)
+    boxed.set_mesh_lods(sc.mesh_lods, sc.mesh_count, sc.lod_count)
+    reqs = _requests(scenes + [boxed], 24, seed=7)
+    assert any(r[0] is boxed for r in reqs) and any(r[0] is not boxed for r in reqs)
+    srv = CullServer(ctx, max_batch_views=24, max_inflight=1, max_batch_candidates=4_000_000)
+    tickets = [srv.submit(*r) for r in reqs]
+    mixed = [srv.wait(t, 60_000) for t in tickets]
+    for k in range(len(reqs)):
+        _same(mixed[k], _plain(ctx, reqs[k]))
+    srv.close()
+
+
 def test_budget_admission_and_rejection(ctx):
     scenes = _scenes(ctx)
     reqs = _requests(scenes, 12, seed=1)
