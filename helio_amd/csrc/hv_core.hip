@@ -312,6 +312,7 @@ extern "C++" hv_status scene_grow(hv_scene* s, uint32_t need, hipStream_t stream
     HV_TRY(s->meta.reserve(cap * 4, true, stream));
     if (s->explicit_aabb) HV_TRY(s->aabb.reserve(cap * 32, true, stream));
     s->capacity = (uint32_t)cap;
+    ++s->storage_gen;
     return HV_OK;
 }
 
@@ -382,6 +383,7 @@ hv_status hv_scene_upload(hv_scene* s, const hv_instance_data* instances, const 
         HV_TRY(s->aabb.reserve((uint64_t)s->capacity * 32, false, st));
         HV_HIP(hipMemcpy(s->aabb.ptr, host.data(), (size_t)s->capacity * 32, hipMemcpyHostToDevice));
         s->explicit_aabb = true;
+        ++s->storage_gen;
     }
     if (count) {
         HV_TRY(s->staging.reserve((uint64_t)count * sizeof(hv_instance_data), false, st));
@@ -520,6 +522,7 @@ hv_status hv_scene_set_mesh_lods(hv_scene* s, const hv_mesh_lod* table, uint32_t
     HV_HIP(hipSetDevice(s->ctx->device));
     HV_TRY(s->mesh_lods.reserve(n * sizeof(hv_mesh_lod), false, s->ctx->stream));
     HV_HIP(hipMemcpy(s->mesh_lods.ptr, table, n * sizeof(hv_mesh_lod), hipMemcpyHostToDevice));
+    ++s->storage_gen;
     s->mesh_count = mesh_count;
     s->lod_count = lod_count;
     return HV_OK;

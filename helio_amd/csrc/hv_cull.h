@@ -131,6 +131,8 @@ class CullBatch {
     std::vector<CullView> host_views_;
     std::vector<uint32_t> host_units_, host_block_view_;
     std::vector<ViewDesc> descs_;
+    std::vector<uint64_t> scene_gens_;  // each view's scene storage_gen at prepare
+    hv_status check_scenes() const;
     DevBuf d_table_, d_stage_, d_ids_, d_args_, d_counters_, d_sub_cnt_, d_row_stats_, d_block_hist_, d_totals_;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;

@@ -1087,6 +1087,8 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     n_units_ = (uint32_t)host_units_.size();
     grid_ = G;
     descs_.assign(views, views + n);
+    scene_gens_.resize(n);
+    for (uint32_t v = 0; v < n; ++v) scene_gens_[v] = views[v].scene->storage_gen;
     max_nb_ = max_nb;
     // Upload the table (ordered before any later launch on the context stream).
     std::vector<uint8_t> buf(table_bytes, 0);
@@ -1147,8 +1149,17 @@ hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed) {
     return HV_OK;
 }
 
+hv_status CullBatch::check_scenes() const {
+    for (size_t v = 0; v < descs_.size(); ++v)
+        if (descs_[v].scene->storage_gen != scene_gens_[v])
+            return set_error(HV_ESTATE, "execute: the scene of view %zu moved or changed shape since prepare (grown, "
+                                        "AABBs switched, new LOD table): prepare again", v);
+    return HV_OK;
+}
+
 hv_status CullBatch::execute(hipStream_t s, float* ms) {
     if (!n_views_) return set_error(HV_ESTATE, "execute: nothing prepared");
+    HV_TRY(check_scenes());
     HV_HIP(hipSetDevice(ctx_->device));
     if (ms && !ev_[0])
         for (auto& e : ev_) HV_HIP(hipEventCreate(&e));
@@ -1157,6 +1168,7 @@ hv_status CullBatch::execute(hipStream_t s, float* ms) {
 
 hv_status CullBatch::execute_graph(hipStream_t s) {
     if (!n_views_) return set_error(HV_ESTATE, "execute_graph: nothing prepared");
+    HV_TRY(check_scenes());
     HV_HIP(hipSetDevice(ctx_->device));
     // The classify variant (occlusion, explicit AABBs) and the LDS size are
     // baked into the graph; grids are the capacity grids.

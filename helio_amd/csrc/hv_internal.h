@@ -115,6 +115,10 @@ struct hv_scene {
     uint32_t mesh_count = 0;
     uint32_t lod_count = 0;
     uint32_t max_mesh_id = 0;    // largest mesh_id ever uploaded (host-side check)
+    // Bumped whenever the device arrays move or change shape (growth, the switch
+    // to explicit AABBs and back, a new LOD table): passes and batches prepared
+    // before it refuse to execute on stale pointers.
+    uint64_t storage_gen = 0;
 };
 
 // Grows the resident arrays to hold `need` instances (x2 steps, contents kept).

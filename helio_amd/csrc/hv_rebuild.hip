@@ -662,6 +662,7 @@ hv_status hv_scene_rebuild_execute(hv_scene_rebuild* rb, const hv_rebuild_inputs
     if (out->scene) {
         out->scene->count = n;
         out->scene->max_mesh_id = max_mesh;
+        if (out->scene->explicit_aabb) ++out->scene->storage_gen;
         out->scene->explicit_aabb = false;
     }
     return HV_OK;

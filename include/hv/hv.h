@@ -274,6 +274,12 @@ hv_status hv_instance_cull_data_from_instances(const hv_instance_data* instances
 /* Instances are converted AoS -> SoA on the device at upload: bounds as     */
 /* float4 (16 B) and a 4-byte meta word (mesh_id | casts_shadow << 31), so   */
 /* the cull reads 20 B per instance instead of 144 + 32 B.                   */
+/* A pass or batch prepared on a scene keeps its device pointers: an upload  */
+/* that grows the scene, switches it to explicit AABBs, a new LOD table or a */
+/* rebuild into it makes the next execute return HV_ESTATE until prepared    */
+/* again (the reference rebuilds its bind groups when a buffer changes,      */
+/* crates/helio-pass-indirect-dispatch/src/lib.rs:200-244). Uploads within   */
+/* the capacity need no new prepare.                                          */
 /* ------------------------------------------------------------------------- */
 typedef struct hv_scene hv_scene;
 

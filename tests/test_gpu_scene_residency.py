@@ -163,3 +163,40 @@ def test_residency_errors(ctx):
     with pytest.raises(HvError):  # explicit AABBs are taken from host arrays only
         ex.upload_device(DeviceBuffer.from_array(ctx, inst[:10]), 10)
     ex.close()
+
+
+def test_prepared_pass_refuses_a_grown_scene(ctx):
+    """A pass prepared before the scene grew (its arrays moved) refuses to
+    execute; prepared again, it gives the oracle's result on the grown scene."""
+    g = np.random.default_rng(21)
+    scene = Scene(ctx, 1024)
+    model = Model()
+    a = _instances(g, 1000)
+    scene.upload(a)
+    model.write(0, a)
+    sc = synth.instance_scene(16, config=0)
+    scene.set_mesh_lods(sc.mesh_lods, sc.mesh_count, sc.lod_count)
+    cam = synth.perspective_view(synth.rng(0, 4), position=(150, 150, -30), direction=(0, 0, 1)).camera
+    p = InstanceCullPass(ctx)
+    p.prepare(scene, cam)
+    p.execute()
+    b = _instances(g, 5000)
+    scene.upload(b, first=1000)  # past the capacity: the arrays move
+    model.write(1000, b)
+    with pytest.raises(HvError):
+        p.execute()
+    p.prepare(scene, cam)
+    p.execute()
+    ids, args, stats = p.read()
+    ov = oracle.view_from_camera(cam, lod_distances=(25.0, 75.0, 200.0), mesh_lods=sc.mesh_lods,
+                                 mesh_count=sc.mesh_count)
+    rids, rargs, rstats = oracle.cull_view(ov, model.bounds, model.meta, 0, len(model.meta), None, None, threads=4)
+    assert np.array_equal(stats, rstats)
+    assert np.array_equal(args_as_u32(args), rargs)
+    assert np.array_equal(canonical_ids(ids, args), rids)
+    c = _instances(g, 100)
+    scene.upload(c, first=17)  # within the capacity: the prepared pass stays valid
+    p.execute()
+    p.close()
+    scene.close()
+
