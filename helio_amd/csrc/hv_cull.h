@@ -132,6 +132,7 @@ class CullBatch {
     std::vector<uint32_t> host_units_, host_block_view_;
     std::vector<ViewDesc> descs_;
     std::vector<uint64_t> scene_gens_;  // each view's scene storage_gen at prepare
+    std::vector<uint64_t> hiz_gens_;    // each view's pyramid storage_gen at prepare (0 without one)
     hv_status check_scenes() const;
     DevBuf d_table_, d_stage_, d_ids_, d_args_, d_counters_, d_sub_cnt_, d_row_stats_, d_block_hist_, d_totals_;
     hipGraph_t graph_ = nullptr;

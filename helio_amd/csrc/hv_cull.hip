@@ -1088,7 +1088,11 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     grid_ = G;
     descs_.assign(views, views + n);
     scene_gens_.resize(n);
-    for (uint32_t v = 0; v < n; ++v) scene_gens_[v] = views[v].scene->storage_gen;
+    hiz_gens_.resize(n);
+    for (uint32_t v = 0; v < n; ++v) {
+        scene_gens_[v] = views[v].scene->storage_gen;
+        hiz_gens_[v] = views[v].hiz ? views[v].hiz->storage_gen : 0u;
+    }
     max_nb_ = max_nb;
     // Upload the table (ordered before any later launch on the context stream).
     std::vector<uint8_t> buf(table_bytes, 0);
@@ -1154,6 +1158,11 @@ hv_status CullBatch::check_scenes() const {
         if (descs_[v].scene->storage_gen != scene_gens_[v])
             return set_error(HV_ESTATE, "execute: the scene of view %zu moved or changed shape since prepare (grown, "
                                         "AABBs switched, new LOD table): prepare again", v);
+    if (cfg_.flags & HV_CULL_OCCLUSION)
+        for (size_t v = 0; v < descs_.size(); ++v)
+            if (descs_[v].hiz && descs_[v].hiz->storage_gen != hiz_gens_[v])
+                return set_error(HV_ESTATE, "execute: the pyramid of view %zu was bound to other depth since prepare: "
+                                            "prepare again", v);
     return HV_OK;
 }
 

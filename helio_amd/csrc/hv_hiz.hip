@@ -240,12 +240,14 @@ void hv_hiz_destroy(hv_hiz* z) {
 
 hv_status hv_hiz_bind_depth(hv_hiz* z, const float* depth_device) {
     if (!z) return set_error(HV_EINVAL, "hv_hiz_bind_depth: NULL pyramid");
+    const float* before = z->level0;
     if (depth_device) {
         z->level0 = depth_device;
     } else {
         HV_TRY(z->own_level0.reserve((uint64_t)z->width * z->height * 4, false, z->ctx->stream));
         z->level0 = z->own_level0.as<float>();
     }
+    if (z->level0 != before) ++z->storage_gen;
     z->last_generation = ~0ull;
     return HV_OK;
 }
@@ -259,6 +261,7 @@ hv_status hv_hiz_upload_depth(hv_hiz* z, const float* depth_host, void* stream) 
     HV_HIP(hipStreamSynchronize(s));
     // New depth content alone does not force a rebuild: like the reference,
     // the camera generation decides (crates/helio-pass-hiz/src/lib.rs:472-478).
+    if (z->level0 != z->own_level0.as<float>()) ++z->storage_gen;
     z->level0 = z->own_level0.as<float>();
     return HV_OK;
 }

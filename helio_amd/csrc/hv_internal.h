@@ -133,4 +133,5 @@ struct hv_hiz {
     uint64_t level_off[16] = {};     // float offsets of level m >= 1 inside `packed`
     uint64_t last_generation = ~0ull;
     bool skip_next = false;
+    uint64_t storage_gen = 0;        // bumped when level0 points elsewhere (prepared passes check it)
 };

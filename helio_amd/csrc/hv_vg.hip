@@ -468,6 +468,8 @@ struct hv_vg_pass {
     VgArgs args{};
     bool clear_indirect = false;
     bool prepared = false;
+    const hv_hiz* hiz = nullptr;   // the pyramid of the last prepare and its storage_gen then
+    uint64_t hiz_gen = 0;
     hipEvent_t ev[3] = {};
     hipEvent_t done = nullptr;
     bool done_recorded = false;
@@ -682,6 +684,8 @@ hv_status hv_vg_prepare(hv_vg_pass* p, const hv_vg_frame_data* vg, const hv_came
     HV_HIP(hipStreamSynchronize(st));
     a.view = p->view.as<VgView>();
     p->clear_indirect = (flags & HV_VG_CLEAR_INDIRECT) != 0;
+    p->hiz = hiz;
+    p->hiz_gen = hiz ? hiz->storage_gen : 0u;
     p->prepared = true;
     return HV_OK;
 }
@@ -727,14 +731,22 @@ static hv_status vg_launch(hv_vg_pass* p, hipStream_t s, float* ms) {
     return HV_OK;
 }
 
+static hv_status vg_check_hiz(const hv_vg_pass* p) {
+    if (p->hiz && p->hiz->storage_gen != p->hiz_gen)
+        return set_error(HV_ESTATE, "hv_vg_execute: the pyramid was bound to other depth since prepare: prepare again");
+    return HV_OK;
+}
+
 hv_status hv_vg_execute(hv_vg_pass* p, void* stream) {
     if (!p || !p->prepared) return set_error(HV_ESTATE, "hv_vg_execute: not prepared");
+    HV_TRY(vg_check_hiz(p));
     HV_HIP(hipSetDevice(p->ctx->device));
     return vg_launch(p, stream ? (hipStream_t)stream : p->ctx->stream, nullptr);
 }
 
 hv_status hv_vg_execute_timed(hv_vg_pass* p, void* stream, float ms[2]) {
     if (!p || !p->prepared || !ms) return set_error(HV_ESTATE, "hv_vg_execute_timed: not prepared");
+    HV_TRY(vg_check_hiz(p));
     HV_HIP(hipSetDevice(p->ctx->device));
     return vg_launch(p, stream ? (hipStream_t)stream : p->ctx->stream, ms);
 }

@@ -279,7 +279,8 @@ hv_status hv_instance_cull_data_from_instances(const hv_instance_data* instances
 /* rebuild into it makes the next execute return HV_ESTATE until prepared    */
 /* again (the reference rebuilds its bind groups when a buffer changes,      */
 /* crates/helio-pass-indirect-dispatch/src/lib.rs:200-244). Uploads within   */
-/* the capacity need no new prepare.                                          */
+/* the capacity need no new prepare. Likewise a pyramid rebound to other    */
+/* depth (hv_hiz_bind_depth) after an occlusion or VG prepare.                */
 /* ------------------------------------------------------------------------- */
 typedef struct hv_scene hv_scene;
 
