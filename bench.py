@@ -305,7 +305,7 @@ def occlusion_leg(ctx, steps: int) -> dict:
     min pyramid, 1,048,576 spheres frustum + sub-pixel + Hi-Z tested."""
     from helio_amd import layouts as L
     from helio_amd import synth
-    from helio_amd.passes import HiZBuildPass, InstanceCullPass, Scene
+    from helio_amd.passes import BatchCull, HiZBuildPass, InstanceCullPass, Scene
 
     view, depth, _ = synth.occluder_view_and_depth(config=1, threads=min(16, os.cpu_count() or 1))
     sc = synth.instance_scene(1_048_576, config=1)
@@ -330,6 +330,22 @@ def occlusion_leg(ctx, steps: int) -> dict:
     p.prepare(scene, view.camera, 0, sc.n, hiz)
     k = np.array([p.execute_timed() for _ in range(steps)])
     ids, args, stats = p.read()
+    # A whole frame as a renderer runs it: the pyramid, then the cull as a
+    # one-view batch replayed from its graph, on one stream, frame after frame.
+    b = BatchCull(ctx, 1, occlusion=True, reversed_z=True)
+    b.prepare([(scene, view.camera, 0, sc.n, hiz)])
+    for _ in range(3):
+        hiz.execute()
+        b.execute(graph=True)
+    b.wait()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hiz.execute()
+        b.execute(graph=True)
+    b.wait()
+    frame_ms = (time.perf_counter() - t0) / steps * 1e3
+    assert np.array_equal(b.read(0)[2], stats)
+    b.close()
     hz = float(np.median(hz_ms))
     cls = float(np.median(k[:, 0]))
     tot = float(np.median(k.sum(1)))
@@ -339,6 +355,8 @@ def occlusion_leg(ctx, steps: int) -> dict:
         "cull_kernels_ms": {"classify": cls, "colscan": float(np.median(k[:, 1])),
                             "scatter": float(np.median(k[:, 2]))},
         "spheres_per_s": sc.n / (tot * 1e-3),
+        "frame_ms": frame_ms,
+        "frame_spheres_per_s": sc.n / (frame_ms * 1e-3),
         "classify_gbs": sc.n * BYTES_PER_TEST / (cls * 1e-3) / 1e9,
         "visible": int(len(ids)),
         "stats": [int(x) for x in stats],
