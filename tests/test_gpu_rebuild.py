@@ -239,3 +239,22 @@ def test_rebuilt_shadow_list_feeds_shadow_cull(ctx):
     assert counts.sum() > 1000
     p.close()
     rb.close()
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_rebuild_random_small_sets(ctx, seed):
+    """Random small object sets (sizes around the workgroup and tile edges,
+    few meshes and materials, many ties) in both modes against the oracle."""
+    g = np.random.default_rng(1000 + seed)
+    n = int(g.choice([1, 2, 63, 64, 65, 255, 256, 257, 511, 2047, 2048, 2049, 4097, 9000]))
+    s = synth.scene_objects(n, mesh_count=int(g.integers(1, 9)), material_count=int(g.integers(0, 13)),
+                            class_count=int(g.integers(1, 5)), hash_pool=int(g.integers(1, 5)),
+                            p_foreign_material_id=float(g.random() * 0.6), seed=seed)
+    rb = SceneRebuild(ctx)
+    rb.bind(s.instances, s.aabbs, s.draws, s.objects, s.materials, group_hidden=HIDDEN)
+    for mode in (L.REBUILD_PERSISTENT, L.REBUILD_OPTIMIZED):
+        rb.execute(mode)
+        want = oracle.scene_rebuild(mode, s.instances, s.aabbs, s.draws, s.objects, s.materials,
+                                    group_hidden=HIDDEN)
+        _compare(rb.read(), want)
+    rb.close()

@@ -227,3 +227,33 @@ def test_python_restatement_small(mode, seed):
     assert [(int(a), int(b)) for a, b in dc[:, 3:5]] == groups
     assert [(int(x["material_class"]), int(x["graph_hash"]), int(x["start"]), int(x["count"]))
             for x in r["ranges"]] == ranges
+
+
+try:
+    from hypothesis import HealthCheck, given, settings
+    from hypothesis import strategies as st
+except ImportError:  # pragma: no cover
+    given = None
+
+
+if given is not None:
+    @settings(max_examples=60, deadline=None, suppress_health_check=[HealthCheck.too_slow])
+    @given(n=st.integers(0, 300), meshes=st.integers(1, 8), materials=st.integers(0, 12),
+           classes=st.integers(1, 4), pool=st.integers(1, 4), foreign=st.floats(0.0, 0.6),
+           seed=st.integers(0, 10_000), mode=st.sampled_from([L.REBUILD_PERSISTENT, L.REBUILD_OPTIMIZED]))
+    def test_oracle_equals_python_restatement_random(n, meshes, materials, classes, pool, foreign, seed, mode):
+        s = synth.scene_objects(n, mesh_count=meshes, material_count=materials, class_count=classes,
+                                hash_pool=pool, p_foreign_material_id=foreign, seed=seed)
+        r = oracle.scene_rebuild(mode, s.instances, s.aabbs, s.draws, s.objects, s.materials, group_hidden=0b101)
+        if n == 0:
+            assert list(r["counts"][:2]) == [0, 0]
+            return
+        order, groups, ranges = _python_rebuild(s, mode)
+        slots = np.empty(n, np.int64)
+        slots[order] = np.arange(n)
+        assert np.array_equal(r["gpu_slots"], slots)
+        dc = r["draw_calls"].view(np.uint32).reshape(-1, 5)
+        assert [(int(a), int(b)) for a, b in dc[:, 3:5]] == groups
+        assert [(int(x["material_class"]), int(x["graph_hash"]), int(x["start"]), int(x["count"]))
+                for x in r["ranges"]] == ranges
+        check_invariants(s, r, mode, 0b101)
