@@ -13,6 +13,7 @@ constexpr uint32_t kMaxViews = 1024;      // views per batch launch (10 bits of 
 constexpr uint32_t kMaxBuckets = 14336;   // mesh_count * lod_count per view (LDS histogram, < 64 KiB with statics)
 constexpr uint32_t kCounterStride = 16;   // u32 counters per view
 constexpr uint32_t kColscanThreads = 256; // threads of a column-scan block
+constexpr uint32_t kStatsUnit = 15;       // slice field of the column-scan unit that sums a view's stats (slices use 0..8)
 constexpr uint32_t kEntryShift = 12;      // stage entry = (bucket << 12) | index in chunk (chunk <= 4096)
 
 // Per-view results (u32, kCounterStride per view), written by the column scan:

@@ -19,7 +19,7 @@
 //               the row's stats, once per row. No barrier inside a row.
 //   k_colscan   work units of (view, bucket slice): per bucket, the exclusive
 //               prefix of the counts over the view's rows, and the bucket
-//               total; unit 0 of a view also sums the stats.
+//               total; one more unit per view sums the row stats.
 //   k_scatter   the same block -> item ranges as classify: per row, cursors =
 //               scan(totals) + row offsets in LDS, args from row 0, then a flat
 //               loop over the row's entries places each id with an LDS atomic.
@@ -42,18 +42,38 @@ typedef __attribute__((address_space(1))) const F4v GlobalF4;
 typedef __attribute__((address_space(1))) const uint32_t GlobalU32;
 typedef __attribute__((address_space(1))) uint32_t GlobalU32Out;
 
-// occlusion_cull.wgsl:158-220 for one sphere (reversed-Z mirror when rev).
-__device__ __forceinline__ bool occluded(const CullView* __restrict__ V, const float* vp, float p00, float p11,
-                                         float camx, float camy, float camz, bool rev, float cx, float cy, float cz,
-                                         float r) {
-    if (r <= 0.0f) return false;
+// Orders this wave's LDS accesses around it. A wave's own LDS operations
+// complete in issue order; the fences keep the compiler from moving them
+// across, so lanes read what other lanes of the wave wrote before it.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// occlusion_cull.wgsl:158-220 for one sphere (reversed-Z mirror when rev), in
+// three steps: occl_probe decides whether the sphere takes the test and, if
+// so, sets its near depth, level and tap indices; the caller then loads the
+// taps (an untested sphere keeps the defaults it was given: texel 0 of level
+// 0) and occl_resolve compares.
+struct OcclProbe {
+    const float* level;
+    HizRectIdx idx;
+    float near_z;
+    bool test;  // false: not tested (r <= 0, behind the camera or off-screen), so not occluded
+};
+
+__device__ __forceinline__ void occl_probe(const CullView* __restrict__ V, const float* vp, float p00, float p11,
+                                           float camx, float camy, float camz, bool rev, float cx, float cy, float cz,
+                                           float r, OcclProbe& q) {
+    if (r <= 0.0f) return;
     const float w = mrow(vp, 3, cx, cy, cz);
-    if (w <= 0.0f) return false;
+    if (w <= 0.0f) return;
     const float ndx = mrow(vp, 0, cx, cy, cz) / w;
     const float ndy = mrow(vp, 1, cx, cy, cz) / w;
     const float u = ndx * 0.5f + 0.5f, vv = ndy * -0.5f + 0.5f;
     const float ndc_r = wmax(fabsf(r * p00 / w), fabsf(r * p11 / w));
-    if (ndx + ndc_r < -1.0f || ndx - ndc_r > 1.0f || ndy + ndc_r < -1.0f || ndy - ndc_r > 1.0f) return false;
+    if (ndx + ndc_r < -1.0f || ndx - ndc_r > 1.0f || ndy + ndc_r < -1.0f || ndy - ndc_r > 1.0f) return;
     // sphere_near_depth (:124-140); 1.0 is the "nearest" fallback in reversed Z.
     float near_z = rev ? 1.0f : 0.0f;
     {
@@ -82,7 +102,15 @@ __device__ __forceinline__ bool occluded(const CullView* __restrict__ V, const f
     p.levels = V->hiz_levels;
 #pragma unroll
     for (int i = 0; i < 12; ++i) p.off[i] = V->hiz_off[i];
-    const float hz = hiz_rect4(hiz_level(p, mip), rev, u0, v0, u1, v1);
+    const HizLevel l = hiz_level(p, mip);
+    q.level = l.base;
+    q.idx = hiz_rect_idx(l, u0, v0, u1, v1);
+    q.near_z = near_z;
+    q.test = true;
+}
+
+__device__ __forceinline__ bool occl_resolve(const HizTaps& t, float near_z, bool rev) {
+    const float hz = hiz_reduce4(t, rev);
     constexpr float bias = 1.0f / 65536.0f;
     return rev ? (near_z < hz - bias) : (near_z > hz + bias);
 }
@@ -259,11 +287,8 @@ __device__ __forceinline__ uint32_t classify(const CullView* __restrict__ V, con
         tiny = unsure_t ? te : tiny;
     }
     o.draw = o.visible & !tiny;
-    o.occl = false;
+    o.occl = false;  // the Hi-Z test, when on, follows in process_item
     const float camx = rc.cam[0], camy = rc.cam[1], camz = rc.cam[2];
-    if (OCCL && o.draw)
-        o.occl = occluded(V, V->vp, V->p00, rc.p11, camx, camy, camz, (V->flags & HV_CULL_REVERSED_Z) != 0u, cx, cy,
-                          cz, r);
     // lod = #{t : sqrt_rn(d2) >= lod_distance[t]}, evaluated without the
     // square root as d2 >= lod_d2[t] (the host picks lod_d2[t] as the least
     // float whose correctly rounded square root reaches the distance).
@@ -278,7 +303,7 @@ __device__ __forceinline__ uint32_t classify(const CullView* __restrict__ V, con
                (uint32_t)(d2 >= rc.lod_d2[6]);
     // mesh_id < 2^14 here (buckets <= 14336), so a 24-bit multiply-add is exact.
     const uint32_t code = __umul24(m & 0x7fffffffu, lod_count) + lod;
-    return (o.draw && !o.occl) ? code : kInvalid;
+    return o.draw ? code : kInvalid;
 }
 
 // Advance v to the view holding `item` (items are visited in order).
@@ -360,13 +385,73 @@ __device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, cons
     for (int i = 0; i < 6; ++i) l1[i] = ldc(&Vk->plane_l1[i]);
     uint32_t pos = 0;
     const bool boxes = AABB && c.aabb != nullptr;  // uniform: the view's scene holds explicit AABBs
+    // Occlusion: every candidate of the item is classified and probed (taps
+    // loaded) before the first is resolved and stored.
+    uint32_t code_o[OCCL ? IPT : 1];
+    Outcome out_o[OCCL ? IPT : 1];
+    if constexpr (OCCL) {
+        // The Hi-Z test runs on the wave's drawable candidates packed into
+        // consecutive lanes through a per-wave LDS slice, 64 at a time: one
+        // probe (seven IEEE divisions and a square root) per 64 drawable
+        // candidates instead of one per round with a few lanes active.
+        __shared__ float4 s_pk[NW][64];
+        __shared__ uint32_t s_oc[NW][64];
+        const CullView* __restrict__ V = c.V;
+        const bool rev = (V->flags & HV_CULL_REVERSED_Z) != 0u;
+        uint32_t slot[IPT];
+        uint32_t total = 0;  // wave-uniform
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const uint32_t j = k * WG + tid;
+            const bool valid = j < n;
+            code_o[k] = classify<OCCL, AABB>(V, Vk, pl, l1, *c.rc, valid, bb[k], mm[k], boxes,
+                                             boxes ? c.aabb + first + (valid ? j : 0u) : nullptr, out_o[k]);
+            const uint64_t dm = __ballot(out_o[k].draw);
+            slot[k] = total + mbcnt(dm);
+            total += (uint32_t)__popcll(dm);
+        }
+        for (uint32_t base = 0; base < total; base += 64u) {
+#pragma unroll
+            for (int k = 0; k < IPT; ++k)
+                if (out_o[k].draw && slot[k] - base < 64u) s_pk[wave][slot[k] - base] = bb[k];
+            wave_lds_sync();
+            bool occ = false;
+            if (lane < total - base) {
+                const float4 sp = s_pk[wave][lane];
+                OcclProbe q;
+                q.level = V->hiz0;
+                q.idx = HizRectIdx{0u, 0u, 0u, 0u};
+                q.near_z = 0.0f;
+                q.test = false;
+                occl_probe(V, V->vp, V->p00, c.rc->p11, c.rc->cam[0], c.rc->cam[1], c.rc->cam[2], rev, sp.x, sp.y,
+                           sp.z, sp.w, q);
+                const HizTaps t = hiz_load4(q.level, q.idx);
+                occ = q.test && occl_resolve(t, q.near_z, rev);
+            }
+            s_oc[wave][lane] = occ ? 1u : 0u;
+            wave_lds_sync();
+#pragma unroll
+            for (int k = 0; k < IPT; ++k)
+                if (out_o[k].draw && slot[k] - base < 64u) out_o[k].occl = s_oc[wave][slot[k] - base] != 0u;
+            wave_lds_sync();  // the next round rewrites both slices
+        }
+#pragma unroll
+        for (int k = 0; k < IPT; ++k)
+            if (out_o[k].occl) code_o[k] = kInvalid;
+    }
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
         const uint32_t j = k * WG + tid;
         const bool valid = j < n;
         Outcome o;
-        const uint32_t code = classify<OCCL, AABB>(c.V, Vk, pl, l1, *c.rc, valid, bb[k], mm[k], boxes,
-                                                   boxes ? c.aabb + first + (valid ? j : 0u) : nullptr, o);
+        uint32_t code;
+        if constexpr (OCCL) {
+            o = out_o[k];
+            code = code_o[k];
+        } else {
+            code = classify<OCCL, AABB>(c.V, Vk, pl, l1, *c.rc, valid, bb[k], mm[k], boxes,
+                                        boxes ? c.aabb + first + (valid ? j : 0u) : nullptr, o);
+        }
         const uint64_t keep = __ballot(code != kInvalid);
         if (code != kInvalid) {
             out[pos + mbcnt(keep)] = (code << kEntryShift) | j;
@@ -538,6 +623,34 @@ __global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __r
         const uint32_t v = code & 1023u, lg = (code >> 10) & 15u, b0 = code >> 14;
         const CullView* __restrict__ V = views + v;
         const uint32_t R = ldc(&V->n_rows), nb = ldc(&V->nb);
+        if (lg == kStatsUnit) {
+            // The view's stats: 32 row groups x 8 words, 16 rows per group in
+            // flight (512 rows per pass).
+            const uint32_t word = tid & 7u, g = tid >> 3;
+            const uint32_t* __restrict__ rs = V->row_stats;
+            uint32_t acc = 0;
+            for (uint32_t r = g; r < R; r += 512u) {
+                uint32_t y[16];
+#pragma unroll
+                for (uint32_t k = 0; k < 16; ++k) y[k] = (r + 32u * k < R) ? rs[(size_t)(r + 32u * k) * 8 + word] : 0u;
+#pragma unroll
+                for (uint32_t k = 0; k < 16; ++k) acc += y[k];
+            }
+            s_part[tid] = acc;
+            __syncthreads();
+            if (tid < 8) {
+                uint32_t t = 0;
+                for (uint32_t q = 0; q < 32; ++q) t += s_part[q * 8 + tid];
+                if (tid == 0) {
+                    V->counters[kCtrVisible] = t;  // survivors
+                    V->counters[0] = V->count;     // stats[0]: instances tested
+                } else {
+                    V->counters[tid] = t;
+                }
+            }
+            __syncthreads();
+            continue;
+        }
         const uint32_t col = tid & ((1u << lg) - 1u), grp = tid >> lg, RG = kColscanThreads >> lg;
         const uint32_t b = b0 + col;
         const bool live = b < nb;
@@ -601,31 +714,6 @@ __global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __r
             a.base_vertex = ml.base_vertex;
             a.first_instance = 0u;
             V->args[b] = a;
-        }
-        if (b0 == 0u) {  // stats of the view: 32 row groups x 8 words
-            __syncthreads();
-            const uint32_t word = tid & 7u, g = tid >> 3;
-            const uint32_t* __restrict__ rs = V->row_stats;
-            uint32_t acc = 0;
-            for (uint32_t r = g; r < R; r += 128u) {
-                uint32_t y[4];
-#pragma unroll
-                for (uint32_t k = 0; k < 4; ++k) y[k] = (r + 32u * k < R) ? rs[(size_t)(r + 32u * k) * 8 + word] : 0u;
-#pragma unroll
-                for (uint32_t k = 0; k < 4; ++k) acc += y[k];
-            }
-            s_part[tid] = acc;
-            __syncthreads();
-            if (tid < 8) {
-                uint32_t t = 0;
-                for (uint32_t q = 0; q < 32; ++q) t += s_part[q * 8 + tid];
-                if (tid == 0) {
-                    V->counters[kCtrVisible] = t;  // survivors
-                    V->counters[0] = V->count;     // stats[0]: instances tested
-                } else {
-                    V->counters[tid] = t;
-                }
-            }
         }
         __syncthreads();
     }
@@ -860,7 +948,8 @@ namespace {
 
 // Upper bound of column-scan units for `views` views of at most `buckets`
 // buckets with `rows` rows in total (see unit_lg: a slice is at least
-// min(256, 1024 / R) buckets wide).
+// min(256, 1024 / R) buckets wide, so a view has at most buckets / 256 +
+// buckets * R / 1024 + 1 bucket units, plus its stats unit).
 uint64_t unit_bound(uint64_t views, uint64_t buckets, uint64_t rows) {
     return views * (buckets / 128 + 2) + (buckets * rows) / 1024 + 1;
 }
@@ -995,6 +1084,7 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
         item += cv.n_items;
         rows_total += cv.n_rows;
         hist_total += (uint64_t)cv.n_rows * nb;
+        host_units_.push_back(v | (kStatsUnit << 10));
         const uint32_t lg = unit_lg(cv.n_rows);
         for (uint32_t b0 = 0; b0 < nb; b0 += (1u << lg)) host_units_.push_back(v | (lg << 10) | (b0 << 14));
     }

@@ -78,18 +78,48 @@ __device__ __forceinline__ uint32_t hiz_texel(float u, uint32_t n) {
     return (f >= 0.0f) ? (f < (float)n ? (uint32_t)(int)f : n - 1u) : 0u;
 }
 
-// The four nearest taps of the rect (u0, v0)-(u1, v1) on one level, reduced
-// in the reference order op(op(t(u0,v0), t(u1,v0)), op(t(u0,v1), t(u1,v1))).
-// The two columns and two rows are computed once for the four taps.
+// The four nearest taps of the rect (u0, v0)-(u1, v1) on one level: a, b on
+// row v0 at columns u0, u1; c, d on row v1. The two columns and two rows are
+// computed once for the four taps. A level holds at most 4096 x 4096 texels,
+// so a texel index fits 32 bits.
+struct HizRectIdx {
+    uint32_t a, b, c, d;
+};
+
+__device__ __forceinline__ HizRectIdx hiz_rect_idx(const HizLevel& l, float u0, float v0, float u1, float v1) {
+    const uint32_t x0 = hiz_texel(u0, l.w), x1 = hiz_texel(u1, l.w);
+    const uint32_t r0 = hiz_texel(v0, l.h) * l.w, r1 = hiz_texel(v1, l.h) * l.w;
+    return HizRectIdx{r0 + x0, r0 + x1, r1 + x0, r1 + x1};
+}
+
+struct HizTaps {
+    float a, b, c, d;
+};
+
 // The pyramid is device memory: the taps are global loads even when the level
 // pointer itself was read from LDS.
-__device__ __forceinline__ float hiz_rect4(const HizLevel& l, bool rev, float u0, float v0, float u1, float v1) {
+__device__ __forceinline__ HizTaps hiz_load4(const float* level, const HizRectIdx& i) {
     typedef __attribute__((address_space(1))) const float GlobalF;
-    const GlobalF* base = (const GlobalF*)l.base;
-    const uint32_t x0 = hiz_texel(u0, l.w), x1 = hiz_texel(u1, l.w);
-    const size_t r0 = (size_t)hiz_texel(v0, l.h) * l.w, r1 = (size_t)hiz_texel(v1, l.h) * l.w;
-    const float a = base[r0 + x0], b = base[r0 + x1], c = base[r1 + x0], d = base[r1 + x1];
-    return rev ? wmin(wmin(a, b), wmin(c, d)) : wmax(wmax(a, b), wmax(c, d));
+    const GlobalF* base = (const GlobalF*)level;
+    HizTaps t;
+    t.a = base[i.a];
+    t.b = base[i.b];
+    t.c = base[i.c];
+    t.d = base[i.d];
+    return t;
+}
+
+__device__ __forceinline__ HizTaps hiz_taps4(const HizLevel& l, float u0, float v0, float u1, float v1) {
+    return hiz_load4(l.base, hiz_rect_idx(l, u0, v0, u1, v1));
+}
+
+// The taps reduced in the reference order op(op(a, b), op(c, d)).
+__device__ __forceinline__ float hiz_reduce4(const HizTaps& t, bool rev) {
+    return rev ? wmin(wmin(t.a, t.b), wmin(t.c, t.d)) : wmax(wmax(t.a, t.b), wmax(t.c, t.d));
+}
+
+__device__ __forceinline__ float hiz_rect4(const HizLevel& l, bool rev, float u0, float v0, float u1, float v1) {
+    return hiz_reduce4(hiz_taps4(l, u0, v0, u1, v1), rev);
 }
 
 }  // namespace dev
