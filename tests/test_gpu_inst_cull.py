@@ -180,3 +180,71 @@ def test_occlusion_parity_config1_full(ctx):
     ref = _oracle(sc, view.camera, pyr=oracle.pyramid(depth, 1), flags=L.CULL_OCCLUSION | L.CULL_REVERSED_Z)
     _check(gpu, ref)
     assert ref[2][4] > 10_000
+
+
+def _dense_in_view(view, n, seed):
+    """n spheres inside the view's frustum, 5-400 m from the camera (log-uniform),
+    radii 0.1-10 m: most pass the frustum and sub-pixel tests, so a wave has
+    more drawable candidates than lanes and the Hi-Z probe takes several packed
+    rounds."""
+    g = synth.rng(1, seed)
+    inv = np.linalg.inv(view.view_proj64.reshape(4, 4).T)  # column-major -> row-major
+    ndc = np.stack([g.uniform(-0.95, 0.95, n), g.uniform(-0.95, 0.95, n), np.full(n, 0.5), np.ones(n)])
+    p = inv @ ndc
+    d = (p[:3] / p[3]).T - view.position
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    t = np.exp(g.uniform(np.log(5.0), np.log(400.0), n))
+    sc = synth.instance_scene(n, config=1, index=seed)
+    sc.bounds[:, :3] = (view.position + d * t[:, None]).astype(np.float32)
+    return sc
+
+
+def _half_wall_depth(view, rev, dist=60.0, width=1920, height=1080):
+    """Depth of a wall `dist` m ahead covering the left half of the screen;
+    the right half is cleared to the far value."""
+    vp = view.view_proj64.reshape(4, 4).T
+    inv = np.linalg.inv(vp)
+    c = inv @ np.array([0.0, 0.0, 0.5, 1.0])
+    fwd = c[:3] / c[3] - view.position
+    fwd /= np.linalg.norm(fwd)
+    clip = vp @ np.append(view.position + fwd * dist, 1.0)
+    depth = np.full((height, width), 0.0 if rev else 1.0, np.float32)
+    depth[:, : width // 2] = np.float32(clip[2] / clip[3])
+    return depth
+
+
+@pytest.mark.parametrize("rev,boxes", [(True, False), (False, False), (True, True)])
+@pytest.mark.parametrize("shape", [(512, 4), (256, 2), (64, 1)])
+def test_occlusion_dense_view_packed_rounds(ctx, rev, boxes, shape):
+    """Occlusion on a view where nearly every candidate reaches the Hi-Z probe:
+    the drawable candidates of a wave fill several packed 64-lane rounds (up to
+    IPT of them), under three launch shapes, sphere and explicit-AABB kernels."""
+    from helio_amd import autotune
+
+    saved = autotune.get_shapes(ctx)
+    autotune.apply(ctx, dict(saved, wg=shape[0], ipt=shape[1]))
+    try:
+        view = synth.perspective_view(synth.rng(1, 50), reversed_z=rev)
+        depth = _half_wall_depth(view, rev)
+        sc = _dense_in_view(view, 150_000, 40 + int(rev) + 2 * int(boxes))
+        aabbs = aabb6 = None
+        if boxes:
+            g = synth.rng(1, 77)
+            aabbs = np.zeros(sc.n, L.AABB)
+            ext = (sc.bounds[:, 3:4] * g.uniform(0.5, 1.5, (sc.n, 3))).astype(np.float32)
+            aabbs["min"] = sc.bounds[:, :3] - ext
+            aabbs["max"] = sc.bounds[:, :3] + ext
+            aabb6 = np.concatenate([aabbs["min"], aabbs["max"]], axis=1).astype(np.float32)
+        hiz = HiZBuildPass(ctx, 1920, 1080, L.HIZ_MIN if rev else L.HIZ_MAX)
+        hiz.upload_depth(depth)
+        hiz.execute()
+        print(f"dense occlusion rev={rev} boxes={boxes} shape={shape}", flush=True)
+        gpu = _run(ctx, sc, view.camera, soa=not boxes, aabbs=aabbs, hiz=hiz, occl=True, rev=rev)
+        flags = L.CULL_OCCLUSION | (L.CULL_REVERSED_Z if rev else 0)
+        ref = _oracle(sc, view.camera, pyr=oracle.pyramid(depth, 1 if rev else 0), flags=flags, aabb6=aabb6)
+        _check(gpu, ref)
+        st = ref[2]
+        assert sc.n - st[1] - st[2] > 0.8 * sc.n, st  # reach the probe
+        assert st[4] > 0.05 * sc.n and st[3] > 0.05 * sc.n, st  # both outcomes well represented
+    finally:
+        autotune.apply(ctx, saved)
