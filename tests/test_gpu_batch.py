@@ -211,3 +211,44 @@ def test_batch_full_size_matches_oracle(ctx):
         _check(got, _oracle(sc, v.camera, f, c))
         visible += len(got[0])
     assert tests > 9_000_000 and visible > 0
+
+
+def test_batch_occlusion_views_own_pyramids(ctx):
+    """Occlusion in a batch: six reversed-Z views with their own cameras and
+    pyramids (two views share one pyramid, one view is empty), run eagerly and
+    replayed from the graph; every view equals the oracle with its pyramid."""
+    from helio_amd.passes import HiZBuildPass
+    from tests.test_gpu_inst_cull import _half_wall_depth
+
+    sc = synth.instance_scene(200_000, config=1, index=9)
+    scene = _scene(ctx, sc)
+    walls = [60.0, 120.0, 30.0, 90.0]
+    cams, pyrs, refs_pyr = [], [], []
+    for i, d in enumerate(walls):
+        view = synth.perspective_view(synth.rng(1, 60 + i), reversed_z=True)
+        depth = _half_wall_depth(view, True, dist=d)
+        hiz = HiZBuildPass(ctx, 1920, 1080, L.HIZ_MIN)
+        hiz.upload_depth(depth)
+        hiz.execute()
+        cams.append(view.camera)
+        pyrs.append(hiz)
+        refs_pyr.append(oracle.pyramid(depth, 1))
+    plan = [(0, 0, 200_000), (1, 10_000, 150_000), (2, 50_000, 97_000), (0, 100_000, 100_000), (3, 5, 0),
+            (3, 1, 199_999)]
+    b = BatchCull(ctx, len(plan), occlusion=True, reversed_z=True)
+    b.prepare([(scene, cams[p], f, c, pyrs[p]) for p, f, c in plan])
+    flags = L.CULL_OCCLUSION | L.CULL_REVERSED_Z
+    refs = []
+    for p, f, c in plan:
+        ov = oracle.view_from_camera(cams[p], flags=flags, mesh_lods=sc.mesh_lods, mesh_count=sc.mesh_count)
+        refs.append(oracle.cull_view(ov, sc.bounds, sc.meta(), f, c, refs_pyr[p], None, threads=8))
+    assert sum(int(r[2][4]) for r in refs) > 1000, [list(r[2]) for r in refs]
+    print(f"occlusion batch of {len(plan)} views", flush=True)
+    b.execute()
+    for i in range(len(plan)):
+        _check(b.read(i), refs[i])
+    b.execute(graph=True)
+    b.execute(graph=True)
+    for i in range(len(plan)):
+        _check(b.read(i), refs[i])
+    b.close()
