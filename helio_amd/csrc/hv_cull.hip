@@ -624,15 +624,16 @@ __global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __r
         const CullView* __restrict__ V = views + v;
         const uint32_t R = ldc(&V->n_rows), nb = ldc(&V->nb);
         if (lg == kStatsUnit) {
-            // The view's stats: 32 row groups x 8 words, 16 rows per group in
-            // flight (512 rows per pass).
+            // The view's stats: SG row groups x 8 words, 16 rows per group in
+            // flight (16 SG rows per pass).
+            constexpr uint32_t SG = kColscanThreads / 8u;
             const uint32_t word = tid & 7u, g = tid >> 3;
             const uint32_t* __restrict__ rs = V->row_stats;
             uint32_t acc = 0;
-            for (uint32_t r = g; r < R; r += 512u) {
+            for (uint32_t r = g; r < R; r += 16u * SG) {
                 uint32_t y[16];
 #pragma unroll
-                for (uint32_t k = 0; k < 16; ++k) y[k] = (r + 32u * k < R) ? rs[(size_t)(r + 32u * k) * 8 + word] : 0u;
+                for (uint32_t k = 0; k < 16; ++k) y[k] = (r + SG * k < R) ? rs[(size_t)(r + SG * k) * 8 + word] : 0u;
 #pragma unroll
                 for (uint32_t k = 0; k < 16; ++k) acc += y[k];
             }
@@ -640,7 +641,7 @@ __global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __r
             __syncthreads();
             if (tid < 8) {
                 uint32_t t = 0;
-                for (uint32_t q = 0; q < 32; ++q) t += s_part[q * 8 + tid];
+                for (uint32_t q = 0; q < SG; ++q) t += s_part[q * 8 + tid];
                 if (tid == 0) {
                     V->counters[kCtrVisible] = t;  // survivors
                     V->counters[0] = V->count;     // stats[0]: instances tested
