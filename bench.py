@@ -12,7 +12,9 @@ visible ids and DrawIndexedIndirect args.
   value          culled instances/s (view x instance tests per second) of the
                  batched path: the 64 views in one hipGraph replay per step,
                  two copies of the batch in flight on two streams (one step's
-                 column scan and scatter overlap the next step's classify);
+                 column scan and scatter overlap the next step's classify;
+                 three copies measured 43 us/step with this leg alone but
+                 52 us/step in the default run, against 45 us for two);
                  one_in_flight gives the same steps on one stream.
   plain          the same 64 views as a per-view launch loop (prepared once,
                  3 eager launches per view), views/s and p50/p99.
@@ -487,8 +489,8 @@ def main() -> int:
     ms_seq = dt_seq / args.steps * 1e3
     log(f"batched, one in flight: {ms_seq:.3f} ms/step")
     # Steady state: IN_FLIGHT prepared copies of the batch replay their graphs
-    # on their own streams, steps alternating, so one step's latency-bound
-    # column scan and scatter overlap the next step's classify. Every step
+    # on their own streams, steps taking turns, so one step's latency-bound
+    # column scan and scatter overlap the next steps' classify. Every step
     # still culls all 64 views.
     batches = [batch] + [BatchCull(ctx, len(views)) for _ in range(IN_FLIGHT - 1)]
     for b in batches[1:]:
@@ -653,7 +655,7 @@ def main() -> int:
             "lod_distances_m": [25, 75, 200],
             "buckets_per_view": sc.mesh_count * sc.lod_count,
             "launch": f"hipGraph replay (classify, column scan, scatter); {IN_FLIGHT} copies of the batch in "
-                      f"flight on {IN_FLIGHT} streams, steps alternating",
+                      f"flight on {IN_FLIGHT} streams, steps taking turns",
             "in_flight": IN_FLIGHT,
             "launch_shape": {"wg": tuned["wg"], "ipt": tuned["ipt"]},
             "visible_per_step": visible,

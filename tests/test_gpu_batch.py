@@ -50,20 +50,21 @@ def test_batch_graph_replay_matches_oracle(ctx):
         _check(b.read(i), _oracle(sc, v.camera, f, c))
 
 
-def test_two_batches_in_flight_on_two_streams(ctx):
-    """Two copies of a batch replaying their graphs alternately on two library
-    streams (the benchmark's steady state): after 5 steps each, every view of
-    both copies equals the oracle."""
+@pytest.mark.parametrize("copies", [2, 3])
+def test_batches_in_flight_on_their_own_streams(ctx, copies):
+    """Copies of a batch replaying their graphs in turn, each on its own library
+    stream (the benchmark's steady state): after 5 steps each, every view of
+    every copy equals the oracle."""
     sc = synth.instance_scene(262_144, config=3, index=2)
     scene = _scene(ctx, sc)
     views = synth.multiview_views(sc.n, max_count=100_000)
     reqs = [(scene, v.camera, f, c) for v, f, c in views]
-    batches = [BatchCull(ctx, len(views)) for _ in range(2)]
-    streams = [Stream(ctx) for _ in range(2)]
+    batches = [BatchCull(ctx, len(views)) for _ in range(copies)]
+    streams = [Stream(ctx) for _ in range(copies)]
     for b in batches:
         b.prepare(reqs)
-    for i in range(10):
-        batches[i % 2].execute(stream=streams[i % 2].handle, graph=True)
+    for i in range(5 * copies):
+        batches[i % copies].execute(stream=streams[i % copies].handle, graph=True)
     for b in batches:
         b.wait()
     for i, (v, f, c) in enumerate(views):
