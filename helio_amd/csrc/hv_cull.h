@@ -10,7 +10,7 @@
 namespace hv {
 
 constexpr uint32_t kMaxViews = 1024;      // views per batch launch (10 bits of a column-scan unit)
-constexpr uint32_t kMaxBuckets = 14336;   // mesh_count * lod_count per view (LDS histogram, < 64 KiB with statics)
+constexpr uint32_t kMaxBuckets = 14336;   // mesh_count * lod_count per view (56 KiB LDS histogram; with the statics well within 160 KiB)
 constexpr uint32_t kCounterStride = 16;   // u32 counters per view
 constexpr uint32_t kColscanThreads = 256; // threads of a column-scan block
 constexpr uint32_t kStatsUnit = 15;       // slice field of the column-scan unit that sums a view's stats (slices use 0..8)

@@ -182,7 +182,7 @@ def test_occlusion_parity_config1_full(ctx):
     assert ref[2][4] > 10_000
 
 
-def _dense_in_view(view, n, seed):
+def _dense_in_view(view, n, seed, mesh_count=256):
     """n spheres inside the view's frustum, 5-400 m from the camera (log-uniform),
     radii 0.1-10 m: most pass the frustum and sub-pixel tests, so a wave has
     more drawable candidates than lanes and the Hi-Z probe takes several packed
@@ -194,7 +194,7 @@ def _dense_in_view(view, n, seed):
     d = (p[:3] / p[3]).T - view.position
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     t = np.exp(g.uniform(np.log(5.0), np.log(400.0), n))
-    sc = synth.instance_scene(n, config=1, index=seed)
+    sc = synth.instance_scene(n, config=1, index=seed, mesh_count=mesh_count)
     sc.bounds[:, :3] = (view.position + d * t[:, None]).astype(np.float32)
     return sc
 
@@ -248,3 +248,38 @@ def test_occlusion_dense_view_packed_rounds(ctx, rev, boxes, shape):
         assert st[4] > 0.05 * sc.n and st[3] > 0.05 * sc.n, st  # both outcomes well represented
     finally:
         autotune.apply(ctx, saved)
+
+
+@pytest.mark.parametrize("occl", [False, True])
+def test_max_buckets(ctx, occl):
+    """The largest bucket table a view may have, 3,584 meshes x 4 LODs = 14,336
+    buckets (a 56 KiB LDS histogram per classify workgroup, plus the occlusion
+    variant's packing slices), with and without the Hi-Z test; one mesh more is
+    refused when the LOD table is set."""
+    from helio_amd._lib import HvError
+
+    view = synth.perspective_view(synth.rng(1, 50), reversed_z=True)
+    sc = _dense_in_view(view, 120_000, 61, mesh_count=3584)
+    assert sc.mesh_count * sc.lod_count == 14336
+    hiz = pyr = None
+    flags = 0
+    if occl:
+        depth = _half_wall_depth(view, True)
+        hiz = HiZBuildPass(ctx, 1920, 1080, L.HIZ_MIN)
+        hiz.upload_depth(depth)
+        hiz.execute()
+        pyr = oracle.pyramid(depth, 1)
+        flags = L.CULL_OCCLUSION | L.CULL_REVERSED_Z
+    print(f"max buckets occl={occl}", flush=True)
+    gpu = _run(ctx, sc, view.camera, hiz=hiz, occl=occl, rev=occl)
+    ref = _oracle(sc, view.camera, pyr=pyr, flags=flags)
+    _check(gpu, ref)
+    assert (ref[1].reshape(-1, 5)[:, 1] > 0).sum() > 10_000  # most buckets drawn into
+    if occl:
+        assert ref[2][4] > 0
+
+    big = synth.instance_scene(1000, config=1, index=62, mesh_count=3585)
+    scene = Scene(ctx, big.n)
+    scene.upload_soa(big.bounds, big.mesh_ids, big.flags)
+    with pytest.raises(HvError):
+        scene.set_mesh_lods(big.mesh_lods, big.mesh_count, big.lod_count)
