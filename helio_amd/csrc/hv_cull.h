@@ -104,7 +104,9 @@ class CullBatch {
     // (ms[0] classify, ms[1] column scan, ms[2] scatter); this synchronises.
     hv_status execute(hipStream_t stream, float* ms = nullptr);
     // The same three kernels captured once into a hipGraph and replayed; the
-    // view table lives in device memory, so a new prepare() needs no re-capture.
+    // view table lives in device memory, so a new prepare() needs no re-capture
+    // unless it moved the table (a larger table is a new allocation, and the
+    // graph's kernels hold the table's address).
     hv_status execute_graph(hipStream_t stream);
     hv_status outputs(uint32_t view, hv_cull_outputs* out) const;
     hv_status read(uint32_t view, uint32_t* ids, uint32_t ids_capacity, uint32_t* id_count,
@@ -138,7 +140,7 @@ class CullBatch {
     DevBuf d_table_, d_stage_, d_ids_, d_args_, d_counters_, d_sub_cnt_, d_row_stats_, d_block_hist_, d_totals_;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
-    uint32_t graph_key_[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t graph_key_[7] = {0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev_[5] = {};
     hipEvent_t done_ = nullptr;    // recorded after every launch
     bool done_recorded_ = false;

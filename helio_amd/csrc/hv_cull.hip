@@ -1245,10 +1245,22 @@ hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed) {
 }
 
 hv_status CullBatch::check_scenes() const {
-    for (size_t v = 0; v < descs_.size(); ++v)
-        if (descs_[v].scene->storage_gen != scene_gens_[v])
+    for (size_t v = 0; v < descs_.size(); ++v) {
+        const ViewDesc& d = descs_[v];
+        if (d.scene->storage_gen != scene_gens_[v])
             return set_error(HV_ESTATE, "execute: the scene of view %zu moved or changed shape since prepare (grown, "
                                         "AABBs switched, new LOD table): prepare again", v);
+        // Uploads within the capacity, swap_remove and truncate keep the
+        // storage but may leave the prepared range past the resident count, or
+        // a mesh id past the LOD table (its bucket would index past the
+        // kernels' histograms).
+        if ((uint64_t)d.first + d.count > d.scene->count)
+            return set_error(HV_ESTATE, "execute: view %zu range [%u, +%u) now exceeds the %u resident instances: "
+                                        "prepare again", v, d.first, d.count, d.scene->count);
+        if (d.scene->count && d.scene->max_mesh_id >= d.scene->mesh_count)
+            return set_error(HV_ESTATE, "execute: the scene of view %zu now holds mesh_id %u but only %u meshes have "
+                                        "LODs", v, d.scene->max_mesh_id, d.scene->mesh_count);
+    }
     if (cfg_.flags & HV_CULL_OCCLUSION)
         for (size_t v = 0; v < descs_.size(); ++v)
             if (descs_[v].hiz && descs_[v].hiz->storage_gen != hiz_gens_[v])
@@ -1273,8 +1285,10 @@ hv_status CullBatch::execute_graph(hipStream_t s) {
     // The classify variant (occlusion, explicit AABBs) and the LDS size are
     // baked into the graph; grids are the capacity grids.
     const bool aabb_any = std::any_of(host_views_.begin(), host_views_.end(), [](const CullView& c) { return c.aabb; });
-    const uint32_t key[6] = {resident1_, lds_cap_, aabb_any ? 1u : 0u, shape.wg * 16u + shape.ipt, ctx_->cu_count,
-                             resident3_};
+    // The table's address is a kernel argument of every captured launch: a
+    // prepare that had to grow the table moved it, and the graph is re-captured.
+    const uint64_t key[7] = {resident1_, lds_cap_, aabb_any ? 1u : 0u, shape.wg * 16u + shape.ipt, ctx_->cu_count,
+                             resident3_, reinterpret_cast<uint64_t>(d_table_.ptr)};
     if (!graph_exec_ || std::memcmp(key, graph_key_, sizeof key) != 0) {
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
         if (graph_) (void)hipGraphDestroy(graph_);

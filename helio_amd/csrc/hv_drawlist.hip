@@ -15,8 +15,11 @@ namespace {
 constexpr uint32_t kCompactThreads = 1024;
 constexpr uint32_t kCompactMax = 1u << 20;  // draws per call (one workgroup walks them)
 
-__global__ __launch_bounds__(1024) void k_compact_draws(const uint32_t* __restrict__ args, uint32_t n,
-                                                        uint32_t* __restrict__ out, uint32_t* __restrict__ count) {
+// `args` and `out` may be the same array (compaction in place), so neither is
+// declared __restrict__: a chunk's loads complete before the barrier that
+// precedes its stores, and every store lands at or before the draw it moves.
+__global__ __launch_bounds__(1024) void k_compact_draws(const uint32_t* args, uint32_t n, uint32_t* out,
+                                                        uint32_t* __restrict__ count) {
     __shared__ uint32_t s_wave[kCompactThreads / 64];
     __shared__ uint32_t s_base;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;

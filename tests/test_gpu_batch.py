@@ -157,6 +157,27 @@ def test_batch_reprepare_after_replay(ctx):
         _check(b.read(i), _oracle(sc, v.camera, f, c))
 
 
+def test_batch_graph_after_the_table_grows(ctx):
+    """A batch captured on a small table, then prepared with many more views
+    (the device table is reallocated at a new address), replays a graph that
+    reads the new table: every view equals the oracle, before and after the
+    growth, and again after going back to the small table."""
+    sc = synth.instance_scene(200_000, config=3, index=3)
+    scene = _scene(ctx, sc)
+    views = synth.multiview_views(sc.n, max_count=60_000)
+    small = [(v, f, min(c, 2_000)) for v, f, c in views[:2]]
+    b = BatchCull(ctx, len(views))
+    for table in (small, views, small):
+        b.prepare([(scene, v.camera, f, c) for v, f, c in table])
+        print(f"graph replay over {len(table)} views", flush=True)
+        b.execute(graph=True)
+        b.execute(graph=True)
+        for i, (v, f, c) in enumerate(table):
+            _check(b.read(i), _oracle(sc, v.camera, f, c))
+    b.close()
+    scene.close()
+
+
 def test_frustum_and_subpixel_boundaries(ctx):
     """Instances placed on the frustum planes (the p-vertex distance at zero,
     give or take an ulp) and radii at the sub-pixel threshold: the kernel's

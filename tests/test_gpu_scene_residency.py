@@ -200,3 +200,40 @@ def test_prepared_pass_refuses_a_grown_scene(ctx):
     p.close()
     scene.close()
 
+
+
+def test_prepared_pass_refuses_a_shrunk_range_or_a_new_mesh_id(ctx):
+    """Updates that keep the scene's storage can still invalidate a prepared
+    pass: a mesh id past the LOD table (its bucket would lie outside the
+    kernels' histograms) or a truncate below the prepared range. Execute
+    refuses both on the host (nothing is launched); prepare refuses them too."""
+    g = np.random.default_rng(22)
+    scene = Scene(ctx, 1024)
+    scene.upload(_instances(g, 1000, meshes=16))
+    sc = synth.instance_scene(16, config=0)
+    lods = sc.mesh_lods[: 16 * sc.lod_count]
+    scene.set_mesh_lods(lods, 16, sc.lod_count)
+    cam = synth.perspective_view(synth.rng(0, 4), position=(150, 150, -30), direction=(0, 0, 1)).camera
+    p = InstanceCullPass(ctx)
+    p.prepare(scene, cam)
+    p.execute()
+    p.read()
+    bad = _instances(g, 1, meshes=16)
+    bad["mesh_id"] = 16
+    scene.upload(bad, first=3)  # within the capacity: no growth
+    with pytest.raises(HvError, match="mesh_id"):
+        p.execute()
+    with pytest.raises(HvError):
+        p.prepare(scene, cam)
+    scene.set_mesh_lods(sc.mesh_lods, sc.mesh_count, sc.lod_count)
+    p.prepare(scene, cam)
+    p.execute()
+    scene.truncate(900)
+    with pytest.raises(HvError, match="exceeds"):
+        p.execute()
+    p.prepare(scene, cam, 0, 900)
+    p.execute()
+    ids, args, stats = p.read()
+    assert stats[0] == 900
+    p.close()
+    scene.close()
