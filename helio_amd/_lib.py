@@ -214,6 +214,7 @@ _SIGS: dict[str, tuple] = {
     "hv_inst_cull_read": (i32, [vp, vp, u32, P(u32), vp, u32, vp]),
     "hv_inst_cull_execute_timed": (i32, [vp, vp, P(f32)]),
     "hv_inst_cull_set_shape": (i32, [vp, u32, u32]),
+    "hv_inst_cull_noalloc": (i32, [vp, P(CullConfig), vp, vp, u32, u32, vp, vp, vp, vp, vp]),
     "hv_batch_create": (i32, [vp, P(CullConfig), u32, P(vp)]),
     "hv_batch_destroy": (None, [vp]),
     "hv_batch_set_shape": (i32, [vp, u32, u32]),

@@ -119,6 +119,9 @@ hv_status hv_ctx_create(int32_t device, hv_ctx** out) {
         hv_ctx_destroy(c);
         return hip_error(e, "hv_ctx_create", __FILE__, __LINE__);
     }
+    // Once per device: the address of the no-allocation cull's scratch array
+    // (a failure is reported by that entry point, which retries the lookup).
+    (void)cull_scratch_init(device);
     *out = c;
     return HV_OK;
 }

@@ -27,6 +27,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <mutex>
+
 #include "hv_cull.h"
 #include "hv_devmath.h"
 
@@ -916,6 +918,84 @@ float sqrt_threshold(float t) {
     return r;
 }
 
+// The argument checks of one view of a batch (prepare's, per view).
+hv_status check_view(const hv_cull_config& cfg, const ViewDesc& d, uint32_t v) {
+    if (!d.scene || !d.camera) return set_error(HV_EINVAL, "prepare: view %u has no scene or camera", v);
+    if (!d.scene->mesh_count) return set_error(HV_ESTATE, "prepare: view %u scene has no mesh LOD table", v);
+    if (d.scene->count && d.scene->max_mesh_id >= d.scene->mesh_count)
+        return set_error(HV_EINVAL, "prepare: view %u scene holds mesh_id %u but only %u meshes have LODs", v,
+                         d.scene->max_mesh_id, d.scene->mesh_count);
+    if (d.scene->mesh_count * d.scene->lod_count > kMaxBuckets)
+        return set_error(HV_ELIMIT, "prepare: view %u has more than %u buckets", v, kMaxBuckets);
+    if (d.scene->lod_count != cfg.lod_count)
+        return set_error(HV_EINVAL, "prepare: scene has %u LODs, pass config %u", d.scene->lod_count, cfg.lod_count);
+    if ((uint64_t)d.first + d.count > d.scene->count)
+        return set_error(HV_EINVAL, "prepare: view %u range [%u, +%u) exceeds the %u resident instances", v, d.first,
+                         d.count, d.scene->count);
+    if (cfg.flags & HV_CULL_OCCLUSION) {
+        if (!d.hiz || !d.hiz->level0)
+            return set_error(HV_ESTATE, "prepare: occlusion enabled but view %u has no bound pyramid", v);
+        const bool rev = (cfg.flags & HV_CULL_REVERSED_Z) != 0;
+        if ((d.hiz->mode == HV_HIZ_MIN) != rev)
+            return set_error(HV_EINVAL, "prepare: pyramid mode does not match HV_CULL_REVERSED_Z");
+    }
+    return HV_OK;
+}
+
+// Everything of a view record that does not depend on the batch layout:
+// planes and their decision margins, camera terms, LOD thresholds, the
+// scene's arrays and the pyramid.
+hv_status fill_view(const hv_cull_config& cfg, const ViewDesc& d, CullView& cv) {
+    HV_TRY(hv_extract_frustum_planes(d.camera->view_proj, cv.planes));
+    for (int i = 0; i < 6; ++i)
+        for (int a = 0; a < 3; ++a) cv.pv_sign[3 * i + a] = cv.planes[4 * i + a] >= 0.0f ? 1.0f : -1.0f;
+    {
+        // Margin of the frustum decisions (see the kernel): 2^-20 (L1max B + |d|max) + 2^-140.
+        double l1max = 0.0, dmax = 0.0;
+        bool finite = true;
+        for (int i = 0; i < 6; ++i) {
+            cv.plane_l1[i] = (std::fabs(cv.planes[4 * i]) + std::fabs(cv.planes[4 * i + 1])) + std::fabs(cv.planes[4 * i + 2]);
+            const double l1 = std::fabs((double)cv.planes[4 * i]) + std::fabs((double)cv.planes[4 * i + 1]) +
+                              std::fabs((double)cv.planes[4 * i + 2]);
+            l1max = std::max(l1max, l1);
+            dmax = std::max(dmax, std::fabs((double)cv.planes[4 * i + 3]));
+            for (int k = 0; k < 4; ++k) finite = finite && std::isfinite(cv.planes[4 * i + k]);
+        }
+        // Rounded up, so the kernel's margin never falls below the bound.
+        cv.sure_kl = std::nextafter((float)(l1max * 0x1p-20), INFINITY);
+        cv.sure_kd = std::nextafter((float)(dmax * 0x1p-20 + 0x1p-140), INFINITY);
+        cv.sure_ok = (finite && l1max <= 0x1p20 && dmax <= 0x1p100) ? 1u : 0u;
+    }
+    std::memcpy(cv.vp, d.camera->view_proj, 64);
+    cv.p00 = d.camera->proj[0];
+    cv.p11 = d.camera->proj[5];
+    std::memcpy(cv.cam, d.camera->position_near, 12);
+    // Thresholds past lod_count - 1 are NaN: d2 >= NaN is false, so the
+    // kernel may compare against all of them.
+    for (uint32_t i = 0; i < 7; ++i)
+        cv.lod_d2[i] = i + 1 < cfg.lod_count ? sqrt_threshold(cfg.lod_distances[i]) : std::nanf("");
+    cv.lod_count = cfg.lod_count;
+    cv.flags = cfg.flags;
+    cv.nb = d.scene->mesh_count * d.scene->lod_count;
+    cv.first = d.first;
+    cv.count = d.count;
+    cv.screen_h = cfg.screen_height;
+    cv.hiz_mip_count = hv_mip_levels(cfg.screen_width, cfg.screen_height);
+    cv.bounds = d.scene->bounds.as<float4>();
+    cv.meta = d.scene->meta.as<uint32_t>();
+    cv.aabb = d.scene->explicit_aabb ? d.scene->aabb.as<hv_instance_aabb>() : nullptr;
+    cv.mesh_lods = d.scene->mesh_lods.as<hv_mesh_lod>();
+    if (cfg.flags & HV_CULL_OCCLUSION) {
+        cv.hiz_w = d.hiz->width;
+        cv.hiz_h = d.hiz->height;
+        cv.hiz_levels = d.hiz->levels;
+        for (uint32_t m = 1; m < d.hiz->levels && m < 12; ++m) cv.hiz_off[m] = (uint32_t)d.hiz->level_off[m];
+        cv.hiz0 = d.hiz->level0;
+        cv.hiz_packed = d.hiz->packed.as<float>();
+    }
+    return HV_OK;
+}
+
 }  // namespace
 
 CullBatch::CullBatch(hv_ctx* ctx, const hv_cull_config& cfg, uint32_t max_views)
@@ -1028,31 +1108,12 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     HV_HIP(hipSetDevice(ctx_->device));
     // The previous launch may still read the table and buffers rewritten below.
     HV_TRY(sync_last());
-    const bool occl = (cfg_.flags & HV_CULL_OCCLUSION) != 0;
     const uint32_t chunk = shape.chunk(), nw = shape.wg / 64;
     uint64_t ids_total = 0, args_total = 0, items_total = 0;
     uint32_t max_nb = 1;
     for (uint32_t v = 0; v < n; ++v) {
         const ViewDesc& d = views[v];
-        if (!d.scene || !d.camera) return set_error(HV_EINVAL, "prepare: view %u has no scene or camera", v);
-        if (!d.scene->mesh_count) return set_error(HV_ESTATE, "prepare: view %u scene has no mesh LOD table", v);
-        if (d.scene->count && d.scene->max_mesh_id >= d.scene->mesh_count)
-            return set_error(HV_EINVAL, "prepare: view %u scene holds mesh_id %u but only %u meshes have LODs", v,
-                             d.scene->max_mesh_id, d.scene->mesh_count);
-        if (d.scene->mesh_count * d.scene->lod_count > kMaxBuckets)
-            return set_error(HV_ELIMIT, "prepare: view %u has more than %u buckets", v, kMaxBuckets);
-        if (d.scene->lod_count != cfg_.lod_count)
-            return set_error(HV_EINVAL, "prepare: scene has %u LODs, pass config %u", d.scene->lod_count, cfg_.lod_count);
-        if ((uint64_t)d.first + d.count > d.scene->count)
-            return set_error(HV_EINVAL, "prepare: view %u range [%u, +%u) exceeds the %u resident instances", v, d.first,
-                             d.count, d.scene->count);
-        if (occl) {
-            if (!d.hiz || !d.hiz->level0)
-                return set_error(HV_ESTATE, "prepare: occlusion enabled but view %u has no bound pyramid", v);
-            const bool rev = (cfg_.flags & HV_CULL_REVERSED_Z) != 0;
-            if ((d.hiz->mode == HV_HIZ_MIN) != rev)
-                return set_error(HV_EINVAL, "prepare: pyramid mode does not match HV_CULL_REVERSED_Z");
-        }
+        HV_TRY(check_view(cfg_, d, v));
         const uint32_t nb = d.scene->mesh_count * d.scene->lod_count;
         max_nb = std::max(max_nb, nb);
         ids_total += d.count;
@@ -1114,53 +1175,7 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     for (uint32_t v = 0; v < n; ++v) {
         const ViewDesc& d = views[v];
         CullView& cv = host_views_[v];
-        HV_TRY(hv_extract_frustum_planes(d.camera->view_proj, cv.planes));
-        for (int i = 0; i < 6; ++i)
-            for (int a = 0; a < 3; ++a) cv.pv_sign[3 * i + a] = cv.planes[4 * i + a] >= 0.0f ? 1.0f : -1.0f;
-        {
-            // Margin of the frustum decisions (see the kernel): 2^-20 (L1max B + |d|max) + 2^-140.
-            double l1max = 0.0, dmax = 0.0;
-            bool finite = true;
-            for (int i = 0; i < 6; ++i) {
-                cv.plane_l1[i] = (std::fabs(cv.planes[4 * i]) + std::fabs(cv.planes[4 * i + 1])) + std::fabs(cv.planes[4 * i + 2]);
-                const double l1 = std::fabs((double)cv.planes[4 * i]) + std::fabs((double)cv.planes[4 * i + 1]) +
-                                  std::fabs((double)cv.planes[4 * i + 2]);
-                l1max = std::max(l1max, l1);
-                dmax = std::max(dmax, std::fabs((double)cv.planes[4 * i + 3]));
-                for (int k = 0; k < 4; ++k) finite = finite && std::isfinite(cv.planes[4 * i + k]);
-            }
-            // Rounded up, so the kernel's margin never falls below the bound.
-            cv.sure_kl = std::nextafter((float)(l1max * 0x1p-20), INFINITY);
-            cv.sure_kd = std::nextafter((float)(dmax * 0x1p-20 + 0x1p-140), INFINITY);
-            cv.sure_ok = (finite && l1max <= 0x1p20 && dmax <= 0x1p100) ? 1u : 0u;
-        }
-        std::memcpy(cv.vp, d.camera->view_proj, 64);
-        cv.p00 = d.camera->proj[0];
-        cv.p11 = d.camera->proj[5];
-        std::memcpy(cv.cam, d.camera->position_near, 12);
-        // Thresholds past lod_count - 1 are NaN: d2 >= NaN is false, so the
-        // kernel may compare against all of them.
-        for (uint32_t i = 0; i < 7; ++i)
-            cv.lod_d2[i] = i + 1 < cfg_.lod_count ? sqrt_threshold(cfg_.lod_distances[i]) : std::nanf("");
-        cv.lod_count = cfg_.lod_count;
-        cv.flags = cfg_.flags;
-        cv.nb = d.scene->mesh_count * d.scene->lod_count;
-        cv.first = d.first;
-        cv.count = d.count;
-        cv.screen_h = cfg_.screen_height;
-        cv.hiz_mip_count = hv_mip_levels(cfg_.screen_width, cfg_.screen_height);
-        cv.bounds = d.scene->bounds.as<float4>();
-        cv.meta = d.scene->meta.as<uint32_t>();
-        cv.aabb = d.scene->explicit_aabb ? d.scene->aabb.as<hv_instance_aabb>() : nullptr;
-        cv.mesh_lods = d.scene->mesh_lods.as<hv_mesh_lod>();
-        if (occl) {
-            cv.hiz_w = d.hiz->width;
-            cv.hiz_h = d.hiz->height;
-            cv.hiz_levels = d.hiz->levels;
-            for (uint32_t m = 1; m < d.hiz->levels && m < 12; ++m) cv.hiz_off[m] = (uint32_t)d.hiz->level_off[m];
-            cv.hiz0 = d.hiz->level0;
-            cv.hiz_packed = d.hiz->packed.as<float>();
-        }
+        HV_TRY(fill_view(cfg_, d, cv));
         cv.ids = d_ids_.as<uint32_t>() + so;
         cv.args = d_args_.as<hv_draw_indexed_indirect>() + ao;
         cv.totals = d_totals_.as<uint32_t>() + ao;
@@ -1371,6 +1386,52 @@ hv_status CullBatch::read(uint32_t v, uint32_t* ids, uint32_t ids_capacity, uint
     return HV_OK;
 }
 
+// ---------------------------------------------------------------------------
+// One view without allocation (hv_inst_cull_noalloc). The scratch of the
+// three kernels (the view table, survivor stage, per-wave counts, row stats,
+// row histograms, bucket totals) lives in one statically sized device array
+// per GPU; its address is looked up once per device at context creation.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) uint8_t scratch[HV_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[kScratchDevices] = {};
+
+// Writes the one-view table (header, view record, column-scan units, block ->
+// view array) into the scratch from kernel arguments, so the call needs no
+// host staging buffer and no copy.
+__global__ __launch_bounds__(256) void k_one_view_table(BatchHdr hdr, CullView view, uint32_t lg, uint32_t* units,
+                                                        uint32_t* block_view, uint32_t n_block_view,
+                                                        uint32_t* table) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t* h = reinterpret_cast<const uint32_t*>(&hdr);
+    const uint32_t* v = reinterpret_cast<const uint32_t*>(&view);
+    constexpr uint32_t HW = sizeof(BatchHdr) / 4, VW = sizeof(CullView) / 4;
+    for (uint32_t i = tid; i < HW + VW; i += 256u) table[i] = i < HW ? h[i] : v[i - HW];
+    // Unit 0 sums the stats; the others cover buckets [b0, b0 + 2^lg).
+    for (uint32_t u = tid; u < hdr.n_units; u += 256u)
+        units[u] = u == 0u ? (kStatsUnit << 10) : ((lg << 10) | (((u - 1u) << lg) << 14));
+    for (uint32_t b = tid; b < n_block_view; b += 256u) block_view[b] = 0u;
+}
+
+inline uint64_t align256(uint64_t x) { return (x + 255u) & ~255ull; }
+}  // namespace
+
+hv_status cull_scratch_init(int device) {
+    if (device < 0 || device >= kScratchDevices)
+        return set_error(HV_ELIMIT, "scratch: device %d beyond the %d supported", device, kScratchDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return HV_OK;
+    // The caller made `device` current: the symbol resolves to that GPU's copy.
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_error(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    g_scratch_ptr[device] = static_cast<uint8_t*>(p);
+    return HV_OK;
+}
+
 }  // namespace hv
 
 using namespace hv;
@@ -1434,6 +1495,105 @@ hv_status hv_inst_cull_read(hv_inst_cull_pass* pass, uint32_t* ids, uint32_t ids
                             hv_draw_indexed_indirect* args, uint32_t args_capacity, uint32_t stats[HV_STAT_COUNT]) {
     if (!pass || !pass->prepared) return set_error(HV_ESTATE, "hv_inst_cull_read: not prepared");
     return pass->batch->read(0, ids, ids_capacity, id_count, args, args_capacity, stats);
+}
+
+hv_status hv_inst_cull_noalloc(hv_ctx* ctx, const hv_cull_config* config, const hv_scene* scene,
+                               const hv_camera_uniforms* camera, uint32_t first, uint32_t count, const hv_hiz* hiz,
+                               uint32_t* ids, hv_draw_indexed_indirect* args, uint32_t* counters, void* stream) {
+    if (!ctx) return set_error(HV_EINVAL, "hv_inst_cull_noalloc: NULL context");
+    HV_TRY(check_config(config));
+    ViewDesc d;
+    d.scene = scene;
+    d.camera = camera;
+    d.first = first;
+    d.count = count;
+    d.hiz = hiz;
+    HV_TRY(check_view(*config, d, 0));
+    if (!args || !counters || (count && !ids)) return set_error(HV_EINVAL, "hv_inst_cull_noalloc: NULL output");
+    const CullShape shape = ctx->cull_shape;
+    if (!classify_fn(shape, false, false) || shape.chunk() > (1u << kEntryShift))
+        return set_error(HV_EINVAL, "hv_inst_cull_noalloc: unsupported launch shape wg=%u ipt=%u", shape.wg, shape.ipt);
+    HV_HIP(hipSetDevice(ctx->device));
+    // The address is normally looked up by hv_ctx_create; a failed lookup
+    // there is retried (and reported) here.
+    if (ctx->device >= kScratchDevices || !g_scratch_ptr[ctx->device]) HV_TRY(cull_scratch_init(ctx->device));
+    uint8_t* const base = g_scratch_ptr[ctx->device];
+    const bool occl = (config->flags & HV_CULL_OCCLUSION) != 0, aabb = scene->explicit_aabb;
+    const uint32_t chunk = shape.chunk(), nw = shape.wg / 64u;
+    const uint32_t nb = scene->mesh_count * scene->lod_count;
+    const uint32_t N = (uint32_t)(((uint64_t)count + chunk - 1u) / chunk);
+    const size_t lds = (size_t)nb * 4u;
+    ClassifyFn fn = classify_fn(shape, occl, aabb);
+    int occ1 = 0, occ3 = 0;
+    HV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ1, reinterpret_cast<const void*>(fn), (int)shape.wg, lds));
+    HV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ3, reinterpret_cast<const void*>(scatter_fn(shape)),
+                                                        (int)scatter_threads(shape), lds));
+    const uint32_t resident1 = std::max(1u, ctx->cu_count * (uint32_t)std::max(occ1, 1));
+    const uint32_t resident3 = std::max(1u, ctx->cu_count * (uint32_t)std::max(occ3, 1));
+    // One view: the G classify blocks each own a row (block_of(0) = 0,
+    // block_of(N - 1) = G - 1).
+    const uint32_t G = N ? std::min(resident1, N) : 0u;
+    const uint32_t R = G;
+    uint32_t lg = 8;
+    while (lg > 0 && (R + (kColscanThreads >> lg) - 1) / (kColscanThreads >> lg) > 8) --lg;
+    const uint32_t n_units = 1u + (nb + (1u << lg) - 1u) / (1u << lg);
+    // Scratch layout: table (header, view, units, block views), then the
+    // kernels' buffers, each on a 256-byte boundary.
+    const uint64_t units_off = sizeof(BatchHdr) + sizeof(CullView);
+    const uint64_t bview_off = units_off + (uint64_t)n_units * 4u;
+    const uint32_t n_bview = std::max(G, 1u);
+    const uint64_t stage_off = align256(bview_off + (uint64_t)n_bview * 4u);
+    const uint64_t sub_off = align256(stage_off + (uint64_t)N * chunk * 4u);
+    const uint64_t rstat_off = align256(sub_off + (uint64_t)N * nw * 4u);
+    const uint64_t hist_off = align256(rstat_off + (uint64_t)R * 32u);
+    const uint64_t tot_off = align256(hist_off + (uint64_t)R * nb * 4u);
+    const uint64_t need = tot_off + (uint64_t)nb * 4u;
+    if (need > HV_NOALLOC_SCRATCH_BYTES)
+        return set_error(HV_ELIMIT,
+                         "hv_inst_cull_noalloc: %u candidates x %u buckets need %llu scratch bytes, more than the %llu "
+                         "reserved",
+                         count, nb, (unsigned long long)need, (unsigned long long)HV_NOALLOC_SCRATCH_BYTES);
+    CullView cv{};
+    HV_TRY(fill_view(*config, d, cv));
+    cv.item_begin = 0;
+    cv.n_items = N;
+    cv.block_begin = 0;
+    cv.n_rows = R;
+    cv.ids = ids;
+    cv.args = args;
+    cv.counters = counters;
+    cv.row_stats = reinterpret_cast<uint32_t*>(base + rstat_off);
+    cv.block_hist = reinterpret_cast<uint32_t*>(base + hist_off);
+    cv.totals = reinterpret_cast<uint32_t*>(base + tot_off);
+    BatchHdr hdr{};
+    hdr.n_views = 1;
+    hdr.n_items = N;
+    hdr.n_rows = R;
+    hdr.n_units = n_units;
+    hdr.grid = G;
+    hdr.chunk = chunk;
+    hdr.sub = shape.ipt * 64u;
+    hdr.waves_log2 = nw == 1 ? 0u : nw == 2 ? 1u : nw == 4 ? 2u : 3u;
+    hdr.stage = reinterpret_cast<uint32_t*>(base + stage_off);
+    hdr.sub_cnt = reinterpret_cast<uint32_t*>(base + sub_off);
+    hdr.units = reinterpret_cast<const uint32_t*>(base + units_off);
+    hdr.block_view = reinterpret_cast<const uint32_t*>(base + bview_off);
+    const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    const BatchHdr* dhdr = reinterpret_cast<const BatchHdr*>(base);
+    const CullView* dviews = reinterpret_cast<const CullView*>(base + sizeof(BatchHdr));
+    hipLaunchKernelGGL(k_one_view_table, dim3(1), dim3(256), 0, s, hdr, cv, lg,
+                       reinterpret_cast<uint32_t*>(base + units_off), reinterpret_cast<uint32_t*>(base + bview_off),
+                       n_bview, reinterpret_cast<uint32_t*>(base));
+    HV_HIP(hipGetLastError());
+    hipLaunchKernelGGL(fn, dim3(std::max(G, 1u)), dim3(shape.wg), lds, s, dhdr, dviews);
+    HV_HIP(hipGetLastError());
+    hipLaunchKernelGGL(dev::k_colscan, dim3(std::max(1u, std::min(n_units, 8u * ctx->cu_count))),
+                       dim3(kColscanThreads), 0, s, dhdr, dviews);
+    HV_HIP(hipGetLastError());
+    hipLaunchKernelGGL(scatter_fn(shape), dim3(std::max(1u, std::min(G, resident3))), dim3(scatter_threads(shape)), lds,
+                       s, dhdr, dviews);
+    HV_HIP(hipGetLastError());
+    return HV_OK;
 }
 
 }  // extern "C"

@@ -89,6 +89,10 @@ struct HizShape {
     uint32_t tile = 64;  // mip-0 tile edge reduced by one workgroup: 32 or 64
 };
 
+// Looks up this GPU's copy of the no-allocation cull's static scratch array
+// (hv_inst_cull_noalloc) once per device; `device` must be current.
+hv_status cull_scratch_init(int device);
+
 }  // namespace hv
 
 struct hv_ctx {

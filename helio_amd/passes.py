@@ -294,6 +294,23 @@ class InstanceCullPass:
         return ids[: n.value].copy(), args, stats
 
 
+def inst_cull_noalloc(ctx: Context, scene: Scene, camera: np.ndarray, ids: "DeviceBuffer", args: "DeviceBuffer",
+                      counters: "DeviceBuffer", first: int = 0, count: int | None = None,
+                      hiz: HiZBuildPass | None = None, lod_distances=(25.0, 75.0, 200.0), occlusion: bool = False,
+                      reversed_z: bool = False, screen=(1920, 1080), stream: int | None = None) -> None:
+    """One view of :class:`InstanceCullPass` that allocates nothing (hv_inst_cull_noalloc):
+    enqueues the cull into the caller's device buffers ``ids`` (count u32),
+    ``args`` (mesh_count * lod_count records) and ``counters`` (9 u32: the 8
+    stats, then the visible count)."""
+    flags = (L.CULL_OCCLUSION if occlusion else 0) | (L.CULL_REVERSED_Z if reversed_z else 0)
+    cfg = cull_config(lod_distances, flags, screen)
+    cam = np.ascontiguousarray(camera, dtype=L.CAMERA).reshape(1)
+    if count is None:
+        count = scene.info()["instances"] - first
+    call("hv_inst_cull_noalloc", ctx.handle, C.byref(cfg), scene.handle, ptr(cam), first, count,
+         None if hiz is None else hiz.handle, ids.ptr, args.ptr, counters.ptr, stream)
+
+
 class BatchCull:
     """Many views in one launch sequence (or one hipGraph replay); each view's
     result equals the plain :class:`InstanceCullPass` result for that view."""

@@ -478,6 +478,29 @@ hv_status hv_inst_cull_read(hv_inst_cull_pass* pass, uint32_t* ids, uint32_t ids
                             uint32_t* id_count, hv_draw_indexed_indirect* args,
                             uint32_t args_capacity, uint32_t stats[HV_STAT_COUNT]);
 
+/* One view of the cull above (hv_inst_cull_prepare + hv_inst_cull_execute in
+ * one call) that allocates nothing: no hipMalloc, no hipFree, no host staging
+ * copy. No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation). Same checks, kernels and results as the pass (bit-exact
+ * ids per bucket, args, stats); the context's launch shape applies.
+ * Inputs: a resident scene, the camera (read during the call), an optional
+ * bound pyramid (HV_CULL_OCCLUSION). Outputs are the caller's device arrays:
+ * ids (count entries), args (mesh_count * lod_count), counters (9 u32:
+ * [0..7] stats, [8] visible ids). Only enqueues on `stream`.
+ * Scratch: one static device array of HV_NOALLOC_SCRATCH_BYTES per GPU the
+ * library runs on, held until the process ends, shared by every call on that
+ * GPU: calls ordered on one stream are safe; calls that overlap on one GPU
+ * (unordered streams) race. A call needs about 4 B per candidate + 4 B per
+ * (row, bucket) with one row per classify workgroup (at most the resident
+ * workgroups, e.g. 1,024), so 2^22 candidates at 1,024 buckets fit; a call
+ * that does not fit returns HV_ELIMIT with the bytes it needs. */
+#define HV_NOALLOC_SCRATCH_BYTES (32ull << 20)
+hv_status hv_inst_cull_noalloc(hv_ctx* ctx, const hv_cull_config* config, const hv_scene* scene,
+                               const hv_camera_uniforms* camera, uint32_t first, uint32_t count,
+                               const hv_hiz* hiz, uint32_t* ids, hv_draw_indexed_indirect* args,
+                               uint32_t* counters, void* stream);
+
 
 /* ------------------------------------------------------------------------- */
 /* Batched multi-view cull: many views (cameras, shadow cascades, cube faces) */
