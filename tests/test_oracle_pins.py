@@ -1,0 +1,631 @@
+"""Independent pins of the oracle where the GPU comparisons cannot help (both
+sides were written from one reading of the shaders): the occlusion geometry
+(occlusion_cull.wgsl:101-208), the sub-pixel rule (indirect_dispatch.wgsl:127-137),
+the distance LOD thresholds and bucket-major compaction layout of the
+per-instance cull, and the VG Hi-Z branch and LOD selection (vg_cull.wgsl:184-235,
+317-337).
+
+Every expectation here comes from outside the oracle: a hand-built camera
+(identity view, perspective_rh projection) and hand-built pyramids with known
+answers, exact arithmetic on chosen coordinates (Pythagorean distances, the
+square root of a rounded square), or a float64 restatement of the shader text
+that flags a disagreement only within a few ulps of a decision boundary. Each
+test is chosen so that one plausible misreading fails it: a flipped screen V
+axis, P00 in place of P11, `>` in place of `>=` at a LOD threshold, clip.w in
+place of the nearest view depth in the VG footprint.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from helio_amd import layouts as L
+from helio_amd import synth
+from tests.vg_util import identity_instance, oracle_vg, vg_uniforms, work_items_for
+
+BIAS = 1.0 / 65536.0
+F32 = np.float32
+
+
+# ---------------------------------------------------------------------------
+# A camera at the origin looking down -Z: view = identity, view_proj = proj.
+# ---------------------------------------------------------------------------
+def _proj_cols(aspect, fov_deg=60.0, near=0.1, far=1000.0, reversed_z=False) -> np.ndarray:
+    """perspective_rh (z in [0, 1]) as four columns; reversed Z swaps near and far."""
+    h = 1.0 / math.tan(math.radians(fov_deg) / 2.0)
+    w = h / aspect
+    zn, zf = (far, near) if reversed_z else (near, far)
+    r = zf / (zn - zf)
+    return np.array([[w, 0, 0, 0], [0, h, 0, 0], [0, 0, r, -1.0], [0, 0, r * zn, 0]], np.float64)
+
+
+def _camera(aspect, fov_deg=60.0, near=0.1, far=1000.0, reversed_z=False) -> np.ndarray:
+    cols = _proj_cols(aspect, fov_deg, near, far, reversed_z)
+    cam = np.zeros(1, L.CAMERA)
+    cam["view"] = np.eye(4, dtype=np.float32).reshape(-1)
+    cam["proj"] = cols.reshape(-1).astype(F32)
+    cam["view_proj"] = cols.reshape(-1).astype(F32)
+    cam["inv_view_proj"] = np.linalg.inv(cols.T).T.reshape(-1).astype(F32)
+    cam["position_near"] = [0.0, 0.0, 0.0, near]
+    cam["forward_far"] = [0.0, 0.0, -1.0, far]
+    return cam
+
+
+def _ndc_depth(t, near=0.1, far=1000.0, reversed_z=False) -> float:
+    """NDC depth of the point at view distance t on the axis (closed form)."""
+    zn, zf = (far, near) if reversed_z else (near, far)
+    r = zf / (zn - zf)
+    return r * (zn - t) / t
+
+
+def _vp64(cam) -> np.ndarray:
+    """view_proj as a row-major float64 matrix (clip = M @ [x, y, z, 1])."""
+    return cam["view_proj"].reshape(4, 4).astype(np.float64).T.copy()
+
+
+# ---------------------------------------------------------------------------
+# Occlusion through the oracle's per-draw pass (occlusion_cull.wgsl:146-208
+# with no frustum pass in front of it).
+# ---------------------------------------------------------------------------
+def _occluded(cam, screen, centers, radii, pyr) -> np.ndarray:
+    centers = np.asarray(centers, F32).reshape(-1, 3)
+    radii = np.asarray(radii, F32).reshape(-1)
+    n = len(radii)
+    inst = np.zeros(n, L.INSTANCE)
+    inst["bounds"][:, :3] = centers
+    inst["bounds"][:, 3] = radii
+    draws = np.zeros((n, 5), np.uint32)
+    draws[:, 3] = np.arange(n, dtype=np.uint32)  # first_instance
+    draws[:, 4] = 1
+    ind = np.zeros((n, 5), np.uint32)
+    ind[:, 1] = 1
+    stats = np.zeros(8, np.uint32)
+    camf = np.ascontiguousarray(cam).view(np.float32)
+    w, h = screen
+    oracle.lib().or_occlusion_cull_draws(camf.ctypes.data, w, h, oracle.lib().or_mip_levels(w, h),
+                                         inst.ctypes.data, draws.ctypes.data, n, C.addressof(pyr),
+                                         ind.ctypes.data, stats.ctypes.data)
+    occ = ind[:, 1] == 0
+    assert stats[4] == occ.sum()
+    return occ
+
+
+def _ndc_to_center(ndx, ndy, t, p00, p11):
+    """View-space point at view distance t whose projection is (ndx, ndy)."""
+    return [ndx * t / p00, ndy * t / p11, -t]
+
+
+@pytest.mark.parametrize("rev", [False, True])
+def test_occlusion_wall_over_the_top_half_pins_the_v_axis(rev):
+    """A wall 20 m away covers the TOP half of the screen (texture rows 0..127:
+    v = 0 is the top, ndc y = +1). Spheres behind it in the upper half are
+    occluded; their mirror images in the lower half, behind nothing, are not;
+    spheres in front of the wall are not. A wall over the LEFT half does the
+    same for the u axis (no flip)."""
+    W = H = 256
+    cam = _camera(1.0, reversed_z=rev)
+    p = float(cam["proj"][0, 5])
+    far_v, wall_v = (0.0, _ndc_depth(20.0, reversed_z=True)) if rev else (1.0, _ndc_depth(20.0))
+    top = np.full((H, W), far_v, F32)
+    top[: H // 2] = wall_v
+    left = np.full((H, W), far_v, F32)
+    left[:, : W // 2] = wall_v
+    mode = 1 if rev else 0
+    # ndc offset 0.25, 30 m away, radius 1: footprint 0.058 ndc, mip 4 (16 x 16 texels).
+    cases = [
+        (top, (0.0, +0.25), 30.0, True),
+        (top, (0.0, -0.25), 30.0, False),
+        (top, (0.0, +0.25), 15.0, False),
+        (top, (+0.3, +0.6), 40.0, True),
+        (top, (-0.6, -0.6), 40.0, False),
+        (left, (-0.25, 0.0), 30.0, True),
+        (left, (+0.25, 0.0), 30.0, False),
+        (left, (-0.25, 0.0), 15.0, False),
+    ]
+    for depth, (nx, ny), t, want in cases:
+        pyr = oracle.pyramid(depth, mode)
+        got = _occluded(cam, (W, H), _ndc_to_center(nx, ny, t, p, p), 1.0, pyr)
+        assert bool(got[0]) == want, (rev, nx, ny, t, want)
+
+
+@pytest.mark.parametrize("rev", [False, True])
+def test_occlusion_edges_fallbacks_and_skips(rev):
+    """Full-screen wall 20 m away: a sphere behind it that straddles a screen
+    edge is still tested (the taps clamp into [0, 1]) and occluded; one whose
+    footprint lies wholly outside the viewport is skipped; a camera inside the
+    sphere takes the near-plane depth (never occluded); a centre behind the
+    camera (clip.w <= 0) is skipped; a zero radius is skipped."""
+    W, H = 320, 180
+    aspect = W / H
+    cam = _camera(aspect, reversed_z=rev)
+    p00, p11 = float(cam["proj"][0, 0]), float(cam["proj"][0, 5])
+    wall = np.full((H, W), _ndc_depth(20.0, reversed_z=rev), F32)
+    pyr = oracle.pyramid(wall, 1 if rev else 0)
+    t, r = 60.0, 3.0
+    nr = max(r * p00 / t, r * p11 / t)
+    centers, radii, want = [], [], []
+    for ex, ey in [(-1, 0), (1, 0), (0, -1), (0, 1)]:
+        # Centre on the edge: half the footprint outside, still tested.
+        centers.append(_ndc_to_center(ex * 1.0, ey * 1.0, t, p00, p11))
+        radii.append(r)
+        want.append(True)
+        # Footprint wholly outside by a margin: skipped.
+        centers.append(_ndc_to_center(ex * (1.0 + 1.5 * nr), ey * (1.0 + 1.5 * nr), t, p00, p11))
+        radii.append(r)
+        want.append(False)
+    centers.append([0.0, 0.0, -1.0]); radii.append(2.0); want.append(False)    # camera inside
+    centers.append([0.0, 0.0, 5.0]); radii.append(10.0); want.append(False)   # behind the camera
+    centers.append([0.0, 0.0, -60.0]); radii.append(0.0); want.append(False)  # no bounds
+    centers.append([0.0, 0.0, -60.0]); radii.append(3.0); want.append(True)   # plain behind-the-wall
+    got = _occluded(cam, (W, H), centers, radii, pyr)
+    assert got.tolist() == want
+
+
+@pytest.mark.parametrize("rev", [False, True])
+def test_occlusion_compare_is_strict_after_the_bias(rev):
+    """occlusion_cull.wgsl:204-208: occluded only when near_z > hiz + 2^-16.
+    A sphere beyond the far plane has its near depth clamped to exactly 1.0
+    (0.0 in reversed Z); against a wall at 1 - 2^-16 (2^-16) the biased depth
+    equals it exactly, so the strict compare keeps it; a wall one more 2^-16
+    nearer occludes it."""
+    W, H = 64, 64
+    cam = _camera(1.0, reversed_z=rev)
+    c, r = [0.0, 0.0, -1100.0], 10.0
+    assert (_ndc_depth(1090.0, reversed_z=rev) > 1.0) if not rev else (_ndc_depth(1090.0, reversed_z=rev) < 0.0)
+    for k, want in [(1, False), (2, True)]:
+        wall = F32(k * BIAS) if rev else F32(1.0 - k * BIAS)
+        assert (wall - F32(BIAS) == F32(0.0)) if (rev and k == 1) else True
+        assert (wall + F32(BIAS) == F32(1.0)) if (not rev and k == 1) else True
+        pyr = oracle.pyramid(np.full((H, W), wall, F32), 1 if rev else 0)
+        assert bool(_occluded(cam, (W, H), c, r, pyr)[0]) == want, (rev, k)
+
+
+def _pyramid64(level0: np.ndarray, mode: int, levels: int) -> list:
+    """hiz_build.wgsl:12-23 in float64: parent = max (min) of the 2x2 taps
+    with min(2x+1, w-1) clamps (the odd trailing row/column is dropped)."""
+    out = [level0.astype(np.float64)]
+    red = np.minimum if mode else np.maximum
+    for _ in range(1, levels):
+        s = out[-1]
+        sh, sw = s.shape
+        dh, dw = max(1, sh // 2), max(1, sw // 2)
+        x0 = 2 * np.arange(dw)
+        y0 = 2 * np.arange(dh)
+        x1 = np.minimum(x0 + 1, sw - 1)
+        y1 = np.minimum(y0 + 1, sh - 1)
+        out.append(red(red(s[np.ix_(y0, x0)], s[np.ix_(y0, x1)]), red(s[np.ix_(y1, x0)], s[np.ix_(y1, x1)])))
+    return out
+
+
+def _near_int(x, eps):
+    return np.abs(x - np.round(x)) < eps
+
+
+def _occlusion64(cam, screen, c, r, levels, mode):
+    """occlusion_cull.wgsl:158-208 in float64, vectorised. Returns (occluded,
+    near_boundary): near_boundary marks spheres within a few ulps of a decision
+    (the viewport skip, the camera-inside test, a mip step, a texel edge of a
+    tap, or the depth compare), where float32 may decide either way."""
+    W, H = screen
+    M = _vp64(cam)
+    p00, p11 = float(cam["proj"][0, 0]), float(cam["proj"][0, 5])
+    cp = cam["position_near"][0, :3].astype(np.float64)
+    rev = mode == 1
+    c = c.astype(np.float64)
+    r = r.astype(np.float64)
+    n = len(r)
+    hom = np.concatenate([c, np.ones((n, 1))], axis=1)
+    clip = hom @ M.T
+    w = clip[:, 3]
+    live = (r > 0) & (w > 0)
+    ws = np.where(live, w, 1.0)
+    ndx, ndy = clip[:, 0] / ws, clip[:, 1] / ws
+    u, v = ndx * 0.5 + 0.5, ndy * -0.5 + 0.5
+    ndc_r = np.maximum(np.abs(r * p00 / ws), np.abs(r * p11 / ws))
+    edges = np.stack([ndx + ndc_r + 1.0, ndx - ndc_r - 1.0, ndy + ndc_r + 1.0, ndy - ndc_r - 1.0])
+    outside = (edges[0] < 0) | (edges[1] > 0) | (edges[2] < 0) | (edges[3] > 0)
+    boundary = live & (np.abs(edges) < 1e-5).any(axis=0)
+    live &= ~outside
+    tc = c - cp
+    dist_sq = (tc * tc).sum(axis=1)
+    inside = dist_sq <= r * r
+    boundary |= live & (np.abs(dist_sq - r * r) <= 1e-5 * r * r)
+    dirn = tc / np.sqrt(np.where(inside, 1.0, dist_sq))[:, None]
+    near = c - dirn * r[:, None]
+    nclip = np.concatenate([near, np.ones((n, 1))], axis=1) @ M.T
+    fallback = 1.0 if rev else 0.0
+    nw = np.where(nclip[:, 3] > 0, nclip[:, 3], 1.0)
+    near_z = np.where(inside | (nclip[:, 3] <= 0), fallback, np.clip(nclip[:, 2] / nw, 0.0, 1.0))
+    r_px = np.abs(r / ws * p11 * (H * 0.5))
+    diam = np.maximum(r_px * 2.0, 1.0)
+    lg = np.log2(diam)
+    boundary |= live & (diam > 1.0) & _near_int(lg, 1e-5)
+    mip = np.minimum(np.ceil(lg - 1e-12).astype(np.int64), len(levels) - 1)
+    mip = np.minimum(mip, oracle.lib().or_mip_levels(W, H) - 1)
+    uh = ndc_r * 0.5
+    us = [np.clip(u - uh, 0, 1), np.clip(u + uh, 0, 1)]
+    vs = [np.clip(v - uh, 0, 1), np.clip(v + uh, 0, 1)]
+    hiz = np.full(n, -np.inf if not rev else np.inf)
+    for m in np.unique(mip[live]):
+        sel = live & (mip == m)
+        lvl = levels[m]
+        hm, wm = lvl.shape
+        for uu in us:
+            for vv in vs:
+                fx, fy = uu[sel] * wm, vv[sel] * hm
+                boundary[np.where(sel)[0]] |= ((_near_int(fx, 1e-3) & (fx > 0) & (fx < wm)) |
+                                               (_near_int(fy, 1e-3) & (fy > 0) & (fy < hm)))
+                x = np.clip(np.floor(fx), 0, wm - 1).astype(np.int64)
+                y = np.clip(np.floor(fy), 0, hm - 1).astype(np.int64)
+                t = lvl[y, x]
+                hiz[sel] = np.minimum(hiz[sel], t) if rev else np.maximum(hiz[sel], t)
+    if rev:
+        occ = live & (near_z < hiz - BIAS)
+        boundary |= live & (np.abs(near_z - (hiz - BIAS)) < 1e-5)
+    else:
+        occ = live & (near_z > hiz + BIAS)
+        boundary |= live & (np.abs(near_z - (hiz + BIAS)) < 1e-5)
+    return occ, boundary
+
+
+@pytest.mark.parametrize("rev", [False, True])
+def test_occlusion_float64_brute_force(rev):
+    """100,000 random spheres (in and around the view, 1-400 m, radii 0.02-30
+    m) against the pyramid of 80 random occluder rectangles: the oracle agrees
+    with a float64 restatement of occlusion_cull.wgsl:158-208 on every sphere
+    not within a few ulps of a decision boundary."""
+    W, H = 320, 180
+    cam = _camera(W / H, reversed_z=rev)
+    p00, p11 = float(cam["proj"][0, 0]), float(cam["proj"][0, 5])
+    g = np.random.default_rng(77 + rev)
+    depth = np.full((H, W), 0.0 if rev else 1.0, F32)
+    for _ in range(80):
+        x0, y0 = g.integers(0, W), g.integers(0, H)
+        x1, y1 = x0 + g.integers(4, W // 2), y0 + g.integers(4, H // 2)
+        d = F32(_ndc_depth(float(np.exp(g.uniform(math.log(2.0), math.log(300.0)))), reversed_z=rev))
+        blk = depth[y0:y1, x0:x1]
+        depth[y0:y1, x0:x1] = np.maximum(blk, d) if rev else np.minimum(blk, d)
+    mode = 1 if rev else 0
+    pyr = oracle.pyramid(depth, mode)
+    levels = _pyramid64(depth, mode, pyr.levels)
+    n = 100_000
+    t = np.exp(g.uniform(math.log(1.0), math.log(400.0), n))
+    nx, ny = g.uniform(-1.3, 1.3, n), g.uniform(-1.3, 1.3, n)
+    c = np.stack([nx * t / p00, ny * t / p11, -t], axis=1).astype(F32)
+    r = np.exp(g.uniform(math.log(0.02), math.log(30.0), n)).astype(F32)
+    got = _occluded(cam, (W, H), c, r, pyr)
+    want, boundary = _occlusion64(cam, (W, H), c, r, levels, mode)
+    bad = (got != want) & ~boundary
+    assert not bad.any(), np.where(bad)[0][:10]
+    assert (got != want).sum() <= 50
+    assert 0.1 * n < want.sum() < 0.9 * n  # both outcomes well represented
+
+
+# ---------------------------------------------------------------------------
+# Sub-pixel rule, LOD thresholds and compaction through the per-instance cull.
+# ---------------------------------------------------------------------------
+MESH_COUNT = 8
+LODS = (25.0, 75.0, 200.0)
+
+
+def _mesh_lods(mesh_count=MESH_COUNT, lod_count=4, seed=3) -> np.ndarray:
+    g = np.random.default_rng(seed)
+    t = np.zeros(mesh_count * lod_count, L.MESH_LOD)
+    t["index_count"] = g.integers(3, 30000, len(t))
+    t["first_index"] = g.integers(0, 1 << 24, len(t))
+    t["base_vertex"] = g.integers(-1000, 1 << 20, len(t))
+    return t
+
+
+def _classify(cam, centers, radii, mesh_ids=None, flags=None):
+    n = len(radii)
+    b = np.zeros((n, 4), F32)
+    b[:, :3] = centers
+    b[:, 3] = radii
+    meta = np.zeros(n, np.uint32) if mesh_ids is None else np.asarray(mesh_ids, np.uint32)
+    if flags is not None:
+        meta = meta | (np.asarray(flags, np.uint32) << 31)
+    ov = oracle.view_from_camera(cam, lod_distances=LODS, mesh_lods=_mesh_lods(), mesh_count=MESH_COUNT)
+    return oracle.classify(ov, b, meta, 0, n)
+
+
+def test_subpixel_boundary_uses_p11_and_is_strict():
+    """indirect_dispatch.wgsl:127-137: culled when |r * proj[1][1] / clip.w| <
+    0.001. With a 16:9 camera P00 = P11 * 9/16, so the radius at which the rule
+    flips tells P11 from P00. On the axis clip.w is the distance exactly; the
+    flip radius r* (the least binary32 r with fl(fl(r * P11) / w) >= 0.001)
+    is within a few ulps of the closed form 0.001 w / P11, r* is drawn and the
+    next float below is sub-pixel."""
+    cam = _camera(16 / 9)
+    p11 = F32(cam["proj"][0, 5])
+    for w in (F32(50.0), F32(100.0), F32(700.0)):
+        r = F32(0.001 * float(w) / float(p11))
+        r = np.nextafter(r, F32(0), dtype=F32) if (r * p11) / w >= F32(0.001) else r
+        while (r * p11) / w < F32(0.001):
+            r = np.nextafter(r, F32(np.inf), dtype=F32)
+        assert abs(float(r) - 0.001 * float(w) / float(p11)) <= 4 * float(np.spacing(r))
+        below = np.nextafter(r, F32(0), dtype=F32)
+        code, stats = _classify(cam, [[0, 0, -w], [0, 0, -w]], [r, below])
+        assert code[0] != 0xFFFFFFFF and code[1] == 0xFFFFFFFF, (w, r)
+        assert stats.tolist()[:4] == [2, 0, 1, 1]
+        # The same pair one P00-equivalent apart: a radius drawn only under P00 is sub-pixel.
+        r00 = F32(0.0011 * float(w) / float(cam["proj"][0, 0]))  # r00 * P11 / w = 0.0011 * 16/9 > 0.001
+        r_sub = F32(0.0009 * float(w) / float(p11))
+        code, _ = _classify(cam, [[0, 0, -w], [0, 0, -w]], [r00, r_sub])
+        assert code[0] != 0xFFFFFFFF and code[1] == 0xFFFFFFFF
+
+
+def test_lod_thresholds_exact_distances():
+    """LOD = #{t : d >= t} with d = sqrt(dx^2 + dy^2 + dz^2). Integer
+    Pythagorean offsets give d = 25, 75, 200 exactly; on the axis
+    sqrt(fl(z * z)) = |z| (a correctly rounded square root of a correctly
+    rounded square), so z = -nextbelow(t) lands one ulp inside the threshold
+    and z = -nextabove(t) one ulp outside."""
+    cam = _camera(16 / 9)
+    pts, want = [], []
+    for (x, z), lod in [((15, -20), 1), ((45, -60), 2), ((120, -160), 3)]:
+        pts.append([x, 0, z]); want.append(lod)
+    for t, lod in zip(LODS, (1, 2, 3)):
+        t32 = F32(t)
+        pts.append([0, 0, -t32]); want.append(lod)
+        pts.append([0, 0, -np.nextafter(t32, F32(0), dtype=F32)]); want.append(lod - 1)
+        pts.append([0, 0, -np.nextafter(t32, F32(np.inf), dtype=F32)]); want.append(lod)
+    pts.append([0, 0, -10.0]); want.append(0)
+    mesh = np.arange(len(pts)) % MESH_COUNT
+    code, stats = _classify(cam, np.array(pts, F32), np.ones(len(pts), F32), mesh)
+    assert stats[3] == len(pts)
+    assert (code % 4).tolist() == want
+    assert (code // 4).tolist() == mesh.tolist()
+
+
+def _scene(n, seed):
+    g = np.random.default_rng(seed)
+    t = np.exp(g.uniform(math.log(0.5), math.log(600.0), n))
+    nx, ny = g.uniform(-1.2, 1.2, n), g.uniform(-1.2, 1.2, n)
+    c = np.stack([nx * t * math.tan(math.radians(30)) * 16 / 9, ny * t * math.tan(math.radians(30)), -t], 1)
+    # A shell of instances near each LOD threshold.
+    k = n // 8
+    tt = np.repeat(np.array(LODS), k // 3 + 1)[:k] * (1 + g.uniform(-1e-3, 1e-3, k))
+    c[:k, 2] = -tt
+    c[:k, :2] *= 0.01
+    r = np.exp(g.uniform(math.log(0.001), math.log(10.0), n))
+    mesh = g.integers(0, MESH_COUNT, n)
+    flags = g.integers(0, 2, n)
+    return c.astype(F32), r.astype(F32), mesh.astype(np.uint32), flags.astype(np.uint32)
+
+
+def _visible64(cam, c, r):
+    """indirect_dispatch.wgsl:79-137 in float64 over sphere_to_aabb boxes (c -+ r
+    rounded to binary32, as the scene builds them): p-vertex AABB frustum test
+    against the Gribb-Hartmann planes of view_proj, then the sub-pixel rule.
+    Returns (frustum, drawn, near_boundary)."""
+    M = _vp64(cam)
+    rows = [M[3] + M[0], M[3] - M[0], M[3] + M[1], M[3] - M[1], M[2], M[3] - M[2]]
+    mn = (c - r[:, None]).astype(F32).astype(np.float64)
+    mx = (c + r[:, None]).astype(F32).astype(np.float64)
+    fr = np.ones(len(r), bool)
+    boundary = np.zeros(len(r), bool)
+    for p in rows:
+        p = p / np.linalg.norm(p[:3])
+        pv = np.where(p[:3] >= 0, mx, mn)
+        d = pv @ p[:3] + p[3]
+        fr &= d >= 0
+        boundary |= np.abs(d) <= 1e-5 * (np.abs(pv) @ np.abs(p[:3]) + abs(p[3]) + 1.0)
+    w = np.concatenate([c.astype(np.float64), np.ones((len(r), 1))], 1) @ M[3]
+    q = np.abs(r.astype(np.float64) * float(cam["proj"][0, 5]) / np.where(w > 0, w, 1.0))
+    sub = (w > 0) & (q < 0.001)
+    boundary |= (w > 0) & (np.abs(q - 0.001) <= 1e-6)
+    return fr, fr & ~sub, boundary
+
+
+def _lod64(c, cam_pos=(0.0, 0.0, 0.0)):
+    d = np.sqrt(((c.astype(np.float64) - np.asarray(cam_pos)) ** 2).sum(1))
+    lod = sum((d >= t).astype(np.int64) for t in LODS)
+    near = np.zeros(len(d), bool)
+    for t in LODS:
+        near |= np.abs(d - t) <= 4e-7 * t
+    return lod, near
+
+
+def test_classify_float64_brute_force():
+    """200,000 instances in and around a 16:9 view, a shell of them within
+    0.1 % of each LOD threshold: frustum, sub-pixel and LOD decisions of the
+    oracle equal a float64 restatement except within a few ulps of a boundary;
+    stats[0] = stats[1] + stats[2] + stats[3] and the shadow slots count flags."""
+    cam = _camera(16 / 9)
+    c, r, mesh, flags = _scene(200_000, 11)
+    code, stats = _classify(cam, c, r, mesh, flags)
+    fr, drawn, b1 = _visible64(cam, c, r)
+    lod, b2 = _lod64(c)
+    got_drawn = code != 0xFFFFFFFF
+    bad = (got_drawn != drawn) & ~b1
+    assert not bad.any(), np.where(bad)[0][:10]
+    both = got_drawn & drawn & ~b2
+    assert np.array_equal((code[both] % 4).astype(np.int64), lod[both])
+    assert np.array_equal(code[both] // 4, mesh[both])
+    assert (b2 & got_drawn).sum() > 0  # the threshold shell reaches the ulp boundary
+    n = len(r)
+    assert stats[0] == n == stats[1] + stats[2] + stats[3]
+    assert abs(int(stats[1]) - int((~fr).sum())) <= b1.sum()
+    assert stats[3] == got_drawn.sum()
+    assert stats[5] == flags.sum() and stats[6] == (flags.astype(bool) & got_drawn).sum()
+    assert stats[4] == stats[7] == 0
+
+
+def test_compaction_layout_is_bucket_major():
+    """The compacted output of the per-instance cull, checked against its
+    definition: args[b] = {index_count, count_b, first_index, base_vertex,
+    first_instance} from the (mesh, LOD) table with first_instance the
+    exclusive prefix of the counts; the ids of bucket b are exactly the drawn
+    candidates whose mesh * 4 + LOD (LOD by float64 distance away from the
+    thresholds) is b, in ascending order; sum of counts = ids = stats[3]."""
+    cam = _camera(16 / 9)
+    c, r, mesh, flags = _scene(60_000, 12)
+    first, count = 7, 59_000
+    n = len(r)
+    b4 = np.zeros((n, 4), F32)
+    b4[:, :3], b4[:, 3] = c, r
+    meta = mesh | (flags << 31)
+    table = _mesh_lods()
+    ov = oracle.view_from_camera(cam, lod_distances=LODS, mesh_lods=table, mesh_count=MESH_COUNT)
+    ids, args, stats = oracle.cull_view(ov, b4, meta, first, count)
+    nb = MESH_COUNT * 4
+    assert args.shape == (nb, 5)
+    assert args[:, 1].sum() == len(ids) == stats[3] - stats[4]
+    assert np.array_equal(args[:, 4], np.concatenate([[0], np.cumsum(args[:, 1])[:-1]]))
+    assert np.array_equal(args[:, 0], table["index_count"])
+    assert np.array_equal(args[:, 2], table["first_index"])
+    assert np.array_equal(args[:, 3].view(np.int32), table["base_vertex"])
+    assert ((ids >= first) & (ids < first + count)).all()
+    _, drawn, b1 = _visible64(cam, c, r)
+    lod, b2 = _lod64(c)
+    cand = np.zeros(n, bool)
+    cand[first:first + count] = True
+    sure = cand & ~b1 & ~b2
+    in_ids = np.zeros(n, bool)
+    in_ids[ids] = True
+    assert np.array_equal(in_ids[sure], drawn[sure])
+    bucket = mesh.astype(np.int64) * 4 + lod
+    for b in range(nb):
+        seg = ids[args[b, 4]: args[b, 4] + args[b, 1]]
+        assert np.all(np.diff(seg.astype(np.int64)) > 0)
+        s = seg[sure[seg]]
+        assert np.all(bucket[s] == b), b
+        want = np.where(sure & drawn & (bucket == b))[0]
+        assert np.array_equal(s, want), b
+
+
+# ---------------------------------------------------------------------------
+# Virtual geometry: the Hi-Z branch (vg_cull.wgsl:184-235) and the LOD select
+# (vg_cull.wgsl:317-337), one object and hand-built pyramids.
+# ---------------------------------------------------------------------------
+VW, VH = 1920, 1080
+
+
+def _vg_one(cam, center, radius, pyr, lod_errors=None, lod_count=1, local_radius=None, reversed_z=False):
+    """One object on an identity instance; its LOD l holds one meshlet at
+    `center` (cone disabled). Returns (counters, selected_lod_plus_one, lod of
+    the published meshlet or None)."""
+    m = np.zeros(lod_count, L.MESHLET)
+    m["center"] = center
+    m["radius"] = radius
+    m["cone_apex"] = center
+    m["cone_cutoff"] = 2.0
+    m["cone_axis"] = [0.0, 0.0, 1.0]
+    m["index_count"] = 3
+    o = np.zeros(1, L.VG_OBJECT)
+    o["lod_count"] = lod_count
+    o["max_meshlet_count"] = 1
+    o["local_bounds"] = [*center, radius if local_radius is None else local_radius]
+    if lod_errors is not None:
+        o["lod_errors"][0, : len(lod_errors)] = lod_errors
+    o["lod_first_meshlets"][0, :lod_count] = np.arange(lod_count)
+    o["lod_meshlet_counts"][0, :lod_count] = 1
+    inst = identity_instance()
+    icd = np.zeros(1, L.INSTANCE_CULL)
+    icd[0] = (1.0, 1.0, 0, 1)
+    wi = work_items_for(o)
+    u = vg_uniforms(1, len(wi), 16, screen=(VW, VH))
+    _, md, ctr, objs = oracle_vg(cam, u, m, o, inst, icd, wi, pyr, 16, reversed_z=reversed_z)
+    return ctr, int(objs["selected_lod_plus_one"][0]), (int(md[0, 2]) if ctr[0] else None)
+
+
+def _pyramid_with_levels(values_by_level: dict, base: float, mode: int = 0):
+    """A 1920 x 1080 pyramid whose level m is filled with values_by_level.get(m, base)."""
+    lib = oracle.lib()
+    levels = min(lib.or_mip_levels(VW, VH), 12)
+    l0 = np.full((VH, VW), values_by_level.get(0, base), F32)
+    packed = np.zeros(int(lib.or_hiz_packed_floats(VW, VH)), F32)
+    off = 0
+    for m in range(1, levels):
+        sz = max(1, VW >> m) * max(1, VH >> m)
+        packed[off: off + sz] = values_by_level.get(m, base)
+        off += sz
+    return oracle.pyramid(l0, mode, packed)
+
+
+def test_vg_hiz_footprint_uses_the_nearest_view_depth():
+    """vg_cull.wgsl:190-219: ndc_r = max(|r P00|, |r P11|) / (clip.w - r) and
+    mip = ceil(log2(max(ndc_r * H, 1))). A sphere of radius 1 with its centre
+    7.667 m down the axis (r / (w - r) = 0.15) covers 1080 * 0.15 * P11 =
+    280.6 px: mip 9. Dividing by clip.w instead gives 244 px (mip 8); taking
+    P00 gives 158 px (mip 8). With only level 9 occluding the meshlet is
+    rejected; with only level 8 occluding it is published."""
+    cam = _camera(VW / VH)
+    p11 = float(cam["proj"][0, 5])
+    r, w = 1.0, 1.0 + 1.0 / 0.15
+    assert math.ceil(math.log2(r / (w - r) * p11 * VH)) == 9
+    assert math.ceil(math.log2(r / w * p11 * VH)) == 8
+    wall = _ndc_depth(3.0)
+    ctr, _, _ = _vg_one(cam, [0.0, 0.0, -w], r, _pyramid_with_levels({9: wall}, 1.0))
+    assert ctr[0] == 0
+    ctr, _, _ = _vg_one(cam, [0.0, 0.0, -w], r, _pyramid_with_levels({8: wall}, 1.0))
+    assert ctr[0] == 1
+
+
+@pytest.mark.parametrize("rev", [False, True])
+def test_vg_hiz_rect_touching_an_edge_is_not_tested(rev):
+    """vg_cull.wgsl:206: the Hi-Z test runs only when the whole uv rect is on
+    screen. Behind a wall that covers every level, a sphere whose rect pokes
+    0.5 % of the screen past an edge is published; moved 0.5 % inward it is
+    rejected. All four edges (v = 0 is the top)."""
+    cam = _camera(VW / VH, reversed_z=rev)
+    p00, p11 = float(cam["proj"][0, 0]), float(cam["proj"][0, 5])
+    pyr = _pyramid_with_levels({}, _ndc_depth(3.0, reversed_z=rev), mode=1 if rev else 0)
+    w, r = 20.0, 1.0
+    uvr = max(r * p00, r * p11) / (w - r) * 0.5
+    for axis, side in [(0, -1), (0, 1), (1, -1), (1, 1)]:
+        for poke, want in [(0.005, 1), (-0.005, 0)]:
+            uv = uvr - poke if side < 0 else 1.0 - uvr + poke  # rect edge past (poke > 0) or inside
+            nd = uv * 2.0 - 1.0 if axis == 0 else (0.5 - uv) * 2.0
+            c = [nd * w / p00, 0.0, -w] if axis == 0 else [0.0, nd * w / p11, -w]
+            ctr, _, _ = _vg_one(cam, c, r, pyr, reversed_z=rev)
+            assert ctr[0] == want, (axis, side, poke)
+
+
+def test_vg_select_lod_distance_and_focal():
+    """vg_cull.wgsl:317-337: closest = max(dist - r_ws, max(near, 1e-4)), focal
+    = |P11| H / 2, level l passes while lod_errors[l] * max_scale * focal /
+    closest <= threshold (0.5 px), stopping at the first failure. An object of
+    radius 10 at 40 m (closest 30 m) with projected errors 0.3 / 0.45 / 0.52 /
+    0.1 px selects level 2 (level 3 fails, so level 4 is never reached);
+    measured from the centre (40 m) level 3 would pass. A camera inside the
+    bounds clamps to the near plane: nothing but level 0 passes."""
+    cam = _camera(VW / VH)
+    focal = abs(float(cam["proj"][0, 5])) * VH * 0.5
+    closest = 30.0
+    errs = [0.0] + [px * closest / focal for px in (0.3, 0.45, 0.52, 0.1)]
+    assert 0.52 * closest / 40.0 < 0.5
+    ctr, sel, lod = _vg_one(cam, [0.0, 0.0, -40.0], 0.5, None, lod_errors=errs, lod_count=5, local_radius=10.0)
+    assert sel == 3 and lod == 2
+    assert ctr[2 + 2] == 1 and ctr[10] == 4
+    errs_near = [0.0] + [1e-9] * 4
+    ctr, sel, lod = _vg_one(cam, [0.0, 0.0, -5.0], 0.5, None, lod_errors=errs_near, lod_count=5, local_radius=10.0)
+    # closest = max(5 - 10, 0.1) = 0.1: 1e-9 * focal / 0.1 = 9.4e-6 px passes every level.
+    assert sel == 5 and lod == 4
+
+
+def test_vg_benchmark_invariants_through_the_oracle():
+    """meshlet_cull_benchmark.rs:284-294,344-368 run through the oracle itself:
+    every one of the 1,048,576 meshlets is attempted ([TOTAL, 0]), and a
+    capacity of TOTAL - 17 reports [TOTAL, 17] (two counters, 1x1 far Hi-Z,
+    hiz_mip_count 1, threshold 2 px)."""
+    total = 1_048_576
+    for objects, cap, want in [(4096, total, [total, 0]), (1, total, [total, 0]), (4096, total - 17, [total, 17])]:
+        cam, sc = synth.benchmark_vg_case(objects, total // objects)
+        icd = np.zeros(len(sc.instances), L.INSTANCE_CULL)
+        icd["max_scale"] = icd["min_scale"] = 1.0
+        icd["cone_cull_enabled"] = icd["valid_transform"] = 1
+        wi = work_items_for(sc.objects)
+        u = vg_uniforms(objects, len(wi), cap, screen=(VW, VH), hiz_mip_count=1, threshold=2.0)
+        pyr = oracle.pyramid(np.ones((1, 1), F32), 0)
+        ind, md, ctr, _ = oracle_vg(cam, u, sc.meshlets, sc.objects, sc.instances, icd, wi, pyr, total,
+                                    counter_len=2)
+        assert ctr.tolist() == want
+        n = min(cap, total)
+        assert np.array_equal(ind[:n, 4], np.arange(n))  # first_instance = slot
+        assert (ind[:n, 1] == 1).all() and (ind[:n, 0] == 3).all()
