@@ -415,6 +415,29 @@ def vg_leg(ctx, steps: int) -> dict:
     return out
 
 
+def side_only(ctx, args) -> int:
+    """Counter runs of the side legs: a device-to-device copy of known size
+    (the scale of the HBM byte counters), then each leg of --legs."""
+    from helio_amd.passes import DeviceBuffer
+
+    src, dst = DeviceBuffer(ctx, CALIBRATION_COPY_BYTES), DeviceBuffer(ctx, CALIBRATION_COPY_BYTES)
+    dst.copy_from(src)
+    src.close()
+    dst.close()
+    legs = [x.strip() for x in args.legs.split(",") if x.strip()]
+    out = {"mode": "side-only", "steps": args.steps, "calibration_copy_bytes": CALIBRATION_COPY_BYTES}
+    fns = {"occlusion": occlusion_leg, "vg": vg_leg, "shadow": shadow_leg}
+    for leg in legs:
+        log(f"{leg} leg")
+        if leg == "rebuild":
+            out[leg] = rebuild_leg(ctx, args.steps, rebuild_objects())
+        else:
+            out[leg] = fns[leg](ctx, args.steps)
+        log(f"{leg}: {out[leg]}")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -434,6 +457,8 @@ def main() -> int:
                     help="classify launch shape, e.g. 512x4 (default: autotuned on the batch)")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched leg and one calibration copy (for counter runs)")
+    ap.add_argument("--side-only", action="store_true",
+                    help="only one calibration copy and the --legs side legs, --steps each (for counter runs)")
     args = ap.parse_args()
     if args.batched_only:
         args.no_serving = args.no_extra = args.no_cpu_baseline = True
@@ -450,6 +475,8 @@ def main() -> int:
     ctx = Context(0)
     cu, hbm = ctx.device_info()
     log(f"device: {cu} CUs, {hbm / 2**30:.0f} GiB")
+    if args.side_only:
+        return side_only(ctx, args)
     sc = synth.instance_scene(args.instances, config=3)
     scene = Scene(ctx, sc.n)
     scene.upload_soa(sc.bounds, sc.mesh_ids, sc.flags)

@@ -68,6 +68,41 @@ def test_camera_inside_cloud_many_visible(ctx):
     assert (ref[1].reshape(-1, 4, 5)[:, :, 1].sum(0) > 0).all()
 
 
+def test_lod_thresholds_exact_and_within_ulps(ctx):
+    """The kernel picks the LOD without a square root (d^2 >= the least float
+    whose square root reaches the threshold); the oracle takes sqrt(d^2) >= t.
+    Instances placed exactly 25 / 75 / 200 m from an off-origin camera
+    (integer Pythagorean offsets) and ~60,000 more within +-16 ulps of each
+    threshold in random directions (so the rounded squared distance lands on,
+    just below and just above each boundary) give bit-identical buckets."""
+    cam_pos = np.array([100.25, 50.5, 300.125], np.float32)
+    view = synth.perspective_view(synth.rng(0, 401), position=cam_pos.astype(np.float64), direction=(0, 0, -1))
+    g = np.random.default_rng(402)
+    pts = [cam_pos + np.array(o, np.float32) for o in
+           [(15, 0, -20), (-15, 0, -20), (0, 0, -25), (45, 0, -60), (0, 12, -75), (120, 0, -160), (0, 0, -200)]]
+    per = 20_000
+    for t in (25.0, 75.0, 200.0):
+        k = g.integers(-16, 17, per)
+        dist = t * (1.0 + k * 2.0 ** -24)
+        d = np.stack([g.uniform(-0.5, 0.5, per), g.uniform(-0.3, 0.3, per), -np.ones(per)], 1)
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        pts.extend((cam_pos + d * dist[:, None]).astype(np.float32))
+    p = np.array(pts, np.float32)
+    sc = synth.instance_scene(len(p), config=0, index=41)
+    sc.bounds[:, :3] = p
+    sc.bounds[:, 3] = 0.5
+    # The distance exactly as the shader rounds it: the boundary must be hit.
+    dx = (p - cam_pos).astype(np.float32)
+    d32 = np.sqrt((dx[:, 0] * dx[:, 0] + dx[:, 1] * dx[:, 1]) + dx[:, 2] * dx[:, 2])
+    for t in (25.0, 75.0, 200.0):
+        t32 = np.float32(t)
+        assert (d32 == t32).sum() > 100 and (d32 < t32).sum() > 1000 and (d32 > t32).sum() > 1000
+    print(f"LOD threshold scene: {sc.n} instances", flush=True)
+    gpu, ref = _run(ctx, sc, view.camera), _oracle(sc, view.camera)
+    _check(gpu, ref)
+    assert ref[2][3] > 0.95 * sc.n
+
+
 @pytest.mark.parametrize("first,count", [(0, 0), (5, 1), (1000, 63), (3, 64), (77, 65), (123, 4097)])
 def test_ragged_ranges(ctx, first, count):
     sc = synth.instance_scene(10_000, config=0, index=3, cube=200.0)
