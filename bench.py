@@ -162,6 +162,17 @@ def serving_leg(ctx, rates, seconds: float, meshlet_fraction: float = 0.25) -> d
                          int(sizes[si]), bool(kw)))
         return reqs
 
+    # Meshlets a meshlet part tests: the selected LOD's range of every object
+    # that passed selection (counters[2 + l] objects chose level l; every
+    # object of this scene has the same per-level meshlet counts).
+    lod_sizes = vgsc.objects["lod_meshlet_counts"][:, :8].astype(np.int64)
+    assert (lod_sizes == lod_sizes[0]).all()
+    lod_sizes = lod_sizes[0]
+
+    def tested_meshlets(req) -> int:
+        vgo = req[0][1][4]
+        return 0 if vgo is None else int((vgo[2][2:10].astype(np.int64) * lod_sizes).sum())
+
     out = {"scenes": [int(x) for x in sizes], "meshlet_scene": int(vg.meshlet_count),
            "meshlet_fraction": meshlet_fraction, "rates": {}, "max_requests_per_rate": 30_000}
     batched = CullServer(ctx, max_batch_views=64, max_inflight=3, max_batch_candidates=64 << 20,
@@ -187,7 +198,7 @@ def serving_leg(ctx, rates, seconds: float, meshlet_fraction: float = 0.25) -> d
                 "meshlet_latency_ms_p99": float(np.percentile(lat[with_vg], 99) * 1e3) if with_vg.any() else None,
                 "mean_batch_views": st["mean_batch_views"],
                 "instances_per_s": float(sum(c for _, c, _ in reqs) / wall),
-                "meshlets_per_s": float(with_vg.sum() * vg.meshlet_count / wall),
+                "meshlets_tested_per_s": float(sum(tested_meshlets(r) for r in reqs) / wall),
             }
             log(f"serving {name} @ {rate}/s: {row[name]['views_per_s']:.0f} views/s, "
                 f"p50 {row[name]['latency_ms_p50']:.3f} ms, p99 {row[name]['latency_ms_p99']:.3f} ms, "
