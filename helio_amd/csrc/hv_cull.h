@@ -12,8 +12,10 @@ namespace hv {
 constexpr uint32_t kMaxViews = 1024;      // views per batch launch (10 bits of a column-scan unit)
 constexpr uint32_t kMaxBuckets = 14336;   // mesh_count * lod_count per view (56 KiB LDS histogram; with the statics well within 160 KiB)
 constexpr uint32_t kCounterStride = 16;   // u32 counters per view
-constexpr uint32_t kColscanThreads = 256; // threads of a column-scan block
-constexpr uint32_t kStatsUnit = 15;       // slice field of the column-scan unit that sums a view's stats (slices use 0..8)
+constexpr uint32_t kColscanThreads = 1024; // threads of a column-scan block
+constexpr uint32_t kColscanRows = 32;     // rows of one bucket a column-scan thread holds in registers
+constexpr uint32_t kColscanMinLg = 5;     // narrowest bucket slice of a column-scan unit: 32 buckets (128-byte rows)
+constexpr uint32_t kStatsSlice = 1023;    // slice field of the column-scan unit that sums a view's stats
 constexpr uint32_t kEntryShift = 12;      // stage entry = (bucket << 12) | index in chunk (chunk <= 4096)
 
 // Per-view results (u32, kCounterStride per view), written by the column scan:
@@ -43,6 +45,8 @@ struct alignas(16) CullView {
     uint32_t n_items;        // work items of this view
     uint32_t block_begin;    // block owning the view's first item (its row 0)
     uint32_t n_rows;         // blocks whose range meets the view (0 when the view is empty)
+    uint32_t col_lg;         // log2 of the bucket slice of the view's column-scan units
+    uint32_t _pad_col;
     uint32_t screen_h;       // occlusion: r_px scale
     uint32_t hiz_mip_count;  // CullParams.hiz_mip_count = mip_levels(screen)
     uint32_t hiz_w, hiz_h, hiz_levels;
@@ -72,7 +76,7 @@ struct alignas(16) BatchHdr {
     uint32_t waves_log2;     // log2(WG / 64)
     uint32_t* stage;         // n_items x chunk packed survivors; wave w of item i fills [(i*W+w)*sub, +count)
     uint32_t* sub_cnt;       // n_items x W survivor counts
-    const uint32_t* units;   // n_units x (view | log2(slice) << 10 | first bucket << 14)
+    const uint32_t* units;   // n_units x (view | slice index << 10); slice kStatsSlice sums the stats
     const uint32_t* block_view;  // G: view holding each block's first item
 };
 

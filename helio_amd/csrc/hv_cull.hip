@@ -117,12 +117,10 @@ __device__ __forceinline__ bool occl_resolve(const HizTaps& t, float near_z, boo
     return rev ? (near_z < hz - bias) : (near_z > hz + bias);
 }
 
-// Per-candidate outcome of the indirect-dispatch predicate (+ Hi-Z).
-struct Outcome {
-    bool visible;   // in the frustum (indirect_dispatch.wgsl:100-113)
-    bool draw;      // visible and not sub-pixel (:127-137)
-    bool occl;      // draw and occluded by the Hi-Z test (occlusion_cull.wgsl:158-220)
-};
+// Mask of the active lanes for which `p` holds. The builtin keeps the mask
+// in scalar registers (HIP's __ballot materialises the bool in a VGPR and
+// compares it again: two more vector instructions per call).
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // sphere_in_frustum (indirect_dispatch.wgsl:79-86).
 __device__ __forceinline__ bool sphere_visible(const float* pl, float cx, float cy, float cz, float r) {
@@ -184,7 +182,7 @@ __device__ __forceinline__ bool frustum_visible(const float* pl, const float* sg
     // non-degenerate. Only a wave holding some other lane runs the exact test.
     const float cmax = fmaxf(fmaxf(fabsf(cx), fabsf(cy)), fabsf(cz));
     const bool sure_box = r > cmax * 0x1p-22f;
-    if (__ballot(!sure_box) != 0ull) {
+    if (ballot(!sure_box) != 0ull) {
         // Empty volatile asm: keeps this rare block a real (uniform) branch
         // instead of being speculated into every wave.
         asm volatile("" ::: "memory");
@@ -200,6 +198,8 @@ __device__ __forceinline__ bool frustum_visible(const float* pl, const float* sg
 // once per row and read from there, so they sit in a few VGPRs instead of
 // pushing the scalar file into spills.
 struct RowLds {
+    float pd[6];       // plane offsets (the fourth component): the addend of each
+                       // plane's first FMA comes from a VGPR, not a scalar
     float vp3[4];      // row 3 of view_proj (clip w)
     float cam[3];
     float p11;
@@ -236,60 +236,65 @@ constexpr float kTinyT = 0.001f;
 constexpr float kTinyLo = 0.001f * (1.0f - 0x1p-20f);
 constexpr float kTinyHi = 0.001f * (1.0f + 0x1p-20f);
 
-// One candidate. Returns its bucket (mesh_id * lod_count + lod) when it is
-// drawn, kInvalid otherwise. Lanes with !valid report all-false. `boxes`
-// (uniform) says the view's scene holds explicit AABBs: a batch that mixes
-// such scenes with sphere-only ones runs the AABB kernel for every view, and
-// the sphere-only views take the sphere path (their `ab` is null).
-template <bool OCCL, bool AABB>
-__device__ __forceinline__ uint32_t classify(const CullView* __restrict__ V, const CullView* __restrict__ Vk,
-                                             const float* pl, const float* l1s, const RowLds& rc, bool valid, float4 b,
-                                             uint32_t m, bool boxes, const hv_instance_aabb* ab, Outcome& o) {
+// The frustum test of one candidate (indirect_dispatch.wgsl:100-113). Lanes
+// with !valid report false. `boxes` (uniform) says the view's scene holds
+// explicit AABBs: a batch that mixes such scenes with sphere-only ones runs
+// the AABB kernel for every view, and the sphere-only views take the sphere
+// path (their `ab` is null).
+template <bool AABB>
+__device__ __forceinline__ bool frustum_part(const CullView* __restrict__ Vk, const float* pl, const float* l1s,
+                                             const RowLds& rc, bool valid, float4 b, bool boxes,
+                                             const hv_instance_aabb* ab) {
     const float cx = b.x, cy = b.y, cz = b.z, r = b.w;
-    if (AABB && boxes) {
-        o.visible = valid && frustum_visible<true>(pl, nullptr, cx, cy, cz, r, ab);
-    } else {
-        const float cmax = fmaxf(fmaxf(fabsf(cx), fabsf(cy)), fabsf(cz));
-        const float bb = cmax + r;
-        // r > 2^-22 max|c| also rules out a degenerate box (see frustum_visible).
-        const bool lane_ok = (r > cmax * 0x1p-22f) & (bb < 0x1p100f);
-        float s[6];
+    if (AABB && boxes) return valid && frustum_visible<true>(pl, nullptr, cx, cy, cz, r, ab);
+    const float cmax = fmaxf(fmaxf(fabsf(cx), fabsf(cy)), fabsf(cz));
+    const float bb = cmax + r;
+    // r > 2^-22 max|c| also rules out a degenerate box (see frustum_visible).
+    const bool lane_ok = (r > cmax * 0x1p-22f) & (bb < 0x1p100f);
+    float s[6];
 #pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            const float l1 = l1s[i];
-            s[i] = __builtin_fmaf(pl[4 * i], cx,
-                                  __builtin_fmaf(pl[4 * i + 1], cy,
-                                                 __builtin_fmaf(pl[4 * i + 2], cz, __builtin_fmaf(l1, r, pl[4 * i + 3]))));
-        }
-        const float smin = fminf(fminf(fminf(s[0], s[1]), s[2]), fminf(fminf(s[3], s[4]), s[5]));
-        const float e = __builtin_fmaf(rc.sure_kl, bb, rc.sure_kd);
-        bool vis = smin >= e;
-        const bool unsure = valid & ((rc.sure_ok == 0u) | !(lane_ok & ((smin >= e) | (smin < -e))));
-        if (__ballot(unsure) != 0ull) {
-            asm volatile("" ::: "memory");
-            // The p-vertex signs are needed only here: loaded on demand.
-            float sg[18];
-#pragma unroll
-            for (int i = 0; i < 18; ++i) sg[i] = ldc(&Vk->pv_sign[i]);
-            const bool ve = frustum_visible<false>(pl, sg, cx, cy, cz, r, nullptr);
-            vis = unsure ? ve : vis;
-        }
-        o.visible = valid & vis;
+    for (int i = 0; i < 6; ++i) {
+        const float l1 = l1s[i];
+        s[i] = __builtin_fmaf(pl[4 * i], cx,
+                              __builtin_fmaf(pl[4 * i + 1], cy,
+                                             __builtin_fmaf(pl[4 * i + 2], cz, __builtin_fmaf(l1, r, rc.pd[i]))));
     }
+    const float smin = fminf(fminf(fminf(s[0], s[1]), s[2]), fminf(fminf(s[3], s[4]), s[5]));
+    const float e = __builtin_fmaf(rc.sure_kl, bb, rc.sure_kd);
+    bool vis = smin >= e;
+    const bool unsure = valid & ((rc.sure_ok == 0u) | !(lane_ok & ((smin >= e) | (smin < -e))));
+    if (ballot(unsure) != 0ull) {
+        asm volatile("" ::: "memory");
+        // The p-vertex signs are needed only here: scalar loads after the
+        // barrier (signs derived from the planes in place would be hoisted
+        // into every candidate's fast path as vector selects).
+        float sg[18];
+#pragma unroll
+        for (int i = 0; i < 18; ++i) sg[i] = ldc(&Vk->pv_sign[i]);
+        const bool ve = frustum_visible<false>(pl, sg, cx, cy, cz, r, nullptr);
+        vis = unsure ? ve : vis;
+    }
+    return valid & vis;
+}
+
+// The rest of the predicate for a frustum-visible candidate (`act`): the
+// sub-pixel cull (indirect_dispatch.wgsl:127-137), then distance LOD. Returns
+// the bucket (mesh_id * lod_count + lod) when drawn, kInvalid otherwise.
+__device__ __forceinline__ uint32_t draw_part(const RowLds& rc, bool act, float4 b, uint32_t m, bool& draw) {
+    const float cx = b.x, cy = b.y, cz = b.z, r = b.w;
     // Sub-pixel cull: w = ((vp[3] x + vp[7] y) + vp[11] z) + vp[15].
     const float w = ((rc.vp3[0] * cx + rc.vp3[1] * cy) + rc.vp3[2] * cz) + rc.vp3[3];
     const float a = r * rc.p11;
     const bool w_ok = (__float_as_uint(w) - 0x1F800000u) < 0x3F000000u;  // w in [2^-64, 2^62)
     const float q = fabsf(a) * __builtin_amdgcn_rcpf(w);
     bool tiny = w_ok & (q < kTinyLo);
-    const bool unsure_t = o.visible & (w > 0.0f) & !(w_ok & ((q < kTinyLo) | (q > kTinyHi)));
-    if (__ballot(unsure_t) != 0ull) {
+    const bool unsure_t = act & (w > 0.0f) & !(w_ok & ((q < kTinyLo) | (q > kTinyHi)));
+    if (ballot(unsure_t) != 0ull) {
         asm volatile("" ::: "memory");
         const bool te = fabsf(a / w) < kTinyT;
         tiny = unsure_t ? te : tiny;
     }
-    o.draw = o.visible & !tiny;
-    o.occl = false;  // the Hi-Z test, when on, follows in process_item
+    draw = act & !tiny;
     const float camx = rc.cam[0], camy = rc.cam[1], camz = rc.cam[2];
     // lod = #{t : sqrt_rn(d2) >= lod_distance[t]}, evaluated without the
     // square root as d2 >= lod_d2[t] (the host picks lod_d2[t] as the least
@@ -305,7 +310,7 @@ __device__ __forceinline__ uint32_t classify(const CullView* __restrict__ V, con
                (uint32_t)(d2 >= rc.lod_d2[6]);
     // mesh_id < 2^14 here (buckets <= 14336), so a 24-bit multiply-add is exact.
     const uint32_t code = __umul24(m & 0x7fffffffu, lod_count) + lod;
-    return o.draw ? code : kInvalid;
+    return draw ? code : kInvalid;
 }
 
 // Advance v to the view holding `item` (items are visited in order).
@@ -355,20 +360,29 @@ __device__ __forceinline__ void load_item(const ItemCtx& c, uint32_t it, float4*
     }
 }
 
-// Per-lane stat counters of a row: tests seen as visible / drawn / occluded,
-// shadow casters, shadow & drawn, shadow & occluded (reduced once per row).
-struct LaneStats {
+// A wave's stat counters over a row: tests seen as visible / drawn /
+// occluded, shadow casters, shadow & drawn, shadow & occluded. Each is a
+// popcount of a ballot, so the counters are wave-uniform (scalar registers).
+struct WaveStats {
     uint32_t vis = 0, draw = 0, occl = 0, sh = 0, sh_draw = 0, sh_occl = 0;
 };
 
 // Classifies the loaded candidates of item `it`: wave-compacted survivors into
-// the wave's stage slots, the row histogram in LDS; `valid` and `kept` count
-// this wave's valid candidates and survivors.
+// the wave's stage slots, the row histogram in LDS; `n_valid` and `kept`
+// count this wave's valid candidates and survivors. The frustum test runs on
+// every candidate; the candidates it keeps (about one in eight in the
+// benchmarked views) are packed into consecutive lanes of the wave through a
+// per-wave LDS slice, 64 at a time, and the rest of the predicate (sub-pixel,
+// LOD, the Hi-Z test), the compaction and the histogram run once per 64 of
+// them instead of once per round of mostly idle lanes.
 template <int WG, int IPT, bool OCCL, bool AABB>
 __device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, const float4* bb, const uint32_t* mm,
-                                             uint32_t* s_hist, LaneStats& ls, uint32_t& n_valid, uint32_t& kept) {
+                                             uint32_t* s_hist, WaveStats& ws, uint32_t& n_valid, uint32_t& kept) {
     constexpr uint32_t NW = WG / 64;
     constexpr uint32_t CHUNK = WG * IPT, SUB = IPT * 64;
+    __shared__ float4 s_pk[NW][64];    // packed candidates: bounds
+    __shared__ uint32_t s_pm[NW][64];  // meta
+    __shared__ uint32_t s_pj[NW][64];  // index in the chunk
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
     const uint32_t base_idx = (it - c.ibeg) * CHUNK;
     const uint32_t rem = c.vcount - base_idx;
@@ -385,103 +399,83 @@ __device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, cons
     for (int i = 0; i < 24; ++i) pl[i] = ldc(&Vk->planes[i]);
 #pragma unroll
     for (int i = 0; i < 6; ++i) l1[i] = ldc(&Vk->plane_l1[i]);
-    uint32_t pos = 0;
     const bool boxes = AABB && c.aabb != nullptr;  // uniform: the view's scene holds explicit AABBs
-    // Occlusion: every candidate of the item is classified and probed (taps
-    // loaded) before the first is resolved and stored.
-    uint32_t code_o[OCCL ? IPT : 1];
-    Outcome out_o[OCCL ? IPT : 1];
-    if constexpr (OCCL) {
-        // The Hi-Z test runs on the wave's drawable candidates packed into
-        // consecutive lanes through a per-wave LDS slice, 64 at a time: one
-        // probe (seven IEEE divisions and a square root) per 64 drawable
-        // candidates instead of one per round with a few lanes active.
-        __shared__ float4 s_pk[NW][64];
-        __shared__ uint32_t s_oc[NW][64];
-        const CullView* __restrict__ V = c.V;
-        const bool rev = (V->flags & HV_CULL_REVERSED_Z) != 0u;
-        uint32_t slot[IPT];
-        uint32_t total = 0;  // wave-uniform
+    // Frustum on every candidate; slot = rank among the wave's visible ones.
+    bool vis[IPT];
+    uint32_t slot[IPT];
+    uint32_t total = 0;  // wave-uniform
 #pragma unroll
-        for (int k = 0; k < IPT; ++k) {
-            const uint32_t j = k * WG + tid;
-            const bool valid = j < n;
-            code_o[k] = classify<OCCL, AABB>(V, Vk, pl, l1, *c.rc, valid, bb[k], mm[k], boxes,
-                                             boxes ? c.aabb + first + (valid ? j : 0u) : nullptr, out_o[k]);
-            const uint64_t dm = __ballot(out_o[k].draw);
-            slot[k] = total + mbcnt(dm);
-            total += (uint32_t)__popcll(dm);
+    for (int k = 0; k < IPT; ++k) {
+        const uint32_t j = k * WG + tid;
+        const bool valid = j < n;
+        vis[k] = frustum_part<AABB>(Vk, pl, l1, *c.rc, valid, bb[k], boxes,
+                                    boxes ? c.aabb + first + (valid ? j : 0u) : nullptr);
+        const uint64_t vm = ballot(vis[k]);
+        slot[k] = total + mbcnt(vm);
+        total += (uint32_t)__builtin_popcountll(vm);
+        // Stat masks straight from compares where possible (a ballot of a
+        // compare is the compare's own lane mask).
+        const uint64_t valm = ballot(j < n);
+        n_valid += (uint32_t)__builtin_popcountll(valm);
+        ws.sh += (uint32_t)__builtin_popcountll(ballot((int32_t)mm[k] < 0) & valm);
+    }
+    ws.vis += total;
+    const CullView* __restrict__ V = c.V;
+    const bool rev = OCCL && (V->flags & HV_CULL_REVERSED_Z) != 0u;
+    uint32_t pos = 0;
+    for (uint32_t base = 0; base < total; base += 64u) {
+#pragma unroll
+        for (int k = 0; k < IPT; ++k)
+            if (vis[k] && slot[k] - base < 64u) {
+                const uint32_t sl = slot[k] - base;
+                s_pk[wave][sl] = bb[k];
+                s_pm[wave][sl] = mm[k];
+                s_pj[wave][sl] = k * WG + tid;
+            }
+        wave_lds_sync();
+        const bool act = lane < total - base;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        uint32_t m = 0u, jj = 0u;
+        if (act) {
+            p = s_pk[wave][lane];
+            m = s_pm[wave][lane];
+            jj = s_pj[wave][lane];
         }
-        for (uint32_t base = 0; base < total; base += 64u) {
-#pragma unroll
-            for (int k = 0; k < IPT; ++k)
-                if (out_o[k].draw && slot[k] - base < 64u) s_pk[wave][slot[k] - base] = bb[k];
-            wave_lds_sync();
+        bool draw;
+        uint32_t code = draw_part(*c.rc, act, p, m, draw);
+        const uint64_t shm = ballot((int32_t)m < 0);
+        const uint64_t dm = ballot(draw);
+        ws.draw += (uint32_t)__builtin_popcountll(dm);
+        ws.sh_draw += (uint32_t)__builtin_popcountll(dm & shm);
+        if constexpr (OCCL) {
+            // occlusion_cull.wgsl:158-220 on the drawn candidates.
             bool occ = false;
-            if (lane < total - base) {
-                const float4 sp = s_pk[wave][lane];
+            if (draw) {
                 OcclProbe q;
                 q.level = V->hiz0;
                 q.idx = HizRectIdx{0u, 0u, 0u, 0u};
                 q.near_z = 0.0f;
                 q.test = false;
-                occl_probe(V, V->vp, V->p00, c.rc->p11, c.rc->cam[0], c.rc->cam[1], c.rc->cam[2], rev, sp.x, sp.y,
-                           sp.z, sp.w, q);
+                occl_probe(V, V->vp, V->p00, c.rc->p11, c.rc->cam[0], c.rc->cam[1], c.rc->cam[2], rev, p.x, p.y,
+                           p.z, p.w, q);
                 const HizTaps t = hiz_load4(q.level, q.idx);
                 occ = q.test && occl_resolve(t, q.near_z, rev);
             }
-            s_oc[wave][lane] = occ ? 1u : 0u;
-            wave_lds_sync();
-#pragma unroll
-            for (int k = 0; k < IPT; ++k)
-                if (out_o[k].draw && slot[k] - base < 64u) out_o[k].occl = s_oc[wave][slot[k] - base] != 0u;
-            wave_lds_sync();  // the next round rewrites both slices
+            const uint64_t om = ballot(occ);
+            ws.occl += (uint32_t)__builtin_popcountll(om);
+            ws.sh_occl += (uint32_t)__builtin_popcountll(om & shm);
+            if (occ) code = kInvalid;
         }
-#pragma unroll
-        for (int k = 0; k < IPT; ++k)
-            if (out_o[k].occl) code_o[k] = kInvalid;
-    }
-#pragma unroll
-    for (int k = 0; k < IPT; ++k) {
-        const uint32_t j = k * WG + tid;
-        const bool valid = j < n;
-        Outcome o;
-        uint32_t code;
-        if constexpr (OCCL) {
-            o = out_o[k];
-            code = code_o[k];
-        } else {
-            code = classify<OCCL, AABB>(c.V, Vk, pl, l1, *c.rc, valid, bb[k], mm[k], boxes,
-                                        boxes ? c.aabb + first + (valid ? j : 0u) : nullptr, o);
-        }
-        const uint64_t keep = __ballot(code != kInvalid);
+        const uint64_t keep = ballot(code != kInvalid);
         if (code != kInvalid) {
-            out[pos + mbcnt(keep)] = (code << kEntryShift) | j;
+            out[pos + mbcnt(keep)] = (code << kEntryShift) | jj;
             atomicAdd(&s_hist[code], 1u);
         }
-        pos += (uint32_t)__popcll(keep);
-        // Valid lanes of this wave in round k: n - (k*WG + wave*64), clamped.
-        const uint32_t w0 = k * WG + wave * 64u;
-        n_valid += n > w0 ? min(n - w0, 64u) : 0u;
-        const bool sh = valid & ((mm[k] >> 31) != 0u);
-        ls.vis += o.visible ? 1u : 0u;
-        ls.draw += o.draw ? 1u : 0u;
-        ls.sh += sh ? 1u : 0u;
-        ls.sh_draw += (sh & o.draw) ? 1u : 0u;
-        if (OCCL) {
-            ls.occl += o.occl ? 1u : 0u;
-            ls.sh_occl += (sh & o.occl) ? 1u : 0u;
-        }
+        pos += (uint32_t)__builtin_popcountll(keep);
+        wave_lds_sync();  // the next round rewrites the slices
     }
     kept += pos;
     if (lane == 0) c.sub_cnt[(size_t)it * NW + wave] = pos;
-}
-
-// Sum of a lane value over the wave.
-__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
-#pragma unroll
-    for (uint32_t off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
 }
 
 template <int WG, int IPT, bool OCCL, bool AABB>
@@ -507,6 +501,8 @@ __global__ __launch_bounds__(WG) void k_classify(const BatchHdr* __restrict__ hd
         const CullView* __restrict__ V = views + v;
         if (tid == 0) {
 #pragma unroll
+            for (int i = 0; i < 6; ++i) s_rc.pd[i] = V->planes[4 * i + 3];
+#pragma unroll
             for (int i = 0; i < 4; ++i) s_rc.vp3[i] = V->vp[4 * i + 3];
 #pragma unroll
             for (int i = 0; i < 3; ++i) s_rc.cam[i] = V->cam[i];
@@ -526,7 +522,7 @@ __global__ __launch_bounds__(WG) void k_classify(const BatchHdr* __restrict__ hd
         const hv_instance_aabb* __restrict__ aabb = ldc(&V->aabb);
         for (uint32_t b = tid; b < nb; b += WG) s_hist[b] = 0u;
         __syncthreads();
-        LaneStats ls;
+        WaveStats ws;
         uint32_t n_valid = 0, kept = 0;
         // One item at a time: latency is covered by the other resident waves
         // (a two-deep register pipeline measured no faster and cost registers).
@@ -535,15 +531,14 @@ __global__ __launch_bounds__(WG) void k_classify(const BatchHdr* __restrict__ hd
             float4 bb[IPT];
             uint32_t mm[IPT];
             load_item<WG, IPT>(ic, item, bb, mm);
-            process_item<WG, IPT, OCCL, AABB>(ic, item, bb, mm, s_hist, ls, n_valid, kept);
+            process_item<WG, IPT, OCCL, AABB>(ic, item, bb, mm, s_hist, ws, n_valid, kept);
         }
         // Row stats (layout of indirect_dispatch.wgsl:69-77): [0] survivors,
         // [1] invalid (not visible), [2] visible but sub-pixel, [3] drawn,
         // [4] occluded, [5] shadow casters, [6] shadow & drawn, [7] shadow &
         // occluded.
-        const uint32_t nvis = wave_sum(ls.vis), ndraw = wave_sum(ls.draw), nsh = wave_sum(ls.sh);
-        const uint32_t nshd = wave_sum(ls.sh_draw);
-        const uint32_t nocc = OCCL ? wave_sum(ls.occl) : 0u, nsho = OCCL ? wave_sum(ls.sh_occl) : 0u;
+        const uint32_t nvis = ws.vis, ndraw = ws.draw, nsh = ws.sh, nshd = ws.sh_draw;
+        const uint32_t nocc = OCCL ? ws.occl : 0u, nsho = OCCL ? ws.sh_occl : 0u;
         if (lane == 0) {
             s_st[wave][0] = kept;
             s_st[wave][1] = n_valid - nvis;
@@ -611,9 +606,40 @@ __device__ void block_excl_scan_lds(uint32_t* s, uint32_t n, uint32_t* s_w) {
     __syncthreads();
 }
 
-// Column scan. A unit covers buckets [b0, b0 + 2^lg) of one view; its 256
-// threads form 256 >> lg row groups per bucket, each owning a contiguous run
-// of rows (the host picks lg so that a run is at most 8 rows).
+// Column scan. A unit is one slice of 2^lg buckets of one view (lg per view,
+// from 5 to 10): its 1024 threads form 1024 >> lg row groups, a thread holds
+// one bucket of one group's contiguous run of rows (at most kColscanRows in
+// registers, so every row is read and written once), and consecutive lanes
+// hold consecutive buckets, so each access of a wave covers at least 128
+// contiguous bytes of a row. Per bucket: the exclusive prefix of the counts
+// over the view's rows (in place) and the total. A unit with slice
+// kStatsSlice sums the view's row stats instead.
+__device__ __forceinline__ void colscan_stats(const CullView* __restrict__ V, uint32_t R, uint32_t* s_part) {
+    constexpr uint32_t SG = kColscanThreads / 8u;  // row groups of 8 words; 16 rows per group in flight
+    const uint32_t tid = threadIdx.x, word = tid & 7u, g = tid >> 3;
+    const uint32_t* __restrict__ rs = V->row_stats;
+    uint32_t acc = 0;
+    for (uint32_t r = g; r < R; r += 16u * SG) {
+        uint32_t y[16];
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) y[k] = (r + SG * k < R) ? rs[(size_t)(r + SG * k) * 8 + word] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) acc += y[k];
+    }
+    s_part[tid] = acc;
+    __syncthreads();
+    if (tid < 8) {
+        uint32_t t = 0;
+        for (uint32_t q = 0; q < SG; ++q) t += s_part[q * 8 + tid];
+        if (tid == 0) {
+            V->counters[kCtrVisible] = t;  // survivors
+            V->counters[0] = V->count;     // stats[0]: instances tested
+        } else {
+            V->counters[tid] = t;
+        }
+    }
+}
+
 __global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __restrict__ hdr,
                                                              const CullView* __restrict__ views) {
     __shared__ uint32_t s_part[kColscanThreads];
@@ -622,101 +648,73 @@ __global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __r
     const uint32_t* __restrict__ units = ldc(&hdr->units);
     for (uint32_t u = blockIdx.x; u < n_units; u += gridDim.x) {
         const uint32_t code = ldc(units + u);
-        const uint32_t v = code & 1023u, lg = (code >> 10) & 15u, b0 = code >> 14;
+        const uint32_t v = code & 1023u, si = code >> 10;
         const CullView* __restrict__ V = views + v;
         const uint32_t R = ldc(&V->n_rows), nb = ldc(&V->nb);
-        if (lg == kStatsUnit) {
-            // The view's stats: SG row groups x 8 words, 16 rows per group in
-            // flight (16 SG rows per pass).
-            constexpr uint32_t SG = kColscanThreads / 8u;
-            const uint32_t word = tid & 7u, g = tid >> 3;
-            const uint32_t* __restrict__ rs = V->row_stats;
-            uint32_t acc = 0;
-            for (uint32_t r = g; r < R; r += 16u * SG) {
-                uint32_t y[16];
-#pragma unroll
-                for (uint32_t k = 0; k < 16; ++k) y[k] = (r + SG * k < R) ? rs[(size_t)(r + SG * k) * 8 + word] : 0u;
-#pragma unroll
-                for (uint32_t k = 0; k < 16; ++k) acc += y[k];
-            }
-            s_part[tid] = acc;
-            __syncthreads();
-            if (tid < 8) {
-                uint32_t t = 0;
-                for (uint32_t q = 0; q < SG; ++q) t += s_part[q * 8 + tid];
-                if (tid == 0) {
-                    V->counters[kCtrVisible] = t;  // survivors
-                    V->counters[0] = V->count;     // stats[0]: instances tested
-                } else {
-                    V->counters[tid] = t;
-                }
-            }
+        if (si == kStatsSlice) {
+            colscan_stats(V, R, s_part);
             __syncthreads();
             continue;
         }
-        const uint32_t col = tid & ((1u << lg) - 1u), grp = tid >> lg, RG = kColscanThreads >> lg;
-        const uint32_t b = b0 + col;
+        const uint32_t lg = ldc(&V->col_lg), RG = kColscanThreads >> lg;
+        const uint32_t col = tid & ((1u << lg) - 1u), grp = tid >> lg;
+        const uint32_t b = (si << lg) + col;
         const bool live = b < nb;
         const uint32_t rpg = (R + RG - 1u) / RG;
-        const uint32_t r0 = min(R, grp * rpg), r1 = min(R, r0 + rpg);
+        const uint32_t r0 = min(R, grp * rpg), n = live ? min(R, r0 + rpg) - r0 : 0u;
+        // Uniform base + a 32-bit lane offset (rows x buckets < 2^32), so
+        // the loads take the scalar-base form with one offset register.
         uint32_t* __restrict__ hist = ldc(&V->block_hist);
-        uint32_t x[8];
+        const uint32_t off = r0 * nb + b;
+        uint32_t x[kColscanRows];
         uint32_t sum = 0;
+        if (rpg <= kColscanRows) {
 #pragma unroll
-        for (uint32_t k = 0; k < 8; ++k) {
-            x[k] = (live && r0 + k < r1) ? hist[(size_t)(r0 + k) * nb + b] : 0u;
-            sum += x[k];
+            for (uint32_t k = 0; k < kColscanRows; ++k) x[k] = k < n ? hist[off + k * nb] : 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < kColscanRows; ++k) {
+                const uint32_t t = x[k];
+                x[k] = sum;
+                sum += t;
+            }
+        } else {
+            // More rows than registers (a grid of many small classify
+            // blocks): sum now, reread when writing.
+            for (uint32_t k = 0; k < n; ++k) sum += hist[off + k * nb];
         }
-        for (uint32_t r = r0 + 8u; r < r1; r += 8u) {
-            uint32_t y[8];
-#pragma unroll
-            for (uint32_t k = 0; k < 8; ++k) y[k] = (live && r + k < r1) ? hist[(size_t)(r + k) * nb + b] : 0u;
-#pragma unroll
-            for (uint32_t k = 0; k < 8; ++k) sum += y[k];
-        }
-        // Inclusive scan over the row groups of each bucket column.
         s_part[tid] = sum;
         __syncthreads();
-        uint32_t inc = sum;
-        for (uint32_t off = 1; off < RG; off <<= 1) {
-            const uint32_t y = grp >= off ? s_part[tid - (off << lg)] : 0u;
-            __syncthreads();
-            inc += y;
-            s_part[tid] = inc;
-            __syncthreads();
+        uint32_t base = 0, total = 0;
+        for (uint32_t g2 = 0; g2 < RG; ++g2) {
+            const uint32_t t = s_part[(g2 << lg) + col];
+            base += g2 < grp ? t : 0u;
+            total += t;
         }
-        if (live && grp == RG - 1u) V->totals[b] = inc;
-        uint32_t run = inc - sum;
-        if (r1 - r0 <= 8u) {
+        if (rpg <= kColscanRows) {
 #pragma unroll
-            for (uint32_t k = 0; k < 8; ++k)
-                if (live && r0 + k < r1) {
-                    hist[(size_t)(r0 + k) * nb + b] = run;
-                    run += x[k];
-                }
+            for (uint32_t k = 0; k < kColscanRows; ++k)
+                if (k < n) hist[off + k * nb] = base + x[k];
         } else {
-            for (uint32_t r = r0; r < r1; r += 8u) {
-                uint32_t y[8];
-#pragma unroll
-                for (uint32_t k = 0; k < 8; ++k) y[k] = (live && r + k < r1) ? hist[(size_t)(r + k) * nb + b] : 0u;
-#pragma unroll
-                for (uint32_t k = 0; k < 8; ++k)
-                    if (live && r + k < r1) {
-                        hist[(size_t)(r + k) * nb + b] = run;
-                        run += y[k];
-                    }
+            uint32_t run = base;
+            for (uint32_t k = 0; k < n; ++k) {
+                const uint32_t t = hist[off + k * nb];
+                hist[off + k * nb] = run;
+                run += t;
             }
         }
-        // An empty view has no row 0 in the scatter: its args are written here.
-        if (R == 0u && live && grp == 0u) {
-            const hv_mesh_lod ml = V->mesh_lods[b];
-            hv_draw_indexed_indirect a;
-            a.index_count = ml.index_count;
-            a.instance_count = 0u;
-            a.first_index = ml.first_index;
-            a.base_vertex = ml.base_vertex;
-            a.first_instance = 0u;
-            V->args[b] = a;
+        if (live && grp == 0u) {
+            V->totals[b] = total;
+            // An empty view has no row 0 in the scatter: its args are written here.
+            if (R == 0u) {
+                const hv_mesh_lod ml = V->mesh_lods[b];
+                hv_draw_indexed_indirect a;
+                a.index_count = ml.index_count;
+                a.instance_count = 0u;
+                a.first_index = ml.first_index;
+                a.base_vertex = ml.base_vertex;
+                a.first_instance = 0u;
+                V->args[b] = a;
+            }
         }
         __syncthreads();
     }
@@ -1028,22 +1026,24 @@ uint32_t CullBatch::max_grid(const hv_ctx* ctx) { return std::max(1u, ctx->cu_co
 namespace {
 
 // Upper bound of column-scan units for `views` views of at most `buckets`
-// buckets with `rows` rows in total (see unit_lg: a slice is at least
-// min(256, 1024 / R) buckets wide, so a view has at most buckets / 256 +
-// buckets * R / 1024 + 1 bucket units, plus its stats unit).
-uint64_t unit_bound(uint64_t views, uint64_t buckets, uint64_t rows) {
-    return views * (buckets / 128 + 2) + (buckets * rows) / 1024 + 1;
+// buckets (slices are at least 2^kColscanMinLg buckets wide; one stats unit
+// per view).
+uint64_t unit_bound(uint64_t views, uint64_t buckets, uint64_t /*rows*/) {
+    return views * ((buckets >> kColscanMinLg) + 2);
 }
 
 // log2 of the bucket slice of a column-scan unit for a view of R rows: the
-// widest power of two (<= 256) that leaves each of the 256 / slice row groups
-// at most 8 rows.
-uint32_t unit_lg(uint32_t R) {
-    if (R == 0) return 8;
-    uint32_t lg = 8;
-    while (lg > 0 && (R + (kColscanThreads >> lg) - 1) / (kColscanThreads >> lg) > 8) --lg;
+// widest slice (at most 1024 buckets) whose row groups hold at most
+// kColscanRows rows each, and no narrower than 2^kColscanMinLg.
+uint32_t col_lg(uint32_t R) {
+    uint32_t lg = 10;
+    while (lg > kColscanMinLg && (R + (kColscanThreads >> lg) - 1) / (kColscanThreads >> lg) > kColscanRows) --lg;
     return lg;
 }
+
+// Column-scan blocks: one 1024-thread block per CU at most (the units are
+// taken grid-stride), so a capacity grid never floods the dispatcher.
+uint32_t colscan_grid(const hv_ctx* ctx, uint32_t units) { return std::max(1u, std::min(units, ctx->cu_count)); }
 
 }  // namespace
 
@@ -1146,9 +1146,9 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
         item += cv.n_items;
         rows_total += cv.n_rows;
         hist_total += (uint64_t)cv.n_rows * nb;
-        host_units_.push_back(v | (kStatsUnit << 10));
-        const uint32_t lg = unit_lg(cv.n_rows);
-        for (uint32_t b0 = 0; b0 < nb; b0 += (1u << lg)) host_units_.push_back(v | (lg << 10) | (b0 << 14));
+        host_units_.push_back(v | (kStatsSlice << 10));
+        cv.col_lg = col_lg(cv.n_rows);
+        for (uint32_t si = 0; (si << cv.col_lg) < nb; ++si) host_units_.push_back(v | (si << 10));
     }
     const hipStream_t st = ctx_->stream;
     HV_TRY(d_stage_.reserve(std::max<uint64_t>(items_total, 1) * chunk * 4, false, st));
@@ -1233,9 +1233,8 @@ hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed) {
     // Fixed (capacity) grids make one captured graph valid for any later table:
     // blocks beyond the table's G (or past the last unit) exit at once.
     const size_t lds = fixed ? (size_t)lds_cap_ : (size_t)max_nb_ * 4;
-    const uint32_t colscan_cap = 8u * ctx_->cu_count;
     const uint32_t g1 = fixed ? resident1_ : std::max(1u, grid_);
-    const uint32_t g2 = fixed ? colscan_cap : std::max(1u, std::min(n_units_, colscan_cap));
+    const uint32_t g2 = colscan_grid(ctx_, fixed ? ~0u : n_units_);
     const uint32_t g3 = fixed ? resident3_ : std::max(1u, std::min(grid_, resident3_));
     if (ms) HV_HIP(hipEventRecord(ev_[0], s));
     hipLaunchKernelGGL(fn, dim3(g1), dim3(shape.wg), lds, s, hdr, views);
@@ -1402,7 +1401,7 @@ uint8_t* g_scratch_ptr[kScratchDevices] = {};
 // Writes the one-view table (header, view record, column-scan units, block ->
 // view array) into the scratch from kernel arguments, so the call needs no
 // host staging buffer and no copy.
-__global__ __launch_bounds__(256) void k_one_view_table(BatchHdr hdr, CullView view, uint32_t lg, uint32_t* units,
+__global__ __launch_bounds__(256) void k_one_view_table(BatchHdr hdr, CullView view, uint32_t* units,
                                                         uint32_t* block_view, uint32_t n_block_view,
                                                         uint32_t* table) {
     const uint32_t tid = threadIdx.x;
@@ -1411,8 +1410,7 @@ __global__ __launch_bounds__(256) void k_one_view_table(BatchHdr hdr, CullView v
     constexpr uint32_t HW = sizeof(BatchHdr) / 4, VW = sizeof(CullView) / 4;
     for (uint32_t i = tid; i < HW + VW; i += 256u) table[i] = i < HW ? h[i] : v[i - HW];
     // Unit 0 sums the stats; the others cover buckets [b0, b0 + 2^lg).
-    for (uint32_t u = tid; u < hdr.n_units; u += 256u)
-        units[u] = u == 0u ? (kStatsUnit << 10) : ((lg << 10) | (((u - 1u) << lg) << 14));
+    for (uint32_t u = tid; u < hdr.n_units; u += 256u) units[u] = u == 0u ? (kStatsSlice << 10) : ((u - 1u) << 10);
     for (uint32_t b = tid; b < n_block_view; b += 256u) block_view[b] = 0u;
 }
 
@@ -1534,8 +1532,7 @@ hv_status hv_inst_cull_noalloc(hv_ctx* ctx, const hv_cull_config* config, const 
     // block_of(N - 1) = G - 1).
     const uint32_t G = N ? std::min(resident1, N) : 0u;
     const uint32_t R = G;
-    uint32_t lg = 8;
-    while (lg > 0 && (R + (kColscanThreads >> lg) - 1) / (kColscanThreads >> lg) > 8) --lg;
+    const uint32_t lg = col_lg(R);
     const uint32_t n_units = 1u + (nb + (1u << lg) - 1u) / (1u << lg);
     // Scratch layout: table (header, view, units, block views), then the
     // kernels' buffers, each on a 256-byte boundary.
@@ -1559,6 +1556,7 @@ hv_status hv_inst_cull_noalloc(hv_ctx* ctx, const hv_cull_config* config, const 
     cv.n_items = N;
     cv.block_begin = 0;
     cv.n_rows = R;
+    cv.col_lg = lg;
     cv.ids = ids;
     cv.args = args;
     cv.counters = counters;
@@ -1581,14 +1579,13 @@ hv_status hv_inst_cull_noalloc(hv_ctx* ctx, const hv_cull_config* config, const 
     const hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     const BatchHdr* dhdr = reinterpret_cast<const BatchHdr*>(base);
     const CullView* dviews = reinterpret_cast<const CullView*>(base + sizeof(BatchHdr));
-    hipLaunchKernelGGL(k_one_view_table, dim3(1), dim3(256), 0, s, hdr, cv, lg,
+    hipLaunchKernelGGL(k_one_view_table, dim3(1), dim3(256), 0, s, hdr, cv,
                        reinterpret_cast<uint32_t*>(base + units_off), reinterpret_cast<uint32_t*>(base + bview_off),
                        n_bview, reinterpret_cast<uint32_t*>(base));
     HV_HIP(hipGetLastError());
     hipLaunchKernelGGL(fn, dim3(std::max(G, 1u)), dim3(shape.wg), lds, s, dhdr, dviews);
     HV_HIP(hipGetLastError());
-    hipLaunchKernelGGL(dev::k_colscan, dim3(std::max(1u, std::min(n_units, 8u * ctx->cu_count))),
-                       dim3(kColscanThreads), 0, s, dhdr, dviews);
+    hipLaunchKernelGGL(dev::k_colscan, dim3(colscan_grid(ctx, n_units)), dim3(kColscanThreads), 0, s, dhdr, dviews);
     HV_HIP(hipGetLastError());
     hipLaunchKernelGGL(scatter_fn(shape), dim3(std::max(1u, std::min(G, resident3))), dim3(scatter_threads(shape)), lds,
                        s, dhdr, dviews);
