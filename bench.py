@@ -327,6 +327,20 @@ def occlusion_leg(ctx, steps: int) -> dict:
     scene.set_mesh_lods(sc.mesh_lods, sc.mesh_count, sc.lod_count)
     hiz = HiZBuildPass(ctx, 1920, 1080, L.HIZ_MIN)
     hiz.upload_depth(depth)
+    # The pyramid tile and this view's classify shape come from the tuner
+    # (cached per device and workload); the context's shapes are restored
+    # after the leg.
+    from helio_amd import autotune
+    saved = autotune.get_shapes(ctx)
+    hz_tune, hz_cached = autotune.hiz_tile(ctx, "hiz:1920x1080:min", hiz)
+
+    def prepare_one(c):
+        bb = BatchCull(c, 1, occlusion=True, reversed_z=True)
+        bb.prepare([(scene, view.camera, 0, sc.n, hiz)])
+        return bb
+
+    hiz.execute()
+    cull_tune, b, cull_cached = autotune.cull_shape(ctx, "occlusion:1920x1080:1048576", prepare_one)
     # Pyramid build time (HIP events on the context stream).
     from helio_amd._lib import call
     hz_ms = []
@@ -345,8 +359,6 @@ def occlusion_leg(ctx, steps: int) -> dict:
     ids, args, stats = p.read()
     # A whole frame as a renderer runs it: the pyramid, then the cull as a
     # one-view batch replayed from its graph, on one stream, frame after frame.
-    b = BatchCull(ctx, 1, occlusion=True, reversed_z=True)
-    b.prepare([(scene, view.camera, 0, sc.n, hiz)])
     for _ in range(3):
         hiz.execute()
         b.execute(graph=True)
@@ -359,6 +371,7 @@ def occlusion_leg(ctx, steps: int) -> dict:
     frame_ms = (time.perf_counter() - t0) / steps * 1e3
     assert np.array_equal(b.read(0)[2], stats)
     b.close()
+    autotune.apply(ctx, saved)
     hz = float(np.median(hz_ms))
     cls = float(np.median(k[:, 0]))
     tot = float(np.median(k.sum(1)))
@@ -373,6 +386,10 @@ def occlusion_leg(ctx, steps: int) -> dict:
         "classify_gbs": sc.n * BYTES_PER_TEST / (cls * 1e-3) / 1e9,
         "visible": int(len(ids)),
         "stats": [int(x) for x in stats],
+        "tuned": {"hiz_tile": int(hz_tune["best_tile"]), "hiz_cached": hz_cached,
+                  "hiz_ms_tile32_tile64": [hz_tune.get("ms_tile32"), hz_tune.get("ms_tile64")],
+                  "launch_shape": {"wg": int(cull_tune["wg"]), "ipt": int(cull_tune["ipt"])},
+                  "launch_cached": cull_cached},
     }
 
 
@@ -402,7 +419,12 @@ def vg_leg(ctx, steps: int) -> dict:
     view, sc, depth = synth.vg_workload(8192)
     hiz = HiZBuildPass(ctx, 1920, 1080, L.HIZ_MIN)
     hiz.upload_depth(depth)
+    from helio_amd import autotune
+    saved = autotune.get_shapes(ctx)
+    hz_tune, _ = autotune.hiz_tile(ctx, "hiz:1920x1080:min", hiz)
     hiz.execute()
+    autotune.apply(ctx, saved)
+    out["hiz_tile"] = int(hz_tune["best_tile"])
     vg = VirtualGeometryPass(ctx)
     vg.set_frame(sc.meshlets, sc.objects, sc.instances)
     vg.prepare(view.camera, 1920, 1080, 0.5, hiz, reversed_z=True)
@@ -466,6 +488,7 @@ def main() -> int:
     ap.add_argument("--serving-seconds", type=float, default=0.5)
     ap.add_argument("--launch-shape", default=None, metavar="WGxIPT",
                     help="classify launch shape, e.g. 512x4 (default: autotuned on the batch)")
+    ap.add_argument("--retune", action="store_true", help="tune launch shapes even when cached for this device")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched leg and one calibration copy (for counter runs)")
     ap.add_argument("--side-only", action="store_true",
@@ -503,17 +526,24 @@ def main() -> int:
     if args.launch_shape:
         wg, ipt = (int(x) for x in args.launch_shape.lower().split("x"))
         autotune.apply(ctx, dict(autotune.get_shapes(ctx), wg=wg, ipt=ipt))
-    batch = BatchCull(ctx, len(views))
-    batch.prepare([(scene, v.camera, f, c) for v, f, c in views])
+    def prepare_batch(c):
+        b = BatchCull(c, len(views))
+        b.prepare([(scene, v.camera, f, c_) for v, f, c_ in views])
+        return b
+
     if args.launch_shape:
+        batch = prepare_batch(ctx)
         tuned = {"wg": wg, "ipt": ipt}
         log(f"launch shape wg={wg} ipt={ipt} (fixed)")
     else:
         # The autotuner times the 16 classify shapes on this batch (median of
-        # the sum of the three kernels) and leaves the batch and the context on
-        # the fastest; every later pass of this run uses it.
-        tuned = autotune.tune_batch(batch, reps=5)["best"]
-        log(f"launch shape wg={tuned['wg']} ipt={tuned['ipt']} ({tuned['total_ms'] * 1e3:.1f} us kernels)")
+        # the sum of the three kernels), or takes the shape cached for this
+        # device and workload, and leaves the context on it; every later pass
+        # of this run uses it.
+        tuned, batch, cached = autotune.cull_shape(ctx, f"multiview64:{sc.n}:{tests_per_step}", prepare_batch,
+                                                   reps=5, retune=args.retune)
+        log(f"launch shape wg={tuned['wg']} ipt={tuned['ipt']} ({tuned['total_ms'] * 1e3:.1f} us kernels"
+            f"{', cached' if cached else ''})")
     # Sequential: one batch, one graph replay per step on one stream.
     for _ in range(args.warmup):
         batch.execute(graph=True)

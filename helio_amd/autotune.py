@@ -44,9 +44,9 @@ def tune_hiz(hiz: HiZBuildPass, reps: int = 10) -> dict:
     return {"best_tile": tile.value, "ms_tile32": ms[0], "ms_tile64": ms[1]}
 
 
-def key(ctx: Context, workload: str) -> str:
-    cu, _ = ctx.device_info()
-    return f"cu{cu}:{workload}"
+def key(cu_count: int, workload: str) -> str:
+    """Cache key: the device's compute-unit count and the workload signature."""
+    return f"cu{int(cu_count)}:{workload}"
 
 
 def load_cache(path: Path = CACHE) -> dict:
@@ -56,13 +56,49 @@ def load_cache(path: Path = CACHE) -> dict:
         return {}
 
 
-def lookup(ctx: Context, workload: str, path: Path = CACHE) -> dict | None:
-    return load_cache(path).get(key(ctx, workload))
+def lookup(cu_count: int, workload: str, path: Path = CACHE) -> dict | None:
+    return load_cache(path).get(key(cu_count, workload))
 
 
-def store(ctx: Context, workload: str, entry: dict, path: Path = CACHE) -> None:
+def store(cu_count: int, workload: str, entry: dict, path: Path = CACHE) -> None:
     data = load_cache(path)
     entry = dict(entry, tuned_at=time.strftime("%Y-%m-%dT%H:%M:%S"))
-    data[key(ctx, workload)] = entry
+    data[key(cu_count, workload)] = entry
     path.parent.mkdir(parents=True, exist_ok=True)
-    path.write_text(json.dumps(data, indent=1, sort_keys=True))
+    tmp = path.with_suffix(".json.tmp")
+    tmp.write_text(json.dumps(data, indent=1, sort_keys=True))
+    tmp.replace(path)
+
+
+def cull_shape(ctx: Context, workload: str, prepare, reps: int = 5, path: Path = CACHE,
+               retune: bool = False) -> tuple[dict, "BatchCull", bool]:
+    """The classify launch shape for `workload` on this device, applied to the
+    context, and a batch prepared with it. `prepare(ctx)` builds and prepares
+    the batch the shape is tuned on. A cached entry is applied before the
+    batch is built; on a miss (or retune) every shape is timed on the batch
+    (hv_autotune_batch, which leaves the batch and the context on the
+    fastest) and the result is stored. Returns (shape, batch, from_cache)."""
+    cu, _ = ctx.device_info()
+    hit = None if retune else lookup(cu, workload, path)
+    if hit is not None:
+        apply(ctx, dict(get_shapes(ctx), wg=hit["wg"], ipt=hit["ipt"]))
+        return hit, prepare(ctx), True
+    batch = prepare(ctx)
+    res = tune_batch(batch, reps)
+    store(cu, workload, dict(res["best"], table=res["table"]), path)
+    return res["best"], batch, False
+
+
+def hiz_tile(ctx: Context, workload: str, hiz: HiZBuildPass, reps: int = 10, path: Path = CACHE,
+             retune: bool = False) -> tuple[dict, bool]:
+    """The Hi-Z tile edge for `workload` (the pyramid's size and mode) on this
+    device, applied to the context: cached, or timed on `hiz` (which must have
+    depth bound) and stored. Returns (entry, from_cache)."""
+    cu, _ = ctx.device_info()
+    hit = None if retune else lookup(cu, workload, path)
+    if hit is not None:
+        apply(ctx, dict(get_shapes(ctx), hiz_tile=hit["best_tile"]))
+        return hit, True
+    res = tune_hiz(hiz, reps)
+    store(cu, workload, res, path)
+    return res, False
