@@ -175,8 +175,14 @@ def serving_leg(ctx, rates, seconds: float, meshlet_fraction: float = 0.25) -> d
 
     out = {"scenes": [int(x) for x in sizes], "meshlet_scene": int(vg.meshlet_count),
            "meshlet_fraction": meshlet_fraction, "rates": {}, "max_requests_per_rate": 30_000}
+    # The batched server grows its slots' pools with the admitted load, each
+    # growth checked against the device's free memory (1 GiB headroom); a
+    # warm-up trace grows them before the measured rates.
     batched = CullServer(ctx, max_batch_views=64, max_inflight=3, max_batch_candidates=64 << 20,
-                         device_budget_bytes=16 << 30, copy_ids=False, use_graph=True, max_vg_draws=16 * vg_cap)
+                         device_budget_bytes=16 << 30, copy_ids=False, use_graph=True, max_vg_draws=16 * vg_cap,
+                         grow_pools=True)
+    warm = trace(2000)
+    batched.run_trace([r for (r, _), _, _ in warm], np.cumsum(np.random.default_rng(1).exponential(1 / 50_000, 2000)))
     plain = CullServer(ctx, max_batch_views=1, max_inflight=1, max_batch_candidates=8 << 20,
                        device_budget_bytes=16 << 30, copy_ids=False, use_graph=False, max_vg_draws=vg_cap)
     for rate in rates:
@@ -205,6 +211,10 @@ def serving_leg(ctx, rates, seconds: float, meshlet_fraction: float = 0.25) -> d
                 f"batch {st['mean_batch_views']:.1f}")
         row["batched_over_plain_views_per_s"] = row["batched"]["views_per_s"] / row["plain"]["views_per_s"]
         out["rates"][str(rate)] = row
+    st = batched.stats()
+    out["batched_server_memory"] = {"grow_pools": True, "pool_bytes": int(st["device_bytes_slots"]),
+                                    "pool_grows": int(st["pool_grows"]), "device_free_bytes": int(st["device_bytes_free"]),
+                                    "memory_waits": int(st["memory_waits"])}
     batched.close()
     plain.close()
     vg.close()

@@ -98,6 +98,8 @@ class ServerConfig(C.Structure):
         ("use_graph", u32),
         ("copy_ids", u32),
         ("max_vg_draws", u32),
+        ("grow_pools", u32),
+        ("device_headroom_bytes", u64),
     ]
 
 
@@ -128,7 +130,8 @@ class ServerStats(C.Structure):
         ("submitted", u64), ("completed", u64), ("rejected", u64), ("batches", u64),
         ("latency_p50_us", C.c_double), ("latency_p99_us", C.c_double), ("latency_max_us", C.c_double),
         ("latency_mean_us", C.c_double), ("mean_batch_views", C.c_double),
-        ("device_bytes_peak", u64), ("device_bytes_slots", u64),
+        ("device_bytes_peak", u64), ("device_bytes_slots", u64), ("device_bytes_free", u64), ("pool_grows", u64),
+        ("memory_waits", u64),
     ]
 
 

@@ -3,13 +3,18 @@
 // concurrent launches (one stream and one CullBatch per slot).
 //
 // One worker thread owns the device side. Each turn it
-//   1. retires slots whose completion event fired: copies results out of the
-//      slot's pinned staging into the requesters' buffers, returns their
-//      device bytes to the budget and wakes the waiters;
+//   1. retires slots whose completion event fired. A slot with published
+//      meshlet commands to return first enqueues their copies (exactly the
+//      published count, now known) into pinned staging on its stream and
+//      records a second event; the worker never waits on a copy. A retired
+//      slot's results go from pinned staging into the requesters' buffers,
+//      their device bytes return to the budget and the waiters wake;
 //   2. fills every free slot from the FIFO queue: requests join the batch while
-//      the batch has view and candidate room and the admitted-bytes total stays
-//      within the budget (no reordering: a head request that does not fit
-//      waits for memory to free up);
+//      the batch has view and candidate room, the admitted-bytes total stays
+//      within the budget and, when pools grow with the load, the device's free
+//      memory (hipMemGetInfo) covers the slot's growth above the headroom (no
+//      reordering: a head request that does not fit waits; one that cannot
+//      fit even alone with nothing in flight fails with HV_ENOMEM);
 //   3. launches the batch (graph replay), enqueues the result copies on the
 //      slot's stream and records the slot's completion event.
 // Requests that arrive while batches are on the GPU are therefore admitted as
@@ -59,6 +64,9 @@ struct Slot {
     std::vector<int32_t> vg_req;     // request -> meshlet request of the launch (-1: none)
     std::vector<uint32_t> vg_off;    // meshlet request -> first slot in the VG outputs
     bool busy = false;
+    bool copying = false;              // meshlet copies enqueued after `done`; `copied` ends them
+    hipEvent_t copied = nullptr;
+    uint64_t grew = 0;                 // pool bytes added by the last launch (grow_pools)
     uint32_t* h_ctr = nullptr;
     hv_draw_indexed_indirect* h_args = nullptr;
     uint32_t* h_ids = nullptr;
@@ -68,6 +76,11 @@ struct Slot {
     VgView* h_vg_views = nullptr;
     uint32_t* h_vg_begin = nullptr;
     uint32_t* h_vg_ctr = nullptr;
+    // Pinned staging of the published meshlet commands (grown on demand).
+    hv_draw_indexed_indirect* h_vg_ind = nullptr;
+    hv_vg_draw* h_vg_md = nullptr;
+    uint64_t vg_stage_cap = 0;         // entries
+    std::vector<uint64_t> vg_stage_off;  // meshlet request -> first staged entry
 };
 
 }  // namespace
@@ -82,6 +95,7 @@ struct hv_server {
     std::vector<Slot> slots;
     uint64_t next_ticket = 1;
     uint64_t admitted_bytes = 0, peak_bytes = 0, slot_bytes = 0;
+    uint64_t headroom = 0, free_bytes = 0, n_grows = 0, n_mem_waits = 0;
     uint64_t n_submitted = 0, n_completed = 0, n_rejected = 0, n_batches = 0, n_batch_views = 0;
     std::vector<double> lat_us;
     bool stop = false;
@@ -112,15 +126,12 @@ static void deliver(hv_server* s, Slot& slot) {
             const uint32_t published = std::min(ctr[0], q->r.vg.capacity);
             q->visible += published;
             if (q->r.vg.counters_out) std::memcpy(q->r.vg.counters_out, ctr, HV_VG_COUNTER_COUNT * 4);
-            // Exactly the published commands, now that the count is known.
-            hipError_t e = hipSuccess;
-            if (q->r.vg.indirect_out && published)
-                e = hipMemcpy(q->r.vg.indirect_out, slot.d_vg_ind.as<hv_draw_indexed_indirect>() + slot.vg_off[k],
-                              (size_t)published * 20, hipMemcpyDeviceToHost);
-            if (e == hipSuccess && q->r.vg.metadata_out && published)
-                e = hipMemcpy(q->r.vg.metadata_out, slot.d_vg_md.as<hv_vg_draw>() + slot.vg_off[k],
-                              (size_t)published * 16, hipMemcpyDeviceToHost);
-            if (e != hipSuccess) q->status = hip_error(e, "hv_server (meshlet results)", __FILE__, __LINE__);
+            // The published commands, staged by stage_vg_results.
+            if (published && (q->r.vg.indirect_out || q->r.vg.metadata_out)) {
+                const uint64_t o = slot.vg_stage_off[k];
+                if (q->r.vg.indirect_out) std::memcpy(q->r.vg.indirect_out, slot.h_vg_ind + o, (size_t)published * 20);
+                if (q->r.vg.metadata_out) std::memcpy(q->r.vg.metadata_out, slot.h_vg_md + o, (size_t)published * 16);
+            }
         }
     }
     const Clock::time_point now = Clock::now();
@@ -134,7 +145,56 @@ static void deliver(hv_server* s, Slot& slot) {
     }
     slot.reqs.clear();
     slot.busy = false;
+    slot.copying = false;
     s->cv_done.notify_all();
+}
+
+// After a slot's launch completed: enqueue the copies of the published
+// meshlet commands of the requests that want them (exactly the published
+// count, read from the staged counters) into pinned staging on the slot's
+// stream, and record `copied`. Sets slot.copying when anything was enqueued.
+static hv_status stage_vg_results(Slot& slot) {
+    const uint32_t n = (uint32_t)slot.reqs.size();
+    slot.vg_stage_off.assign(slot.vg_off.size(), 0);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const int32_t k = slot.vg_req[i];
+        const hv_vg_request& r = slot.reqs[i]->r.vg;
+        if (k < 0 || (!r.indirect_out && !r.metadata_out)) continue;
+        slot.vg_stage_off[k] = total;
+        total += std::min(slot.h_vg_ctr[(size_t)k * HV_VG_COUNTER_COUNT], r.capacity);
+    }
+    if (!total) return HV_OK;
+    if (total > slot.vg_stage_cap) {
+        // The slot's stream is idle (its launch completed): the old staging is free.
+        uint64_t cap = std::max<uint64_t>(slot.vg_stage_cap, 1u << 14);
+        while (cap < total) cap *= 2;
+        if (slot.h_vg_ind) (void)hipHostFree(slot.h_vg_ind);
+        if (slot.h_vg_md) (void)hipHostFree(slot.h_vg_md);
+        slot.h_vg_ind = nullptr;
+        slot.h_vg_md = nullptr;
+        slot.vg_stage_cap = 0;
+        HV_HIP(hipHostMalloc((void**)&slot.h_vg_ind, (size_t)cap * 20, hipHostMallocDefault));
+        HV_HIP(hipHostMalloc((void**)&slot.h_vg_md, (size_t)cap * 16, hipHostMallocDefault));
+        slot.vg_stage_cap = cap;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        const int32_t k = slot.vg_req[i];
+        const hv_vg_request& r = slot.reqs[i]->r.vg;
+        if (k < 0 || (!r.indirect_out && !r.metadata_out)) continue;
+        const uint32_t published = std::min(slot.h_vg_ctr[(size_t)k * HV_VG_COUNTER_COUNT], r.capacity);
+        if (!published) continue;
+        const uint64_t o = slot.vg_stage_off[k];
+        if (r.indirect_out)
+            HV_HIP(hipMemcpyAsync(slot.h_vg_ind + o, slot.d_vg_ind.as<hv_draw_indexed_indirect>() + slot.vg_off[k],
+                                  (size_t)published * 20, hipMemcpyDeviceToHost, slot.stream));
+        if (r.metadata_out)
+            HV_HIP(hipMemcpyAsync(slot.h_vg_md + o, slot.d_vg_md.as<hv_vg_draw>() + slot.vg_off[k],
+                                  (size_t)published * 16, hipMemcpyDeviceToHost, slot.stream));
+    }
+    HV_HIP(hipEventRecord(slot.copied, slot.stream));
+    slot.copying = true;
+    return HV_OK;
 }
 
 static hv_status launch_slot(hv_server* s, Slot& slot) {
@@ -163,8 +223,18 @@ static hv_status launch_slot(hv_server* s, Slot& slot) {
         io += q->r.view.count;
     }
     const uint32_t nv = (uint32_t)vd.size();
+    slot.grew = 0;
     if (nv) {
+        const uint64_t before = slot.batch->device_bytes();
+        if (s->cfg.grow_pools) {
+            // Pools sized from the admitted batch (the admission checked the
+            // device's free memory for this growth).
+            uint32_t max_nb = 1;
+            for (uint32_t i = 0; i < n; ++i) max_nb = std::max(max_nb, slot.reqs[i]->nb);
+            HV_TRY(slot.batch->reserve(nv, io, max_nb));
+        }
         HV_TRY(slot.batch->prepare(vd.data(), nv));
+        slot.grew = slot.batch->device_bytes() - before;
         HV_TRY(s->cfg.use_graph ? slot.batch->execute_graph(slot.stream) : slot.batch->execute(slot.stream));
         // Result copies ride on the same stream, into pinned staging.
         hv_cull_outputs o0;
@@ -247,49 +317,106 @@ static void worker_main(hv_server* s) {
         bool any_busy = false;
         for (Slot& sl : s->slots) {
             if (!sl.busy) continue;
-            const hipError_t q = hipEventQuery(sl.done);
+            const bool was_copying = sl.copying;
+            const hipError_t q = hipEventQuery(was_copying ? sl.copied : sl.done);
+            hv_status st = HV_OK;
             if (q == hipSuccess) {
                 lk.unlock();
-                deliver(s, sl);
-                lk.lock();
-            } else if (q != hipErrorNotReady) {
-                s->worker_error = hip_error(q, "hipEventQuery", __FILE__, __LINE__);
-                for (Req* r : sl.reqs) {
-                    r->status = s->worker_error;
-                    r->done = true;
-                    s->admitted_bytes -= r->bytes;
+                if (!was_copying) st = stage_vg_results(sl);
+                if (st == HV_OK && sl.copying && !was_copying) {
+                    // Copies just enqueued: retired when `copied` fires.
+                    lk.lock();
+                    any_busy = true;
+                    continue;
                 }
-                sl.reqs.clear();
-                sl.busy = false;
-                s->cv_done.notify_all();
+                if (st == HV_OK) deliver(s, sl);
+                lk.lock();
+                if (st == HV_OK) continue;
+            } else if (q != hipErrorNotReady) {
+                st = hip_error(q, "hipEventQuery", __FILE__, __LINE__);
             } else {
                 any_busy = true;
+                continue;
             }
+            s->worker_error = st;
+            for (Req* r : sl.reqs) {
+                r->status = st;
+                r->done = true;
+                s->admitted_bytes -= r->bytes;
+            }
+            sl.reqs.clear();
+            sl.busy = false;
+            sl.copying = false;
+            s->cv_done.notify_all();
         }
-        // 2. Fill free slots from the queue (FIFO, budget-limited).
-        bool launched = false;
+        // 2. Fill free slots from the queue (FIFO, budget-limited; with growing
+        // pools also limited by the device's free memory, queried at most once
+        // per pass).
+        bool launched = false, have_free = false, head_mem_blocked = false;
+        uint64_t free_now = 0;
+        auto device_free = [&]() -> uint64_t {
+            if (!have_free) {
+                size_t f = 0, t = 0;
+                if (hipMemGetInfo(&f, &t) == hipSuccess) free_now = f;
+                have_free = true;
+                s->free_bytes = free_now;
+            }
+            return free_now;
+        };
         for (Slot& sl : s->slots) {
             if (sl.busy || s->queue.empty()) continue;
-            uint64_t cand = 0, vg_draws = 0;
+            uint64_t cand = 0, vg_draws = 0, growth = 0;
+            uint32_t views = 0, max_nb = 1;
+            bool mem_cut = false;
             while (!s->queue.empty() && sl.reqs.size() < s->cfg.max_batch_views) {
                 Req* q = s->queue.front();
                 if (cand + q->r.view.count > s->cfg.max_batch_candidates) break;
                 if (q->r.vg.scene && vg_draws + q->r.vg.capacity > s->cfg.max_vg_draws) break;
                 if (s->admitted_bytes + q->bytes > s->cfg.device_budget_bytes) break;
+                if (s->cfg.grow_pools && q->r.view.scene) {
+                    // The slot's pools after this request joins; buffers grow by
+                    // doubling, so a growth never allocates more than twice that.
+                    const uint64_t need = CullBatch::bytes_for(cand + q->r.view.count, views + 1,
+                                                               std::max(max_nb, q->nb), sl.batch->shape.chunk(),
+                                                               CullBatch::max_grid(s->ctx));
+                    const uint64_t have = sl.batch->device_bytes();
+                    const uint64_t g = need > have ? 2 * need - have : 0;
+                    if (g > growth) {
+                        if (device_free() < g + s->headroom) {
+                            mem_cut = true;
+                            break;
+                        }
+                        growth = g;
+                    }
+                }
                 s->queue.pop_front();
                 s->admitted_bytes += q->bytes;
                 s->peak_bytes = std::max(s->peak_bytes, s->admitted_bytes);
                 cand += q->r.view.count;
+                if (q->r.view.scene) {
+                    ++views;
+                    max_nb = std::max(max_nb, q->nb);
+                }
                 if (q->r.vg.scene) vg_draws += q->r.vg.capacity;
                 sl.reqs.push_back(q);
             }
-            if (sl.reqs.empty()) break;  // head request waits for memory
+            if (mem_cut) ++s->n_mem_waits;
+            if (sl.reqs.empty()) {
+                if (mem_cut) {  // another free slot may hold larger pools already
+                    head_mem_blocked = true;
+                    continue;
+                }
+                break;  // head request waits for the budget
+            }
+            free_now -= std::min(free_now, growth);
             sl.busy = true;
             ++s->n_batches;
             s->n_batch_views += sl.reqs.size();
             lk.unlock();
             const hv_status st = launch_slot(s, sl);
             lk.lock();
+            s->slot_bytes += sl.grew;
+            if (sl.grew) ++s->n_grows;
             if (st != HV_OK) {
                 for (Req* r : sl.reqs) {
                     r->status = st;
@@ -303,6 +430,18 @@ static void worker_main(hv_server* s) {
                 launched = true;
                 any_busy = true;
             }
+        }
+        // A head request that no free slot can grow to hold, with nothing in
+        // flight whose completion could change that: it can never be served.
+        if (head_mem_blocked && !launched && !any_busy && !s->queue.empty()) {
+            Req* q = s->queue.front();
+            s->queue.pop_front();
+            q->status = HV_ENOMEM;
+            q->t_done = Clock::now();
+            q->done = true;
+            ++s->n_rejected;
+            s->cv_done.notify_all();
+            continue;
         }
         if (s->stop && s->queue.empty() && !any_busy) break;
         if (launched) continue;
@@ -333,14 +472,18 @@ hv_status hv_server_create(hv_ctx* ctx, const hv_server_config* cfg, hv_server**
     s->ctx = ctx;
     s->cfg = *cfg;
     s->slots.resize(cfg->max_inflight);
+    s->headroom = cfg->device_headroom_bytes ? cfg->device_headroom_bytes : (1ull << 30);
     hv_status st = HV_OK;
     for (Slot& sl : s->slots) {
         sl.batch = new CullBatch(ctx, cfg->cull, cfg->max_batch_views);
-        st = sl.batch->reserve(cfg->max_batch_views, cfg->max_batch_candidates, cfg->max_buckets);
-        if (st != HV_OK) break;
-        s->slot_bytes += sl.batch->device_bytes();
+        if (!cfg->grow_pools) {
+            st = sl.batch->reserve(cfg->max_batch_views, cfg->max_batch_candidates, cfg->max_buckets);
+            if (st != HV_OK) break;
+            s->slot_bytes += sl.batch->device_bytes();
+        }
         hipError_t e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming);
         if (e == hipSuccess)
             e = hipHostMalloc((void**)&sl.h_ctr, (size_t)cfg->max_batch_views * kCounterStride * 4, hipHostMallocDefault);
         if (e == hipSuccess)
@@ -371,6 +514,12 @@ hv_status hv_server_create(hv_ctx* ctx, const hv_server_config* cfg, hv_server**
             break;
         }
     }
+    if (st == HV_OK) {
+        size_t f = 0, t = 0;
+        const hipError_t e = hipMemGetInfo(&f, &t);
+        if (e != hipSuccess) st = hip_error(e, "hipMemGetInfo", __FILE__, __LINE__);
+        s->free_bytes = f;
+    }
     if (st != HV_OK) {
         hv_server_destroy(s);
         return st;
@@ -393,6 +542,9 @@ void hv_server_destroy(hv_server* s) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
         delete sl.batch;
         if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.copied) (void)hipEventDestroy(sl.copied);
+        if (sl.h_vg_ind) (void)hipHostFree(sl.h_vg_ind);
+        if (sl.h_vg_md) (void)hipHostFree(sl.h_vg_md);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
         if (sl.h_ctr) (void)hipHostFree(sl.h_ctr);
         if (sl.h_args) (void)hipHostFree(sl.h_args);
@@ -499,6 +651,9 @@ hv_status hv_server_stats_get(hv_server* s, hv_server_stats* out) {
     out->mean_batch_views = s->n_batches ? (double)s->n_batch_views / s->n_batches : 0.0;
     out->device_bytes_peak = s->peak_bytes;
     out->device_bytes_slots = s->slot_bytes;
+    out->device_bytes_free = s->free_bytes;
+    out->pool_grows = s->n_grows;
+    out->memory_waits = s->n_mem_waits;
     if (!s->lat_us.empty()) {
         std::vector<double> v = s->lat_us;
         std::sort(v.begin(), v.end());

@@ -644,7 +644,7 @@ class CullServer:
     def __init__(self, ctx: Context, max_batch_views: int = 64, max_inflight: int = 2,
                  max_batch_candidates: int = 1 << 24, device_budget_bytes: int = 1 << 30, max_buckets: int = 1024,
                  lod_distances=(25.0, 75.0, 200.0), use_graph: bool = True, copy_ids: bool = True,
-                 max_vg_draws: int = 0):
+                 max_vg_draws: int = 0, grow_pools: bool = False, device_headroom_bytes: int = 0):
         self.ctx = ctx
         c = ServerConfig()
         c.cull = cull_config(lod_distances)
@@ -652,6 +652,7 @@ class CullServer:
         c.max_batch_candidates, c.device_budget_bytes = max_batch_candidates, device_budget_bytes
         c.max_buckets, c.use_graph, c.copy_ids = max_buckets, int(use_graph), int(copy_ids)
         c.max_vg_draws = max_vg_draws
+        c.grow_pools, c.device_headroom_bytes = int(grow_pools), int(device_headroom_bytes)
         h = C.c_void_p()
         call("hv_server_create", ctx.handle, C.byref(c), C.byref(h))
         self.handle = h

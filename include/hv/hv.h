@@ -790,6 +790,14 @@ typedef struct hv_server_config {
     uint32_t copy_ids;              /* copy visible ids back for requests that give ids_out */
     uint32_t max_vg_draws;          /* meshlet draws per launch (sizes each slot's VG outputs);
                                        0: requests with a meshlet part are refused */
+    uint32_t grow_pools;            /* 0: each slot's cull pools are allocated for max_batch_candidates at
+                                       creation (HV_ENOMEM if they do not fit the device's free memory);
+                                       1: pools start empty and grow with the batches admitted, each growth
+                                       only while the device's free memory (hipMemGetInfo) stays above
+                                       device_headroom_bytes; a batch that would not fit waits, and a head
+                                       request that cannot fit even alone, with nothing in flight, fails
+                                       with HV_ENOMEM */
+    uint64_t device_headroom_bytes; /* device memory left free by pool growth (0: 1 GiB) */
 } hv_server_config;
 
 /* Meshlet part of a server request: VirtualGeometryPass semantics
@@ -822,6 +830,9 @@ typedef struct hv_server_stats {
     double mean_batch_views;
     uint64_t device_bytes_peak;   /* peak admitted-request bytes */
     uint64_t device_bytes_slots;  /* slot buffers held by the server */
+    uint64_t device_bytes_free;   /* device free memory at the last admission check */
+    uint64_t pool_grows;          /* slot pool growths (grow_pools) */
+    uint64_t memory_waits;        /* admission passes cut short by device memory */
 } hv_server_stats;
 
 typedef struct hv_server hv_server;

@@ -222,3 +222,38 @@ def test_meshlet_overflow_in_server(ctx):
     assert len(np.unique(keys[:, 0].astype(np.uint64) * 1024 + keys[:, 1])) == 50_000
     srv.close()
     vgs.close()
+
+
+# ---- admission by device memory ----------------------------------------------
+
+def test_pools_grow_with_the_admitted_load(ctx):
+    """grow_pools: the slots start without cull pools and grow them for the
+    batches they admit, each growth checked against the device's free memory;
+    results equal the plain pass, and the growth is reported."""
+    scenes = _scenes(ctx)
+    reqs = _requests(scenes, 40, seed=7)
+    srv = CullServer(ctx, max_batch_views=16, max_inflight=2, max_batch_candidates=8_000_000,
+                     device_budget_bytes=1 << 34, grow_pools=True, device_headroom_bytes=256 << 20)
+    assert srv.stats()["device_bytes_slots"] == 0 and srv.stats()["device_bytes_free"] > 0
+    tickets = [srv.submit(*r) for r in reqs]
+    for t, r in zip(tickets, reqs):
+        _same(srv.wait(t, 60_000), _plain(ctx, r))
+    st = srv.stats()
+    assert st["pool_grows"] >= 1 and st["device_bytes_slots"] > 0 and st["rejected"] == 0
+    srv.close()
+
+
+def test_request_that_can_never_fit_the_free_memory_is_refused(ctx):
+    """A headroom above the device's memory leaves no room for any pool: the
+    head request, with nothing in flight, fails with HV_ENOMEM instead of
+    waiting forever; meshlet-free requests are the ones that need pools."""
+    scenes = _scenes(ctx)
+    (s, cam, f, c), = _requests(scenes, 1, seed=8)
+    srv = CullServer(ctx, max_batch_views=4, max_inflight=1, max_batch_candidates=1_000_000,
+                     device_budget_bytes=1 << 34, grow_pools=True, device_headroom_bytes=1 << 50)
+    t = srv.submit(s, cam, f, c)
+    with pytest.raises(HvError):
+        srv.wait(t, 60_000)
+    st = srv.stats()
+    assert st["rejected"] == 1 and st["memory_waits"] >= 1 and st["device_bytes_slots"] == 0
+    srv.close()
