@@ -39,3 +39,16 @@ def canonical_ids(ids: np.ndarray, args: np.ndarray) -> np.ndarray:
 def args_as_u32(args: np.ndarray) -> np.ndarray:
     """Structured DRAW_INDEXED_INDIRECT array -> (n, 5) u32 view."""
     return np.ascontiguousarray(args).view(np.uint32).reshape(-1, 5)
+
+
+def check_cull_invariants(ids: np.ndarray, args: np.ndarray, stats: np.ndarray) -> None:
+    """The compaction contract of one view, independent of any reference:
+    bucket counts sum to the ids, first_instance is their exclusive prefix,
+    stats[0] = stats[1] + stats[2] + stats[3], and the ids are the drawn
+    candidates that the Hi-Z test kept (stats[3] - stats[4])."""
+    a = args_as_u32(args) if args.dtype.names else np.asarray(args)
+    cnt = a[:, 1].astype(np.int64)
+    assert cnt.sum() == len(ids)
+    assert np.array_equal(a[:, 4].astype(np.int64), np.concatenate([[0], np.cumsum(cnt)[:-1]]))
+    assert int(stats[0]) == int(stats[1]) + int(stats[2]) + int(stats[3])
+    assert len(ids) == int(stats[3]) - int(stats[4])

@@ -9,7 +9,7 @@ import oracle
 from helio_amd import layouts as L
 from helio_amd import synth
 from helio_amd.passes import BatchCull, InstanceCullPass, Scene, Stream, extract_frustum_planes
-from tests.conftest import args_as_u32, canonical_ids
+from tests.conftest import args_as_u32, canonical_ids, check_cull_invariants
 
 pytestmark = pytest.mark.gpu
 
@@ -29,6 +29,7 @@ def _oracle(sc, cam, first, count):
 def _check(got, ref):
     ids, args, stats = got
     rids, rargs, rstats = ref
+    check_cull_invariants(ids, args, stats)
     assert np.array_equal(stats, rstats), (stats, rstats)
     assert np.array_equal(args_as_u32(args), rargs)
     assert np.array_equal(canonical_ids(ids, args), rids)

@@ -12,7 +12,7 @@ import oracle
 from helio_amd import layouts as L
 from helio_amd import synth
 from helio_amd.passes import HiZBuildPass, InstanceCullPass, Scene
-from tests.conftest import args_as_u32, canonical_ids
+from tests.conftest import args_as_u32, canonical_ids, check_cull_invariants
 
 pytestmark = pytest.mark.gpu
 
@@ -43,6 +43,7 @@ def _oracle(sc, cam, first=0, count=None, pyr=None, flags=0, aabb6=None, lod=(25
 def _check(gpu, ref):
     ids, args, stats = gpu
     rids, rargs, rstats = ref
+    check_cull_invariants(ids, args, stats)
     assert np.array_equal(stats, rstats), (stats, rstats)
     assert np.array_equal(args_as_u32(args), rargs)
     assert len(ids) == len(rids)

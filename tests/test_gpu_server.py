@@ -8,7 +8,7 @@ import pytest
 from helio_amd import synth
 from helio_amd._lib import HvError
 from helio_amd.passes import CullServer, InstanceCullPass, Scene
-from tests.conftest import args_as_u32, canonical_ids
+from tests.conftest import args_as_u32, canonical_ids, check_cull_invariants
 
 pytestmark = pytest.mark.gpu
 
@@ -51,6 +51,7 @@ def _plain(ctx, req):
 def _same(a, b):
     ids, args, st = a
     rids, rargs, rst = b
+    check_cull_invariants(ids, args, st)
     assert np.array_equal(st, rst)
     assert np.array_equal(args_as_u32(args), args_as_u32(rargs))
     assert np.array_equal(canonical_ids(ids, args), canonical_ids(rids, rargs))

@@ -19,7 +19,7 @@ import oracle
 from helio_amd import layouts as L
 from helio_amd import synth
 from helio_amd.passes import CullServer, Scene, VgScene, build_vg_work_items
-from tests.conftest import args_as_u32, canonical_ids
+from tests.conftest import args_as_u32, canonical_ids, check_cull_invariants
 from tests.vg_util import oracle_vg, vg_uniforms
 
 pytestmark = pytest.mark.gpu
@@ -95,6 +95,7 @@ def test_serving_workload_full_size_matches_oracle(ctx):
         assert stats[8] == len(rids), k
         assert np.array_equal(args_as_u32(args), rargs), k
         if q["sampled"]:
+            check_cull_invariants(ids[: stats[8]], args, stats[:8])
             assert np.array_equal(canonical_ids(ids[: stats[8]], args), rids), k
             n_ids += 1
     icd = _oracle_icd(vgsc.instances)
