@@ -10,13 +10,16 @@
 //                 :354-405): wave 0 decodes the 64 items at once (one lane
 //                 each, recomputing the object's LOD so no cull waits for a
 //                 select workgroup), compacts the live spans into LDS with the
-//                 instance transforms, then all 512 threads cull the flattened
-//                 meshlets (cull_meshlet, :135-264), each thread loading its
-//                 next descriptor while it tests the current one (the per-view
-//                 uniforms are read from an LDS copy). Survivors are staged in LDS
-//                 and ONE slot reservation (a single-address atomic, which
-//                 serialises device-wide) covers the workgroup; the
-//                 capacity-overflow count is derived from the same reservation.
+//                 instance transforms, then all 512 threads run the frustum and
+//                 cone tests of cull_meshlet (:135-182) over the flattened
+//                 meshlets, each thread loading its next descriptor while it
+//                 tests the current one (the per-view uniforms are read from an
+//                 LDS copy); the survivors of sparse waves are queued in LDS and
+//                 take the Hi-Z test (:184-240) afterwards in dense waves.
+//                 Survivors are staged in LDS and ONE slot reservation (a
+//                 single-address atomic, which serialises device-wide) covers
+//                 the workgroup; the capacity-overflow count is derived from the
+//                 same reservation.
 // A meshlet's work is therefore two dependent rounds of loads (work item ->
 // object and transform, then descriptor -> Hi-Z taps) instead of a chain per
 // 64-meshlet span, and partial spans (LODs smaller than max_meshlet_count, or
@@ -100,17 +103,29 @@ __device__ __forceinline__ void vload(const float* src, float* dst) {
     for (int i = 0; i < N; ++i) dst[i] = src[i];
 }
 
-// cull_meshlet (vg_cull.wgsl:135-264) for one lane; true = publish.
+// The cone test's cosine, |t.a| / (|t||a|), computed as the reference does
+// (normalised axis and direction, IEEE divisions and square roots) lies within
+// about 12 units of 2^-24 of the real cosine of the two float vectors; the
+// estimate dot(t, a) * rsq(a.a) * rsq(t.t) within about 14 (v_rsq_f32 is good
+// to 1 ulp, and the inputs are kept in the normal range). A margin of 2^-16
+// (256 such units) therefore decides every comparison the estimate is clear
+// of; the rest take the reference's arithmetic.
+constexpr float kConeMargin = 0x1p-16f;
+
+// The frustum and cone tests of cull_meshlet (vg_cull.wgsl:135-182) for one
+// lane; false = culled. Returns the world sphere for the Hi-Z test.
 // The view `a` is the workgroup's LDS copy. Compiler-only fences (asm with a
 // memory clobber) keep each section's LDS reads after the previous section
 // and inside the caller's loop, so the uniforms are re-read where they are
 // used (broadcast LDS reads) instead of ~60 of them staying live in registers
 // across the workgroup loop.
-__device__ __forceinline__ bool cull_meshlet(const VgView& a, const float* xf, float4 q0, float4 q1, float4 q2) {
+__device__ __forceinline__ bool cull_front(const VgView& a, const float* xf, float4 q0, float4 q1, float4 q2,
+                                           float& cx, float& cy, float& cz, float& wr) {
     asm volatile("" ::: "memory");
-    const float cx = xrow(xf, 0, q0.x, q0.y, q0.z), cy = xrow(xf, 1, q0.x, q0.y, q0.z),
-                cz = xrow(xf, 2, q0.x, q0.y, q0.z);
-    const float wr = wmax(q0.w * xf[kXfScale], 0.0f);
+    cx = xrow(xf, 0, q0.x, q0.y, q0.z);
+    cy = xrow(xf, 1, q0.x, q0.y, q0.z);
+    cz = xrow(xf, 2, q0.x, q0.y, q0.z);
+    wr = wmax(q0.w * xf[kXfScale], 0.0f);
     {
         float pl[24];
         vload<24>(a.planes, pl);
@@ -126,20 +141,37 @@ __device__ __forceinline__ bool cull_meshlet(const VgView& a, const float* xf, f
         const float ax = (xf[12] * q2.x + xf[15] * q2.y) + xf[18] * q2.z;
         const float ay = (xf[13] * q2.x + xf[16] * q2.y) + xf[19] * q2.z;
         const float az = (xf[14] * q2.x + xf[17] * q2.y) + xf[20] * q2.z;
-        const float alen = sqrtf(dot3(ax, ay, az, ax, ay, az));
-        const float nax = ax / alen, nay = ay / alen, naz = az / alen;
         const float px = xrow(xf, 0, q1.x, q1.y, q1.z), py = xrow(xf, 1, q1.x, q1.y, q1.z),
                     pz = xrow(xf, 2, q1.x, q1.y, q1.z);
         const float tax = px - camx, tay = py - camy, taz = pz - camz;
         const float ad2 = dot3(tax, tay, taz, tax, tay, taz);
         if (ad2 > 1.0e-12f) {
-            const float s = sqrtf(ad2);
-            if (dot3(tax / s, tay / s, taz / s, nax, nay, naz) >= q1.w) return false;
+            const float da = dot3(ax, ay, az, ax, ay, az);
+            bool cull = false, decided = false;
+            if (da >= 0x1p-100f && da <= 0x1p100f && ad2 <= 0x1p100f) {
+                const float est =
+                    dot3(tax, tay, taz, ax, ay, az) * (__builtin_amdgcn_rsqf(da) * __builtin_amdgcn_rsqf(ad2));
+                cull = est >= q1.w + kConeMargin;
+                decided = cull || est < q1.w - kConeMargin;
+            }
+            if (!decided) {
+                const float alen = sqrtf(da);
+                const float nax = ax / alen, nay = ay / alen, naz = az / alen;
+                const float s = sqrtf(ad2);
+                cull = dot3(tax / s, tay / s, taz / s, nax, nay, naz) >= q1.w;
+            }
+            if (cull) return false;
         }
     }
-    // Conservative Hi-Z test.
+    return true;
+}
+
+// The conservative Hi-Z test of cull_meshlet (vg_cull.wgsl:184-240) for the
+// world sphere (cx, cy, cz, wr); false = occluded.
+__device__ __forceinline__ bool cull_hiz(const VgView& a, float cx, float cy, float cz, float wr) {
     asm volatile("" ::: "memory");
-    if (a.has_hiz == 0u) return true;
+    const float tcx = cx - a.camx, tcy = cy - a.camy, tcz = cz - a.camz;
+    const float cam_dist_sq = dot3(tcx, tcy, tcz, tcx, tcy, tcz);
     float vp[16];
     vload<16>(a.vp, vp);
     const float w = mrow(vp, 3, cx, cy, cz);
@@ -192,6 +224,7 @@ constexpr uint32_t kVgSpans = 64;
 constexpr uint32_t kVgMinSpans = 8;
 constexpr uint32_t kVgTargetBlocks = 512;
 constexpr uint32_t kVgUnroll = 4;
+constexpr uint32_t kVgDenseLanes = 48;  // front survivors of a wave that take the Hi-Z test in place
 
 struct VgLds {
     float xf[kVgSpans][kXfWords];
@@ -199,9 +232,11 @@ struct VgLds {
     uint32_t lod[kVgSpans];
     uint32_t pref[kVgSpans + 1];      // exclusive prefix of the spans' meshlet counts; [K] = total
     uint16_t stage[kVgSpans * 64];    // survivors: span << 6 | meshlet within the span
+    uint16_t queue[kVgSpans * 64];    // frustum and cone survivors awaiting the Hi-Z test (same form)
+    uint8_t owner[kVgSpans * 64];     // flattened meshlet -> its span
     VgView v;                         // the per-view uniforms
     uint32_t hist[8];
-    uint32_t nspans, nsurv, base, max_lod;
+    uint32_t nspans, nsurv, nqueue, base, max_lod;
 };
 
 // One select workgroup: kVgThreads objects (cs_select_objects). Writes
@@ -230,14 +265,29 @@ __device__ __forceinline__ void vg_select_block(const VgArgs& a, uint32_t block,
     }
 }
 
+// Appends the lanes of `vis` to an LDS list (one LDS atomic per wave).
+__device__ __forceinline__ void wave_append(bool vis, uint32_t* count, uint16_t* list, uint16_t entry) {
+    const uint64_t m = __builtin_amdgcn_ballot_w64(vis);
+    if (m) {
+        uint32_t wb = 0;
+        if ((threadIdx.x & 63u) == 0u) wb = atomicAdd(count, (uint32_t)__popcll(m));
+        wb = __shfl(wb, 0);
+        if (vis) list[wb + mbcnt(m)] = entry;
+    }
+}
+
 // One cull workgroup: kVgSpans work items (cs_cull_meshlets for each).
 //   A. wave 0 decodes one work item per lane: the object's LOD (recomputed
 //      with select_lod, so the cull does not wait for the select workgroups),
 //      the span of the selected LOD, the instance's transform; live spans are
-//      compacted into LDS with their meshlet-count prefix;
+//      compacted into LDS with their meshlet-count prefix, and each flattened
+//      meshlet gets its span in an owner table;
 //   B. the kVgThreads threads walk the flattened meshlets of the live spans,
-//      the next descriptor loading while the current one is tested, and
-//      stage survivors in LDS;
+//      the next descriptor loading while the current one is tested by the
+//      frustum and cone tests. Without a pyramid the survivors are staged for
+//      publication; with one they are queued, and the Hi-Z test (the larger
+//      half of the arithmetic) then runs over the queue in dense waves, so
+//      lanes culled by the frustum or the cone do not idle through it;
 //   C. one slot reservation for the workgroup, then the survivors' commands and
 //      metadata are written to consecutive slots.
 __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, VgLds& L) {
@@ -267,7 +317,7 @@ __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, V
                 }
             }
         }
-        const uint64_t m = __ballot(ok);
+        const uint64_t m = __builtin_amdgcn_ballot_w64(ok);
         const uint32_t k = mbcnt(m);
         uint32_t v = ok ? n : 0u;
 #pragma unroll
@@ -299,20 +349,25 @@ __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, V
             L.nspans = K;
             L.pref[K] = v;
             L.nsurv = 0u;
+            L.nqueue = 0u;
         }
     }
     __syncthreads();
     const uint32_t K = L.nspans, T = L.pref[K];
+    // The owner table: wave w fills the entries of spans w, w + 8, ...
+    for (uint32_t k = tid >> 6; k < K; k += kVgThreads / 64u) {
+        const uint32_t p0 = L.pref[k];
+        if (p0 + lane < L.pref[k + 1]) L.owner[p0 + lane] = (uint8_t)k;
+    }
+    __syncthreads();
     // Flattened meshlet f -> (span, meshlet within the span, meshlet index),
     // and the descriptor fetched one step ahead of the cull that uses it.
     auto locate = [&](uint32_t f, uint32_t& k, uint32_t& li) -> uint32_t {
-        k = 0;
-#pragma unroll
-        for (uint32_t step = 32; step; step >>= 1)
-            if (k + step < K && L.pref[k + step] <= f) k += step;  // largest k with pref[k] <= f
+        k = L.owner[f];
         li = f - L.pref[k];
         return __float_as_uint(L.xf[k][kXfFirst]) + li;
     };
+    const bool hiz = v.has_hiz != 0u;
     uint32_t nk = 0, nli = 0;
     bool nact = false;
     float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0, n2 = n0;
@@ -343,16 +398,43 @@ __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, V
                 n2 = mp[2];
             }
         }
-        const bool vis = cact && cull_meshlet(v, L.xf[ck], c0, c1, c2);
-        const uint64_t m = __ballot(vis);
-        if (m) {
-            uint32_t wb = 0;
-            if (lane == 0) wb = atomicAdd(&L.nsurv, (uint32_t)__popcll(m));
-            wb = __shfl(wb, 0);
-            if (vis) L.stage[wb + mbcnt(m)] = (uint16_t)((ck << 6) | cli);
+        float cx, cy, cz, wr;
+        bool vis = cact && cull_front(v, L.xf[ck], c0, c1, c2, cx, cy, cz, wr);
+        // A wave whose lanes mostly survived tests them at once; a sparse one
+        // queues its survivors for the dense Hi-Z pass.
+        const bool dense = __popcll(__builtin_amdgcn_ballot_w64(vis)) >= kVgDenseLanes;
+        if (hiz && !dense) {
+            wave_append(vis, &L.nqueue, L.queue, (uint16_t)((ck << 6) | cli));
+        } else {
+            if (hiz) vis = vis && cull_hiz(v, cx, cy, cz, wr);
+            wave_append(vis, &L.nsurv, L.stage, (uint16_t)((ck << 6) | cli));
         }
     }
     __syncthreads();
+    if (hiz) {
+        // The queued spheres again from their descriptors (L2-resident: the
+        // front loop just read them), then the Hi-Z test on dense waves.
+        const uint32_t nq = L.nqueue;
+#pragma unroll 1
+        for (uint32_t e0 = 0; e0 < nq; e0 += kVgThreads) {
+            const uint32_t e = e0 + tid;
+            bool vis = false;
+            uint16_t en = 0;
+            if (e < nq) {
+                en = L.queue[e];
+                const uint32_t k = en >> 6u;
+                const float* xf = L.xf[k];
+                const uint32_t mi = __float_as_uint(xf[kXfFirst]) + (en & 63u);
+                const float4 q0 = reinterpret_cast<const float4*>(a.meshlets + mi)[0];
+                const float cx = xrow(xf, 0, q0.x, q0.y, q0.z), cy = xrow(xf, 1, q0.x, q0.y, q0.z),
+                            cz = xrow(xf, 2, q0.x, q0.y, q0.z);
+                const float wr = wmax(q0.w * xf[kXfScale], 0.0f);
+                vis = cull_hiz(v, cx, cy, cz, wr);
+            }
+            wave_append(vis, &L.nsurv, L.stage, en);
+        }
+        __syncthreads();
+    }
     if (tid == 0) {
         const uint32_t tot = L.nsurv;
         uint32_t b = 0;

@@ -14,7 +14,7 @@ import oracle
 from helio_amd import layouts as L
 from helio_amd import synth
 from helio_amd.passes import HiZBuildPass, VirtualGeometryPass, instance_cull_data
-from tests.vg_util import oracle_vg, vg_uniforms
+from tests.vg_util import cone_cutoff_scene, oracle_cone_scene, oracle_vg, vg_uniforms
 
 pytestmark = pytest.mark.gpu
 
@@ -169,3 +169,26 @@ def test_vg_dirty_instance_update(ctx):
     ref = oracle_vg(view.camera, u, sc.meshlets, sc.objects, inst, icd, wi, oracle.pyramid(depth, 1), maxd,
                     reversed_z=True)
     _compare(got, ref)
+
+
+
+def test_cone_cutoffs_at_the_computed_cosine(ctx):
+    """Cutoffs equal to each meshlet's cone cosine, a few ulps either side and
+    inside / outside the 2^-16 margin within which the kernel's estimate
+    defers to the reference's arithmetic (vg_cull.wgsl:160-181): the
+    published set equals the oracle's (which the CPU suite pins to the
+    binary32 cosine)."""
+    sc = cone_cutoff_scene()
+    n = len(sc.meshlets)
+    hiz = HiZBuildPass(ctx, 1, 1, L.HIZ_MAX)
+    hiz.upload_depth(np.ones((1, 1), np.float32))
+    hiz.execute()
+    vg = VirtualGeometryPass(ctx, publication_limit=n)
+    vg.set_frame(sc.meshlets, sc.objects, sc.instances, max_draw_count=n)
+    vg.prepare(sc.camera, 1920, 1080, 0.5, hiz)
+    vg.execute()
+    got = vg.read()
+    ref = oracle_cone_scene(sc)
+    _compare(got, ref)
+    assert 0 < ref[2][0] < n
+    vg.close()

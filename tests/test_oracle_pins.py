@@ -2,8 +2,8 @@
 sides were written from one reading of the shaders): the occlusion geometry
 (occlusion_cull.wgsl:101-208), the sub-pixel rule (indirect_dispatch.wgsl:127-137),
 the distance LOD thresholds and bucket-major compaction layout of the
-per-instance cull, and the VG Hi-Z branch and LOD selection (vg_cull.wgsl:184-235,
-317-337).
+per-instance cull, the VG Hi-Z branch and LOD selection (vg_cull.wgsl:184-235,
+317-337) and the VG cone test (vg_cull.wgsl:160-181).
 
 Every expectation here comes from outside the oracle: a hand-built camera
 (identity view, perspective_rh projection) and hand-built pyramids with known
@@ -25,7 +25,7 @@ import pytest
 import oracle
 from helio_amd import layouts as L
 from helio_amd import synth
-from tests.vg_util import identity_instance, oracle_vg, vg_uniforms, work_items_for
+from tests.vg_util import cone_cutoff_scene, identity_instance, oracle_cone_scene, oracle_vg, vg_uniforms, work_items_for
 
 BIAS = 1.0 / 65536.0
 F32 = np.float32
@@ -629,3 +629,19 @@ def test_vg_benchmark_invariants_through_the_oracle():
         n = min(cap, total)
         assert np.array_equal(ind[:n, 4], np.arange(n))  # first_instance = slot
         assert (ind[:n, 1] == 1).all() and (ind[:n, 0] == 3).all()
+
+
+def test_vg_cone_cutoff_against_the_binary32_cosine():
+    """vg_cull.wgsl:160-181: a meshlet is cone-culled exactly when
+    dot(normalize(apex - camera), normalize(axis)) >= cutoff, in binary32 with
+    both normalisations as x / sqrt(x . x), and only for cutoffs <= 1.
+    Cutoffs at the cosine, 1-3 ulps either side and up to 3e-5 away pin the
+    comparison's direction, its strictness and the operand order."""
+    sc = cone_cutoff_scene()
+    ind, md, ctr, _ = oracle_cone_scene(sc)
+    n = len(sc.meshlets)
+    culled = np.ones(n, bool)
+    culled[md[: ctr[0], 1]] = False
+    want = (sc.cutoff <= sc.cosine) & (sc.cutoff <= 1.0)
+    assert np.array_equal(culled, want)
+    assert 0 < want.sum() < n

@@ -100,3 +100,73 @@ def single_meshlet_case(camera, apex, axis, cutoff, radius=1.0) -> int:
     u = vg_uniforms(1, len(wi), 16, hiz_mip_count=1)
     _, _, ctr, _ = oracle_vg(cam, u, m, o, inst, icd, wi, pyr, 16)
     return int(ctr[0])
+
+
+def cone_cos32(apex, axis):
+    """The cone test's cosine in binary32, in the order vg_cull.wgsl:160-181
+    computes it for an identity instance and a camera at the origin:
+    normalize(axis) . normalize(apex - camera), each normalisation as x / sqrt(x . x)."""
+    f = np.float32
+    ax, ay, az = (axis[:, i].astype(f) for i in range(3))
+    tx, ty, tz = (apex[:, i].astype(f) for i in range(3))
+    alen = np.sqrt((ax * ax + ay * ay) + az * az)
+    s = np.sqrt((tx * tx + ty * ty) + tz * tz)
+    return ((tx / s) * (ax / alen) + (ty / s) * (ay / alen)) + (tz / s) * (az / alen)
+
+
+class ConeScene:
+    pass
+
+
+def cone_cutoff_scene(n_geo: int = 256, seed: int = 31) -> ConeScene:
+    """One object of meshlets in front of a camera at the origin (identity
+    instance, cone culling enabled), each geometry repeated with cutoffs at its
+    binary32 cone cosine d, d +- 1..3 ulps and d +- {1e-6, 5e-6, 1.4e-5, 1.6e-5,
+    3e-5}. A meshlet is cone-culled exactly when cutoff <= min(d, 1)."""
+    g = np.random.default_rng(seed)
+    z = -g.uniform(5.0, 50.0, n_geo)
+    apex = np.stack([z * g.uniform(-0.5, 0.5, n_geo), z * g.uniform(-0.3, 0.3, n_geo), z], 1).astype(np.float32)
+    axis = g.normal(size=(n_geo, 3)).astype(np.float32)
+    d = cone_cos32(apex, axis)
+    cut = [(d + np.float32(o)).astype(np.float32)
+           for o in (0.0, 1e-6, -1e-6, 5e-6, -5e-6, 1.4e-5, -1.4e-5, 1.6e-5, -1.6e-5, 3e-5, -3e-5)]
+    for k in (1, 2, 3):
+        up, dn = d.copy(), d.copy()
+        for _ in range(k):
+            up = np.nextafter(up, np.float32(np.inf))
+            dn = np.nextafter(dn, np.float32(-np.inf))
+        cut += [up, dn]
+    cut = np.concatenate(cut).astype(np.float32)
+    n = len(cut)
+    reps = n // n_geo
+    m = np.zeros(n, L.MESHLET)
+    m["center"] = np.tile(apex, (reps, 1))
+    m["radius"] = 0.05
+    m["cone_apex"] = np.tile(apex, (reps, 1))
+    m["cone_axis"] = np.tile(axis, (reps, 1))
+    m["cone_cutoff"] = cut
+    m["index_count"] = 3
+    m["first_index"] = np.arange(n)
+    o = np.zeros(1, L.VG_OBJECT)
+    o["lod_count"] = 1
+    o["max_meshlet_count"] = n
+    o["local_bounds"] = [0.0, 0.0, -27.5, 40.0]
+    o["lod_meshlet_counts"][0, 0] = n
+    view = synth.look_to_rh(np.zeros(3), np.array([0.0, 0.0, -1.0]), np.array([0.0, 1.0, 0.0]))
+    proj = synth.perspective_rh(math.radians(60.0), 16 / 9, 0.1, 1000.0)
+    sc = ConeScene()
+    sc.meshlets, sc.objects, sc.instances = m, o, identity_instance()
+    sc.camera = synth.camera_uniforms(view, proj, np.zeros(3), 0.1, 1000.0)
+    sc.cutoff, sc.cosine = cut, np.tile(d, reps)
+    return sc
+
+
+def oracle_cone_scene(sc: ConeScene):
+    """The oracle's select + cull of a cone_cutoff_scene under a far 1x1 pyramid."""
+    icd = np.zeros(1, L.INSTANCE_CULL)
+    icd[0] = (1.0, 1.0, 1, 1)
+    wi = work_items_for(sc.objects)
+    n = len(sc.meshlets)
+    u = vg_uniforms(1, len(wi), n, hiz_mip_count=1)
+    pyr = oracle.pyramid(np.ones((1, 1), np.float32), 0)
+    return oracle_vg(sc.camera, u, sc.meshlets, sc.objects, sc.instances, icd, wi, pyr, n)
