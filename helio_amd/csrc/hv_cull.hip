@@ -1234,7 +1234,10 @@ hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed) {
     // blocks beyond the table's G (or past the last unit) exit at once.
     const size_t lds = fixed ? (size_t)lds_cap_ : (size_t)max_nb_ * 4;
     const uint32_t g1 = fixed ? resident1_ : std::max(1u, grid_);
-    const uint32_t g2 = colscan_grid(ctx_, fixed ? ~0u : n_units_);
+    // The column scan takes its units grid-stride: a capacity grid needs no
+    // more blocks than the units a table within the capacity can have.
+    const uint32_t g2 = colscan_grid(
+        ctx_, fixed ? (uint32_t)std::min<uint64_t>(unit_bound(max_views_, lds_cap_ / 4u, 0), ~0u) : n_units_);
     const uint32_t g3 = fixed ? resident3_ : std::max(1u, std::min(grid_, resident3_));
     if (ms) HV_HIP(hipEventRecord(ev_[0], s));
     hipLaunchKernelGGL(fn, dim3(g1), dim3(shape.wg), lds, s, hdr, views);
