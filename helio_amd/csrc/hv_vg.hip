@@ -225,6 +225,7 @@ constexpr uint32_t kVgMinSpans = 8;
 constexpr uint32_t kVgTargetBlocks = 512;
 constexpr uint32_t kVgUnroll = 4;
 constexpr uint32_t kVgDenseLanes = 48;  // front survivors of a wave that take the Hi-Z test in place
+constexpr uint32_t kVgQueueSpheres = 512;  // queued spheres kept in LDS (later ones are recomputed)
 
 struct VgLds {
     float xf[kVgSpans][kXfWords];
@@ -233,6 +234,7 @@ struct VgLds {
     uint32_t pref[kVgSpans + 1];      // exclusive prefix of the spans' meshlet counts; [K] = total
     uint16_t stage[kVgSpans * 64];    // survivors: span << 6 | meshlet within the span
     uint16_t queue[kVgSpans * 64];    // frustum and cone survivors awaiting the Hi-Z test (same form)
+    float4 qs[kVgQueueSpheres];       // the world spheres of the first queued ones
     uint8_t owner[kVgSpans * 64];     // flattened meshlet -> its span
     VgView v;                         // the per-view uniforms
     uint32_t hist[8];
@@ -273,6 +275,22 @@ __device__ __forceinline__ void wave_append(bool vis, uint32_t* count, uint16_t*
         if ((threadIdx.x & 63u) == 0u) wb = atomicAdd(count, (uint32_t)__popcll(m));
         wb = __shfl(wb, 0);
         if (vis) list[wb + mbcnt(m)] = entry;
+    }
+}
+
+// The same, keeping the sphere beside the entry while there is room.
+__device__ __forceinline__ void wave_queue(bool vis, uint32_t* count, uint16_t* list, float4* spheres, uint16_t entry,
+                                           float4 sphere) {
+    const uint64_t m = __builtin_amdgcn_ballot_w64(vis);
+    if (m) {
+        uint32_t wb = 0;
+        if ((threadIdx.x & 63u) == 0u) wb = atomicAdd(count, (uint32_t)__popcll(m));
+        wb = __shfl(wb, 0);
+        const uint32_t at = wb + mbcnt(m);
+        if (vis) {
+            list[at] = entry;
+            if (at < kVgQueueSpheres) spheres[at] = sphere;
+        }
     }
 }
 
@@ -404,7 +422,7 @@ __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, V
         // queues its survivors for the dense Hi-Z pass.
         const bool dense = __popcll(__builtin_amdgcn_ballot_w64(vis)) >= kVgDenseLanes;
         if (hiz && !dense) {
-            wave_append(vis, &L.nqueue, L.queue, (uint16_t)((ck << 6) | cli));
+            wave_queue(vis, &L.nqueue, L.queue, L.qs, (uint16_t)((ck << 6) | cli), make_float4(cx, cy, cz, wr));
         } else {
             if (hiz) vis = vis && cull_hiz(v, cx, cy, cz, wr);
             wave_append(vis, &L.nsurv, L.stage, (uint16_t)((ck << 6) | cli));
@@ -412,8 +430,8 @@ __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, V
     }
     __syncthreads();
     if (hiz) {
-        // The queued spheres again from their descriptors (L2-resident: the
-        // front loop just read them), then the Hi-Z test on dense waves.
+        // The queued spheres (from LDS, or again from their descriptors past
+        // the first kVgQueueSpheres), then the Hi-Z test on dense waves.
         const uint32_t nq = L.nqueue;
 #pragma unroll 1
         for (uint32_t e0 = 0; e0 < nq; e0 += kVgThreads) {
@@ -422,14 +440,18 @@ __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, V
             uint16_t en = 0;
             if (e < nq) {
                 en = L.queue[e];
-                const uint32_t k = en >> 6u;
-                const float* xf = L.xf[k];
-                const uint32_t mi = __float_as_uint(xf[kXfFirst]) + (en & 63u);
-                const float4 q0 = reinterpret_cast<const float4*>(a.meshlets + mi)[0];
-                const float cx = xrow(xf, 0, q0.x, q0.y, q0.z), cy = xrow(xf, 1, q0.x, q0.y, q0.z),
-                            cz = xrow(xf, 2, q0.x, q0.y, q0.z);
-                const float wr = wmax(q0.w * xf[kXfScale], 0.0f);
-                vis = cull_hiz(v, cx, cy, cz, wr);
+                float4 q;
+                if (e < kVgQueueSpheres) {
+                    q = L.qs[e];
+                } else {
+                    const uint32_t k = en >> 6u;
+                    const float* xf = L.xf[k];
+                    const uint32_t mi = __float_as_uint(xf[kXfFirst]) + (en & 63u);
+                    const float4 q0 = reinterpret_cast<const float4*>(a.meshlets + mi)[0];
+                    q = make_float4(xrow(xf, 0, q0.x, q0.y, q0.z), xrow(xf, 1, q0.x, q0.y, q0.z),
+                                    xrow(xf, 2, q0.x, q0.y, q0.z), wmax(q0.w * xf[kXfScale], 0.0f));
+                }
+                vis = cull_hiz(v, q.x, q.y, q.z, q.w);
             }
             wave_append(vis, &L.nsurv, L.stage, en);
         }
