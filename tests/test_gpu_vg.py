@@ -112,11 +112,13 @@ def _compare(got, ref):
         assert np.array_equal(_rows(ind, md), ref_rows)
 
 
+@pytest.mark.parametrize("index", [1, 3, 5])
 @pytest.mark.parametrize("rev", [True, False])
-def test_vg_parity_with_hiz(ctx, rev):
-    """Config-2 recipe at 1,024 objects x 512 meshlets: frustum + cone + Hi-Z + LOD."""
-    view, sc, depth = synth.vg_workload(1024, index=1, reversed_z=rev, boxes=400)
-    print(f"vg parity rev={rev}", flush=True)
+def test_vg_parity_with_hiz(ctx, rev, index):
+    """Config-2 recipe at 1,024 objects x 512 meshlets: frustum + cone + Hi-Z + LOD
+    (three seeded scenes per depth convention)."""
+    view, sc, depth = synth.vg_workload(1024, index=index, reversed_z=rev, boxes=400)
+    print(f"vg parity rev={rev} index={index}", flush=True)
     got, ref, _ = _vg_case(ctx, sc, view, depth, rev)
     _compare(got, ref)
     assert ref[2][0] > 1000, "too few visible meshlets to pin the path"
@@ -172,13 +174,14 @@ def test_vg_dirty_instance_update(ctx):
 
 
 
-def test_cone_cutoffs_at_the_computed_cosine(ctx):
+@pytest.mark.parametrize("seed", [31, 32, 33])
+def test_cone_cutoffs_at_the_computed_cosine(ctx, seed):
     """Cutoffs equal to each meshlet's cone cosine, a few ulps either side and
     inside / outside the 2^-16 margin within which the kernel's estimate
     defers to the reference's arithmetic (vg_cull.wgsl:160-181): the
     published set equals the oracle's (which the CPU suite pins to the
     binary32 cosine)."""
-    sc = cone_cutoff_scene()
+    sc = cone_cutoff_scene(seed=seed)
     n = len(sc.meshlets)
     hiz = HiZBuildPass(ctx, 1, 1, L.HIZ_MAX)
     hiz.upload_depth(np.ones((1, 1), np.float32))
