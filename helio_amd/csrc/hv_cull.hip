@@ -338,7 +338,11 @@ struct ItemCtx {
     uint32_t ibeg, vcount, vfirst;
 };
 
-// Loads the IPT candidates of work item `it` held by this thread.
+// Loads the IPT candidates of work item `it` held by this thread. A lane past
+// the item's end (j >= n, n >= 1) loads the item's last candidate instead of
+// branching around the load; every use masks it out by j < n. The item's base
+// is uniform and the lane offsets are below 2^16 bytes, so the loads take the
+// scalar-base form with one 32-bit offset register.
 template <int WG, int IPT>
 __device__ __forceinline__ void load_item(const ItemCtx& c, uint32_t it, float4* bb, uint32_t* mm) {
     constexpr uint32_t CHUNK = WG * IPT;
@@ -346,17 +350,15 @@ __device__ __forceinline__ void load_item(const ItemCtx& c, uint32_t it, float4*
     const uint32_t rem = c.vcount - base_idx;
     const uint32_t n = rem < CHUNK ? rem : CHUNK;
     const uint32_t first = c.vfirst + base_idx;
+    typedef __attribute__((address_space(1))) const char GlobalB;
+    GlobalB* bsrc = reinterpret_cast<GlobalB*>(c.bounds + first);
+    GlobalB* msrc = reinterpret_cast<GlobalB*>(c.meta + first);
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
-        const uint32_t j = k * WG + threadIdx.x;
-        if (j < n) {
-            const F4v q = c.bounds[first + j];
-            bb[k] = make_float4(q.x, q.y, q.z, q.w);
-            mm[k] = c.meta[first + j];
-        } else {
-            bb[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-            mm[k] = 0u;
-        }
+        const uint32_t j = min(k * WG + threadIdx.x, n - 1u);
+        const F4v q = *reinterpret_cast<GlobalF4*>(bsrc + j * 16u);
+        bb[k] = make_float4(q.x, q.y, q.z, q.w);
+        mm[k] = *reinterpret_cast<GlobalU32*>(msrc + j * 4u);
     }
 }
 
