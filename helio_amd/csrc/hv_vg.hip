@@ -303,9 +303,11 @@ __device__ __forceinline__ void wave_queue(bool vis, uint32_t* count, uint16_t* 
 //   B. the kVgThreads threads walk the flattened meshlets of the live spans,
 //      the next descriptor loading while the current one is tested by the
 //      frustum and cone tests. Without a pyramid the survivors are staged for
-//      publication; with one they are queued, and the Hi-Z test (the larger
-//      half of the arithmetic) then runs over the queue in dense waves, so
-//      lanes culled by the frustum or the cone do not idle through it;
+//      publication. With one, a wave whose survivors fill at least
+//      kVgDenseLanes lanes takes the Hi-Z test (the larger half of the
+//      arithmetic) at once; a sparser wave queues its survivors with their
+//      world spheres, and the Hi-Z test then runs over the queue in dense
+//      waves, so lanes culled by the frustum or the cone do not idle through it;
 //   C. one slot reservation for the workgroup, then the survivors' commands and
 //      metadata are written to consecutive slots.
 __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, VgLds& L) {
