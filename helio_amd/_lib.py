@@ -270,6 +270,7 @@ _SIGS: dict[str, tuple] = {
     "hv_ctx_set_shapes": (i32, [vp, u32, u32, u32]),
     "hv_ctx_get_shapes": (i32, [vp, P(u32), P(u32), P(u32)]),
     "hv_autotune_batch": (i32, [vp, u32, P(TuneResult), u32, P(u32), P(TuneResult)]),
+    "hv_autotune_batch_replay": (i32, [vp, u32, P(TuneResult), u32, P(u32), P(TuneResult)]),
     "hv_autotune_hiz": (i32, [vp, u32, P(f32), P(u32)]),
 }
 

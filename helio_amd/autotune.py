@@ -28,10 +28,14 @@ def apply(ctx: Context, shapes: dict) -> None:
     call("hv_ctx_set_shapes", ctx.handle, int(shapes["wg"]), int(shapes["ipt"]), int(shapes["hiz_tile"]))
 
 
-def tune_batch(batch: BatchCull, reps: int = 5) -> dict:
+def tune_batch(batch: BatchCull, reps: int = 5, replay: bool = False) -> dict:
+    """Times every launch shape on `batch`: eager launches (the three kernels'
+    sum), or with `replay` replays of its captured graph (as a renderer runs
+    it every frame)."""
     res = (TuneResult * 16)()
     n, best = C.c_uint32(), TuneResult()
-    call("hv_autotune_batch", batch.handle, reps, res, 16, C.byref(n), C.byref(best))
+    call("hv_autotune_batch_replay" if replay else "hv_autotune_batch", batch.handle, reps, res, 16, C.byref(n),
+         C.byref(best))
     return {"best": {"wg": best.wg, "ipt": best.ipt, "classify_ms": best.classify_ms, "total_ms": best.total_ms},
             "table": [{"wg": r.wg, "ipt": r.ipt, "classify_ms": r.classify_ms, "total_ms": r.total_ms}
                       for r in res[: n.value]]}
@@ -71,20 +75,24 @@ def store(cu_count: int, workload: str, entry: dict, path: Path = CACHE) -> None
 
 
 def cull_shape(ctx: Context, workload: str, prepare, reps: int = 5, path: Path = CACHE,
-               retune: bool = False) -> tuple[dict, "BatchCull", bool]:
+               retune: bool = False, replay: bool = False) -> tuple[dict, "BatchCull", bool]:
     """The classify launch shape for `workload` on this device, applied to the
     context, and a batch prepared with it. `prepare(ctx)` builds and prepares
     the batch the shape is tuned on. A cached entry is applied before the
     batch is built; on a miss (or retune) every shape is timed on the batch
     (hv_autotune_batch, which leaves the batch and the context on the
-    fastest) and the result is stored. Returns (shape, batch, from_cache)."""
+    fastest) and the result is stored; with `replay` the shapes are timed as
+    graph replays (the cache keeps the two kinds apart). Returns (shape,
+    batch, from_cache)."""
     cu, _ = ctx.device_info()
+    if replay:
+        workload = f"{workload}:replay"
     hit = None if retune else lookup(cu, workload, path)
     if hit is not None:
         apply(ctx, dict(get_shapes(ctx), wg=hit["wg"], ipt=hit["ipt"]))
         return hit, prepare(ctx), True
     batch = prepare(ctx)
-    res = tune_batch(batch, reps)
+    res = tune_batch(batch, reps, replay)
     store(cu, workload, dict(res["best"], table=res["table"]), path)
     return res["best"], batch, False
 

@@ -21,6 +21,83 @@ static hv_status set_shape(CullBatch* b, uint32_t wg, uint32_t ipt) {
     return HV_OK;
 }
 
+namespace {
+
+// Times every (wg, ipt) on the prepared batch: eager launches with an event
+// between the kernels (total = their sum), or replays of the batch's
+// captured graph (total = events around the replay). Leaves the batch and the
+// context on the fastest total.
+hv_status autotune(hv_batch* b, uint32_t reps, bool replay, hv_tune_result* results, uint32_t capacity,
+                   uint32_t* count, hv_tune_result* best) {
+    if (!b || !b->prepared) return set_error(HV_ESTATE, "hv_autotune_batch: prepare the batch first");
+    if (reps == 0) reps = 5;
+    static const uint32_t kWg[] = {64, 128, 256, 512};
+    static const uint32_t kIpt[] = {1, 2, 4, 8};
+    CullBatch* cb = b->batch;
+    hipStream_t s = cb->ctx()->stream;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    if (replay) {
+        HV_HIP(hipEventCreate(&ev[0]));
+        hipError_t e = hipEventCreate(&ev[1]);
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(ev[0]);
+            return hip_error(e, "hipEventCreate", __FILE__, __LINE__);
+        }
+    }
+    hv_tune_result top{0, 0, 1e30f, 1e30f};
+    uint32_t n = 0;
+    hv_status st = HV_OK;
+    for (uint32_t wg : kWg)
+        for (uint32_t ipt : kIpt) {
+            if (st == HV_OK) st = set_shape(cb, wg, ipt);
+            if (st == HV_OK) st = cb->reprepare();
+            std::vector<float> t1, tt;
+            if (st == HV_OK && !replay) {
+                float ms[3];
+                st = cb->execute(s, ms);  // warm-up
+                for (uint32_t r = 0; r < reps && st == HV_OK; ++r) {
+                    st = cb->execute(s, ms);
+                    t1.push_back(ms[0]);
+                    tt.push_back(ms[0] + ms[1] + ms[2]);
+                }
+            } else if (st == HV_OK) {
+                st = cb->execute_graph(s);  // capture and warm-up
+                for (uint32_t r = 0; r < reps && st == HV_OK; ++r) {
+                    hipError_t e = hipEventRecord(ev[0], s);
+                    if (e == hipSuccess) st = cb->execute_graph(s);
+                    if (e == hipSuccess && st == HV_OK) e = hipEventRecord(ev[1], s);
+                    if (e == hipSuccess && st == HV_OK) e = hipEventSynchronize(ev[1]);
+                    float ms = 0.0f;
+                    if (e == hipSuccess && st == HV_OK) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+                    if (e != hipSuccess && st == HV_OK) st = hip_error(e, "autotune replay", __FILE__, __LINE__);
+                    t1.push_back(0.0f);
+                    tt.push_back(ms);
+                }
+            }
+            if (st != HV_OK) break;
+            std::sort(t1.begin(), t1.end());
+            std::sort(tt.begin(), tt.end());
+            const hv_tune_result res{wg, ipt, t1[t1.size() / 2], tt[tt.size() / 2]};
+            if (results && n < capacity) results[n] = res;
+            ++n;
+            if (res.total_ms < top.total_ms) top = res;
+        }
+    if (replay) {  // every replay was waited for through its event
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+    }
+    HV_TRY(st);
+    HV_TRY(set_shape(cb, top.wg, top.ipt));
+    HV_TRY(cb->reprepare());
+    cb->ctx()->cull_shape.wg = top.wg;
+    cb->ctx()->cull_shape.ipt = top.ipt;
+    if (count) *count = n;
+    if (best) *best = top;
+    return HV_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 hv_status hv_batch_create(hv_ctx* ctx, const hv_cull_config* config, uint32_t max_views, hv_batch** out) {
@@ -130,40 +207,12 @@ hv_status hv_ctx_get_shapes(hv_ctx* ctx, uint32_t* wg, uint32_t* ipt, uint32_t* 
 
 hv_status hv_autotune_batch(hv_batch* b, uint32_t reps, hv_tune_result* results, uint32_t capacity, uint32_t* count,
                             hv_tune_result* best) {
-    if (!b || !b->prepared) return set_error(HV_ESTATE, "hv_autotune_batch: prepare the batch first");
-    if (reps == 0) reps = 5;
-    static const uint32_t kWg[] = {64, 128, 256, 512};
-    static const uint32_t kIpt[] = {1, 2, 4, 8};
-    CullBatch* cb = b->batch;
-    hipStream_t s = cb->ctx()->stream;
-    hv_tune_result top{0, 0, 1e30f, 1e30f};
-    uint32_t n = 0;
-    for (uint32_t wg : kWg)
-        for (uint32_t ipt : kIpt) {
-            HV_TRY(set_shape(cb, wg, ipt));
-            HV_TRY(cb->reprepare());
-            float ms[3];
-            HV_TRY(cb->execute(s, ms));  // warm-up
-            std::vector<float> t1, tt;
-            for (uint32_t r = 0; r < reps; ++r) {
-                HV_TRY(cb->execute(s, ms));
-                t1.push_back(ms[0]);
-                tt.push_back(ms[0] + ms[1] + ms[2]);
-            }
-            std::sort(t1.begin(), t1.end());
-            std::sort(tt.begin(), tt.end());
-            const hv_tune_result res{wg, ipt, t1[t1.size() / 2], tt[tt.size() / 2]};
-            if (results && n < capacity) results[n] = res;
-            ++n;
-            if (res.total_ms < top.total_ms) top = res;
-        }
-    HV_TRY(set_shape(cb, top.wg, top.ipt));
-    HV_TRY(cb->reprepare());
-    cb->ctx()->cull_shape.wg = top.wg;
-    cb->ctx()->cull_shape.ipt = top.ipt;
-    if (count) *count = n;
-    if (best) *best = top;
-    return HV_OK;
+    return autotune(b, reps, false, results, capacity, count, best);
+}
+
+hv_status hv_autotune_batch_replay(hv_batch* b, uint32_t reps, hv_tune_result* results, uint32_t capacity,
+                                   uint32_t* count, hv_tune_result* best) {
+    return autotune(b, reps, true, results, capacity, count, best);
 }
 
 }  // extern "C"

@@ -873,6 +873,11 @@ hv_status hv_ctx_get_shapes(hv_ctx* ctx, uint32_t* wg, uint32_t* ipt, uint32_t* 
  * NULL. */
 hv_status hv_autotune_batch(hv_batch* batch, uint32_t reps, hv_tune_result* results, uint32_t capacity,
                             uint32_t* count, hv_tune_result* best);
+/* The same, each shape timed as a replay of the batch's captured graph (the
+ * three kernels back to back, as a renderer replays them every frame):
+ * total_ms is the median from events around the replay, classify_ms is 0. */
+hv_status hv_autotune_batch_replay(hv_batch* batch, uint32_t reps, hv_tune_result* results, uint32_t capacity,
+                                   uint32_t* count, hv_tune_result* best);
 /* Times the pyramid build with 32x32 and 64x64 tiles (ms[0], ms[1]) and makes
  * the faster the context default. */
 hv_status hv_autotune_hiz(hv_hiz* hiz, uint32_t reps, float ms[2], uint32_t* best_tile);

@@ -95,3 +95,31 @@ def test_pyramid_tiles_and_the_tuned_tile_are_bit_exact(ctx, mode, tmp_path):
     assert np.array_equal(hiz.read_packed(), ref)
     autotune.apply(ctx, saved)
     hiz.close()
+
+
+def test_replay_tuned_shape_matches_the_oracle(ctx, tmp_path):
+    """hv_autotune_batch_replay: every shape timed as a graph replay; the
+    kept shape's replayed output is the oracle's, and the replay-tuned entry
+    is cached apart from the eager one."""
+    sc = synth.instance_scene(200_000, config=3, index=6)
+    scene = _scene(ctx, sc)
+    views = synth.multiview_views(sc.n, max_count=60_000)[:6]
+    saved = autotune.get_shapes(ctx)
+
+    def prepare(c):
+        bb = BatchCull(c, len(views))
+        bb.prepare([(scene, v.camera, f, c_) for v, f, c_ in views])
+        return bb
+
+    cache = tmp_path / "shapes.json"
+    best, b, cached = autotune.cull_shape(ctx, "test-replay", prepare, reps=3, path=cache, replay=True)
+    assert not cached and best["classify_ms"] == 0.0 and best["total_ms"] > 0.0
+    entry = autotune.lookup(ctx.device_info()[0], "test-replay:replay", cache)
+    assert entry is not None and len(entry["table"]) == 16
+    assert all(r["total_ms"] > 0.0 for r in entry["table"])
+    assert autotune.lookup(ctx.device_info()[0], "test-replay", cache) is None
+    b.execute(graph=True)
+    _check(b, sc, views)
+    b.close()
+    autotune.apply(ctx, saved)
+    scene.close()
