@@ -351,7 +351,8 @@ def occlusion_leg(ctx, steps: int) -> dict:
 
     hiz.execute()
     # The frame replays the cull from its graph, so the shape is tuned on replays.
-    cull_tune, b, cull_cached = autotune.cull_shape(ctx, "occlusion:1920x1080:1048576", prepare_one, replay=True)
+    cull_tune, b, cull_cached = autotune.cull_shape(ctx, "occlusion:1920x1080:1048576", prepare_one, reps=20,
+                                                    replay=True)
     # Pyramid build time (HIP events on the context stream).
     from helio_amd._lib import call
     hz_ms = []
