@@ -35,8 +35,6 @@
 namespace hv {
 namespace dev {
 
-constexpr uint32_t kInvalid = 0xffffffffu;
-
 // Global-address-space views of pointers read from the view table (plain
 // pointers loaded from memory compile to flat accesses).
 typedef float F4v __attribute__((ext_vector_type(4)));
@@ -236,21 +234,27 @@ constexpr float kTinyT = 0.001f;
 constexpr float kTinyLo = 0.001f * (1.0f - 0x1p-20f);
 constexpr float kTinyHi = 0.001f * (1.0f + 0x1p-20f);
 
-// The frustum test of one candidate (indirect_dispatch.wgsl:100-113). Lanes
-// with !valid report false. `boxes` (uniform) says the view's scene holds
-// explicit AABBs: a batch that mixes such scenes with sphere-only ones runs
-// the AABB kernel for every view, and the sphere-only views take the sphere
-// path (their `ab` is null).
+// Lane masks are carried between the steps below instead of per-lane bools:
+// a ballot taken right at a compare is the compare's own mask, while a bool
+// that lives across a branch is rebuilt as a vector register and compared
+// again each time it is balloted; __builtin_amdgcn_inverse_ballot_w64 turns
+// a mask back into a lane predicate at no cost.
+__device__ __forceinline__ bool lane_in(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+
+// The frustum test of one candidate per lane (indirect_dispatch.wgsl:100-113)
+// for the lanes of `valm`; returns the mask of the visible ones. `sure`
+// (uniform): the view's planes admit the margin decisions. `boxes` (uniform)
+// says the view's scene holds explicit AABBs: a batch that mixes such scenes
+// with sphere-only ones runs the AABB kernel for every view, and the
+// sphere-only views take the sphere path (their `ab` is null).
 template <bool AABB>
-__device__ __forceinline__ bool frustum_part(const CullView* __restrict__ Vk, const float* pl, const float* l1s,
-                                             const RowLds& rc, bool valid, float4 b, bool boxes,
-                                             const hv_instance_aabb* ab) {
+__device__ __forceinline__ uint64_t frustum_part(const CullView* __restrict__ Vk, const float* pl, const float* l1s,
+                                                 const RowLds& rc, bool sure, uint64_t valm, float4 b, bool boxes,
+                                                 const hv_instance_aabb* ab) {
     const float cx = b.x, cy = b.y, cz = b.z, r = b.w;
-    if (AABB && boxes) return valid && frustum_visible<true>(pl, nullptr, cx, cy, cz, r, ab);
+    if (AABB && boxes) return ballot(frustum_visible<true>(pl, nullptr, cx, cy, cz, r, ab)) & valm;
     const float cmax = fmaxf(fmaxf(fabsf(cx), fabsf(cy)), fabsf(cz));
     const float bb = cmax + r;
-    // r > 2^-22 max|c| also rules out a degenerate box (see frustum_visible).
-    const bool lane_ok = (r > cmax * 0x1p-22f) & (bb < 0x1p100f);
     float s[6];
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
@@ -261,9 +265,14 @@ __device__ __forceinline__ bool frustum_part(const CullView* __restrict__ Vk, co
     }
     const float smin = fminf(fminf(fminf(s[0], s[1]), s[2]), fminf(fminf(s[3], s[4]), s[5]));
     const float e = __builtin_fmaf(rc.sure_kl, bb, rc.sure_kd);
-    bool vis = smin >= e;
-    const bool unsure = valid & ((rc.sure_ok == 0u) | !(lane_ok & ((smin >= e) | (smin < -e))));
-    if (ballot(unsure) != 0ull) {
+    // Decided lanes: r > 2^-22 max|c| (which also rules out a degenerate box,
+    // see frustum_visible), finite bounded inputs, and a minimum clear of the
+    // margin on either side.
+    const uint64_t ok = ballot(r > cmax * 0x1p-22f) & ballot(bb < 0x1p100f);
+    const uint64_t in = ballot(smin >= e), out = ballot(smin < -e);
+    uint64_t vis = in & valm;
+    const uint64_t unsure = sure ? valm & ~(ok & (in | out)) : valm;
+    if (unsure != 0ull) {
         asm volatile("" ::: "memory");
         // The p-vertex signs are needed only here: scalar loads after the
         // barrier (signs derived from the planes in place would be hoisted
@@ -272,29 +281,30 @@ __device__ __forceinline__ bool frustum_part(const CullView* __restrict__ Vk, co
 #pragma unroll
         for (int i = 0; i < 18; ++i) sg[i] = ldc(&Vk->pv_sign[i]);
         const bool ve = frustum_visible<false>(pl, sg, cx, cy, cz, r, nullptr);
-        vis = unsure ? ve : vis;
+        vis = (vis & ~unsure) | (ballot(ve) & unsure);
     }
-    return valid & vis;
+    return vis;
 }
 
-// The rest of the predicate for a frustum-visible candidate (`act`): the
+// The rest of the predicate for the frustum-visible candidates of `act`: the
 // sub-pixel cull (indirect_dispatch.wgsl:127-137), then distance LOD. Returns
-// the bucket (mesh_id * lod_count + lod) when drawn, kInvalid otherwise.
-__device__ __forceinline__ uint32_t draw_part(const RowLds& rc, bool act, float4 b, uint32_t m, bool& draw) {
+// the bucket (mesh_id * lod_count + lod) of each lane and sets `drawn` to the
+// mask of the lanes drawn (the bucket of the others is meaningless).
+__device__ __forceinline__ uint32_t draw_part(const RowLds& rc, uint64_t act, float4 b, uint32_t m, uint64_t& drawn) {
     const float cx = b.x, cy = b.y, cz = b.z, r = b.w;
     // Sub-pixel cull: w = ((vp[3] x + vp[7] y) + vp[11] z) + vp[15].
     const float w = ((rc.vp3[0] * cx + rc.vp3[1] * cy) + rc.vp3[2] * cz) + rc.vp3[3];
     const float a = r * rc.p11;
-    const bool w_ok = (__float_as_uint(w) - 0x1F800000u) < 0x3F000000u;  // w in [2^-64, 2^62)
     const float q = fabsf(a) * __builtin_amdgcn_rcpf(w);
-    bool tiny = w_ok & (q < kTinyLo);
-    const bool unsure_t = act & (w > 0.0f) & !(w_ok & ((q < kTinyLo) | (q > kTinyHi)));
-    if (ballot(unsure_t) != 0ull) {
+    const uint64_t w_ok = ballot((__float_as_uint(w) - 0x1F800000u) < 0x3F000000u);  // w in [2^-64, 2^62)
+    const uint64_t lo = ballot(q < kTinyLo), hi = ballot(q > kTinyHi);
+    uint64_t tiny = w_ok & lo;
+    const uint64_t unsure = act & ballot(w > 0.0f) & ~(w_ok & (lo | hi));
+    if (unsure != 0ull) {
         asm volatile("" ::: "memory");
-        const bool te = fabsf(a / w) < kTinyT;
-        tiny = unsure_t ? te : tiny;
+        tiny = (tiny & ~unsure) | (ballot(fabsf(a / w) < kTinyT) & unsure);
     }
-    draw = act & !tiny;
+    drawn = act & ~tiny;
     const float camx = rc.cam[0], camy = rc.cam[1], camz = rc.cam[2];
     // lod = #{t : sqrt_rn(d2) >= lod_distance[t]}, evaluated without the
     // square root as d2 >= lod_d2[t] (the host picks lod_d2[t] as the least
@@ -309,8 +319,7 @@ __device__ __forceinline__ uint32_t draw_part(const RowLds& rc, bool act, float4
         lod += (uint32_t)(d2 >= rc.lod_d2[3]) + (uint32_t)(d2 >= rc.lod_d2[4]) + (uint32_t)(d2 >= rc.lod_d2[5]) +
                (uint32_t)(d2 >= rc.lod_d2[6]);
     // mesh_id < 2^14 here (buckets <= 14336), so a 24-bit multiply-add is exact.
-    const uint32_t code = __umul24(m & 0x7fffffffu, lod_count) + lod;
-    return draw ? code : kInvalid;
+    return __umul24(m & 0x7fffffffu, lod_count) + lod;
 }
 
 // Advance v to the view holding `item` (items are visited in order).
@@ -338,13 +347,14 @@ struct ItemCtx {
     uint32_t ibeg, vcount, vfirst;
 };
 
-// Loads the IPT candidates of work item `it` held by this thread. A lane past
-// the item's end (j >= n, n >= 1) loads the item's last candidate instead of
-// branching around the load; every use masks it out by j < n. The item's base
-// is uniform and the lane offsets are below 2^16 bytes, so the loads take the
-// scalar-base form with one 32-bit offset register.
+// Loads slice k (candidates k WG + [0, WG)) of work item `it` held by this
+// thread. A lane past the item's end (j >= n, n >= 1) loads the item's last
+// candidate instead of branching around the load; every use masks it out by
+// j < n. The item's base is uniform and the lane offsets are below 2^16
+// bytes, so the loads take the scalar-base form with one 32-bit offset
+// register.
 template <int WG, int IPT>
-__device__ __forceinline__ void load_item(const ItemCtx& c, uint32_t it, float4* bb, uint32_t* mm) {
+__device__ __forceinline__ void load_slice(const ItemCtx& c, uint32_t it, int k, float4& b, uint32_t& m) {
     constexpr uint32_t CHUNK = WG * IPT;
     const uint32_t base_idx = (it - c.ibeg) * CHUNK;
     const uint32_t rem = c.vcount - base_idx;
@@ -353,13 +363,10 @@ __device__ __forceinline__ void load_item(const ItemCtx& c, uint32_t it, float4*
     typedef __attribute__((address_space(1))) const char GlobalB;
     GlobalB* bsrc = reinterpret_cast<GlobalB*>(c.bounds + first);
     GlobalB* msrc = reinterpret_cast<GlobalB*>(c.meta + first);
-#pragma unroll
-    for (int k = 0; k < IPT; ++k) {
-        const uint32_t j = min(k * WG + threadIdx.x, n - 1u);
-        const F4v q = *reinterpret_cast<GlobalF4*>(bsrc + j * 16u);
-        bb[k] = make_float4(q.x, q.y, q.z, q.w);
-        mm[k] = *reinterpret_cast<GlobalU32*>(msrc + j * 4u);
-    }
+    const uint32_t j = min(k * WG + threadIdx.x, n - 1u);
+    const F4v q = *reinterpret_cast<GlobalF4*>(bsrc + j * 16u);
+    b = make_float4(q.x, q.y, q.z, q.w);
+    m = *reinterpret_cast<GlobalU32*>(msrc + j * 4u);
 }
 
 // A wave's stat counters over a row: tests seen as visible / drawn /
@@ -372,19 +379,24 @@ struct WaveStats {
 // Classifies the loaded candidates of item `it`: wave-compacted survivors into
 // the wave's stage slots, the row histogram in LDS; `n_valid` and `kept`
 // count this wave's valid candidates and survivors. The frustum test runs on
-// every candidate; the candidates it keeps (about one in eight in the
-// benchmarked views) are packed into consecutive lanes of the wave through a
-// per-wave LDS slice, 64 at a time, and the rest of the predicate (sub-pixel,
-// LOD, the Hi-Z test), the compaction and the histogram run once per 64 of
-// them instead of once per round of mostly idle lanes.
+// every candidate, slice by slice; the candidates it keeps (about one in
+// eight in the benchmarked views) are packed into a per-wave LDS list at
+// once, and the rest of the predicate (sub-pixel, LOD, the Hi-Z test), the
+// compaction and the histogram then run over that list 64 at a time instead
+// of once per round of mostly idle lanes. A slice's registers are free once
+// it is packed, so when the row has a next item (`more`) the slice's
+// registers take that item's slice at once: its loads are in flight while
+// the rest of this item is classified, at no register cost.
 template <int WG, int IPT, bool OCCL, bool AABB>
-__device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, const float4* bb, const uint32_t* mm,
+__device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, bool more, float4* bb, uint32_t* mm,
                                              uint32_t* s_hist, WaveStats& ws, uint32_t& n_valid, uint32_t& kept) {
     constexpr uint32_t NW = WG / 64;
     constexpr uint32_t CHUNK = WG * IPT, SUB = IPT * 64;
-    __shared__ float4 s_pk[NW][64];    // packed candidates: bounds
-    __shared__ uint32_t s_pm[NW][64];  // meta
-    __shared__ uint32_t s_pj[NW][64];  // index in the chunk
+    // The packed candidates of each wave: bounds, and meta with the index in
+    // the chunk in bits 14..25 (mesh ids are below 14,336 here: at most
+    // kMaxBuckets buckets, each mesh with >= 1 LOD).
+    __shared__ float4 s_pk[NW][SUB];
+    __shared__ uint32_t s_pm[NW][SUB];
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
     const uint32_t base_idx = (it - c.ibeg) * CHUNK;
     const uint32_t rem = c.vcount - base_idx;
@@ -402,57 +414,51 @@ __device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, cons
 #pragma unroll
     for (int i = 0; i < 6; ++i) l1[i] = ldc(&Vk->plane_l1[i]);
     const bool boxes = AABB && c.aabb != nullptr;  // uniform: the view's scene holds explicit AABBs
-    // Frustum on every candidate; slot = rank among the wave's visible ones.
-    bool vis[IPT];
-    uint32_t slot[IPT];
+    const bool sure = ldc(&Vk->sure_ok) != 0u;
+    // Frustum on every candidate; its survivors appended to the wave's list.
     uint32_t total = 0;  // wave-uniform
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
         const uint32_t j = k * WG + tid;
-        const bool valid = j < n;
-        vis[k] = frustum_part<AABB>(Vk, pl, l1, *c.rc, valid, bb[k], boxes,
-                                    boxes ? c.aabb + first + (valid ? j : 0u) : nullptr);
-        const uint64_t vm = ballot(vis[k]);
-        slot[k] = total + mbcnt(vm);
-        total += (uint32_t)__builtin_popcountll(vm);
-        // Stat masks straight from compares where possible (a ballot of a
-        // compare is the compare's own lane mask).
         const uint64_t valm = ballot(j < n);
+        const uint64_t vis = frustum_part<AABB>(Vk, pl, l1, *c.rc, sure, valm, bb[k], boxes,
+                                                boxes ? c.aabb + first + (j < n ? j : 0u) : nullptr);
         n_valid += (uint32_t)__builtin_popcountll(valm);
         ws.sh += (uint32_t)__builtin_popcountll(ballot((int32_t)mm[k] < 0) & valm);
+        if (lane_in(vis)) {
+            const uint32_t sl = total + mbcnt(vis);
+            s_pk[wave][sl] = bb[k];
+            s_pm[wave][sl] = (mm[k] & 0x80003fffu) | (j << 14);
+        }
+        total += (uint32_t)__builtin_popcountll(vis);
+        // (Not in the occlusion kernel: its Hi-Z taps, loaded later in this
+        // item, would wait for these loads too, in order.)
+        if (!OCCL && more) load_slice<WG, IPT>(c, it + 1u, k, bb[k], mm[k]);
     }
     ws.vis += total;
     const CullView* __restrict__ V = c.V;
     const bool rev = OCCL && (V->flags & HV_CULL_REVERSED_Z) != 0u;
     uint32_t pos = 0;
+    wave_lds_sync();
     for (uint32_t base = 0; base < total; base += 64u) {
-#pragma unroll
-        for (int k = 0; k < IPT; ++k)
-            if (vis[k] && slot[k] - base < 64u) {
-                const uint32_t sl = slot[k] - base;
-                s_pk[wave][sl] = bb[k];
-                s_pm[wave][sl] = mm[k];
-                s_pj[wave][sl] = k * WG + tid;
-            }
-        wave_lds_sync();
-        const bool act = lane < total - base;
+        const uint64_t act = ballot(lane < total - base);
         float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-        uint32_t m = 0u, jj = 0u;
-        if (act) {
-            p = s_pk[wave][lane];
-            m = s_pm[wave][lane];
-            jj = s_pj[wave][lane];
+        uint32_t pm = 0u;
+        if (lane_in(act)) {
+            p = s_pk[wave][base + lane];
+            pm = s_pm[wave][base + lane];
         }
-        bool draw;
-        uint32_t code = draw_part(*c.rc, act, p, m, draw);
+        const uint32_t m = pm & 0x80003fffu, jj = (pm >> 14) & 0xfffu;
+        uint64_t dm;
+        const uint32_t code = draw_part(*c.rc, act, p, m, dm);
         const uint64_t shm = ballot((int32_t)m < 0);
-        const uint64_t dm = ballot(draw);
         ws.draw += (uint32_t)__builtin_popcountll(dm);
         ws.sh_draw += (uint32_t)__builtin_popcountll(dm & shm);
+        uint64_t keep = dm;
         if constexpr (OCCL) {
             // occlusion_cull.wgsl:158-220 on the drawn candidates.
             bool occ = false;
-            if (draw) {
+            if (lane_in(dm)) {
                 OcclProbe q;
                 q.level = V->hiz0;
                 q.idx = HizRectIdx{0u, 0u, 0u, 0u};
@@ -463,19 +469,18 @@ __device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, cons
                 const HizTaps t = hiz_load4(q.level, q.idx);
                 occ = q.test && occl_resolve(t, q.near_z, rev);
             }
-            const uint64_t om = ballot(occ);
+            const uint64_t om = ballot(occ) & dm;
             ws.occl += (uint32_t)__builtin_popcountll(om);
             ws.sh_occl += (uint32_t)__builtin_popcountll(om & shm);
-            if (occ) code = kInvalid;
+            keep = dm & ~om;
         }
-        const uint64_t keep = ballot(code != kInvalid);
-        if (code != kInvalid) {
+        if (lane_in(keep)) {
             out[pos + mbcnt(keep)] = (code << kEntryShift) | jj;
             atomicAdd(&s_hist[code], 1u);
         }
         pos += (uint32_t)__builtin_popcountll(keep);
-        wave_lds_sync();  // the next round rewrites the slices
     }
+    wave_lds_sync();  // the next item rewrites the list
     kept += pos;
     if (lane == 0) c.sub_cnt[(size_t)it * NW + wave] = pos;
 }
@@ -526,14 +531,18 @@ __global__ __launch_bounds__(WG) void k_classify(const BatchHdr* __restrict__ hd
         __syncthreads();
         WaveStats ws;
         uint32_t n_valid = 0, kept = 0;
-        // One item at a time: latency is covered by the other resident waves
-        // (a two-deep register pipeline measured no faster and cost registers).
+        // The row's first item is loaded here; each item then loads the next
+        // one slice by slice as it frees the registers (the occlusion kernel
+        // loads every item here).
         const ItemCtx ic{V, &s_rc, bounds, meta, aabb, stage, sub_cnt, ibeg, vcount, vfirst};
+        const uint32_t row_first = item;
+        float4 bb[IPT];
+        uint32_t mm[IPT];
         for (; item < seg_end; ++item) {
-            float4 bb[IPT];
-            uint32_t mm[IPT];
-            load_item<WG, IPT>(ic, item, bb, mm);
-            process_item<WG, IPT, OCCL, AABB>(ic, item, bb, mm, s_hist, ws, n_valid, kept);
+            if (OCCL || item == row_first)
+#pragma unroll
+                for (int k = 0; k < IPT; ++k) load_slice<WG, IPT>(ic, item, k, bb[k], mm[k]);
+            process_item<WG, IPT, OCCL, AABB>(ic, item, item + 1u < seg_end, bb, mm, s_hist, ws, n_valid, kept);
         }
         // Row stats (layout of indirect_dispatch.wgsl:69-77): [0] survivors,
         // [1] invalid (not visible), [2] visible but sub-pixel, [3] drawn,

@@ -310,16 +310,23 @@ __device__ __forceinline__ void wave_queue(bool vis, uint32_t* count, uint16_t* 
 //      waves, so lanes culled by the frustum or the cone do not idle through it;
 //   C. one slot reservation for the workgroup, then the survivors' commands and
 //      metadata are written to consecutive slots.
-__device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, VgLds& L) {
+// Wave 0's work item of cull workgroup `block` (one per lane); `has` is false
+// for lanes past the workgroup's spans or the work-item list.
+__device__ __forceinline__ hv_vg_work_item vg_work_item(const VgArgs& a, uint32_t block, bool& has) {
+    const uint32_t lane = threadIdx.x & 63u, work = block * a.spans_per_block + lane;
+    has = lane < a.spans_per_block && work < a.work_item_count && work < a.work_item_buf_count;
+    return has ? a.work_items[work] : hv_vg_work_item{0u, 0u};
+}
+
+__device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, VgLds& L, hv_vg_work_item item,
+                                              bool has_item) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const VgView& v = L.v;
     if (tid < 64u) {
-        const uint32_t work = block * a.spans_per_block + lane;
         uint32_t first = 0, n = 0, ii = 0, lod = 0;
         bool ok = false;
         hv_instance_cull_data d{};
-        if (lane < a.spans_per_block && work < a.work_item_count && work < a.work_item_buf_count) {
-            const hv_vg_work_item item = a.work_items[work];
+        if (has_item) {
             if (item.object_index < a.object_count_buf) {
                 const hv_vg_object* o = a.objects + item.object_index;
                 uint32_t lc = 0;
@@ -459,8 +466,8 @@ __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, V
         }
         __syncthreads();
     }
+    const uint32_t tot = L.nsurv;
     if (tid == 0) {
-        const uint32_t tot = L.nsurv;
         uint32_t b = 0;
         if (tot) {
             b = atomicAdd(&a.counters[0], tot);
@@ -469,27 +476,33 @@ __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, V
         }
         L.base = b;
     }
-    __syncthreads();
-    const uint32_t tot = L.nsurv, b = L.base;
-    // Survivors below the capacity: the descriptors' draw fields are loaded
-    // kVgUnroll at a time before any store, so the reloads overlap.
-    const uint32_t keep = b < a.capacity ? min(tot, a.capacity - b) : 0u;
-    for (uint32_t e0 = 0; e0 < keep; e0 += kVgThreads * kVgUnroll) {
-        uint4 q3[kVgUnroll];
-        uint32_t mi[kVgUnroll], kk[kVgUnroll];
+    // The survivors' draw fields, kVgUnroll per thread at a time: the first
+    // batch is loaded while the reservation is in flight, every batch before
+    // any of its stores, so the loads overlap.
+    uint4 q3[kVgUnroll];
+    uint32_t mi[kVgUnroll], kk[kVgUnroll];
+    auto load_draws = [&](uint32_t e0, uint32_t lim) {
 #pragma unroll
         for (uint32_t u = 0; u < kVgUnroll; ++u) {
             const uint32_t e = e0 + u * kVgThreads + tid;
             kk[u] = 0;
             mi[u] = 0;
             q3[u] = make_uint4(0, 0, 0, 0);
-            if (e < keep) {
+            if (e < lim) {
                 const uint32_t en = L.stage[e];
                 kk[u] = en >> 6;
                 mi[u] = __float_as_uint(L.xf[kk[u]][kXfFirst]) + (en & 63u);
                 q3[u] = reinterpret_cast<const uint4*>(a.meshlets + mi[u])[3];
             }
         }
+    };
+    load_draws(0u, tot);
+    __syncthreads();
+    const uint32_t b = L.base;
+    // Survivors below the capacity.
+    const uint32_t keep = b < a.capacity ? min(tot, a.capacity - b) : 0u;
+    for (uint32_t e0 = 0; e0 < keep; e0 += kVgThreads * kVgUnroll) {
+        if (e0) load_draws(e0, keep);
 #pragma unroll
         for (uint32_t u = 0; u < kVgUnroll; ++u) {
             const uint32_t e = e0 + u * kVgThreads + tid;
@@ -517,11 +530,17 @@ __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, V
 // the rest cull work items. The cull recomputes each object's LOD with the
 // same select_lod, so the two halves do not depend on each other.
 __device__ __forceinline__ void vg_block(const VgArgs& a, uint32_t block, VgLds& L) {
+    const bool cull = block >= a.select_blocks;
+    // A cull workgroup's work items do not depend on the view: their loads
+    // are in flight beside the view copy.
+    bool has_item = false;
+    hv_vg_work_item item{0u, 0u};
+    if (cull && threadIdx.x < 64u) item = vg_work_item(a, block - a.select_blocks, has_item);
     if (threadIdx.x < kVgViewWords)
         reinterpret_cast<uint32_t*>(&L.v)[threadIdx.x] = reinterpret_cast<const uint32_t*>(a.view)[threadIdx.x];
     __syncthreads();
-    if (block < a.select_blocks) vg_select_block(a, block, L);
-    else vg_cull_block(a, block - a.select_blocks, L);
+    if (!cull) vg_select_block(a, block, L);
+    else vg_cull_block(a, block - a.select_blocks, L, item, has_item);
 }
 
 __global__ __launch_bounds__(kVgThreads) void k_vg(VgArgs a) {
