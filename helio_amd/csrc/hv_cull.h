@@ -63,6 +63,12 @@ struct alignas(16) CullView {
     uint32_t* row_stats;               // n_rows x 8: [0] survivors, [1..7] stats 1..7
     uint32_t* block_hist;              // n_rows x nb: bucket counts, then offsets within the bucket
     uint32_t* totals;                  // nb: bucket totals
+    // Single-view batches (no column scan): the rows add their bucket counts
+    // to bucket_cnt (= totals) with device atomics and keep the returned
+    // offsets; the scatter clears bucket_cnt_next, the set the next launch
+    // uses (the two sets alternate between launches). NULL: column scan.
+    uint32_t* bucket_cnt;
+    uint32_t* bucket_cnt_next;
 };
 
 struct alignas(16) BatchHdr {
@@ -125,7 +131,7 @@ class CullBatch {
     CullShape shape;
 
   private:
-    hv_status launch(hipStream_t stream, float* ms, bool fixed_grids);
+    hv_status launch(hipStream_t stream, float* ms, bool fixed_grids, uint32_t parity);
     hv_status ensure_occupancy(bool aabb);
     hv_ctx* ctx_;
     hv_cull_config cfg_;
@@ -142,9 +148,16 @@ class CullBatch {
     std::vector<uint64_t> hiz_gens_;    // each view's pyramid storage_gen at prepare (0 without one)
     hv_status check_scenes() const;
     DevBuf d_table_, d_stage_, d_ids_, d_args_, d_counters_, d_sub_cnt_, d_row_stats_, d_block_hist_, d_totals_;
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t graph_exec_ = nullptr;
-    uint64_t graph_key_[7] = {0, 0, 0, 0, 0, 0, 0};
+    // One graph per counter-set parity (a single-view batch alternates the
+    // two; other batches use graph 0 only).
+    hipGraph_t graph_[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};
+    uint64_t graph_key_[2][8] = {};
+    // Single-view batch: device counters instead of the column scan, two
+    // sets alternating by launch parity (see CullView::bucket_cnt).
+    bool atomic_ = false;
+    uint32_t parity_ = 0;
+    DevBuf d_bcnt_;
     hipEvent_t ev_[5] = {};
     hipEvent_t done_ = nullptr;    // recorded after every launch
     bool done_recorded_ = false;

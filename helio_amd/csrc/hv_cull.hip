@@ -563,7 +563,17 @@ __global__ __launch_bounds__(WG) void k_classify(const BatchHdr* __restrict__ hd
         __syncthreads();
         const uint32_t r = blockIdx.x - ldc(&V->block_begin);
         uint32_t* __restrict__ hrow = ldc(&V->block_hist) + (size_t)r * nb;
-        for (uint32_t b = tid; b < nb; b += WG) hrow[b] = s_hist[b];
+        uint32_t* __restrict__ bcnt = ldc(&V->bucket_cnt);
+        if (bcnt) {
+            // A single view has no column scan: the row takes its range in
+            // each of its buckets from the bucket's counter (rows in any order).
+            for (uint32_t b = tid; b < nb; b += WG) {
+                const uint32_t h = s_hist[b];
+                hrow[b] = h ? atomicAdd(bcnt + b, h) : 0u;
+            }
+        } else {
+            for (uint32_t b = tid; b < nb; b += WG) hrow[b] = s_hist[b];
+        }
         if (tid < 8) {
             uint32_t sum = 0;
 #pragma unroll
@@ -625,8 +635,11 @@ __device__ void block_excl_scan_lds(uint32_t* s, uint32_t n, uint32_t* s_w) {
 // contiguous bytes of a row. Per bucket: the exclusive prefix of the counts
 // over the view's rows (in place) and the total. A unit with slice
 // kStatsSlice sums the view's row stats instead.
-__device__ __forceinline__ void colscan_stats(const CullView* __restrict__ V, uint32_t R, uint32_t* s_part) {
-    constexpr uint32_t SG = kColscanThreads / 8u;  // row groups of 8 words; 16 rows per group in flight
+// The view's stats from its R row-stats records, by an NT-thread block
+// (s_part: NT words): counters [0..7] and the visible count.
+template <uint32_t NT>
+__device__ __forceinline__ void view_stats(const CullView* __restrict__ V, uint32_t R, uint32_t* s_part) {
+    constexpr uint32_t SG = NT / 8u;  // row groups of 8 words; 16 rows per group in flight
     const uint32_t tid = threadIdx.x, word = tid & 7u, g = tid >> 3;
     const uint32_t* __restrict__ rs = V->row_stats;
     uint32_t acc = 0;
@@ -663,7 +676,7 @@ __global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __r
         const CullView* __restrict__ V = views + v;
         const uint32_t R = ldc(&V->n_rows), nb = ldc(&V->nb);
         if (si == kStatsSlice) {
-            colscan_stats(V, R, s_part);
+            view_stats<kColscanThreads>(V, R, s_part);
             __syncthreads();
             continue;
         }
@@ -756,6 +769,7 @@ __global__ __launch_bounds__(NT) void k_scatter(const BatchHdr* __restrict__ hdr
     constexpr uint32_t KB = IPT >= 4 ? 1u : 4u / IPT;  // sub-chunks per batch: 4 entries per lane (IPT for IPT > 4)
     extern __shared__ uint32_t s_cur[];
     __shared__ uint32_t s_w[NWS];
+    __shared__ uint32_t s_part[NT];  // the view's stats (single view, row 0)
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
     const uint32_t G = ldc(&hdr->grid);
     const uint32_t n_views = ldc(&hdr->n_views), chunk = ldc(&hdr->chunk), sub = ldc(&hdr->sub);
@@ -815,6 +829,13 @@ __global__ __launch_bounds__(NT) void k_scatter(const BatchHdr* __restrict__ hdr
         for (uint32_t k = 0; k < KR; ++k)
             if (tid + NT * k < nb) cursor(tid + NT * k, offr[k]);
         for (uint32_t bb = tid + NT * KR; bb < nb; bb += NT) cursor(bb, offs[bb]);
+        // A single view without a column scan: row 0's block sums the rows'
+        // stats and clears the counter set of the next launch.
+        uint32_t* __restrict__ bnext = ldc(&V->bucket_cnt_next);
+        if (bnext && r == 0u) {
+            for (uint32_t bb = tid; bb < nb; bb += NT) bnext[bb] = 0u;
+            view_stats<NT>(V, ldc(&V->n_rows), s_part);
+        }
         __syncthreads();
         // Place the ids, batch by batch; the next batch's loads are in flight
         // while the current one is placed (two register sets, no copies).
@@ -1014,20 +1035,22 @@ CullBatch::~CullBatch() {
     (void)hipSetDevice(ctx_->device);
     (void)hipStreamSynchronize(ctx_->stream);
     (void)hipDeviceSynchronize();
-    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
+    for (int p = 0; p < 2; ++p) {
+        if (graph_exec_[p]) (void)hipGraphExecDestroy(graph_exec_[p]);
+        if (graph_[p]) (void)hipGraphDestroy(graph_[p]);
+    }
     for (auto& e : ev_)
         if (e) (void)hipEventDestroy(e);
     if (done_) (void)hipEventDestroy(done_);
     for (DevBuf* b : {&d_table_, &d_stage_, &d_ids_, &d_args_, &d_counters_, &d_sub_cnt_, &d_row_stats_, &d_block_hist_,
-                      &d_totals_})
+                      &d_totals_, &d_bcnt_})
         b->release();
 }
 
 uint64_t CullBatch::device_bytes() const {
     uint64_t t = 0;
     for (const DevBuf* b : {&d_table_, &d_stage_, &d_ids_, &d_args_, &d_counters_, &d_sub_cnt_, &d_row_stats_,
-                            &d_block_hist_, &d_totals_})
+                            &d_block_hist_, &d_totals_, &d_bcnt_})
         t += b->bytes;
     return t;
 }
@@ -1083,7 +1106,8 @@ hv_status CullBatch::reserve(uint32_t views, uint64_t candidates, uint32_t bucke
     HV_TRY(d_row_stats_.reserve(rows * 32, false, st));
     HV_TRY(d_block_hist_.reserve(std::max<uint64_t>(rows * buckets, 1) * 4, false, st));
     HV_TRY(d_counters_.reserve(nv * kCounterStride * 4, false, st));
-    HV_TRY(d_table_.reserve(sizeof(BatchHdr) + nv * sizeof(CullView) + unit_bound(nv, buckets, rows) * 4 +
+    HV_TRY(d_bcnt_.reserve((uint64_t)2 * buckets * 4, false, st));
+    HV_TRY(d_table_.reserve(sizeof(BatchHdr) + (nv + 1) * sizeof(CullView) + unit_bound(nv, buckets, rows) * 4 +
                                 (uint64_t)max_grid(ctx_) * 4,
                             false, st));
     lds_cap_ = std::max(lds_cap_, buckets * 4u);
@@ -1177,7 +1201,13 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
         while (v + 1 < n && it >= host_views_[v].item_begin + host_views_[v].n_items) ++v;
         host_block_view_[b] = v;
     }
-    const uint64_t units_off = sizeof(BatchHdr) + (uint64_t)std::max(n, 1u) * sizeof(CullView);
+    // A single-view batch with candidates skips the column scan: its rows
+    // count with device atomics into one of two counter sets, and the table
+    // holds one view record per set (the launch of parity p reads record p).
+    atomic_ = n == 1 && ids_total > 0;
+    if (atomic_) HV_TRY(d_bcnt_.reserve((uint64_t)2 * max_nb * 4, false, st));
+    const uint32_t n_rec = atomic_ ? 2u : std::max(n, 1u);
+    const uint64_t units_off = sizeof(BatchHdr) + (uint64_t)n_rec * sizeof(CullView);
     const uint64_t bview_off = units_off + std::max<uint64_t>(host_units_.size(), 1) * 4;
     const uint64_t table_bytes = bview_off + (uint64_t)host_block_view_.size() * 4;
     HV_TRY(d_table_.reserve(table_bytes, false, st));
@@ -1228,6 +1258,19 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     hdr.block_view = reinterpret_cast<const uint32_t*>(d_table_.as<uint8_t>() + bview_off);
     std::memcpy(buf.data(), &hdr, sizeof hdr);
     if (n) std::memcpy(buf.data() + sizeof hdr, host_views_.data(), (size_t)n * sizeof(CullView));
+    if (atomic_) {
+        for (uint32_t p = 0; p < 2; ++p) {
+            CullView cv = host_views_[0];
+            cv.bucket_cnt = d_bcnt_.as<uint32_t>() + (uint64_t)p * max_nb;
+            cv.bucket_cnt_next = d_bcnt_.as<uint32_t>() + (uint64_t)(p ^ 1u) * max_nb;
+            cv.totals = cv.bucket_cnt;
+            std::memcpy(buf.data() + sizeof hdr + p * sizeof(CullView), &cv, sizeof cv);
+        }
+        // The first launch (parity 0) counts from zero; every scatter clears
+        // the other set for the launch after it.
+        parity_ = 0;
+        HV_HIP(hipMemsetAsync(d_bcnt_.ptr, 0, (size_t)max_nb * 4, st));
+    }
     if (n_units_) std::memcpy(buf.data() + units_off, host_units_.data(), (size_t)n_units_ * 4);
     std::memcpy(buf.data() + bview_off, host_block_view_.data(), host_block_view_.size() * 4);
     HV_HIP(hipMemcpyAsync(d_table_.ptr, buf.data(), table_bytes, hipMemcpyHostToDevice, st));
@@ -1235,12 +1278,14 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     return HV_OK;
 }
 
-hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed) {
+hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed, uint32_t parity) {
     const bool occl = (cfg_.flags & HV_CULL_OCCLUSION) != 0;
     const bool aabb_any = std::any_of(host_views_.begin(), host_views_.end(), [](const CullView& c) { return c.aabb; });
     ClassifyFn fn = classify_fn(shape, occl, aabb_any);
     const BatchHdr* hdr = d_table_.as<BatchHdr>();
-    const CullView* views = reinterpret_cast<const CullView*>(d_table_.as<uint8_t>() + sizeof(BatchHdr));
+    // A single-view batch reads the view record of this launch's counter set.
+    const CullView* views = reinterpret_cast<const CullView*>(d_table_.as<uint8_t>() + sizeof(BatchHdr)) +
+                            (atomic_ ? parity : 0u);
     // Fixed (capacity) grids make one captured graph valid for any later table:
     // blocks beyond the table's G (or past the last unit) exit at once.
     const size_t lds = fixed ? (size_t)lds_cap_ : (size_t)max_nb_ * 4;
@@ -1254,9 +1299,11 @@ hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed) {
     hipLaunchKernelGGL(fn, dim3(g1), dim3(shape.wg), lds, s, hdr, views);
     HV_HIP(hipGetLastError());
     if (ms) HV_HIP(hipEventRecord(ev_[1], s));
-    hipLaunchKernelGGL(dev::k_colscan, dim3(g2), dim3(kColscanThreads), 0, s, hdr, views);
-    HV_HIP(hipGetLastError());
-    if (ms) HV_HIP(hipEventRecord(ev_[2], s));
+    if (!atomic_) {
+        hipLaunchKernelGGL(dev::k_colscan, dim3(g2), dim3(kColscanThreads), 0, s, hdr, views);
+        HV_HIP(hipGetLastError());
+        if (ms) HV_HIP(hipEventRecord(ev_[2], s));
+    }
     hipLaunchKernelGGL(scatter_fn(shape), dim3(g3), dim3(scatter_threads(shape)), lds, s, hdr, views);
     HV_HIP(hipGetLastError());
     if (!fixed) {  // eager: record completion now (a replay records it lazily)
@@ -1267,7 +1314,13 @@ hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed) {
     if (ms) {
         HV_HIP(hipEventRecord(ev_[3], s));
         HV_HIP(hipEventSynchronize(ev_[3]));
-        for (int i = 0; i < 3; ++i) HV_HIP(hipEventElapsedTime(&ms[i], ev_[i], ev_[i + 1]));
+        if (atomic_) {  // no column scan: ms[1] = 0
+            HV_HIP(hipEventElapsedTime(&ms[0], ev_[0], ev_[1]));
+            ms[1] = 0.0f;
+            HV_HIP(hipEventElapsedTime(&ms[2], ev_[1], ev_[3]));
+        } else {
+            for (int i = 0; i < 3; ++i) HV_HIP(hipEventElapsedTime(&ms[i], ev_[i], ev_[i + 1]));
+        }
     }
     return HV_OK;
 }
@@ -1303,7 +1356,9 @@ hv_status CullBatch::execute(hipStream_t s, float* ms) {
     HV_HIP(hipSetDevice(ctx_->device));
     if (ms && !ev_[0])
         for (auto& e : ev_) HV_HIP(hipEventCreate(&e));
-    return launch(s, ms, false);
+    HV_TRY(launch(s, ms, false, parity_));
+    if (atomic_) parity_ ^= 1u;
+    return HV_OK;
 }
 
 hv_status CullBatch::execute_graph(hipStream_t s) {
@@ -1315,18 +1370,22 @@ hv_status CullBatch::execute_graph(hipStream_t s) {
     const bool aabb_any = std::any_of(host_views_.begin(), host_views_.end(), [](const CullView& c) { return c.aabb; });
     // The table's address is a kernel argument of every captured launch: a
     // prepare that had to grow the table moved it, and the graph is re-captured.
-    const uint64_t key[7] = {resident1_, lds_cap_, aabb_any ? 1u : 0u, shape.wg * 16u + shape.ipt, ctx_->cu_count,
-                             resident3_, reinterpret_cast<uint64_t>(d_table_.ptr)};
-    if (!graph_exec_ || std::memcmp(key, graph_key_, sizeof key) != 0) {
-        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-        if (graph_) (void)hipGraphDestroy(graph_);
-        graph_exec_ = nullptr;
-        graph_ = nullptr;
+    // A single-view batch alternates two graphs (one per counter set).
+    const uint32_t p = atomic_ ? parity_ : 0u;
+    const uint64_t key[8] = {resident1_, lds_cap_, aabb_any ? 1u : 0u, shape.wg * 16u + shape.ipt, ctx_->cu_count,
+                             resident3_, reinterpret_cast<uint64_t>(d_table_.ptr), atomic_ ? 1u : 0u};
+    hipGraph_t& graph = graph_[p];
+    hipGraphExec_t& graph_exec = graph_exec_[p];
+    if (!graph_exec || std::memcmp(key, graph_key_[p], sizeof key) != 0) {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
         hipStream_t cap;
         HV_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
         hv_status st = HV_OK;
-        if (e == hipSuccess) st = launch(cap, nullptr, true);
+        if (e == hipSuccess) st = launch(cap, nullptr, true, p);
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(cap, &g);
         (void)hipStreamDestroy(cap);
@@ -1336,14 +1395,15 @@ hv_status CullBatch::execute_graph(hipStream_t s) {
             return st;
         }
         if (e2 != hipSuccess) return hip_error(e2, "hipStreamEndCapture", __FILE__, __LINE__);
-        graph_ = g;
-        HV_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
-        std::memcpy(graph_key_, key, sizeof key);
+        graph = g;
+        HV_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+        std::memcpy(graph_key_[p], key, sizeof key);
     }
     // No completion event per replay (it costs a queue packet each time): the
     // stream is remembered and sync_last() records the event on it when a
     // later call needs the launch finished.
-    HV_HIP(hipGraphLaunch(graph_exec_, s));
+    HV_HIP(hipGraphLaunch(graph_exec, s));
+    if (atomic_) parity_ ^= 1u;
     pending_ = true;
     pending_stream_ = s;
     return HV_OK;
