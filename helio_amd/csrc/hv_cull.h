@@ -15,10 +15,10 @@ constexpr uint32_t kCounterStride = 16;   // u32 counters per view
 constexpr uint32_t kColscanThreads = 1024; // threads of a column-scan block
 constexpr uint32_t kColscanRows = 32;     // rows of one bucket a column-scan thread holds in registers
 constexpr uint32_t kColscanMinLg = 5;     // narrowest bucket slice of a column-scan unit: 32 buckets (128-byte rows)
-constexpr uint32_t kStatsSlice = 1023;    // slice field of the column-scan unit that sums a view's stats
+constexpr uint32_t kBaseThreads = 1024;   // threads of a bucket-base block (one view at a time)
 constexpr uint32_t kEntryShift = 12;      // stage entry = (bucket << 12) | index in chunk (chunk <= 4096)
 
-// Per-view results (u32, kCounterStride per view), written by the column scan:
+// Per-view results (u32, kCounterStride per view), written by k_bases:
 //   [0..7] stats (layout of indirect_dispatch.wgsl:69-77), [8] visible ids.
 enum : uint32_t { kCtrVisible = 8 };
 
@@ -63,6 +63,7 @@ struct alignas(16) CullView {
     uint32_t* row_stats;               // n_rows x 8: [0] survivors, [1..7] stats 1..7
     uint32_t* block_hist;              // n_rows x nb: bucket counts, then offsets within the bucket
     uint32_t* totals;                  // nb: bucket totals
+    uint32_t* bases;                   // nb: first id slot of each bucket (exclusive scan of the totals)
     // Single-view batches (no column scan): the rows add their bucket counts
     // to bucket_cnt (= totals) with device atomics and keep the returned
     // offsets; the scatter clears bucket_cnt_next, the set the next launch
@@ -75,14 +76,14 @@ struct alignas(16) BatchHdr {
     uint32_t n_views;
     uint32_t n_items;
     uint32_t n_rows;
-    uint32_t n_units;        // column-scan work units
+    uint32_t n_units;        // column-scan work units (view | bucket slice << 10)
     uint32_t grid;           // G: blocks that own items (classify and scatter)
     uint32_t chunk;          // candidates per work item (WG * IPT)
     uint32_t sub;            // stage slots per (item, wave) = IPT * 64
     uint32_t waves_log2;     // log2(WG / 64)
     uint32_t* stage;         // n_items x chunk packed survivors; wave w of item i fills [(i*W+w)*sub, +count)
     uint32_t* sub_cnt;       // n_items x W survivor counts
-    const uint32_t* units;   // n_units x (view | slice index << 10); slice kStatsSlice sums the stats
+    const uint32_t* units;   // n_units x (view | slice index << 10)
     const uint32_t* block_view;  // G: view holding each block's first item
 };
 
@@ -147,7 +148,8 @@ class CullBatch {
     std::vector<uint64_t> scene_gens_;  // each view's scene storage_gen at prepare
     std::vector<uint64_t> hiz_gens_;    // each view's pyramid storage_gen at prepare (0 without one)
     hv_status check_scenes() const;
-    DevBuf d_table_, d_stage_, d_ids_, d_args_, d_counters_, d_sub_cnt_, d_row_stats_, d_block_hist_, d_totals_;
+    DevBuf d_table_, d_stage_, d_ids_, d_args_, d_counters_, d_sub_cnt_, d_row_stats_, d_block_hist_, d_totals_,
+        d_bases_;
     // One graph per counter-set parity (a single-view batch alternates the
     // two; other batches use graph 0 only).
     hipGraph_t graph_[2] = {nullptr, nullptr};

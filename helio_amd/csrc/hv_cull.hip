@@ -19,10 +19,13 @@
 //               the row's stats, once per row. No barrier inside a row.
 //   k_colscan   work units of (view, bucket slice): per bucket, the exclusive
 //               prefix of the counts over the view's rows, and the bucket
-//               total; one more unit per view sums the row stats.
+//               total (not run for a batch of one view: its rows take their
+//               offsets from device counters in classify).
+//   k_bases     one block per view: the bucket bases (exclusive scan of the
+//               totals), the DrawIndexedIndirect args and the view's stats.
 //   k_scatter   the same block -> item ranges as classify: per row, cursors =
-//               scan(totals) + row offsets in LDS, args from row 0, then a flat
-//               loop over the row's entries places each id with an LDS atomic.
+//               bases + row offsets in LDS, then a flat loop over the row's
+//               entries places each id with an LDS atomic.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -609,32 +612,13 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* s_w, u
     return base + inc - x;
 }
 
-// In-place exclusive scan of s[0..n) by an NT-thread block (s_w: NT / 64 words).
-template <int NT>
-__device__ void block_excl_scan_lds(uint32_t* s, uint32_t n, uint32_t* s_w) {
-    const uint32_t tid = threadIdx.x;
-    const uint32_t per = (n + NT - 1u) / NT;
-    const uint32_t beg = min(n, tid * per), end = min(n, beg + per);
-    uint32_t local = 0;
-    for (uint32_t i = beg; i < end; ++i) local += s[i];
-    uint32_t total;
-    uint32_t run = block_excl_scan<NT>(local, s_w, total);
-    for (uint32_t i = beg; i < end; ++i) {
-        const uint32_t t = s[i];
-        s[i] = run;
-        run += t;
-    }
-    __syncthreads();
-}
-
 // Column scan. A unit is one slice of 2^lg buckets of one view (lg per view,
 // from 5 to 10): its 1024 threads form 1024 >> lg row groups, a thread holds
 // one bucket of one group's contiguous run of rows (at most kColscanRows in
 // registers, so every row is read and written once), and consecutive lanes
 // hold consecutive buckets, so each access of a wave covers at least 128
 // contiguous bytes of a row. Per bucket: the exclusive prefix of the counts
-// over the view's rows (in place) and the total. A unit with slice
-// kStatsSlice sums the view's row stats instead.
+// over the view's rows (in place) and the total.
 // The view's stats from its R row-stats records, by an NT-thread block
 // (s_part: NT words): counters [0..7] and the visible count.
 template <uint32_t NT>
@@ -675,11 +659,6 @@ __global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __r
         const uint32_t v = code & 1023u, si = code >> 10;
         const CullView* __restrict__ V = views + v;
         const uint32_t R = ldc(&V->n_rows), nb = ldc(&V->nb);
-        if (si == kStatsSlice) {
-            view_stats<kColscanThreads>(V, R, s_part);
-            __syncthreads();
-            continue;
-        }
         const uint32_t lg = ldc(&V->col_lg), RG = kColscanThreads >> lg;
         const uint32_t col = tid & ((1u << lg) - 1u), grp = tid >> lg;
         const uint32_t b = (si << lg) + col;
@@ -726,21 +705,56 @@ __global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __r
                 run += t;
             }
         }
-        if (live && grp == 0u) {
-            V->totals[b] = total;
-            // An empty view has no row 0 in the scatter: its args are written here.
-            if (R == 0u) {
-                const hv_mesh_lod ml = V->mesh_lods[b];
-                hv_draw_indexed_indirect a;
-                a.index_count = ml.index_count;
-                a.instance_count = 0u;
-                a.first_index = ml.first_index;
-                a.base_vertex = ml.base_vertex;
-                a.first_instance = 0u;
-                V->args[b] = a;
-            }
-        }
+        if (live && grp == 0u) V->totals[b] = total;
         __syncthreads();
+    }
+}
+
+// Per view (views taken grid-stride): the bucket bases (exclusive scan of the
+// bucket totals), the DrawIndexedIndirect args (indirect_dispatch.wgsl's
+// per-draw record: the bucket's mesh LOD range, instance_count = total,
+// first_instance = base) and the view's stats from its rows; a one-view batch
+// also clears the counter set of its next launch (unused by this one). Every
+// row's scatter then starts from the bases instead of scanning the totals.
+__global__ __launch_bounds__(kBaseThreads) void k_bases(const BatchHdr* __restrict__ hdr,
+                                                        const CullView* __restrict__ views) {
+    constexpr uint32_t T = kBaseThreads;
+    __shared__ uint32_t s_w[T / 64];
+    __shared__ uint32_t s_part[T];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t n_views = ldc(&hdr->n_views);
+    for (uint32_t v = blockIdx.x; v < n_views; v += gridDim.x) {
+        const CullView* __restrict__ V = views + v;
+        const uint32_t nb = ldc(&V->nb);
+        const uint32_t* __restrict__ totals = ldc(&V->totals);
+        uint32_t* __restrict__ bases = ldc(&V->bases);
+        const hv_mesh_lod* __restrict__ mesh_lods = ldc(&V->mesh_lods);
+        hv_draw_indexed_indirect* __restrict__ args = ldc(&V->args);
+        // Thread t scans buckets [t per, (t + 1) per) (per = 1 up to 1024 buckets).
+        const uint32_t per = (nb + T - 1u) / T;
+        const uint32_t beg = min(nb, tid * per), end = min(nb, beg + per);
+        uint32_t local = 0;
+        for (uint32_t b = beg; b < end; ++b) local += totals[b];
+        uint32_t sum;
+        uint32_t run = block_excl_scan<T>(local, s_w, sum);
+        for (uint32_t b = beg; b < end; ++b) {
+            const uint32_t t = totals[b];
+            const hv_mesh_lod ml = mesh_lods[b];
+            hv_draw_indexed_indirect a;
+            a.index_count = ml.index_count;
+            a.instance_count = t;
+            a.first_index = ml.first_index;
+            a.base_vertex = ml.base_vertex;
+            a.first_instance = run;
+            args[b] = a;
+            bases[b] = run;
+            run += t;
+        }
+        uint32_t* __restrict__ bnext = ldc(&V->bucket_cnt_next);
+        if (bnext)
+            for (uint32_t b = tid; b < nb; b += T) bnext[b] = 0u;
+        view_stats<T>(V, ldc(&V->n_rows), s_part);
+        __syncthreads();  // s_w and s_part are reused by the next view
     }
 }
 
@@ -764,12 +778,9 @@ __device__ __forceinline__ void scatter_load(const uint32_t* __restrict__ stage,
 
 template <int NT, int IPT>
 __global__ __launch_bounds__(NT) void k_scatter(const BatchHdr* __restrict__ hdr, const CullView* __restrict__ views) {
-    constexpr uint32_t KR = 1024u / NT;  // offsets held in registers: the first 1024 buckets
     constexpr uint32_t NWS = NT / 64;     // scatter waves
     constexpr uint32_t KB = IPT >= 4 ? 1u : 4u / IPT;  // sub-chunks per batch: 4 entries per lane (IPT for IPT > 4)
     extern __shared__ uint32_t s_cur[];
-    __shared__ uint32_t s_w[NWS];
-    __shared__ uint32_t s_part[NT];  // the view's stats (single view, row 0)
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
     const uint32_t G = ldc(&hdr->grid);
     const uint32_t n_views = ldc(&hdr->n_views), chunk = ldc(&hdr->chunk), sub = ldc(&hdr->sub);
@@ -787,7 +798,7 @@ __global__ __launch_bounds__(NT) void k_scatter(const BatchHdr* __restrict__ hdr
         const uint32_t nb = ldc(&V->nb), ibeg = ldc(&V->item_begin), vfirst = ldc(&V->first);
         const uint32_t seg_end = min(i_end, ibeg + ldc(&V->n_items));
         const uint32_t r = cb - ldc(&V->block_begin);
-        const uint32_t* __restrict__ totals = ldc(&V->totals);
+        const uint32_t* __restrict__ bases = ldc(&V->bases);
         const uint32_t* __restrict__ offs = ldc(&V->block_hist) + (size_t)r * nb;
         uint32_t* __restrict__ ids = ldc(&V->ids);
         const size_t gs0 = (size_t)item << wl, gs1 = (size_t)seg_end << wl;
@@ -795,47 +806,8 @@ __global__ __launch_bounds__(NT) void k_scatter(const BatchHdr* __restrict__ hdr
         uint32_t cA[KB], eA[KB][IPT], cB[KB], eB[KB][IPT];
 #pragma unroll
         for (uint32_t k = 0; k < KB; ++k) scatter_load<NT, IPT>(stage, sub_cnt, gs0 + wave + k * NWS, gs1, sub, cA[k], eA[k]);
-        // Bucket totals and this row's offsets (the first 1024 buckets' offsets
-        // in registers; larger tables reread theirs below).
-        uint32_t offr[KR];
-#pragma unroll
-        for (uint32_t k = 0; k < KR; ++k) {
-            const uint32_t bb = tid + NT * k;
-            offr[k] = bb < nb ? offs[bb] : 0u;
-            if (bb < nb) s_cur[bb] = totals[bb];
-        }
-        for (uint32_t bb = tid + NT * KR; bb < nb; bb += NT) s_cur[bb] = totals[bb];
-        __syncthreads();
-        block_excl_scan_lds<NT>(s_cur, nb, s_w);
-        // Cursors: bucket base (scan of the totals) + this row's offsets; row 0
-        // writes the args.
-        const hv_mesh_lod* __restrict__ mesh_lods = ldc(&V->mesh_lods);
-        hv_draw_indexed_indirect* __restrict__ args = ldc(&V->args);
-        auto cursor = [&](uint32_t bb, uint32_t off) {
-            const uint32_t base = s_cur[bb];
-            if (r == 0u) {
-                const hv_mesh_lod ml = mesh_lods[bb];
-                hv_draw_indexed_indirect a;
-                a.index_count = ml.index_count;
-                a.instance_count = totals[bb];
-                a.first_index = ml.first_index;
-                a.base_vertex = ml.base_vertex;
-                a.first_instance = base;
-                args[bb] = a;
-            }
-            s_cur[bb] = base + off;
-        };
-#pragma unroll
-        for (uint32_t k = 0; k < KR; ++k)
-            if (tid + NT * k < nb) cursor(tid + NT * k, offr[k]);
-        for (uint32_t bb = tid + NT * KR; bb < nb; bb += NT) cursor(bb, offs[bb]);
-        // A single view without a column scan: row 0's block sums the rows'
-        // stats and clears the counter set of the next launch.
-        uint32_t* __restrict__ bnext = ldc(&V->bucket_cnt_next);
-        if (bnext && r == 0u) {
-            for (uint32_t bb = tid; bb < nb; bb += NT) bnext[bb] = 0u;
-            view_stats<NT>(V, ldc(&V->n_rows), s_part);
-        }
+        // Cursors: the bucket's base (k_bases) + this row's offset in it.
+        for (uint32_t bb = tid; bb < nb; bb += NT) s_cur[bb] = bases[bb] + offs[bb];
         __syncthreads();
         // Place the ids, batch by batch; the next batch's loads are in flight
         // while the current one is placed (two register sets, no copies).
@@ -1043,14 +1015,14 @@ CullBatch::~CullBatch() {
         if (e) (void)hipEventDestroy(e);
     if (done_) (void)hipEventDestroy(done_);
     for (DevBuf* b : {&d_table_, &d_stage_, &d_ids_, &d_args_, &d_counters_, &d_sub_cnt_, &d_row_stats_, &d_block_hist_,
-                      &d_totals_, &d_bcnt_})
+                      &d_totals_, &d_bases_, &d_bcnt_})
         b->release();
 }
 
 uint64_t CullBatch::device_bytes() const {
     uint64_t t = 0;
     for (const DevBuf* b : {&d_table_, &d_stage_, &d_ids_, &d_args_, &d_counters_, &d_sub_cnt_, &d_row_stats_,
-                            &d_block_hist_, &d_totals_, &d_bcnt_})
+                            &d_block_hist_, &d_totals_, &d_bases_, &d_bcnt_})
         t += b->bytes;
     return t;
 }
@@ -1060,8 +1032,7 @@ uint32_t CullBatch::max_grid(const hv_ctx* ctx) { return std::max(1u, ctx->cu_co
 namespace {
 
 // Upper bound of column-scan units for `views` views of at most `buckets`
-// buckets (slices are at least 2^kColscanMinLg buckets wide; one stats unit
-// per view).
+// buckets (slices are at least 2^kColscanMinLg buckets wide).
 uint64_t unit_bound(uint64_t views, uint64_t buckets, uint64_t /*rows*/) {
     return views * ((buckets >> kColscanMinLg) + 2);
 }
@@ -1086,7 +1057,7 @@ uint64_t CullBatch::bytes_for(uint64_t candidates, uint32_t views, uint32_t buck
     const uint64_t items = candidates / chunk + views;
     const uint64_t rows = std::min(items, max_rows + views);
     return items * chunk * 4 + items * 8 * 4 + candidates * 4 + rows * (buckets + 8ull) * 4 +
-           (uint64_t)views * (buckets * 24ull + kCounterStride * 4 + sizeof(CullView)) +
+           (uint64_t)views * (buckets * 28ull + kCounterStride * 4 + sizeof(CullView)) +
            unit_bound(views, buckets, rows) * 4 + max_rows * 4;
 }
 
@@ -1103,6 +1074,7 @@ hv_status CullBatch::reserve(uint32_t views, uint64_t candidates, uint32_t bucke
     HV_TRY(d_ids_.reserve(std::max<uint64_t>(candidates, 1) * 4, false, st));
     HV_TRY(d_args_.reserve(nv * buckets * sizeof(hv_draw_indexed_indirect), false, st));
     HV_TRY(d_totals_.reserve(nv * buckets * 4, false, st));
+    HV_TRY(d_bases_.reserve(nv * buckets * 4, false, st));
     HV_TRY(d_row_stats_.reserve(rows * 32, false, st));
     HV_TRY(d_block_hist_.reserve(std::max<uint64_t>(rows * buckets, 1) * 4, false, st));
     HV_TRY(d_counters_.reserve(nv * kCounterStride * 4, false, st));
@@ -1181,7 +1153,6 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
         item += cv.n_items;
         rows_total += cv.n_rows;
         hist_total += (uint64_t)cv.n_rows * nb;
-        host_units_.push_back(v | (kStatsSlice << 10));
         cv.col_lg = col_lg(cv.n_rows);
         for (uint32_t si = 0; (si << cv.col_lg) < nb; ++si) host_units_.push_back(v | (si << 10));
     }
@@ -1191,6 +1162,7 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     HV_TRY(d_ids_.reserve(std::max<uint64_t>(ids_total, 1) * 4, false, st));
     HV_TRY(d_args_.reserve(std::max<uint64_t>(args_total, 1) * sizeof(hv_draw_indexed_indirect), false, st));
     HV_TRY(d_totals_.reserve(std::max<uint64_t>(args_total, 1) * 4, false, st));
+    HV_TRY(d_bases_.reserve(std::max<uint64_t>(args_total, 1) * 4, false, st));
     HV_TRY(d_row_stats_.reserve(std::max<uint64_t>(rows_total, 1) * 32, false, st));
     HV_TRY(d_block_hist_.reserve(std::max<uint64_t>(hist_total, 1) * 4, false, st));
     HV_TRY(d_counters_.reserve((uint64_t)std::max(n, 1u) * kCounterStride * 4, false, st));
@@ -1220,6 +1192,7 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
         cv.ids = d_ids_.as<uint32_t>() + so;
         cv.args = d_args_.as<hv_draw_indexed_indirect>() + ao;
         cv.totals = d_totals_.as<uint32_t>() + ao;
+        cv.bases = d_bases_.as<uint32_t>() + ao;
         cv.counters = d_counters_.as<uint32_t>() + (uint64_t)v * kCounterStride;
         cv.row_stats = d_row_stats_.as<uint32_t>() + ro * 8;
         cv.block_hist = d_block_hist_.as<uint32_t>() + ho;
@@ -1302,8 +1275,12 @@ hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed, uint32_t parit
     if (!atomic_) {
         hipLaunchKernelGGL(dev::k_colscan, dim3(g2), dim3(kColscanThreads), 0, s, hdr, views);
         HV_HIP(hipGetLastError());
-        if (ms) HV_HIP(hipEventRecord(ev_[2], s));
     }
+    // One bucket-base block per view (a capacity grid: one per view the batch can hold).
+    const uint32_t g4 = fixed ? std::min(max_views_, kMaxViews) : std::max(1u, n_views_);
+    hipLaunchKernelGGL(dev::k_bases, dim3(std::max(1u, g4)), dim3(kBaseThreads), 0, s, hdr, views);
+    HV_HIP(hipGetLastError());
+    if (ms) HV_HIP(hipEventRecord(ev_[2], s));
     hipLaunchKernelGGL(scatter_fn(shape), dim3(g3), dim3(scatter_threads(shape)), lds, s, hdr, views);
     HV_HIP(hipGetLastError());
     if (!fixed) {  // eager: record completion now (a replay records it lazily)
@@ -1314,13 +1291,7 @@ hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed, uint32_t parit
     if (ms) {
         HV_HIP(hipEventRecord(ev_[3], s));
         HV_HIP(hipEventSynchronize(ev_[3]));
-        if (atomic_) {  // no column scan: ms[1] = 0
-            HV_HIP(hipEventElapsedTime(&ms[0], ev_[0], ev_[1]));
-            ms[1] = 0.0f;
-            HV_HIP(hipEventElapsedTime(&ms[2], ev_[1], ev_[3]));
-        } else {
-            for (int i = 0; i < 3; ++i) HV_HIP(hipEventElapsedTime(&ms[i], ev_[i], ev_[i + 1]));
-        }
+        for (int i = 0; i < 3; ++i) HV_HIP(hipEventElapsedTime(&ms[i], ev_[i], ev_[i + 1]));
     }
     return HV_OK;
 }
@@ -1483,8 +1454,8 @@ __global__ __launch_bounds__(256) void k_one_view_table(BatchHdr hdr, CullView v
     const uint32_t* v = reinterpret_cast<const uint32_t*>(&view);
     constexpr uint32_t HW = sizeof(BatchHdr) / 4, VW = sizeof(CullView) / 4;
     for (uint32_t i = tid; i < HW + VW; i += 256u) table[i] = i < HW ? h[i] : v[i - HW];
-    // Unit 0 sums the stats; the others cover buckets [b0, b0 + 2^lg).
-    for (uint32_t u = tid; u < hdr.n_units; u += 256u) units[u] = u == 0u ? (kStatsSlice << 10) : ((u - 1u) << 10);
+    // Unit u covers buckets [u 2^lg, (u + 1) 2^lg).
+    for (uint32_t u = tid; u < hdr.n_units; u += 256u) units[u] = u << 10;
     for (uint32_t b = tid; b < n_block_view; b += 256u) block_view[b] = 0u;
 }
 
@@ -1607,7 +1578,7 @@ hv_status hv_inst_cull_noalloc(hv_ctx* ctx, const hv_cull_config* config, const 
     const uint32_t G = N ? std::min(resident1, N) : 0u;
     const uint32_t R = G;
     const uint32_t lg = col_lg(R);
-    const uint32_t n_units = 1u + (nb + (1u << lg) - 1u) / (1u << lg);
+    const uint32_t n_units = (nb + (1u << lg) - 1u) / (1u << lg);
     // Scratch layout: table (header, view, units, block views), then the
     // kernels' buffers, each on a 256-byte boundary.
     const uint64_t units_off = sizeof(BatchHdr) + sizeof(CullView);
@@ -1618,7 +1589,8 @@ hv_status hv_inst_cull_noalloc(hv_ctx* ctx, const hv_cull_config* config, const 
     const uint64_t rstat_off = align256(sub_off + (uint64_t)N * nw * 4u);
     const uint64_t hist_off = align256(rstat_off + (uint64_t)R * 32u);
     const uint64_t tot_off = align256(hist_off + (uint64_t)R * nb * 4u);
-    const uint64_t need = tot_off + (uint64_t)nb * 4u;
+    const uint64_t bases_off = align256(tot_off + (uint64_t)nb * 4u);
+    const uint64_t need = bases_off + (uint64_t)nb * 4u;
     if (need > HV_NOALLOC_SCRATCH_BYTES)
         return set_error(HV_ELIMIT,
                          "hv_inst_cull_noalloc: %u candidates x %u buckets need %llu scratch bytes, more than the %llu "
@@ -1637,6 +1609,7 @@ hv_status hv_inst_cull_noalloc(hv_ctx* ctx, const hv_cull_config* config, const 
     cv.row_stats = reinterpret_cast<uint32_t*>(base + rstat_off);
     cv.block_hist = reinterpret_cast<uint32_t*>(base + hist_off);
     cv.totals = reinterpret_cast<uint32_t*>(base + tot_off);
+    cv.bases = reinterpret_cast<uint32_t*>(base + bases_off);
     BatchHdr hdr{};
     hdr.n_views = 1;
     hdr.n_items = N;
@@ -1660,6 +1633,8 @@ hv_status hv_inst_cull_noalloc(hv_ctx* ctx, const hv_cull_config* config, const 
     hipLaunchKernelGGL(fn, dim3(std::max(G, 1u)), dim3(shape.wg), lds, s, dhdr, dviews);
     HV_HIP(hipGetLastError());
     hipLaunchKernelGGL(dev::k_colscan, dim3(colscan_grid(ctx, n_units)), dim3(kColscanThreads), 0, s, dhdr, dviews);
+    HV_HIP(hipGetLastError());
+    hipLaunchKernelGGL(dev::k_bases, dim3(1), dim3(kBaseThreads), 0, s, dhdr, dviews);
     HV_HIP(hipGetLastError());
     hipLaunchKernelGGL(scatter_fn(shape), dim3(std::max(1u, std::min(G, resident3))), dim3(scatter_threads(shape)), lds,
                        s, dhdr, dviews);
