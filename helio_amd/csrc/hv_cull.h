@@ -15,10 +15,12 @@ constexpr uint32_t kCounterStride = 16;   // u32 counters per view
 constexpr uint32_t kColscanThreads = 1024; // threads of a column-scan block
 constexpr uint32_t kColscanRows = 32;     // rows of one bucket a column-scan thread holds in registers
 constexpr uint32_t kColscanMinLg = 5;     // narrowest bucket slice of a column-scan unit: 32 buckets (128-byte rows)
-constexpr uint32_t kBaseThreads = 1024;   // threads of a bucket-base block (one view at a time)
+constexpr uint32_t kStatsSlice = 1023;    // slice field of the column-scan unit that sums a view's stats
+constexpr uint32_t kBaseThreads = 1024;   // threads of a bucket-base block (one-view batches)
 constexpr uint32_t kEntryShift = 12;      // stage entry = (bucket << 12) | index in chunk (chunk <= 4096)
 
-// Per-view results (u32, kCounterStride per view), written by k_bases:
+// Per-view results (u32, kCounterStride per view), written by the column scan
+// (k_bases for a one-view batch):
 //   [0..7] stats (layout of indirect_dispatch.wgsl:69-77), [8] visible ids.
 enum : uint32_t { kCtrVisible = 8 };
 
@@ -63,7 +65,7 @@ struct alignas(16) CullView {
     uint32_t* row_stats;               // n_rows x 8: [0] survivors, [1..7] stats 1..7
     uint32_t* block_hist;              // n_rows x nb: bucket counts, then offsets within the bucket
     uint32_t* totals;                  // nb: bucket totals
-    uint32_t* bases;                   // nb: first id slot of each bucket (exclusive scan of the totals)
+    uint32_t* bases;                   // one-view batch: nb first id slots per bucket (scan of the totals)
     // Single-view batches (no column scan): the rows add their bucket counts
     // to bucket_cnt (= totals) with device atomics and keep the returned
     // offsets; the scatter clears bucket_cnt_next, the set the next launch
@@ -83,7 +85,7 @@ struct alignas(16) BatchHdr {
     uint32_t waves_log2;     // log2(WG / 64)
     uint32_t* stage;         // n_items x chunk packed survivors; wave w of item i fills [(i*W+w)*sub, +count)
     uint32_t* sub_cnt;       // n_items x W survivor counts
-    const uint32_t* units;   // n_units x (view | slice index << 10)
+    const uint32_t* units;   // n_units x (view | slice index << 10); slice kStatsSlice sums the stats
     const uint32_t* block_view;  // G: view holding each block's first item
 };
 
@@ -150,11 +152,12 @@ class CullBatch {
     hv_status check_scenes() const;
     DevBuf d_table_, d_stage_, d_ids_, d_args_, d_counters_, d_sub_cnt_, d_row_stats_, d_block_hist_, d_totals_,
         d_bases_;
-    // One graph per counter-set parity (a single-view batch alternates the
-    // two; other batches use graph 0 only).
-    hipGraph_t graph_[2] = {nullptr, nullptr};
-    hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};
-    uint64_t graph_key_[2][8] = {};
+    // Captured graphs: [0] column-scan batches, [1] and [2] one-view batches
+    // by counter-set parity (kept apart, so a batch that alternates between
+    // one and several views never re-captures).
+    hipGraph_t graph_[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t graph_exec_[3] = {nullptr, nullptr, nullptr};
+    uint64_t graph_key_[3][8] = {};
     // Single-view batch: device counters instead of the column scan, two
     // sets alternating by launch parity (see CullView::bucket_cnt).
     bool atomic_ = false;

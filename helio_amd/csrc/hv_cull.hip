@@ -19,13 +19,14 @@
 //               the row's stats, once per row. No barrier inside a row.
 //   k_colscan   work units of (view, bucket slice): per bucket, the exclusive
 //               prefix of the counts over the view's rows, and the bucket
-//               total (not run for a batch of one view: its rows take their
-//               offsets from device counters in classify).
-//   k_bases     one block per view: the bucket bases (exclusive scan of the
-//               totals), the DrawIndexedIndirect args and the view's stats.
+//               total; one more unit per view sums the row stats.
 //   k_scatter   the same block -> item ranges as classify: per row, cursors =
-//               bases + row offsets in LDS, then a flat loop over the row's
-//               entries places each id with an LDS atomic.
+//               scan(totals) + row offsets in LDS, args from row 0, then a flat
+//               loop over the row's entries places each id with an LDS atomic.
+// A batch of one view with candidates takes k_classify -> k_bases ->
+// k_scatter instead: its rows take their bucket offsets from device counters
+// in classify, and k_bases computes the view's bucket bases, args and stats
+// once for the scatter.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -389,7 +390,10 @@ struct WaveStats {
 // of once per round of mostly idle lanes. A slice's registers are free once
 // it is packed, so when the row has a next item (`more`) the slice's
 // registers take that item's slice at once: its loads are in flight while
-// the rest of this item is classified, at no register cost.
+// the rest of this item is classified, at no register cost. The occlusion
+// kernel does not prefetch (its Hi-Z tap loads would wait behind those loads,
+// in order), so it keeps the item in registers and packs each round's 64
+// candidates from them into a 64-entry list (a quarter of the LDS).
 template <int WG, int IPT, bool OCCL, bool AABB>
 __device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, bool more, float4* bb, uint32_t* mm,
                                              uint32_t* s_hist, WaveStats& ws, uint32_t& n_valid, uint32_t& kept) {
@@ -398,8 +402,9 @@ __device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, bool
     // The packed candidates of each wave: bounds, and meta with the index in
     // the chunk in bits 14..25 (mesh ids are below 14,336 here: at most
     // kMaxBuckets buckets, each mesh with >= 1 LOD).
-    __shared__ float4 s_pk[NW][SUB];
-    __shared__ uint32_t s_pm[NW][SUB];
+    constexpr uint32_t LIST = OCCL ? 64u : SUB;
+    __shared__ float4 s_pk[NW][LIST];
+    __shared__ uint32_t s_pm[NW][LIST];
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
     const uint32_t base_idx = (it - c.ibeg) * CHUNK;
     const uint32_t rem = c.vcount - base_idx;
@@ -418,8 +423,11 @@ __device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, bool
     for (int i = 0; i < 6; ++i) l1[i] = ldc(&Vk->plane_l1[i]);
     const bool boxes = AABB && c.aabb != nullptr;  // uniform: the view's scene holds explicit AABBs
     const bool sure = ldc(&Vk->sure_ok) != 0u;
-    // Frustum on every candidate; its survivors appended to the wave's list.
+    // Frustum on every candidate; its survivors appended to the wave's list
+    // (occlusion kernel: their masks and ranks kept for the rounds).
     uint32_t total = 0;  // wave-uniform
+    uint64_t visk[IPT];
+    uint32_t slotk[IPT];
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
         const uint32_t j = k * WG + tid;
@@ -428,14 +436,15 @@ __device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, bool
                                                 boxes ? c.aabb + first + (j < n ? j : 0u) : nullptr);
         n_valid += (uint32_t)__builtin_popcountll(valm);
         ws.sh += (uint32_t)__builtin_popcountll(ballot((int32_t)mm[k] < 0) & valm);
-        if (lane_in(vis)) {
+        if constexpr (OCCL) {
+            visk[k] = vis;
+            slotk[k] = total + mbcnt(vis);
+        } else if (lane_in(vis)) {
             const uint32_t sl = total + mbcnt(vis);
             s_pk[wave][sl] = bb[k];
             s_pm[wave][sl] = (mm[k] & 0x80003fffu) | (j << 14);
         }
         total += (uint32_t)__builtin_popcountll(vis);
-        // (Not in the occlusion kernel: its Hi-Z taps, loaded later in this
-        // item, would wait for these loads too, in order.)
         if (!OCCL && more) load_slice<WG, IPT>(c, it + 1u, k, bb[k], mm[k]);
     }
     ws.vis += total;
@@ -444,12 +453,24 @@ __device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, bool
     uint32_t pos = 0;
     wave_lds_sync();
     for (uint32_t base = 0; base < total; base += 64u) {
+        if constexpr (OCCL) {
+            // This round's 64 candidates, packed from the registers.
+#pragma unroll
+            for (int k = 0; k < IPT; ++k)
+                if (lane_in(visk[k]) && slotk[k] - base < 64u) {
+                    const uint32_t sl = slotk[k] - base;
+                    s_pk[wave][sl] = bb[k];
+                    s_pm[wave][sl] = (mm[k] & 0x80003fffu) | ((k * WG + tid) << 14);
+                }
+            wave_lds_sync();
+        }
+        const uint32_t li = OCCL ? lane : base + lane;
         const uint64_t act = ballot(lane < total - base);
         float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
         uint32_t pm = 0u;
         if (lane_in(act)) {
-            p = s_pk[wave][base + lane];
-            pm = s_pm[wave][base + lane];
+            p = s_pk[wave][li];
+            pm = s_pm[wave][li];
         }
         const uint32_t m = pm & 0x80003fffu, jj = (pm >> 14) & 0xfffu;
         uint64_t dm;
@@ -482,6 +503,7 @@ __device__ __forceinline__ void process_item(const ItemCtx& c, uint32_t it, bool
             atomicAdd(&s_hist[code], 1u);
         }
         pos += (uint32_t)__builtin_popcountll(keep);
+        if constexpr (OCCL) wave_lds_sync();  // the next round rewrites the list
     }
     wave_lds_sync();  // the next item rewrites the list
     kept += pos;
@@ -538,14 +560,20 @@ __global__ __launch_bounds__(WG) void k_classify(const BatchHdr* __restrict__ hd
         // one slice by slice as it frees the registers (the occlusion kernel
         // loads every item here).
         const ItemCtx ic{V, &s_rc, bounds, meta, aabb, stage, sub_cnt, ibeg, vcount, vfirst};
-        const uint32_t row_first = item;
         float4 bb[IPT];
         uint32_t mm[IPT];
-        for (; item < seg_end; ++item) {
-            if (OCCL || item == row_first)
+        if constexpr (!OCCL) {
+            if (item < seg_end)
 #pragma unroll
                 for (int k = 0; k < IPT; ++k) load_slice<WG, IPT>(ic, item, k, bb[k], mm[k]);
-            process_item<WG, IPT, OCCL, AABB>(ic, item, item + 1u < seg_end, bb, mm, s_hist, ws, n_valid, kept);
+            for (; item < seg_end; ++item)
+                process_item<WG, IPT, OCCL, AABB>(ic, item, item + 1u < seg_end, bb, mm, s_hist, ws, n_valid, kept);
+        } else {
+            for (; item < seg_end; ++item) {
+#pragma unroll
+                for (int k = 0; k < IPT; ++k) load_slice<WG, IPT>(ic, item, k, bb[k], mm[k]);
+                process_item<WG, IPT, OCCL, AABB>(ic, item, false, bb, mm, s_hist, ws, n_valid, kept);
+            }
         }
         // Row stats (layout of indirect_dispatch.wgsl:69-77): [0] survivors,
         // [1] invalid (not visible), [2] visible but sub-pixel, [3] drawn,
@@ -612,13 +640,32 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* s_w, u
     return base + inc - x;
 }
 
+// In-place exclusive scan of s[0..n) by an NT-thread block (s_w: NT / 64 words).
+template <int NT>
+__device__ void block_excl_scan_lds(uint32_t* s, uint32_t n, uint32_t* s_w) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t per = (n + NT - 1u) / NT;
+    const uint32_t beg = min(n, tid * per), end = min(n, beg + per);
+    uint32_t local = 0;
+    for (uint32_t i = beg; i < end; ++i) local += s[i];
+    uint32_t total;
+    uint32_t run = block_excl_scan<NT>(local, s_w, total);
+    for (uint32_t i = beg; i < end; ++i) {
+        const uint32_t t = s[i];
+        s[i] = run;
+        run += t;
+    }
+    __syncthreads();
+}
+
 // Column scan. A unit is one slice of 2^lg buckets of one view (lg per view,
 // from 5 to 10): its 1024 threads form 1024 >> lg row groups, a thread holds
 // one bucket of one group's contiguous run of rows (at most kColscanRows in
 // registers, so every row is read and written once), and consecutive lanes
 // hold consecutive buckets, so each access of a wave covers at least 128
 // contiguous bytes of a row. Per bucket: the exclusive prefix of the counts
-// over the view's rows (in place) and the total.
+// over the view's rows (in place) and the total. A unit with slice
+// kStatsSlice sums the view's row stats instead.
 // The view's stats from its R row-stats records, by an NT-thread block
 // (s_part: NT words): counters [0..7] and the visible count.
 template <uint32_t NT>
@@ -659,6 +706,11 @@ __global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __r
         const uint32_t v = code & 1023u, si = code >> 10;
         const CullView* __restrict__ V = views + v;
         const uint32_t R = ldc(&V->n_rows), nb = ldc(&V->nb);
+        if (si == kStatsSlice) {
+            view_stats<kColscanThreads>(V, R, s_part);
+            __syncthreads();
+            continue;
+        }
         const uint32_t lg = ldc(&V->col_lg), RG = kColscanThreads >> lg;
         const uint32_t col = tid & ((1u << lg) - 1u), grp = tid >> lg;
         const uint32_t b = (si << lg) + col;
@@ -705,17 +757,32 @@ __global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __r
                 run += t;
             }
         }
-        if (live && grp == 0u) V->totals[b] = total;
+        if (live && grp == 0u) {
+            V->totals[b] = total;
+            // An empty view has no row 0 in the scatter: its args are written here.
+            if (R == 0u) {
+                const hv_mesh_lod ml = V->mesh_lods[b];
+                hv_draw_indexed_indirect a;
+                a.index_count = ml.index_count;
+                a.instance_count = 0u;
+                a.first_index = ml.first_index;
+                a.base_vertex = ml.base_vertex;
+                a.first_instance = 0u;
+                V->args[b] = a;
+            }
+        }
         __syncthreads();
     }
 }
 
-// Per view (views taken grid-stride): the bucket bases (exclusive scan of the
-// bucket totals), the DrawIndexedIndirect args (indirect_dispatch.wgsl's
-// per-draw record: the bucket's mesh LOD range, instance_count = total,
-// first_instance = base) and the view's stats from its rows; a one-view batch
-// also clears the counter set of its next launch (unused by this one). Every
-// row's scatter then starts from the bases instead of scanning the totals.
+// One-view batches (counter mode), per view: the bucket bases (exclusive
+// scan of the bucket totals), the DrawIndexedIndirect args (the bucket's mesh
+// LOD range, instance_count = total, first_instance = base), the view's stats
+// from its rows, and the clear of the counter set of the next launch (unused
+// by this one). Every row's scatter then starts from the bases instead of
+// scanning the totals. (Batches of several views keep the scan in the
+// scatter: a fourth kernel in their chain measured slower with two batches
+// in flight, its blocks holding CUs the other batch's classify needs.)
 __global__ __launch_bounds__(kBaseThreads) void k_bases(const BatchHdr* __restrict__ hdr,
                                                         const CullView* __restrict__ views) {
     constexpr uint32_t T = kBaseThreads;
@@ -776,11 +843,13 @@ __device__ __forceinline__ void scatter_load(const uint32_t* __restrict__ stage,
     }
 }
 
-template <int NT, int IPT>
+template <int NT, int IPT, bool BASES>
 __global__ __launch_bounds__(NT) void k_scatter(const BatchHdr* __restrict__ hdr, const CullView* __restrict__ views) {
+    constexpr uint32_t KR = 1024u / NT;  // offsets held in registers: the first 1024 buckets
     constexpr uint32_t NWS = NT / 64;     // scatter waves
     constexpr uint32_t KB = IPT >= 4 ? 1u : 4u / IPT;  // sub-chunks per batch: 4 entries per lane (IPT for IPT > 4)
     extern __shared__ uint32_t s_cur[];
+    __shared__ uint32_t s_w[NWS];
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
     const uint32_t G = ldc(&hdr->grid);
     const uint32_t n_views = ldc(&hdr->n_views), chunk = ldc(&hdr->chunk), sub = ldc(&hdr->sub);
@@ -798,7 +867,7 @@ __global__ __launch_bounds__(NT) void k_scatter(const BatchHdr* __restrict__ hdr
         const uint32_t nb = ldc(&V->nb), ibeg = ldc(&V->item_begin), vfirst = ldc(&V->first);
         const uint32_t seg_end = min(i_end, ibeg + ldc(&V->n_items));
         const uint32_t r = cb - ldc(&V->block_begin);
-        const uint32_t* __restrict__ bases = ldc(&V->bases);
+        const uint32_t* __restrict__ totals = ldc(&V->totals);
         const uint32_t* __restrict__ offs = ldc(&V->block_hist) + (size_t)r * nb;
         uint32_t* __restrict__ ids = ldc(&V->ids);
         const size_t gs0 = (size_t)item << wl, gs1 = (size_t)seg_end << wl;
@@ -806,8 +875,46 @@ __global__ __launch_bounds__(NT) void k_scatter(const BatchHdr* __restrict__ hdr
         uint32_t cA[KB], eA[KB][IPT], cB[KB], eB[KB][IPT];
 #pragma unroll
         for (uint32_t k = 0; k < KB; ++k) scatter_load<NT, IPT>(stage, sub_cnt, gs0 + wave + k * NWS, gs1, sub, cA[k], eA[k]);
-        // Cursors: the bucket's base (k_bases) + this row's offset in it.
-        for (uint32_t bb = tid; bb < nb; bb += NT) s_cur[bb] = bases[bb] + offs[bb];
+        if constexpr (BASES) {
+            // Cursors: the bucket's base (k_bases) + this row's offset in it.
+            const uint32_t* __restrict__ bases = ldc(&V->bases);
+            for (uint32_t bb = tid; bb < nb; bb += NT) s_cur[bb] = bases[bb] + offs[bb];
+        } else {
+            // Bucket totals and this row's offsets (the first 1024 buckets'
+            // offsets in registers; larger tables reread theirs below).
+            uint32_t offr[KR];
+#pragma unroll
+            for (uint32_t k = 0; k < KR; ++k) {
+                const uint32_t bb = tid + NT * k;
+                offr[k] = bb < nb ? offs[bb] : 0u;
+                if (bb < nb) s_cur[bb] = totals[bb];
+            }
+            for (uint32_t bb = tid + NT * KR; bb < nb; bb += NT) s_cur[bb] = totals[bb];
+            __syncthreads();
+            block_excl_scan_lds<NT>(s_cur, nb, s_w);
+            // Cursors: bucket base (scan of the totals) + this row's offsets;
+            // row 0 writes the args.
+            const hv_mesh_lod* __restrict__ mesh_lods = ldc(&V->mesh_lods);
+            hv_draw_indexed_indirect* __restrict__ args = ldc(&V->args);
+            auto cursor = [&](uint32_t bb, uint32_t off) {
+                const uint32_t base = s_cur[bb];
+                if (r == 0u) {
+                    const hv_mesh_lod ml = mesh_lods[bb];
+                    hv_draw_indexed_indirect a;
+                    a.index_count = ml.index_count;
+                    a.instance_count = totals[bb];
+                    a.first_index = ml.first_index;
+                    a.base_vertex = ml.base_vertex;
+                    a.first_instance = base;
+                    args[bb] = a;
+                }
+                s_cur[bb] = base + off;
+            };
+#pragma unroll
+            for (uint32_t k = 0; k < KR; ++k)
+                if (tid + NT * k < nb) cursor(tid + NT * k, offr[k]);
+            for (uint32_t bb = tid + NT * KR; bb < nb; bb += NT) cursor(bb, offs[bb]);
+        }
         __syncthreads();
         // Place the ids, batch by batch; the next batch's loads are in flight
         // while the current one is placed (two register sets, no copies).
@@ -876,19 +983,22 @@ ClassifyFn pick_ipt(uint32_t ipt, bool occl, bool aabb) {
 using ScatterFn = void (*)(const BatchHdr*, const CullView*);
 
 template <int NT>
-ScatterFn scatter_ipt(uint32_t ipt) {
+ScatterFn scatter_ipt(uint32_t ipt, bool bases) {
     switch (ipt) {
-        case 1: return dev::k_scatter<NT, 1>;
-        case 2: return dev::k_scatter<NT, 2>;
-        case 4: return dev::k_scatter<NT, 4>;
-        default: return dev::k_scatter<NT, 8>;
+        case 1: return bases ? dev::k_scatter<NT, 1, true> : dev::k_scatter<NT, 1, false>;
+        case 2: return bases ? dev::k_scatter<NT, 2, true> : dev::k_scatter<NT, 2, false>;
+        case 4: return bases ? dev::k_scatter<NT, 4, true> : dev::k_scatter<NT, 4, false>;
+        default: return bases ? dev::k_scatter<NT, 8, true> : dev::k_scatter<NT, 8, false>;
     }
 }
 
 // Scatter variant for a classify shape: 512 threads beside a 512-wide
 // classify (one scatter block per classify block), else 256; IPT entries per
 // lane per sub-chunk.
-ScatterFn scatter_fn(const CullShape& s) { return s.wg >= 512 ? scatter_ipt<512>(s.ipt) : scatter_ipt<256>(s.ipt); }
+// `bases`: the one-view variant (cursors from k_bases).
+ScatterFn scatter_fn(const CullShape& s, bool bases = false) {
+    return s.wg >= 512 ? scatter_ipt<512>(s.ipt, bases) : scatter_ipt<256>(s.ipt, bases);
+}
 uint32_t scatter_threads(const CullShape& s) { return s.wg >= 512 ? 512u : 256u; }
 
 ClassifyFn classify_fn(const CullShape& s, bool occl, bool aabb) {
@@ -1007,7 +1117,7 @@ CullBatch::~CullBatch() {
     (void)hipSetDevice(ctx_->device);
     (void)hipStreamSynchronize(ctx_->stream);
     (void)hipDeviceSynchronize();
-    for (int p = 0; p < 2; ++p) {
+    for (int p = 0; p < 3; ++p) {
         if (graph_exec_[p]) (void)hipGraphExecDestroy(graph_exec_[p]);
         if (graph_[p]) (void)hipGraphDestroy(graph_[p]);
     }
@@ -1057,7 +1167,7 @@ uint64_t CullBatch::bytes_for(uint64_t candidates, uint32_t views, uint32_t buck
     const uint64_t items = candidates / chunk + views;
     const uint64_t rows = std::min(items, max_rows + views);
     return items * chunk * 4 + items * 8 * 4 + candidates * 4 + rows * (buckets + 8ull) * 4 +
-           (uint64_t)views * (buckets * 28ull + kCounterStride * 4 + sizeof(CullView)) +
+           (uint64_t)views * (buckets * 24ull + kCounterStride * 4 + sizeof(CullView)) +
            unit_bound(views, buckets, rows) * 4 + max_rows * 4;
 }
 
@@ -1154,6 +1264,7 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
         rows_total += cv.n_rows;
         hist_total += (uint64_t)cv.n_rows * nb;
         cv.col_lg = col_lg(cv.n_rows);
+        host_units_.push_back(v | (kStatsSlice << 10));
         for (uint32_t si = 0; (si << cv.col_lg) < nb; ++si) host_units_.push_back(v | (si << 10));
     }
     const hipStream_t st = ctx_->stream;
@@ -1272,16 +1383,14 @@ hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed, uint32_t parit
     hipLaunchKernelGGL(fn, dim3(g1), dim3(shape.wg), lds, s, hdr, views);
     HV_HIP(hipGetLastError());
     if (ms) HV_HIP(hipEventRecord(ev_[1], s));
-    if (!atomic_) {
+    // Bucket offsets: the column scan, or (one view) the bucket-base block.
+    if (atomic_)
+        hipLaunchKernelGGL(dev::k_bases, dim3(1), dim3(kBaseThreads), 0, s, hdr, views);
+    else
         hipLaunchKernelGGL(dev::k_colscan, dim3(g2), dim3(kColscanThreads), 0, s, hdr, views);
-        HV_HIP(hipGetLastError());
-    }
-    // One bucket-base block per view (a capacity grid: one per view the batch can hold).
-    const uint32_t g4 = fixed ? std::min(max_views_, kMaxViews) : std::max(1u, n_views_);
-    hipLaunchKernelGGL(dev::k_bases, dim3(std::max(1u, g4)), dim3(kBaseThreads), 0, s, hdr, views);
     HV_HIP(hipGetLastError());
     if (ms) HV_HIP(hipEventRecord(ev_[2], s));
-    hipLaunchKernelGGL(scatter_fn(shape), dim3(g3), dim3(scatter_threads(shape)), lds, s, hdr, views);
+    hipLaunchKernelGGL(scatter_fn(shape, atomic_), dim3(g3), dim3(scatter_threads(shape)), lds, s, hdr, views);
     HV_HIP(hipGetLastError());
     if (!fixed) {  // eager: record completion now (a replay records it lazily)
         HV_HIP(hipEventRecord(done_, s));
@@ -1342,12 +1451,12 @@ hv_status CullBatch::execute_graph(hipStream_t s) {
     // The table's address is a kernel argument of every captured launch: a
     // prepare that had to grow the table moved it, and the graph is re-captured.
     // A single-view batch alternates two graphs (one per counter set).
-    const uint32_t p = atomic_ ? parity_ : 0u;
+    const uint32_t p = atomic_ ? parity_ : 0u, gi = atomic_ ? 1u + parity_ : 0u;
     const uint64_t key[8] = {resident1_, lds_cap_, aabb_any ? 1u : 0u, shape.wg * 16u + shape.ipt, ctx_->cu_count,
                              resident3_, reinterpret_cast<uint64_t>(d_table_.ptr), atomic_ ? 1u : 0u};
-    hipGraph_t& graph = graph_[p];
-    hipGraphExec_t& graph_exec = graph_exec_[p];
-    if (!graph_exec || std::memcmp(key, graph_key_[p], sizeof key) != 0) {
+    hipGraph_t& graph = graph_[gi];
+    hipGraphExec_t& graph_exec = graph_exec_[gi];
+    if (!graph_exec || std::memcmp(key, graph_key_[gi], sizeof key) != 0) {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
         graph_exec = nullptr;
@@ -1368,7 +1477,7 @@ hv_status CullBatch::execute_graph(hipStream_t s) {
         if (e2 != hipSuccess) return hip_error(e2, "hipStreamEndCapture", __FILE__, __LINE__);
         graph = g;
         HV_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
-        std::memcpy(graph_key_[p], key, sizeof key);
+        std::memcpy(graph_key_[gi], key, sizeof key);
     }
     // No completion event per replay (it costs a queue packet each time): the
     // stream is remembered and sync_last() records the event on it when a
@@ -1454,8 +1563,8 @@ __global__ __launch_bounds__(256) void k_one_view_table(BatchHdr hdr, CullView v
     const uint32_t* v = reinterpret_cast<const uint32_t*>(&view);
     constexpr uint32_t HW = sizeof(BatchHdr) / 4, VW = sizeof(CullView) / 4;
     for (uint32_t i = tid; i < HW + VW; i += 256u) table[i] = i < HW ? h[i] : v[i - HW];
-    // Unit u covers buckets [u 2^lg, (u + 1) 2^lg).
-    for (uint32_t u = tid; u < hdr.n_units; u += 256u) units[u] = u << 10;
+    // Unit 0 sums the stats; the others cover buckets [b0, b0 + 2^lg).
+    for (uint32_t u = tid; u < hdr.n_units; u += 256u) units[u] = u == 0u ? (kStatsSlice << 10) : ((u - 1u) << 10);
     for (uint32_t b = tid; b < n_block_view; b += 256u) block_view[b] = 0u;
 }
 
@@ -1578,7 +1687,7 @@ hv_status hv_inst_cull_noalloc(hv_ctx* ctx, const hv_cull_config* config, const 
     const uint32_t G = N ? std::min(resident1, N) : 0u;
     const uint32_t R = G;
     const uint32_t lg = col_lg(R);
-    const uint32_t n_units = (nb + (1u << lg) - 1u) / (1u << lg);
+    const uint32_t n_units = 1u + (nb + (1u << lg) - 1u) / (1u << lg);
     // Scratch layout: table (header, view, units, block views), then the
     // kernels' buffers, each on a 256-byte boundary.
     const uint64_t units_off = sizeof(BatchHdr) + sizeof(CullView);
@@ -1589,8 +1698,7 @@ hv_status hv_inst_cull_noalloc(hv_ctx* ctx, const hv_cull_config* config, const 
     const uint64_t rstat_off = align256(sub_off + (uint64_t)N * nw * 4u);
     const uint64_t hist_off = align256(rstat_off + (uint64_t)R * 32u);
     const uint64_t tot_off = align256(hist_off + (uint64_t)R * nb * 4u);
-    const uint64_t bases_off = align256(tot_off + (uint64_t)nb * 4u);
-    const uint64_t need = bases_off + (uint64_t)nb * 4u;
+    const uint64_t need = tot_off + (uint64_t)nb * 4u;
     if (need > HV_NOALLOC_SCRATCH_BYTES)
         return set_error(HV_ELIMIT,
                          "hv_inst_cull_noalloc: %u candidates x %u buckets need %llu scratch bytes, more than the %llu "
@@ -1609,7 +1717,6 @@ hv_status hv_inst_cull_noalloc(hv_ctx* ctx, const hv_cull_config* config, const 
     cv.row_stats = reinterpret_cast<uint32_t*>(base + rstat_off);
     cv.block_hist = reinterpret_cast<uint32_t*>(base + hist_off);
     cv.totals = reinterpret_cast<uint32_t*>(base + tot_off);
-    cv.bases = reinterpret_cast<uint32_t*>(base + bases_off);
     BatchHdr hdr{};
     hdr.n_views = 1;
     hdr.n_items = N;
@@ -1633,8 +1740,6 @@ hv_status hv_inst_cull_noalloc(hv_ctx* ctx, const hv_cull_config* config, const 
     hipLaunchKernelGGL(fn, dim3(std::max(G, 1u)), dim3(shape.wg), lds, s, dhdr, dviews);
     HV_HIP(hipGetLastError());
     hipLaunchKernelGGL(dev::k_colscan, dim3(colscan_grid(ctx, n_units)), dim3(kColscanThreads), 0, s, dhdr, dviews);
-    HV_HIP(hipGetLastError());
-    hipLaunchKernelGGL(dev::k_bases, dim3(1), dim3(kBaseThreads), 0, s, dhdr, dviews);
     HV_HIP(hipGetLastError());
     hipLaunchKernelGGL(scatter_fn(shape), dim3(std::max(1u, std::min(G, resident3))), dim3(scatter_threads(shape)), lds,
                        s, dhdr, dviews);
