@@ -16,11 +16,9 @@ constexpr uint32_t kColscanThreads = 1024; // threads of a column-scan block
 constexpr uint32_t kColscanRows = 32;     // rows of one bucket a column-scan thread holds in registers
 constexpr uint32_t kColscanMinLg = 5;     // narrowest bucket slice of a column-scan unit: 32 buckets (128-byte rows)
 constexpr uint32_t kStatsSlice = 1023;    // slice field of the column-scan unit that sums a view's stats
-constexpr uint32_t kBaseThreads = 1024;   // threads of a bucket-base block (one-view batches)
 constexpr uint32_t kEntryShift = 12;      // stage entry = (bucket << 12) | index in chunk (chunk <= 4096)
 
-// Per-view results (u32, kCounterStride per view), written by the column scan
-// (k_bases for a one-view batch):
+// Per-view results (u32, kCounterStride per view), written by the column scan:
 //   [0..7] stats (layout of indirect_dispatch.wgsl:69-77), [8] visible ids.
 enum : uint32_t { kCtrVisible = 8 };
 
@@ -65,20 +63,13 @@ struct alignas(16) CullView {
     uint32_t* row_stats;               // n_rows x 8: [0] survivors, [1..7] stats 1..7
     uint32_t* block_hist;              // n_rows x nb: bucket counts, then offsets within the bucket
     uint32_t* totals;                  // nb: bucket totals
-    uint32_t* bases;                   // one-view batch: nb first id slots per bucket (scan of the totals)
-    // Single-view batches (no column scan): the rows add their bucket counts
-    // to bucket_cnt (= totals) with device atomics and keep the returned
-    // offsets; the scatter clears bucket_cnt_next, the set the next launch
-    // uses (the two sets alternate between launches). NULL: column scan.
-    uint32_t* bucket_cnt;
-    uint32_t* bucket_cnt_next;
 };
 
 struct alignas(16) BatchHdr {
     uint32_t n_views;
     uint32_t n_items;
     uint32_t n_rows;
-    uint32_t n_units;        // column-scan work units (view | bucket slice << 10)
+    uint32_t n_units;        // column-scan work units
     uint32_t grid;           // G: blocks that own items (classify and scatter)
     uint32_t chunk;          // candidates per work item (WG * IPT)
     uint32_t sub;            // stage slots per (item, wave) = IPT * 64
@@ -134,7 +125,7 @@ class CullBatch {
     CullShape shape;
 
   private:
-    hv_status launch(hipStream_t stream, float* ms, bool fixed_grids, uint32_t parity);
+    hv_status launch(hipStream_t stream, float* ms, bool fixed_grids);
     hv_status ensure_occupancy(bool aabb);
     hv_ctx* ctx_;
     hv_cull_config cfg_;
@@ -150,19 +141,10 @@ class CullBatch {
     std::vector<uint64_t> scene_gens_;  // each view's scene storage_gen at prepare
     std::vector<uint64_t> hiz_gens_;    // each view's pyramid storage_gen at prepare (0 without one)
     hv_status check_scenes() const;
-    DevBuf d_table_, d_stage_, d_ids_, d_args_, d_counters_, d_sub_cnt_, d_row_stats_, d_block_hist_, d_totals_,
-        d_bases_;
-    // Captured graphs: [0] column-scan batches, [1] and [2] one-view batches
-    // by counter-set parity (kept apart, so a batch that alternates between
-    // one and several views never re-captures).
-    hipGraph_t graph_[3] = {nullptr, nullptr, nullptr};
-    hipGraphExec_t graph_exec_[3] = {nullptr, nullptr, nullptr};
-    uint64_t graph_key_[3][8] = {};
-    // Single-view batch: device counters instead of the column scan, two
-    // sets alternating by launch parity (see CullView::bucket_cnt).
-    bool atomic_ = false;
-    uint32_t parity_ = 0;
-    DevBuf d_bcnt_;
+    DevBuf d_table_, d_stage_, d_ids_, d_args_, d_counters_, d_sub_cnt_, d_row_stats_, d_block_hist_, d_totals_;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    uint64_t graph_key_[7] = {0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev_[5] = {};
     hipEvent_t done_ = nullptr;    // recorded after every launch
     bool done_recorded_ = false;

@@ -23,10 +23,6 @@
 //   k_scatter   the same block -> item ranges as classify: per row, cursors =
 //               scan(totals) + row offsets in LDS, args from row 0, then a flat
 //               loop over the row's entries places each id with an LDS atomic.
-// A batch of one view with candidates takes k_classify -> k_bases ->
-// k_scatter instead: its rows take their bucket offsets from device counters
-// in classify, and k_bases computes the view's bucket bases, args and stats
-// once for the scatter.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -594,17 +590,7 @@ __global__ __launch_bounds__(WG) void k_classify(const BatchHdr* __restrict__ hd
         __syncthreads();
         const uint32_t r = blockIdx.x - ldc(&V->block_begin);
         uint32_t* __restrict__ hrow = ldc(&V->block_hist) + (size_t)r * nb;
-        uint32_t* __restrict__ bcnt = ldc(&V->bucket_cnt);
-        if (bcnt) {
-            // A single view has no column scan: the row takes its range in
-            // each of its buckets from the bucket's counter (rows in any order).
-            for (uint32_t b = tid; b < nb; b += WG) {
-                const uint32_t h = s_hist[b];
-                hrow[b] = h ? atomicAdd(bcnt + b, h) : 0u;
-            }
-        } else {
-            for (uint32_t b = tid; b < nb; b += WG) hrow[b] = s_hist[b];
-        }
+        for (uint32_t b = tid; b < nb; b += WG) hrow[b] = s_hist[b];
         if (tid < 8) {
             uint32_t sum = 0;
 #pragma unroll
@@ -666,11 +652,8 @@ __device__ void block_excl_scan_lds(uint32_t* s, uint32_t n, uint32_t* s_w) {
 // contiguous bytes of a row. Per bucket: the exclusive prefix of the counts
 // over the view's rows (in place) and the total. A unit with slice
 // kStatsSlice sums the view's row stats instead.
-// The view's stats from its R row-stats records, by an NT-thread block
-// (s_part: NT words): counters [0..7] and the visible count.
-template <uint32_t NT>
-__device__ __forceinline__ void view_stats(const CullView* __restrict__ V, uint32_t R, uint32_t* s_part) {
-    constexpr uint32_t SG = NT / 8u;  // row groups of 8 words; 16 rows per group in flight
+__device__ __forceinline__ void colscan_stats(const CullView* __restrict__ V, uint32_t R, uint32_t* s_part) {
+    constexpr uint32_t SG = kColscanThreads / 8u;  // row groups of 8 words; 16 rows per group in flight
     const uint32_t tid = threadIdx.x, word = tid & 7u, g = tid >> 3;
     const uint32_t* __restrict__ rs = V->row_stats;
     uint32_t acc = 0;
@@ -707,7 +690,7 @@ __global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __r
         const CullView* __restrict__ V = views + v;
         const uint32_t R = ldc(&V->n_rows), nb = ldc(&V->nb);
         if (si == kStatsSlice) {
-            view_stats<kColscanThreads>(V, R, s_part);
+            colscan_stats(V, R, s_part);
             __syncthreads();
             continue;
         }
@@ -775,56 +758,6 @@ __global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __r
     }
 }
 
-// One-view batches (counter mode), per view: the bucket bases (exclusive
-// scan of the bucket totals), the DrawIndexedIndirect args (the bucket's mesh
-// LOD range, instance_count = total, first_instance = base), the view's stats
-// from its rows, and the clear of the counter set of the next launch (unused
-// by this one). Every row's scatter then starts from the bases instead of
-// scanning the totals. (Batches of several views keep the scan in the
-// scatter: a fourth kernel in their chain measured slower with two batches
-// in flight, its blocks holding CUs the other batch's classify needs.)
-__global__ __launch_bounds__(kBaseThreads) void k_bases(const BatchHdr* __restrict__ hdr,
-                                                        const CullView* __restrict__ views) {
-    constexpr uint32_t T = kBaseThreads;
-    __shared__ uint32_t s_w[T / 64];
-    __shared__ uint32_t s_part[T];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t n_views = ldc(&hdr->n_views);
-    for (uint32_t v = blockIdx.x; v < n_views; v += gridDim.x) {
-        const CullView* __restrict__ V = views + v;
-        const uint32_t nb = ldc(&V->nb);
-        const uint32_t* __restrict__ totals = ldc(&V->totals);
-        uint32_t* __restrict__ bases = ldc(&V->bases);
-        const hv_mesh_lod* __restrict__ mesh_lods = ldc(&V->mesh_lods);
-        hv_draw_indexed_indirect* __restrict__ args = ldc(&V->args);
-        // Thread t scans buckets [t per, (t + 1) per) (per = 1 up to 1024 buckets).
-        const uint32_t per = (nb + T - 1u) / T;
-        const uint32_t beg = min(nb, tid * per), end = min(nb, beg + per);
-        uint32_t local = 0;
-        for (uint32_t b = beg; b < end; ++b) local += totals[b];
-        uint32_t sum;
-        uint32_t run = block_excl_scan<T>(local, s_w, sum);
-        for (uint32_t b = beg; b < end; ++b) {
-            const uint32_t t = totals[b];
-            const hv_mesh_lod ml = mesh_lods[b];
-            hv_draw_indexed_indirect a;
-            a.index_count = ml.index_count;
-            a.instance_count = t;
-            a.first_index = ml.first_index;
-            a.base_vertex = ml.base_vertex;
-            a.first_instance = run;
-            args[b] = a;
-            bases[b] = run;
-            run += t;
-        }
-        uint32_t* __restrict__ bnext = ldc(&V->bucket_cnt_next);
-        if (bnext)
-            for (uint32_t b = tid; b < nb; b += T) bnext[b] = 0u;
-        view_stats<T>(V, ldc(&V->n_rows), s_part);
-        __syncthreads();  // s_w and s_part are reused by the next view
-    }
-}
-
 // Scatter: block b revisits the items it classified, row by row. NT threads
 // (the classify block size when that is 512, else 256: one scatter block per
 // classify block). A sub-chunk is one classify wave's survivors of one item
@@ -843,7 +776,7 @@ __device__ __forceinline__ void scatter_load(const uint32_t* __restrict__ stage,
     }
 }
 
-template <int NT, int IPT, bool BASES>
+template <int NT, int IPT>
 __global__ __launch_bounds__(NT) void k_scatter(const BatchHdr* __restrict__ hdr, const CullView* __restrict__ views) {
     constexpr uint32_t KR = 1024u / NT;  // offsets held in registers: the first 1024 buckets
     constexpr uint32_t NWS = NT / 64;     // scatter waves
@@ -875,46 +808,40 @@ __global__ __launch_bounds__(NT) void k_scatter(const BatchHdr* __restrict__ hdr
         uint32_t cA[KB], eA[KB][IPT], cB[KB], eB[KB][IPT];
 #pragma unroll
         for (uint32_t k = 0; k < KB; ++k) scatter_load<NT, IPT>(stage, sub_cnt, gs0 + wave + k * NWS, gs1, sub, cA[k], eA[k]);
-        if constexpr (BASES) {
-            // Cursors: the bucket's base (k_bases) + this row's offset in it.
-            const uint32_t* __restrict__ bases = ldc(&V->bases);
-            for (uint32_t bb = tid; bb < nb; bb += NT) s_cur[bb] = bases[bb] + offs[bb];
-        } else {
-            // Bucket totals and this row's offsets (the first 1024 buckets'
-            // offsets in registers; larger tables reread theirs below).
-            uint32_t offr[KR];
+        // Bucket totals and this row's offsets (the first 1024 buckets' offsets
+        // in registers; larger tables reread theirs below).
+        uint32_t offr[KR];
 #pragma unroll
-            for (uint32_t k = 0; k < KR; ++k) {
-                const uint32_t bb = tid + NT * k;
-                offr[k] = bb < nb ? offs[bb] : 0u;
-                if (bb < nb) s_cur[bb] = totals[bb];
-            }
-            for (uint32_t bb = tid + NT * KR; bb < nb; bb += NT) s_cur[bb] = totals[bb];
-            __syncthreads();
-            block_excl_scan_lds<NT>(s_cur, nb, s_w);
-            // Cursors: bucket base (scan of the totals) + this row's offsets;
-            // row 0 writes the args.
-            const hv_mesh_lod* __restrict__ mesh_lods = ldc(&V->mesh_lods);
-            hv_draw_indexed_indirect* __restrict__ args = ldc(&V->args);
-            auto cursor = [&](uint32_t bb, uint32_t off) {
-                const uint32_t base = s_cur[bb];
-                if (r == 0u) {
-                    const hv_mesh_lod ml = mesh_lods[bb];
-                    hv_draw_indexed_indirect a;
-                    a.index_count = ml.index_count;
-                    a.instance_count = totals[bb];
-                    a.first_index = ml.first_index;
-                    a.base_vertex = ml.base_vertex;
-                    a.first_instance = base;
-                    args[bb] = a;
-                }
-                s_cur[bb] = base + off;
-            };
-#pragma unroll
-            for (uint32_t k = 0; k < KR; ++k)
-                if (tid + NT * k < nb) cursor(tid + NT * k, offr[k]);
-            for (uint32_t bb = tid + NT * KR; bb < nb; bb += NT) cursor(bb, offs[bb]);
+        for (uint32_t k = 0; k < KR; ++k) {
+            const uint32_t bb = tid + NT * k;
+            offr[k] = bb < nb ? offs[bb] : 0u;
+            if (bb < nb) s_cur[bb] = totals[bb];
         }
+        for (uint32_t bb = tid + NT * KR; bb < nb; bb += NT) s_cur[bb] = totals[bb];
+        __syncthreads();
+        block_excl_scan_lds<NT>(s_cur, nb, s_w);
+        // Cursors: bucket base (scan of the totals) + this row's offsets; row 0
+        // writes the args.
+        const hv_mesh_lod* __restrict__ mesh_lods = ldc(&V->mesh_lods);
+        hv_draw_indexed_indirect* __restrict__ args = ldc(&V->args);
+        auto cursor = [&](uint32_t bb, uint32_t off) {
+            const uint32_t base = s_cur[bb];
+            if (r == 0u) {
+                const hv_mesh_lod ml = mesh_lods[bb];
+                hv_draw_indexed_indirect a;
+                a.index_count = ml.index_count;
+                a.instance_count = totals[bb];
+                a.first_index = ml.first_index;
+                a.base_vertex = ml.base_vertex;
+                a.first_instance = base;
+                args[bb] = a;
+            }
+            s_cur[bb] = base + off;
+        };
+#pragma unroll
+        for (uint32_t k = 0; k < KR; ++k)
+            if (tid + NT * k < nb) cursor(tid + NT * k, offr[k]);
+        for (uint32_t bb = tid + NT * KR; bb < nb; bb += NT) cursor(bb, offs[bb]);
         __syncthreads();
         // Place the ids, batch by batch; the next batch's loads are in flight
         // while the current one is placed (two register sets, no copies).
@@ -983,22 +910,19 @@ ClassifyFn pick_ipt(uint32_t ipt, bool occl, bool aabb) {
 using ScatterFn = void (*)(const BatchHdr*, const CullView*);
 
 template <int NT>
-ScatterFn scatter_ipt(uint32_t ipt, bool bases) {
+ScatterFn scatter_ipt(uint32_t ipt) {
     switch (ipt) {
-        case 1: return bases ? dev::k_scatter<NT, 1, true> : dev::k_scatter<NT, 1, false>;
-        case 2: return bases ? dev::k_scatter<NT, 2, true> : dev::k_scatter<NT, 2, false>;
-        case 4: return bases ? dev::k_scatter<NT, 4, true> : dev::k_scatter<NT, 4, false>;
-        default: return bases ? dev::k_scatter<NT, 8, true> : dev::k_scatter<NT, 8, false>;
+        case 1: return dev::k_scatter<NT, 1>;
+        case 2: return dev::k_scatter<NT, 2>;
+        case 4: return dev::k_scatter<NT, 4>;
+        default: return dev::k_scatter<NT, 8>;
     }
 }
 
 // Scatter variant for a classify shape: 512 threads beside a 512-wide
 // classify (one scatter block per classify block), else 256; IPT entries per
 // lane per sub-chunk.
-// `bases`: the one-view variant (cursors from k_bases).
-ScatterFn scatter_fn(const CullShape& s, bool bases = false) {
-    return s.wg >= 512 ? scatter_ipt<512>(s.ipt, bases) : scatter_ipt<256>(s.ipt, bases);
-}
+ScatterFn scatter_fn(const CullShape& s) { return s.wg >= 512 ? scatter_ipt<512>(s.ipt) : scatter_ipt<256>(s.ipt); }
 uint32_t scatter_threads(const CullShape& s) { return s.wg >= 512 ? 512u : 256u; }
 
 ClassifyFn classify_fn(const CullShape& s, bool occl, bool aabb) {
@@ -1117,22 +1041,20 @@ CullBatch::~CullBatch() {
     (void)hipSetDevice(ctx_->device);
     (void)hipStreamSynchronize(ctx_->stream);
     (void)hipDeviceSynchronize();
-    for (int p = 0; p < 3; ++p) {
-        if (graph_exec_[p]) (void)hipGraphExecDestroy(graph_exec_[p]);
-        if (graph_[p]) (void)hipGraphDestroy(graph_[p]);
-    }
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
     for (auto& e : ev_)
         if (e) (void)hipEventDestroy(e);
     if (done_) (void)hipEventDestroy(done_);
     for (DevBuf* b : {&d_table_, &d_stage_, &d_ids_, &d_args_, &d_counters_, &d_sub_cnt_, &d_row_stats_, &d_block_hist_,
-                      &d_totals_, &d_bases_, &d_bcnt_})
+                      &d_totals_})
         b->release();
 }
 
 uint64_t CullBatch::device_bytes() const {
     uint64_t t = 0;
     for (const DevBuf* b : {&d_table_, &d_stage_, &d_ids_, &d_args_, &d_counters_, &d_sub_cnt_, &d_row_stats_,
-                            &d_block_hist_, &d_totals_, &d_bases_, &d_bcnt_})
+                            &d_block_hist_, &d_totals_})
         t += b->bytes;
     return t;
 }
@@ -1142,7 +1064,8 @@ uint32_t CullBatch::max_grid(const hv_ctx* ctx) { return std::max(1u, ctx->cu_co
 namespace {
 
 // Upper bound of column-scan units for `views` views of at most `buckets`
-// buckets (slices are at least 2^kColscanMinLg buckets wide).
+// buckets (slices are at least 2^kColscanMinLg buckets wide; one stats unit
+// per view).
 uint64_t unit_bound(uint64_t views, uint64_t buckets, uint64_t /*rows*/) {
     return views * ((buckets >> kColscanMinLg) + 2);
 }
@@ -1184,12 +1107,10 @@ hv_status CullBatch::reserve(uint32_t views, uint64_t candidates, uint32_t bucke
     HV_TRY(d_ids_.reserve(std::max<uint64_t>(candidates, 1) * 4, false, st));
     HV_TRY(d_args_.reserve(nv * buckets * sizeof(hv_draw_indexed_indirect), false, st));
     HV_TRY(d_totals_.reserve(nv * buckets * 4, false, st));
-    HV_TRY(d_bases_.reserve(nv * buckets * 4, false, st));
     HV_TRY(d_row_stats_.reserve(rows * 32, false, st));
     HV_TRY(d_block_hist_.reserve(std::max<uint64_t>(rows * buckets, 1) * 4, false, st));
     HV_TRY(d_counters_.reserve(nv * kCounterStride * 4, false, st));
-    HV_TRY(d_bcnt_.reserve((uint64_t)2 * buckets * 4, false, st));
-    HV_TRY(d_table_.reserve(sizeof(BatchHdr) + (nv + 1) * sizeof(CullView) + unit_bound(nv, buckets, rows) * 4 +
+    HV_TRY(d_table_.reserve(sizeof(BatchHdr) + nv * sizeof(CullView) + unit_bound(nv, buckets, rows) * 4 +
                                 (uint64_t)max_grid(ctx_) * 4,
                             false, st));
     lds_cap_ = std::max(lds_cap_, buckets * 4u);
@@ -1263,8 +1184,8 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
         item += cv.n_items;
         rows_total += cv.n_rows;
         hist_total += (uint64_t)cv.n_rows * nb;
-        cv.col_lg = col_lg(cv.n_rows);
         host_units_.push_back(v | (kStatsSlice << 10));
+        cv.col_lg = col_lg(cv.n_rows);
         for (uint32_t si = 0; (si << cv.col_lg) < nb; ++si) host_units_.push_back(v | (si << 10));
     }
     const hipStream_t st = ctx_->stream;
@@ -1273,7 +1194,6 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     HV_TRY(d_ids_.reserve(std::max<uint64_t>(ids_total, 1) * 4, false, st));
     HV_TRY(d_args_.reserve(std::max<uint64_t>(args_total, 1) * sizeof(hv_draw_indexed_indirect), false, st));
     HV_TRY(d_totals_.reserve(std::max<uint64_t>(args_total, 1) * 4, false, st));
-    HV_TRY(d_bases_.reserve(std::max<uint64_t>(args_total, 1) * 4, false, st));
     HV_TRY(d_row_stats_.reserve(std::max<uint64_t>(rows_total, 1) * 32, false, st));
     HV_TRY(d_block_hist_.reserve(std::max<uint64_t>(hist_total, 1) * 4, false, st));
     HV_TRY(d_counters_.reserve((uint64_t)std::max(n, 1u) * kCounterStride * 4, false, st));
@@ -1284,13 +1204,7 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
         while (v + 1 < n && it >= host_views_[v].item_begin + host_views_[v].n_items) ++v;
         host_block_view_[b] = v;
     }
-    // A single-view batch with candidates skips the column scan: its rows
-    // count with device atomics into one of two counter sets, and the table
-    // holds one view record per set (the launch of parity p reads record p).
-    atomic_ = n == 1 && ids_total > 0;
-    if (atomic_) HV_TRY(d_bcnt_.reserve((uint64_t)2 * max_nb * 4, false, st));
-    const uint32_t n_rec = atomic_ ? 2u : std::max(n, 1u);
-    const uint64_t units_off = sizeof(BatchHdr) + (uint64_t)n_rec * sizeof(CullView);
+    const uint64_t units_off = sizeof(BatchHdr) + (uint64_t)std::max(n, 1u) * sizeof(CullView);
     const uint64_t bview_off = units_off + std::max<uint64_t>(host_units_.size(), 1) * 4;
     const uint64_t table_bytes = bview_off + (uint64_t)host_block_view_.size() * 4;
     HV_TRY(d_table_.reserve(table_bytes, false, st));
@@ -1303,7 +1217,6 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
         cv.ids = d_ids_.as<uint32_t>() + so;
         cv.args = d_args_.as<hv_draw_indexed_indirect>() + ao;
         cv.totals = d_totals_.as<uint32_t>() + ao;
-        cv.bases = d_bases_.as<uint32_t>() + ao;
         cv.counters = d_counters_.as<uint32_t>() + (uint64_t)v * kCounterStride;
         cv.row_stats = d_row_stats_.as<uint32_t>() + ro * 8;
         cv.block_hist = d_block_hist_.as<uint32_t>() + ho;
@@ -1342,19 +1255,6 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     hdr.block_view = reinterpret_cast<const uint32_t*>(d_table_.as<uint8_t>() + bview_off);
     std::memcpy(buf.data(), &hdr, sizeof hdr);
     if (n) std::memcpy(buf.data() + sizeof hdr, host_views_.data(), (size_t)n * sizeof(CullView));
-    if (atomic_) {
-        for (uint32_t p = 0; p < 2; ++p) {
-            CullView cv = host_views_[0];
-            cv.bucket_cnt = d_bcnt_.as<uint32_t>() + (uint64_t)p * max_nb;
-            cv.bucket_cnt_next = d_bcnt_.as<uint32_t>() + (uint64_t)(p ^ 1u) * max_nb;
-            cv.totals = cv.bucket_cnt;
-            std::memcpy(buf.data() + sizeof hdr + p * sizeof(CullView), &cv, sizeof cv);
-        }
-        // The first launch (parity 0) counts from zero; every scatter clears
-        // the other set for the launch after it.
-        parity_ = 0;
-        HV_HIP(hipMemsetAsync(d_bcnt_.ptr, 0, (size_t)max_nb * 4, st));
-    }
     if (n_units_) std::memcpy(buf.data() + units_off, host_units_.data(), (size_t)n_units_ * 4);
     std::memcpy(buf.data() + bview_off, host_block_view_.data(), host_block_view_.size() * 4);
     HV_HIP(hipMemcpyAsync(d_table_.ptr, buf.data(), table_bytes, hipMemcpyHostToDevice, st));
@@ -1362,14 +1262,12 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     return HV_OK;
 }
 
-hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed, uint32_t parity) {
+hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed) {
     const bool occl = (cfg_.flags & HV_CULL_OCCLUSION) != 0;
     const bool aabb_any = std::any_of(host_views_.begin(), host_views_.end(), [](const CullView& c) { return c.aabb; });
     ClassifyFn fn = classify_fn(shape, occl, aabb_any);
     const BatchHdr* hdr = d_table_.as<BatchHdr>();
-    // A single-view batch reads the view record of this launch's counter set.
-    const CullView* views = reinterpret_cast<const CullView*>(d_table_.as<uint8_t>() + sizeof(BatchHdr)) +
-                            (atomic_ ? parity : 0u);
+    const CullView* views = reinterpret_cast<const CullView*>(d_table_.as<uint8_t>() + sizeof(BatchHdr));
     // Fixed (capacity) grids make one captured graph valid for any later table:
     // blocks beyond the table's G (or past the last unit) exit at once.
     const size_t lds = fixed ? (size_t)lds_cap_ : (size_t)max_nb_ * 4;
@@ -1383,14 +1281,10 @@ hv_status CullBatch::launch(hipStream_t s, float* ms, bool fixed, uint32_t parit
     hipLaunchKernelGGL(fn, dim3(g1), dim3(shape.wg), lds, s, hdr, views);
     HV_HIP(hipGetLastError());
     if (ms) HV_HIP(hipEventRecord(ev_[1], s));
-    // Bucket offsets: the column scan, or (one view) the bucket-base block.
-    if (atomic_)
-        hipLaunchKernelGGL(dev::k_bases, dim3(1), dim3(kBaseThreads), 0, s, hdr, views);
-    else
-        hipLaunchKernelGGL(dev::k_colscan, dim3(g2), dim3(kColscanThreads), 0, s, hdr, views);
+    hipLaunchKernelGGL(dev::k_colscan, dim3(g2), dim3(kColscanThreads), 0, s, hdr, views);
     HV_HIP(hipGetLastError());
     if (ms) HV_HIP(hipEventRecord(ev_[2], s));
-    hipLaunchKernelGGL(scatter_fn(shape, atomic_), dim3(g3), dim3(scatter_threads(shape)), lds, s, hdr, views);
+    hipLaunchKernelGGL(scatter_fn(shape), dim3(g3), dim3(scatter_threads(shape)), lds, s, hdr, views);
     HV_HIP(hipGetLastError());
     if (!fixed) {  // eager: record completion now (a replay records it lazily)
         HV_HIP(hipEventRecord(done_, s));
@@ -1436,9 +1330,7 @@ hv_status CullBatch::execute(hipStream_t s, float* ms) {
     HV_HIP(hipSetDevice(ctx_->device));
     if (ms && !ev_[0])
         for (auto& e : ev_) HV_HIP(hipEventCreate(&e));
-    HV_TRY(launch(s, ms, false, parity_));
-    if (atomic_) parity_ ^= 1u;
-    return HV_OK;
+    return launch(s, ms, false);
 }
 
 hv_status CullBatch::execute_graph(hipStream_t s) {
@@ -1450,22 +1342,18 @@ hv_status CullBatch::execute_graph(hipStream_t s) {
     const bool aabb_any = std::any_of(host_views_.begin(), host_views_.end(), [](const CullView& c) { return c.aabb; });
     // The table's address is a kernel argument of every captured launch: a
     // prepare that had to grow the table moved it, and the graph is re-captured.
-    // A single-view batch alternates two graphs (one per counter set).
-    const uint32_t p = atomic_ ? parity_ : 0u, gi = atomic_ ? 1u + parity_ : 0u;
-    const uint64_t key[8] = {resident1_, lds_cap_, aabb_any ? 1u : 0u, shape.wg * 16u + shape.ipt, ctx_->cu_count,
-                             resident3_, reinterpret_cast<uint64_t>(d_table_.ptr), atomic_ ? 1u : 0u};
-    hipGraph_t& graph = graph_[gi];
-    hipGraphExec_t& graph_exec = graph_exec_[gi];
-    if (!graph_exec || std::memcmp(key, graph_key_[gi], sizeof key) != 0) {
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        graph_exec = nullptr;
-        graph = nullptr;
+    const uint64_t key[7] = {resident1_, lds_cap_, aabb_any ? 1u : 0u, shape.wg * 16u + shape.ipt, ctx_->cu_count,
+                             resident3_, reinterpret_cast<uint64_t>(d_table_.ptr)};
+    if (!graph_exec_ || std::memcmp(key, graph_key_, sizeof key) != 0) {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        graph_exec_ = nullptr;
+        graph_ = nullptr;
         hipStream_t cap;
         HV_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
         hv_status st = HV_OK;
-        if (e == hipSuccess) st = launch(cap, nullptr, true, p);
+        if (e == hipSuccess) st = launch(cap, nullptr, true);
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(cap, &g);
         (void)hipStreamDestroy(cap);
@@ -1475,15 +1363,14 @@ hv_status CullBatch::execute_graph(hipStream_t s) {
             return st;
         }
         if (e2 != hipSuccess) return hip_error(e2, "hipStreamEndCapture", __FILE__, __LINE__);
-        graph = g;
-        HV_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
-        std::memcpy(graph_key_[gi], key, sizeof key);
+        graph_ = g;
+        HV_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+        std::memcpy(graph_key_, key, sizeof key);
     }
     // No completion event per replay (it costs a queue packet each time): the
     // stream is remembered and sync_last() records the event on it when a
     // later call needs the launch finished.
-    HV_HIP(hipGraphLaunch(graph_exec, s));
-    if (atomic_) parity_ ^= 1u;
+    HV_HIP(hipGraphLaunch(graph_exec_, s));
     pending_ = true;
     pending_stream_ = s;
     return HV_OK;

@@ -530,9 +530,7 @@ hv_status hv_batch_prepare(hv_batch* batch, const hv_view_request* views, uint32
  * completion on `stream`, which must stay valid until then. */
 hv_status hv_batch_execute(hv_batch* batch, void* stream, uint32_t use_graph);
 /* Eager execute with HIP events around the three kernels; synchronises and
- * returns ms[0] classify, ms[1] bucket offsets and bases (the column scan and
- * the per-view bucket-base kernel; a batch of one view has no column scan, its
- * rows take their offsets from device counters), ms[2] scatter. */
+ * returns ms[0] classify, ms[1] column scan (bucket offsets), ms[2] scatter. */
 hv_status hv_batch_execute_timed(hv_batch* batch, void* stream, float ms[3]);
 hv_status hv_batch_outputs(const hv_batch* batch, uint32_t view, hv_cull_outputs* out);
 hv_status hv_batch_read(hv_batch* batch, uint32_t view, uint32_t* ids, uint32_t ids_capacity, uint32_t* id_count,
