@@ -35,7 +35,9 @@ struct VgArgs {
     uint32_t meshlet_count, object_count_buf, instance_count, icd_count, work_item_buf_count;
     uint32_t capacity, counter_len;
     uint32_t object_count, work_item_count;
-    uint32_t select_blocks;    // leading workgroups that select objects
+    uint32_t select_blocks;    // workgroups that select objects
+    uint32_t select_last;      // 1: the select workgroups follow the cull workgroups (the whole grid is
+                               // resident at once); 0: they lead
     uint32_t spans_per_block;  // work items per cull workgroup (8..64, a power of two)
     uint32_t write_selection;  // select workgroups store selected_lod_plus_one (the pass: 1;
                                // concurrent requests on one scene: 0, the cull recomputes it)

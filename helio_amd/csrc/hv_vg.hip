@@ -526,21 +526,28 @@ __device__ __forceinline__ void vg_cull_block(const VgArgs& a, uint32_t block, V
     }
 }
 
-// Select and cull in one launch: blocks [0, select_blocks) select objects,
-// the rest cull work items. The cull recomputes each object's LOD with the
-// same select_lod, so the two halves do not depend on each other.
+// Select and cull in one launch: select_blocks workgroups select objects,
+// ceil(work_item_count / spans_per_block) cull work items. The cull
+// recomputes each object's LOD with the same select_lod, so the two halves do
+// not depend on each other. When the whole grid is resident at once the
+// select workgroups come last: the part of a CU holding one more workgroup
+// than the rest then holds select workgroups, not cull workgroups (which set
+// the launch's length); otherwise they come first, so that they finish and
+// leave their slots to the last cull workgroups early.
 __device__ __forceinline__ void vg_block(const VgArgs& a, uint32_t block, VgLds& L) {
-    const bool cull = block >= a.select_blocks;
+    const uint32_t n_cull = a.work_item_count / a.spans_per_block + (a.work_item_count % a.spans_per_block != 0u);
+    const uint32_t cull_begin = a.select_last ? 0u : a.select_blocks;
+    const bool cull = block >= cull_begin && block - cull_begin < n_cull;
     // A cull workgroup's work items do not depend on the view: their loads
     // are in flight beside the view copy.
     bool has_item = false;
     hv_vg_work_item item{0u, 0u};
-    if (cull && threadIdx.x < 64u) item = vg_work_item(a, block - a.select_blocks, has_item);
+    if (cull && threadIdx.x < 64u) item = vg_work_item(a, block - cull_begin, has_item);
     if (threadIdx.x < kVgViewWords)
         reinterpret_cast<uint32_t*>(&L.v)[threadIdx.x] = reinterpret_cast<const uint32_t*>(a.view)[threadIdx.x];
     __syncthreads();
-    if (!cull) vg_select_block(a, block, L);
-    else vg_cull_block(a, block - a.select_blocks, L, item, has_item);
+    if (!cull) vg_select_block(a, a.select_last ? block - n_cull : block, L);
+    else vg_cull_block(a, block - cull_begin, L, item, has_item);
 }
 
 __global__ __launch_bounds__(kVgThreads) void k_vg(VgArgs a) {
@@ -595,6 +602,7 @@ struct hv_vg_pass {
     bool prepared = false;
     const hv_hiz* hiz = nullptr;   // the pyramid of the last prepare and its storage_gen then
     uint64_t hiz_gen = 0;
+    uint32_t resident_blocks = 0;  // k_vg workgroups resident at once on the device (0: not queried yet)
     hipEvent_t ev[3] = {};
     hipEvent_t done = nullptr;
     bool done_recorded = false;
@@ -842,6 +850,12 @@ static hv_status vg_launch(hv_vg_pass* p, hipStream_t s, float* ms) {
         vg_grid(p->object_count, p->work_item_count, a.select_blocks, a.spans_per_block, grid);
         if (grid > 0x7fffffffull) return set_error(HV_ELIMIT, "hv_vg_execute: %llu workgroups exceed the grid limit",
                                                    (unsigned long long)grid);
+        if (!p->resident_blocks) {
+            int occ = 0;
+            HV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(k_vg), kVgThreads, 0));
+            p->resident_blocks = std::max(1u, p->ctx->cu_count) * (uint32_t)std::max(occ, 1);
+        }
+        a.select_last = grid <= p->resident_blocks ? 1u : 0u;
         hipLaunchKernelGGL(k_vg, dim3((uint32_t)grid), dim3(kVgThreads), 0, s, a);
         HV_HIP(hipGetLastError());
     }
