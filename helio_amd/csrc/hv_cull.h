@@ -62,7 +62,8 @@ struct alignas(16) CullView {
     uint32_t* counters;                // kCounterStride
     uint32_t* row_stats;               // n_rows x 8: [0] survivors, [1..7] stats 1..7
     uint32_t* block_hist;              // n_rows x nb: bucket counts, then offsets within the bucket
-    uint32_t* totals;                  // nb: bucket totals
+    uint32_t* totals;                  // 2 nb: bucket totals, then (when one column-scan unit holds
+                                       // every bucket of the view) the bucket bases
 };
 
 struct alignas(16) BatchHdr {

@@ -19,10 +19,14 @@
 //               the row's stats, once per row. No barrier inside a row.
 //   k_colscan   work units of (view, bucket slice): per bucket, the exclusive
 //               prefix of the counts over the view's rows, and the bucket
-//               total; one more unit per view sums the row stats.
+//               total; a unit whose slice holds every bucket of its view also
+//               writes the bucket bases (the scan of the totals); one more
+//               unit per view sums the row stats.
 //   k_scatter   the same block -> item ranges as classify: per row, cursors =
-//               scan(totals) + row offsets in LDS, args from row 0, then a flat
-//               loop over the row's entries places each id with an LDS atomic.
+//               bucket bases (read, or the scan of the totals in LDS when the
+//               column scan split the view's buckets) + row offsets, args from
+//               row 0, then a flat loop over the row's entries places each id
+//               with an LDS atomic.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -644,6 +648,10 @@ __device__ void block_excl_scan_lds(uint32_t* s, uint32_t n, uint32_t* s_w) {
     __syncthreads();
 }
 
+// True when a view's column-scan slice (2^lg buckets) covers all nb of its
+// buckets: its single unit then also writes the bucket bases and the args.
+__device__ __forceinline__ bool bases_here(uint32_t lg, uint32_t nb) { return (1u << lg) >= nb; }
+
 // Column scan. A unit is one slice of 2^lg buckets of one view (lg per view,
 // from 5 to 10): its 1024 threads form 1024 >> lg row groups, a thread holds
 // one bucket of one group's contiguous run of rows (at most kColscanRows in
@@ -681,6 +689,7 @@ __device__ __forceinline__ void colscan_stats(const CullView* __restrict__ V, ui
 __global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __restrict__ hdr,
                                                              const CullView* __restrict__ views) {
     __shared__ uint32_t s_part[kColscanThreads];
+    __shared__ uint32_t s_w[kColscanThreads / 64u];
     const uint32_t tid = threadIdx.x;
     const uint32_t n_units = ldc(&hdr->n_units);
     const uint32_t* __restrict__ units = ldc(&hdr->units);
@@ -740,8 +749,19 @@ __global__ __launch_bounds__(kColscanThreads) void k_colscan(const BatchHdr* __r
                 run += t;
             }
         }
+        // A unit that holds every bucket of the view also writes the bucket
+        // bases (the exclusive scan of the totals) after the totals, and the
+        // scatter takes them as they are. Thread order is bucket order within
+        // group 0 and the other groups add nothing, so a scan over the block
+        // is the scan over the buckets.
+        uint32_t bbase = 0;
+        if (bases_here(lg, nb)) {
+            uint32_t all;
+            bbase = block_excl_scan<kColscanThreads>(live && grp == 0u ? total : 0u, s_w, all);
+        }
         if (live && grp == 0u) {
             V->totals[b] = total;
+            if (bases_here(lg, nb)) V->totals[nb + b] = bbase;
             // An empty view has no row 0 in the scatter: its args are written here.
             if (R == 0u) {
                 const hv_mesh_lod ml = V->mesh_lods[b];
@@ -808,6 +828,27 @@ __global__ __launch_bounds__(NT) void k_scatter(const BatchHdr* __restrict__ hdr
         uint32_t cA[KB], eA[KB][IPT], cB[KB], eB[KB][IPT];
 #pragma unroll
         for (uint32_t k = 0; k < KB; ++k) scatter_load<NT, IPT>(stage, sub_cnt, gs0 + wave + k * NWS, gs1, sub, cA[k], eA[k]);
+        const hv_mesh_lod* __restrict__ mesh_lods = ldc(&V->mesh_lods);
+        hv_draw_indexed_indirect* __restrict__ args = ldc(&V->args);
+        if (bases_here(ldc(&V->col_lg), nb)) {
+            // The column scan wrote the bucket bases after the totals:
+            // cursors = bases + this row's offsets; row 0 writes the args.
+            const uint32_t* __restrict__ bases = totals + nb;
+            for (uint32_t bb = tid; bb < nb; bb += NT) {
+                const uint32_t base = bases[bb];
+                if (r == 0u) {
+                    const hv_mesh_lod ml = mesh_lods[bb];
+                    hv_draw_indexed_indirect a;
+                    a.index_count = ml.index_count;
+                    a.instance_count = totals[bb];
+                    a.first_index = ml.first_index;
+                    a.base_vertex = ml.base_vertex;
+                    a.first_instance = base;
+                    args[bb] = a;
+                }
+                s_cur[bb] = base + offs[bb];
+            }
+        } else {
         // Bucket totals and this row's offsets (the first 1024 buckets' offsets
         // in registers; larger tables reread theirs below).
         uint32_t offr[KR];
@@ -822,8 +863,6 @@ __global__ __launch_bounds__(NT) void k_scatter(const BatchHdr* __restrict__ hdr
         block_excl_scan_lds<NT>(s_cur, nb, s_w);
         // Cursors: bucket base (scan of the totals) + this row's offsets; row 0
         // writes the args.
-        const hv_mesh_lod* __restrict__ mesh_lods = ldc(&V->mesh_lods);
-        hv_draw_indexed_indirect* __restrict__ args = ldc(&V->args);
         auto cursor = [&](uint32_t bb, uint32_t off) {
             const uint32_t base = s_cur[bb];
             if (r == 0u) {
@@ -842,6 +881,7 @@ __global__ __launch_bounds__(NT) void k_scatter(const BatchHdr* __restrict__ hdr
         for (uint32_t k = 0; k < KR; ++k)
             if (tid + NT * k < nb) cursor(tid + NT * k, offr[k]);
         for (uint32_t bb = tid + NT * KR; bb < nb; bb += NT) cursor(bb, offs[bb]);
+        }
         __syncthreads();
         // Place the ids, batch by batch; the next batch's loads are in flight
         // while the current one is placed (two register sets, no copies).
@@ -1090,7 +1130,7 @@ uint64_t CullBatch::bytes_for(uint64_t candidates, uint32_t views, uint32_t buck
     const uint64_t items = candidates / chunk + views;
     const uint64_t rows = std::min(items, max_rows + views);
     return items * chunk * 4 + items * 8 * 4 + candidates * 4 + rows * (buckets + 8ull) * 4 +
-           (uint64_t)views * (buckets * 24ull + kCounterStride * 4 + sizeof(CullView)) +
+           (uint64_t)views * (buckets * 28ull + kCounterStride * 4 + sizeof(CullView)) +
            unit_bound(views, buckets, rows) * 4 + max_rows * 4;
 }
 
@@ -1106,7 +1146,7 @@ hv_status CullBatch::reserve(uint32_t views, uint64_t candidates, uint32_t bucke
     HV_TRY(d_sub_cnt_.reserve(items * nw * 4, false, st));
     HV_TRY(d_ids_.reserve(std::max<uint64_t>(candidates, 1) * 4, false, st));
     HV_TRY(d_args_.reserve(nv * buckets * sizeof(hv_draw_indexed_indirect), false, st));
-    HV_TRY(d_totals_.reserve(nv * buckets * 4, false, st));
+    HV_TRY(d_totals_.reserve(nv * buckets * 8, false, st));  // totals, then bases
     HV_TRY(d_row_stats_.reserve(rows * 32, false, st));
     HV_TRY(d_block_hist_.reserve(std::max<uint64_t>(rows * buckets, 1) * 4, false, st));
     HV_TRY(d_counters_.reserve(nv * kCounterStride * 4, false, st));
@@ -1193,7 +1233,7 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     HV_TRY(d_sub_cnt_.reserve(std::max<uint64_t>(items_total, 1) * nw * 4, false, st));
     HV_TRY(d_ids_.reserve(std::max<uint64_t>(ids_total, 1) * 4, false, st));
     HV_TRY(d_args_.reserve(std::max<uint64_t>(args_total, 1) * sizeof(hv_draw_indexed_indirect), false, st));
-    HV_TRY(d_totals_.reserve(std::max<uint64_t>(args_total, 1) * 4, false, st));
+    HV_TRY(d_totals_.reserve(std::max<uint64_t>(args_total, 1) * 8, false, st));  // totals, then bases
     HV_TRY(d_row_stats_.reserve(std::max<uint64_t>(rows_total, 1) * 32, false, st));
     HV_TRY(d_block_hist_.reserve(std::max<uint64_t>(hist_total, 1) * 4, false, st));
     HV_TRY(d_counters_.reserve((uint64_t)std::max(n, 1u) * kCounterStride * 4, false, st));
@@ -1216,7 +1256,7 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
         HV_TRY(fill_view(cfg_, d, cv));
         cv.ids = d_ids_.as<uint32_t>() + so;
         cv.args = d_args_.as<hv_draw_indexed_indirect>() + ao;
-        cv.totals = d_totals_.as<uint32_t>() + ao;
+        cv.totals = d_totals_.as<uint32_t>() + 2 * ao;
         cv.counters = d_counters_.as<uint32_t>() + (uint64_t)v * kCounterStride;
         cv.row_stats = d_row_stats_.as<uint32_t>() + ro * 8;
         cv.block_hist = d_block_hist_.as<uint32_t>() + ho;
@@ -1585,7 +1625,7 @@ hv_status hv_inst_cull_noalloc(hv_ctx* ctx, const hv_cull_config* config, const 
     const uint64_t rstat_off = align256(sub_off + (uint64_t)N * nw * 4u);
     const uint64_t hist_off = align256(rstat_off + (uint64_t)R * 32u);
     const uint64_t tot_off = align256(hist_off + (uint64_t)R * nb * 4u);
-    const uint64_t need = tot_off + (uint64_t)nb * 4u;
+    const uint64_t need = tot_off + (uint64_t)nb * 8u;  // totals, then bases
     if (need > HV_NOALLOC_SCRATCH_BYTES)
         return set_error(HV_ELIMIT,
                          "hv_inst_cull_noalloc: %u candidates x %u buckets need %llu scratch bytes, more than the %llu "
