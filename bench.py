@@ -549,13 +549,17 @@ def main() -> int:
         log(f"launch shape wg={wg} ipt={ipt} (fixed)")
     else:
         # The autotuner times the 16 classify shapes on this batch (median of
-        # the sum of the three kernels), or takes the shape cached for this
-        # device and workload, and leaves the context on it; every later pass
-        # of this run uses it.
-        tuned, batch, cached = autotune.cull_shape(ctx, f"multiview64:{sc.n}:{tests_per_step}", prepare_batch,
-                                                   reps=5, retune=args.retune)
-        log(f"launch shape wg={tuned['wg']} ipt={tuned['ipt']} ({tuned['total_ms'] * 1e3:.1f} us kernels"
-            f"{', cached' if cached else ''})")
+        # the sum of the three kernels), then the three fastest of them as the
+        # steady state runs them (IN_FLIGHT copies on their own streams), or
+        # takes the shape cached for this device and workload, and leaves the
+        # context on it; every later pass of this run uses it.
+        tuned, batch, cached = autotune.cull_shape_in_flight(ctx, f"multiview64:{sc.n}:{tests_per_step}",
+                                                             prepare_batch, in_flight=IN_FLIGHT, reps=5,
+                                                             retune=args.retune)
+        log(f"launch shape wg={tuned['wg']} ipt={tuned['ipt']} ({tuned['total_ms'] * 1e3:.1f} us kernels, "
+            f"{tuned['step_ms'] * 1e3:.1f} us per step with {IN_FLIGHT} in flight"
+            f"{', cached' if cached else ''}; candidates "
+            + ", ".join(f"{c['wg']}x{c['ipt']} {c['step_ms'] * 1e3:.1f}" for c in tuned["candidates"]) + ")")
     # Sequential: one batch, one graph replay per step on one stream.
     for _ in range(args.warmup):
         batch.execute(graph=True)

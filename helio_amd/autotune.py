@@ -13,7 +13,7 @@ import time
 from pathlib import Path
 
 from ._lib import TuneResult, call
-from .passes import BatchCull, Context, HiZBuildPass
+from .passes import BatchCull, Context, HiZBuildPass, Stream
 
 CACHE = Path(__file__).resolve().parent / "tuned" / "shapes.json"
 
@@ -95,6 +95,60 @@ def cull_shape(ctx: Context, workload: str, prepare, reps: int = 5, path: Path =
     res = tune_batch(batch, reps, replay)
     store(cu, workload, dict(res["best"], table=res["table"]), path)
     return res["best"], batch, False
+
+
+def steady_step_ms(ctx: Context, prepare, in_flight: int, steps: int, warmup: int = 4) -> float:
+    """Milliseconds per step with `in_flight` prepared copies of the batch
+    replaying their graphs on their own streams, steps taking turns (one
+    step's column scan and scatter beside the next steps' classify)."""
+    copies = [prepare(ctx) for _ in range(in_flight)]
+    streams = [Stream(ctx) for _ in copies]
+    try:
+        for i in range(max(warmup, in_flight)):
+            copies[i % in_flight].execute(stream=streams[i % in_flight].handle, graph=True)
+        for b in copies:
+            b.wait()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            copies[i % in_flight].execute(stream=streams[i % in_flight].handle, graph=True)
+        for b in copies:
+            b.wait()
+        return (time.perf_counter() - t0) / steps * 1e3
+    finally:
+        for b in copies:
+            b.close()
+        for st in streams:
+            st.close()
+
+
+def cull_shape_in_flight(ctx: Context, workload: str, prepare, in_flight: int = 2, candidates: int = 3,
+                         steps: int = 60, reps: int = 5, path: Path = CACHE,
+                         retune: bool = False) -> tuple[dict, "BatchCull", bool]:
+    """The classify launch shape for batches replayed `in_flight` at a time
+    (steady_step_ms): the `candidates` shapes with the smallest eager
+    three-kernel sum (cull_shape's table) are each timed in that mode and the
+    fastest is applied, cached under its own key. Returns (shape, batch,
+    from_cache); the entry holds every candidate's step time."""
+    cu, _ = ctx.device_info()
+    wl = f"{workload}:in_flight{in_flight}"
+    hit = None if retune else lookup(cu, wl, path)
+    if hit is not None:
+        apply(ctx, dict(get_shapes(ctx), wg=hit["wg"], ipt=hit["ipt"]))
+        return hit, prepare(ctx), True
+    eager, batch, _ = cull_shape(ctx, workload, prepare, reps, path, retune)
+    batch.close()
+    table = (lookup(cu, workload, path) or {}).get("table") or [eager]
+    cands = sorted(table, key=lambda r: r["total_ms"])[:max(1, candidates)]
+    timed = []
+    for c in cands:
+        apply(ctx, dict(get_shapes(ctx), wg=c["wg"], ipt=c["ipt"]))
+        timed.append({"wg": c["wg"], "ipt": c["ipt"], "total_ms": c["total_ms"],
+                      "step_ms": steady_step_ms(ctx, prepare, in_flight, steps)})
+    best = min(timed, key=lambda r: r["step_ms"])
+    apply(ctx, dict(get_shapes(ctx), wg=best["wg"], ipt=best["ipt"]))
+    entry = dict(best, in_flight=in_flight, candidates=timed)
+    store(cu, wl, entry, path)
+    return entry, prepare(ctx), False
 
 
 def hiz_tile(ctx: Context, workload: str, hiz: HiZBuildPass, reps: int = 10, path: Path = CACHE,

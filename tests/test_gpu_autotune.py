@@ -74,6 +74,43 @@ def test_every_shape_and_the_tuned_one_match_the_oracle(ctx, tmp_path):
     scene.close()
 
 
+def test_in_flight_tuning_keeps_the_oracle_results(ctx, tmp_path):
+    """The steady-state tuner (the fastest eager shapes timed with two copies
+    in flight) leaves the context on its pick, caches it under its own key,
+    and a batch prepared with it culls as the oracle does."""
+    sc = synth.instance_scene(200_000, config=3, index=6)
+    scene = _scene(ctx, sc)
+    views = synth.multiview_views(sc.n, max_count=50_000)[:10]
+    saved = autotune.get_shapes(ctx)
+
+    def prepare(c):
+        bb = BatchCull(c, len(views))
+        bb.prepare([(scene, v.camera, f, c_) for v, f, c_ in views])
+        return bb
+
+    cache = tmp_path / "shapes.json"
+    best, b, cached = autotune.cull_shape_in_flight(ctx, "test-in-flight", prepare, in_flight=2, candidates=2,
+                                                    steps=8, reps=2, path=cache)
+    assert not cached and len(best["candidates"]) == 2
+    assert min(c["step_ms"] for c in best["candidates"]) == best["step_ms"] > 0
+    shapes = autotune.get_shapes(ctx)
+    assert (shapes["wg"], shapes["ipt"]) == (best["wg"], best["ipt"])
+    b.execute(graph=True)
+    _check(b, sc, views)
+    b.close()
+    cu = ctx.device_info()[0]
+    assert autotune.lookup(cu, "test-in-flight:in_flight2", cache)["wg"] == best["wg"]
+    assert len(autotune.lookup(cu, "test-in-flight", cache)["table"]) == 16
+    autotune.apply(ctx, dict(saved, wg=64, ipt=1))
+    hit, b, cached = autotune.cull_shape_in_flight(ctx, "test-in-flight", prepare, path=cache)
+    assert cached and (hit["wg"], hit["ipt"]) == (best["wg"], best["ipt"])
+    b.execute()
+    _check(b, sc, views)
+    b.close()
+    autotune.apply(ctx, saved)
+    scene.close()
+
+
 @pytest.mark.parametrize("mode", [L.HIZ_MIN, L.HIZ_MAX])
 def test_pyramid_tiles_and_the_tuned_tile_are_bit_exact(ctx, mode, tmp_path):
     saved = autotune.get_shapes(ctx)
