@@ -130,6 +130,7 @@ class CullBatch {
     hv_status ensure_occupancy(bool aabb);
     hv_ctx* ctx_;
     hv_cull_config cfg_;
+    float lod_d2_[7];  // the config's squared-distance LOD thresholds (computed once)
     uint32_t max_views_;
     uint32_t n_views_ = 0, n_items_ = 0, n_rows_ = 0, n_units_ = 0, max_nb_ = 0;
     uint32_t grid_ = 0;                       // G of the prepared table

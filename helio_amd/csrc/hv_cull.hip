@@ -1018,10 +1018,16 @@ hv_status check_view(const hv_cull_config& cfg, const ViewDesc& d, uint32_t v) {
     return HV_OK;
 }
 
+// The squared-distance LOD thresholds of a config (see CullView::lod_d2).
+void lod_thresholds(const hv_cull_config& cfg, float out[7]) {
+    for (uint32_t i = 0; i < 7; ++i)
+        out[i] = i + 1 < cfg.lod_count ? sqrt_threshold(cfg.lod_distances[i]) : std::nanf("");
+}
+
 // Everything of a view record that does not depend on the batch layout:
-// planes and their decision margins, camera terms, LOD thresholds, the
-// scene's arrays and the pyramid.
-hv_status fill_view(const hv_cull_config& cfg, const ViewDesc& d, CullView& cv) {
+// planes and their decision margins, camera terms, LOD thresholds (`lod_d2`,
+// from lod_thresholds of the same config), the scene's arrays and the pyramid.
+hv_status fill_view(const hv_cull_config& cfg, const ViewDesc& d, CullView& cv, const float* lod_d2) {
     HV_TRY(hv_extract_frustum_planes(d.camera->view_proj, cv.planes));
     for (int i = 0; i < 6; ++i)
         for (int a = 0; a < 3; ++a) cv.pv_sign[3 * i + a] = cv.planes[4 * i + a] >= 0.0f ? 1.0f : -1.0f;
@@ -1048,8 +1054,7 @@ hv_status fill_view(const hv_cull_config& cfg, const ViewDesc& d, CullView& cv) 
     std::memcpy(cv.cam, d.camera->position_near, 12);
     // Thresholds past lod_count - 1 are NaN: d2 >= NaN is false, so the
     // kernel may compare against all of them.
-    for (uint32_t i = 0; i < 7; ++i)
-        cv.lod_d2[i] = i + 1 < cfg.lod_count ? sqrt_threshold(cfg.lod_distances[i]) : std::nanf("");
+    std::memcpy(cv.lod_d2, lod_d2, sizeof cv.lod_d2);
     cv.lod_count = cfg.lod_count;
     cv.flags = cfg.flags;
     cv.nb = d.scene->mesh_count * d.scene->lod_count;
@@ -1075,7 +1080,9 @@ hv_status fill_view(const hv_cull_config& cfg, const ViewDesc& d, CullView& cv) 
 }  // namespace
 
 CullBatch::CullBatch(hv_ctx* ctx, const hv_cull_config& cfg, uint32_t max_views)
-    : shape(ctx->cull_shape), ctx_(ctx), cfg_(cfg), max_views_(max_views) {}
+    : shape(ctx->cull_shape), ctx_(ctx), cfg_(cfg), max_views_(max_views) {
+    lod_thresholds(cfg_, lod_d2_);
+}
 
 CullBatch::~CullBatch() {
     (void)hipSetDevice(ctx_->device);
@@ -1253,7 +1260,7 @@ hv_status CullBatch::prepare(const ViewDesc* views, uint32_t n) {
     for (uint32_t v = 0; v < n; ++v) {
         const ViewDesc& d = views[v];
         CullView& cv = host_views_[v];
-        HV_TRY(fill_view(cfg_, d, cv));
+        HV_TRY(fill_view(cfg_, d, cv, lod_d2_));
         cv.ids = d_ids_.as<uint32_t>() + so;
         cv.args = d_args_.as<hv_draw_indexed_indirect>() + ao;
         cv.totals = d_totals_.as<uint32_t>() + 2 * ao;
@@ -1632,7 +1639,9 @@ hv_status hv_inst_cull_noalloc(hv_ctx* ctx, const hv_cull_config* config, const 
                          "reserved",
                          count, nb, (unsigned long long)need, (unsigned long long)HV_NOALLOC_SCRATCH_BYTES);
     CullView cv{};
-    HV_TRY(fill_view(*config, d, cv));
+    float lod_d2[7];
+    lod_thresholds(*config, lod_d2);
+    HV_TRY(fill_view(*config, d, cv, lod_d2));
     cv.item_begin = 0;
     cv.n_items = N;
     cv.block_begin = 0;

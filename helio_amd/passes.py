@@ -7,7 +7,9 @@ reference; errors surface as :class:`helio_amd._lib.HvError`.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -15,6 +17,30 @@ from . import layouts as L
 from ._lib import (CullConfig, CullOutputs, DrawBuffers, RebuildInputs, RebuildOutputs, ServerConfig, ServerRequest,
                    ServerStats, ShadowCullBuffers,
                    ShadowMatrixBuffers, VgFrameData, VgOutputs, VgRequest, ViewRequest, call, load, ptr)
+
+# Every live library object, in creation order. At interpreter exit they are
+# closed newest first (servers, batches and passes before the scenes and
+# buffers they use, contexts last) before module teardown, which would
+# otherwise destroy them in an arbitrary order.
+_live: list = []
+
+
+def _track(obj) -> None:
+    _live.append(weakref.ref(obj))
+    if len(_live) >= 4096 and len(_live) & (len(_live) - 1) == 0:
+        _live[:] = [r for r in _live if r() is not None]
+
+
+@atexit.register
+def _close_live() -> None:
+    for r in reversed(_live):
+        o = r()
+        if o is not None:
+            try:
+                o.close()
+            except Exception:
+                pass
+    _live.clear()
 
 
 class Context:
@@ -25,6 +51,7 @@ class Context:
         h = C.c_void_p()
         call("hv_ctx_create", device, C.byref(h))
         self.handle = h
+        _track(self)
 
     def close(self) -> None:
         if self.handle:
@@ -87,6 +114,7 @@ class Stream:
         h = C.c_void_p()
         call("hv_stream_create", ctx.handle, C.byref(h))
         self.handle = h.value
+        _track(self)
 
     def synchronize(self) -> None:
         call("hv_stream_synchronize", self.handle)
@@ -111,6 +139,7 @@ class Scene:
         h = C.c_void_p()
         call("hv_scene_create", ctx.handle, capacity, C.byref(h))
         self.handle = h
+        _track(self)
 
     def close(self) -> None:
         if self.handle:
@@ -174,6 +203,7 @@ class HiZBuildPass:
         h = C.c_void_p()
         call("hv_hiz_create", ctx.handle, width, height, mode, C.byref(h))
         self.handle = h
+        _track(self)
         self.width, self.height, self.mode = width, height, mode
 
     def close(self) -> None:
@@ -245,6 +275,7 @@ class InstanceCullPass:
         h = C.c_void_p()
         call("hv_inst_cull_create", ctx.handle, C.byref(self.config), C.byref(h))
         self.handle = h
+        _track(self)
 
     def close(self) -> None:
         if self.handle:
@@ -323,6 +354,7 @@ class BatchCull:
         h = C.c_void_p()
         call("hv_batch_create", ctx.handle, C.byref(self.config), max_views, C.byref(h))
         self.handle = h
+        _track(self)
         self.n = 0
 
     def close(self) -> None:
@@ -402,6 +434,7 @@ class VirtualGeometryPass:
         h = C.c_void_p()
         call("hv_vg_create", ctx.handle, publication_limit, C.byref(h))
         self.handle = h
+        _track(self)
         self.version = 0
         self.instance_version = 0
         self._frame = None
@@ -491,6 +524,7 @@ class DeviceBuffer:
         p = C.c_void_p()
         call("hv_device_alloc", ctx.handle, max(self.nbytes, 4), C.byref(p))
         self.ptr = p.value
+        _track(self)
 
     @classmethod
     def from_array(cls, ctx: Context, a: np.ndarray) -> "DeviceBuffer":
@@ -548,6 +582,7 @@ class IndirectDispatchPass:
         h = C.c_void_p()
         call("hv_indirect_dispatch_create", ctx.handle, C.byref(h))
         self.handle = h
+        _track(self)
 
     def close(self):
         if self.handle:
@@ -581,6 +616,7 @@ class OcclusionCullPass:
         h = C.c_void_p()
         call("hv_occlusion_cull_create", ctx.handle, width, height, C.byref(h))
         self.handle = h
+        _track(self)
 
     def close(self):
         if self.handle:
@@ -622,6 +658,7 @@ class VgScene:
         h = C.c_void_p()
         call("hv_vg_scene_create", ctx.handle, C.byref(fd), C.byref(h))
         self.handle = h
+        _track(self)
         self.meshlet_count = len(keep[0])
 
     def close(self) -> None:
@@ -656,6 +693,7 @@ class CullServer:
         h = C.c_void_p()
         call("hv_server_create", ctx.handle, C.byref(c), C.byref(h))
         self.handle = h
+        _track(self)
         self._live = {}
 
     def close(self) -> None:
@@ -775,6 +813,7 @@ class ShadowMatrixPass:
         h = C.c_void_p()
         call("hv_shadow_matrix_create", ctx.handle, shadow_atlas_size, C.byref(h))
         self.handle = h
+        _track(self)
 
     def close(self):
         if self.handle:
@@ -810,6 +849,7 @@ class ShadowCullPass:
         h = C.c_void_p()
         call("hv_shadow_cull_create", ctx.handle, max_draws_per_face, plane_mode, C.byref(h))
         self.handle = h
+        _track(self)
 
     def close(self):
         if self.handle:
@@ -859,6 +899,7 @@ class SceneRebuild:
         h = C.c_void_p()
         call("hv_scene_rebuild_create", ctx.handle, C.byref(h))
         self.handle = h
+        _track(self)
         self.n = 0
 
     def close(self):
