@@ -6,7 +6,10 @@ meshlet part from the same camera) arriving as a Poisson stream at 20,000
 requests/s through the batched, graph-replayed server (run_trace). Every
 request's stats and args, and the ids and published meshlets of a sample
 that includes the 8M-instance scene, ortho views and meshlet parts, equal the
-oracle's (oracle.cull_view, or_vg_cull), not the plain HIP pass."""
+oracle's (oracle.cull_view, or_vg_cull), not the plain HIP pass. The same
+trace also runs through the benchmark's server configuration at saturation
+(six slots of up to 128 views, all requests arriving within 1.6 ms), whose
+results must equal the same oracle results."""
 from __future__ import annotations
 
 import concurrent.futures as cf
@@ -61,7 +64,7 @@ def test_serving_workload_full_size_matches_oracle(ctx):
 
     g = synth.rng(4, 2000)
     n_req = 320
-    reqs = []
+    specs = []
     for k in range(n_req):
         si = 7 if k % 40 == 0 else int(g.integers(0, len(SIZES)))
         ortho = g.random() < 0.25
@@ -71,9 +74,31 @@ def test_serving_workload_full_size_matches_oracle(ctx):
             v = synth.perspective_view(g)
         with_vg = g.random() < 0.25 or k % 40 == 20
         sampled = k % 8 == 0 or (with_vg and k % 3 == 0)
-        kw = dict(vg=vg, vg_capacity=VG_CAP, vg_outputs=sampled) if with_vg else {}
-        r, keep = CullServer.request(dev[si], v.camera, 0, int(SIZES[si]), want_ids=sampled, **kw)
-        reqs.append(dict(r=r, keep=keep, si=si, cam=v.camera, vg=with_vg, sampled=sampled, ortho=ortho))
+        specs.append(dict(si=si, cam=v.camera, vg=with_vg, sampled=sampled, ortho=ortho))
+
+    def requests():
+        out = []
+        for q in specs:
+            kw = dict(vg=vg, vg_capacity=VG_CAP, vg_outputs=q["sampled"]) if q["vg"] else {}
+            r, keep = CullServer.request(dev[q["si"]], q["cam"], 0, int(SIZES[q["si"]]), want_ids=q["sampled"], **kw)
+            out.append(dict(q, r=r, keep=keep))
+        return out
+
+    # The same trace through the benchmark's server at saturation: six slots of
+    # up to 128 views / 2^27 candidates, pools grown with the admitted load,
+    # every request arriving within 1.6 ms (batches of dozens of views).
+    reqs_sat = requests()
+    srv = CullServer(ctx, max_batch_views=128, max_inflight=6, max_batch_candidates=128 << 20,
+                     device_budget_bytes=16 << 30, copy_ids=True, use_graph=True, max_vg_draws=16 * VG_CAP,
+                     grow_pools=True)
+    lat, wall = srv.run_trace([q["r"] for q in reqs_sat], np.arange(n_req) * 5e-6)
+    st = srv.stats()
+    print(f"saturated trace: {n_req} requests in {wall * 1e3:.1f} ms, mean batch {st['mean_batch_views']:.2f}",
+          flush=True)
+    assert st["completed"] >= n_req and st["rejected"] == 0 and (lat > 0).all()
+    assert st["mean_batch_views"] > 8
+    srv.close()
+    reqs = requests()
     srv = CullServer(ctx, max_batch_views=64, max_inflight=3, max_batch_candidates=64 << 20,
                      device_budget_bytes=16 << 30, copy_ids=True, use_graph=True, max_vg_draws=16 * VG_CAP)
     arrivals = np.cumsum(np.random.default_rng(20_000).exponential(1.0 / 20_000, n_req))
@@ -90,14 +115,15 @@ def test_serving_workload_full_size_matches_oracle(ctx):
         sc = host[q["si"]]
         ov = oracle.view_from_camera(q["cam"], mesh_lods=sc.mesh_lods, mesh_count=sc.mesh_count)
         rids, rargs, rstats = oracle.cull_view(ov, sc.bounds, metas[q["si"]], 0, sc.n, threads=16)
-        _, ids, args, stats, _ = q["keep"]
-        assert np.array_equal(stats[:8], rstats), (k, stats, rstats)
-        assert stats[8] == len(rids), k
-        assert np.array_equal(args_as_u32(args), rargs), k
-        if q["sampled"]:
-            check_cull_invariants(ids[: stats[8]], args, stats[:8])
-            assert np.array_equal(canonical_ids(ids[: stats[8]], args), rids), k
-            n_ids += 1
+        for qq in (q, reqs_sat[k]):
+            _, ids, args, stats, _ = qq["keep"]
+            assert np.array_equal(stats[:8], rstats), (k, stats, rstats)
+            assert stats[8] == len(rids), k
+            assert np.array_equal(args_as_u32(args), rargs), k
+            if qq["sampled"]:
+                check_cull_invariants(ids[: stats[8]], args, stats[:8])
+                assert np.array_equal(canonical_ids(ids[: stats[8]], args), rids), k
+        n_ids += q["sampled"]
     icd = _oracle_icd(vgsc.instances)
     wi = build_vg_work_items(vgsc.objects)
     u = vg_uniforms(len(vgsc.objects), len(wi), VG_CAP, screen=(1920, 1080), threshold=0.5)
@@ -105,18 +131,21 @@ def test_serving_workload_full_size_matches_oracle(ctx):
     def vg_ref(q):
         return oracle_vg(q["cam"], u, vgsc.meshlets, vgsc.objects, vgsc.instances, icd, wi, None, VG_CAP)
 
-    vq = [q for q in reqs if q["vg"]]
+    vk = [k for k, q in enumerate(reqs) if q["vg"]]
     with cf.ThreadPoolExecutor(16) as ex:
-        refs = list(ex.map(vg_ref, vq))
+        refs = list(ex.map(vg_ref, [reqs[k] for k in vk]))
+    vq = [reqs[k] for k in vk]
     n_vg = 0
-    for q, (rind, rmd, rctr, _) in zip(vq, refs):
-        vgo = q["keep"][4]
-        assert np.array_equal(vgo[2], rctr), (vgo[2], rctr)
-        if q["sampled"]:
-            n = int(min(rctr[0], VG_CAP))
-            got = _vg_rows_u32(args_as_u32(vgo[0][:n]), np.ascontiguousarray(vgo[1][:n]).view(np.uint32).reshape(-1, 4))
-            assert np.array_equal(got, _vg_rows_u32(rind[:n], rmd[:n]))
-            n_vg += n > 0
+    for k, (rind, rmd, rctr, _) in zip(vk, refs):
+        for q in (reqs[k], reqs_sat[k]):
+            vgo = q["keep"][4]
+            assert np.array_equal(vgo[2], rctr), (vgo[2], rctr)
+            if q["sampled"]:
+                n = int(min(rctr[0], VG_CAP))
+                got = _vg_rows_u32(args_as_u32(vgo[0][:n]),
+                                   np.ascontiguousarray(vgo[1][:n]).view(np.uint32).reshape(-1, 4))
+                assert np.array_equal(got, _vg_rows_u32(rind[:n], rmd[:n]))
+        n_vg += int(reqs[k]["sampled"] and min(rctr[0], VG_CAP) > 0)
     assert n_ids >= 40 and n_vg >= 30  # sampled meshlet parts that published something
     assert any(q["sampled"] and q["si"] == 7 for q in reqs)
     assert any(q["sampled"] and q["ortho"] for q in reqs)
