@@ -275,3 +275,38 @@ def test_batch_occlusion_views_own_pyramids(ctx):
     for i in range(len(plan)):
         _check(b.read(i), refs[i])
     b.close()
+
+
+def test_batch_of_the_most_views(ctx):
+    """The largest batch, 1,024 views (a column-scan unit keeps the view in 10
+    bits), of uneven ranges (empty, one candidate, up to 5,000, and a few
+    ending at the scene's end), eager and replayed: every view equals the
+    oracle. A batch of 1,025 views can be neither created nor prepared."""
+    from helio_amd._lib import HvError
+
+    sc = synth.instance_scene(300_000, config=3, index=11)
+    scene = _scene(ctx, sc)
+    g = np.random.default_rng(1024)
+    cams = [synth.perspective_view(synth.rng(3, 100 + i)).camera for i in range(16)]
+    table = []
+    for i in range(1024):
+        c = 0 if i % 97 == 0 else 1 if i % 89 == 0 else int(np.exp(g.uniform(0, np.log(5000))))
+        f = sc.n - c if i % 101 == 0 else int(g.integers(0, sc.n - c + 1))
+        table.append((cams[i % 16], f, c))
+    with pytest.raises(HvError):
+        BatchCull(ctx, 1025)
+    b = BatchCull(ctx, 1024)
+    b.prepare([(scene, cam, f, c) for cam, f, c in table])
+    print(f"batch of {len(table)} views, {sum(c for _, _, c in table)} tests", flush=True)
+    refs = [_oracle(sc, cam, f, c) for cam, f, c in table]
+    b.execute()
+    for i in range(len(table)):
+        _check(b.read(i), refs[i])
+    b.execute(graph=True)
+    b.execute(graph=True)
+    for i in range(len(table)):
+        _check(b.read(i), refs[i])
+    with pytest.raises(HvError):
+        b.prepare([(scene, cams[0], 0, 10)] * 1025)
+    b.close()
+    scene.close()
