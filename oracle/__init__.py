@@ -6,13 +6,16 @@ this module, and only as the checker. The product (helio_amd/) never does.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "_build" / "libhv_oracle.so"
+# HV_ORACLE_LIB points the checker at another build of this restatement (the
+# mutation tests load deliberately altered copies through it).
+LIB_PATH = Path(os.environ["HV_ORACLE_LIB"]) if os.environ.get("HV_ORACLE_LIB") else HERE / "_build" / "libhv_oracle.so"
 
 u32, f32, vp = C.c_uint32, C.c_float, C.c_void_p
 
